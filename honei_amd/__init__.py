@@ -1,0 +1,255 @@
+"""honei_amd -- MI355X (gfx950) backend for HONEI's ELL sparse-solve path.
+
+The product is the shared library ``honei_amd/lib/libhonei_hip.so`` (HIP
+kernels + C++ host code) behind the C ABI in ``include/honei_hip.h``.  This
+module is a thin ctypes binding used by the tests and the benchmark; it
+raises immediately when the library is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libhonei_hip.so")
+
+F32, F64, BF16 = 0, 1, 2
+SUCCESS = 0
+ERRORS = {1: "INVALID_ARGUMENT", 2: "SIZE_MISMATCH", 3: "OUT_OF_MEMORY", 4: "RUNTIME",
+          5: "NOT_CONVERGED", 6: "BUDGET", 7: "NO_DEVICE"}
+
+_sz = C.c_size_t
+_p = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_u64 = C.c_uint64
+
+# name -> argtypes (restype is int unless listed in _RESTYPE)
+_SIGS = {
+    "honei_hip_device_count": [C.POINTER(_i)],
+    "honei_hip_init": [_i],
+    "honei_hip_shutdown": [],
+    "honei_hip_malloc": [C.POINTER(_p), _sz],
+    "honei_hip_free": [_p],
+    "honei_hip_upload": [_p, _p, _sz, _p],
+    "honei_hip_download": [_p, _p, _sz, _p],
+    "honei_hip_copy": [_p, _p, _sz, _p],
+    "honei_hip_fill_zero": [_p, _sz, _p],
+    "honei_hip_synchronize": [_p],
+    "honei_hip_stream_create": [C.POINTER(_p)],
+    "honei_hip_stream_destroy": [_p],
+    "honei_hip_event_create": [C.POINTER(_p)],
+    "honei_hip_event_destroy": [_p],
+    "honei_hip_event_record": [_p, _p],
+    "honei_hip_event_elapsed_ms": [C.POINTER(C.c_float), _p, _p],
+    "honei_hip_convert_double_float": [_p, _p, _sz, _p],
+    "honei_hip_convert_float_double": [_p, _p, _sz, _p],
+    "honei_hip_upload_indices": [_p, _p, _sz, _u64],
+    "honei_hip_upload_values_bf16": [_p, _p, _sz],
+    "honei_hip_product_smell_dv_float": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_product_smell_dv_double": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_product_smell_dv_bf16": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_defect_smell_dv_float": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_defect_smell_dv_double": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_defect_norm_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
+    "honei_hip_jacobi_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
+    "honei_hip_dot_product_two_float": [C.POINTER(C.c_float), _p, _p, _sz, _p],
+    "honei_hip_dot_product_two_double": [C.POINTER(_d), _p, _p, _sz, _p],
+    "honei_hip_norm_l2_one_float": [C.POINTER(C.c_float), _p, _sz, _p],
+    "honei_hip_norm_l2_one_double": [C.POINTER(_d), _p, _sz, _p],
+    "honei_hip_dot_product_dev": [_i, _p, _p, _p, _sz, _p],
+    "honei_hip_norm_l2_dev": [_i, _p, _p, _sz, _p],
+    "honei_hip_axpy_norm_dev": [_i, _p, _p, _d, _p, _sz, _p],
+    "honei_hip_scaled_sum_two": [_i, _p, _p, _d, _sz, _p],
+    "honei_hip_scaled_sum_three_s": [_i, _p, _p, _p, _d, _sz, _p],
+    "honei_hip_scaled_sum_three": [_i, _p, _p, _p, _sz, _p],
+    "honei_hip_sum_two": [_i, _p, _p, _sz, _p],
+    "honei_hip_difference_two": [_i, _p, _p, _sz, _p],
+    "honei_hip_difference_three": [_i, _p, _p, _p, _sz, _p],
+    "honei_hip_scale_one": [_i, _p, _d, _sz, _p],
+    "honei_hip_element_product_three": [_i, _p, _p, _p, _sz, _p],
+    "honei_hip_element_inverse_one": [_i, _p, _sz, _p],
+    "honei_hip_restriction": [_i, _p, _sz, _p, _sz, _p, _p],
+    "honei_hip_prolongation": [_i, _p, _sz, _p, _sz, _p, _p],
+}
+_RESTYPE = {"honei_hip_last_error": C.c_char_p, "honei_hip_version": C.c_char_p}
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, fn, msg):
+        super().__init__(f"{fn} failed: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the backend library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP backend library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _i
+        for name, rt in _RESTYPE.items():
+            f = getattr(L, name)
+            f.argtypes = []
+            f.restype = rt
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS) + list(_RESTYPE)
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != SUCCESS:
+        raise HipError(rc, name, lib().honei_hip_last_error().decode())
+    return rc
+
+
+def last_error() -> str:
+    return lib().honei_hip_last_error().decode()
+
+
+def init(device: int = 0):
+    call("honei_hip_init", device)
+
+
+def device_count() -> int:
+    n = _i(0)
+    rc = lib().honei_hip_device_count(C.byref(n))
+    return n.value if rc == SUCCESS else 0
+
+
+def synchronize(stream=None):
+    call("honei_hip_synchronize", stream)
+
+
+class Event:
+    def __init__(self):
+        p = _p()
+        call("honei_hip_event_create", C.byref(p))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        call("honei_hip_event_record", self.ptr, stream)
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = C.c_float()
+        call("honei_hip_event_elapsed_ms", C.byref(ms), self.ptr, stop.ptr)
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().honei_hip_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Owning device buffer with a numpy dtype (test and bench plumbing)."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = _p()
+        call("honei_hip_malloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            call("honei_hip_upload", d.ptr, a.ctypes.data_as(_p), d.nbytes, None)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        call("honei_hip_upload", self.ptr, a.ctypes.data_as(_p), self.nbytes, None)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            call("honei_hip_download", out.ctypes.data_as(_p), self.ptr, self.nbytes, None)
+        return out
+
+    def zero(self, stream=None):
+        call("honei_hip_fill_zero", self.ptr, self.nbytes, stream)
+
+    def free(self):
+        if self.ptr:
+            lib().honei_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceEll:
+    """ELL matrix resident in HBM: int32 Aj, uint32 Arl, values f32/f64/bf16.
+
+    Built from the reference's host layout (uint64 Aj/Arl, same stride and
+    slot order); indices are narrowed and range-checked on upload."""
+
+    def __init__(self, rows, cols, stride, ncpr, threads, Aj, Ax, Arl, vtype=None):
+        self.rows, self.cols, self.stride, self.ncpr, self.threads = (int(rows), int(cols), int(stride),
+                                                                      int(ncpr), int(threads))
+        Ax = np.ascontiguousarray(Ax)
+        if vtype is None:
+            vtype = F64 if Ax.dtype == np.float64 else F32
+        self.vtype = vtype
+        n = Ax.size
+        self.Aj = DeviceArray(n, np.int32)
+        call("honei_hip_upload_indices", self.Aj.ptr, np.ascontiguousarray(Aj, np.uint64).ctypes.data_as(_p),
+             n, max(self.cols, 1))
+        self.Arl = DeviceArray(self.rows, np.uint32)
+        call("honei_hip_upload_indices", self.Arl.ptr, np.ascontiguousarray(Arl, np.uint64).ctypes.data_as(_p),
+             self.rows, self.ncpr + 1)
+        if vtype == BF16:
+            self.Ax = DeviceArray(n, np.uint16)
+            call("honei_hip_upload_values_bf16", self.Ax.ptr,
+                 np.ascontiguousarray(Ax, np.float32).ctypes.data_as(_p), n)
+        else:
+            self.Ax = DeviceArray.from_host(Ax, np.float64 if vtype == F64 else np.float32)
+
+    @classmethod
+    def from_oracle(cls, A, vtype=None):
+        return cls(A.rows, A.cols, A.stride, A.ncpr, A.threads, A.Aj, A.Ax, A.Arl, vtype)
+
+    @property
+    def vec_dtype(self):
+        return np.float64 if self.vtype == F64 else np.float32
+
+    def product(self, y: DeviceArray, x: DeviceArray, row_start=0, row_end=None, blocksize=0, stream=None):
+        fn = {F32: "honei_hip_product_smell_dv_float", F64: "honei_hip_product_smell_dv_double",
+              BF16: "honei_hip_product_smell_dv_bf16"}[self.vtype]
+        call(fn, x.ptr, y.ptr, self.Aj.ptr, self.Ax.ptr, self.Arl.ptr, row_start,
+             self.rows if row_end is None else row_end, self.ncpr, self.stride, blocksize, self.threads, stream)
+
+    def defect(self, r: DeviceArray, rhs: DeviceArray, x: DeviceArray, stream=None):
+        fn = {F32: "honei_hip_defect_smell_dv_float", F64: "honei_hip_defect_smell_dv_double"}[self.vtype]
+        call(fn, rhs.ptr, r.ptr, self.Aj.ptr, self.Ax.ptr, self.Arl.ptr, x.ptr, 0, self.rows, self.ncpr,
+             self.stride, 0, self.threads, stream)
+
+    def defect_norm(self, r, rhs, x, norm_dev: DeviceArray, stream=None):
+        call("honei_hip_defect_norm_smell_dv", self.vtype, rhs.ptr, r.ptr, self.Aj.ptr, self.Ax.ptr, self.Arl.ptr,
+             x.ptr, self.rows, self.ncpr, self.stride, self.threads, norm_dev.ptr, stream)
+
+    def jacobi(self, x_new, x, rhs, dinv, acc=None, stream=None):
+        call("honei_hip_jacobi_smell_dv", self.vtype, x_new.ptr, None if x is None else x.ptr, rhs.ptr, dinv.ptr,
+             self.Aj.ptr, self.Ax.ptr, self.Arl.ptr, self.rows, self.ncpr, self.stride, self.threads,
+             None if acc is None else acc.ptr, stream)

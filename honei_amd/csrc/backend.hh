@@ -1,0 +1,96 @@
+// backend.hh -- internal C++ launch API of the MI355X backend.
+//
+// Every function enqueues work on `stream` and returns a hipError_t; none of
+// them synchronises.  Device pointers are borrowed.  ELL matrices are in the
+// reference's column-major layout (la/sparse_matrix_ell.hh:44-129): entry k
+// of lane t of row r sits at slot r*T + t + k*stride, with 32-bit column
+// indices and 32-bit row lengths (Arl, in units of T-wide chunks).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "hip_common.hh"
+
+namespace honei {
+namespace hipx {
+
+enum ValueType { VT_F32 = 0, VT_F64 = 1, VT_BF16 = 2 };
+
+struct EllView {
+    int64_t rows, cols, stride, ncpr;
+    int threads;
+    const int32_t *Aj;
+    const void *Ax;
+    const uint32_t *Arl;
+    ValueType vtype;
+};
+
+// What the ELL row kernel does with the row sum s = sum_j A_rj x_j.
+enum EllMode {
+    ELL_PRODUCT = 0,   // out[r - row_start] = s
+    ELL_DEFECT = 1,    // out[r] = rhs[r] - s            (+ ||out||^2 if norm)
+    ELL_JACOBI = 2,    // out[r] = x[r] + dinv[r]*(rhs[r] - s)   (+ acc[r] += out[r])
+    ELL_JACOBI0 = 3,   // as JACOBI with x = dinv .* rhs formed on the fly
+};
+
+struct EllArgs {
+    EllMode mode = ELL_PRODUCT;
+    void *out = nullptr;          // y / r / x_new (vector type)
+    const void *x = nullptr;      // input vector (unused for JACOBI0)
+    const void *rhs = nullptr;
+    const void *dinv = nullptr;
+    void *acc = nullptr;          // JACOBI*: acc += out when non-null
+    double *norm2 = nullptr;      // DEFECT: ||out||^2 into this device double
+    int64_t row_start = 0, row_end = 0;
+    int block = 256;
+};
+
+// Vector element type follows the matrix: f32 and bf16 matrices act on f32
+// vectors, f64 matrices on f64 vectors.
+hipError_t ell_apply(const EllView &A, const EllArgs &a, const ReduceSlots &slots, hipStream_t s);
+
+// ---- BLAS-1 / reductions (double accumulation, deterministic order) -------
+// Each reduction writes its result into a device double (out) and, when
+// out_host_pinned is non-null, nowhere else; callers read it back when needed.
+hipError_t dot(const void *x, const void *y, int64_t n, ValueType vt, double *out,
+               const ReduceSlots &slots, hipStream_t s);
+hipError_t norm2(const void *x, int64_t n, ValueType vt, double *out, const ReduceSlots &slots,
+                 hipStream_t s);
+// x += b*y ; b either a host value (b_dev null) or read from device memory.
+hipError_t axpy(void *x, const void *y, double b, const double *b_dev, int64_t n, ValueType vt,
+                hipStream_t s);
+// r = x + b*y
+hipError_t scaled_sum3(void *r, const void *x, const void *y, double b, int64_t n, ValueType vt,
+                       hipStream_t s);
+// a += b .* c
+hipError_t scaled_sum_v(void *a, const void *b, const void *c, int64_t n, ValueType vt, hipStream_t s);
+hipError_t vsum(void *x, const void *y, int64_t n, ValueType vt, hipStream_t s);          // x += y
+hipError_t vdiff2(void *x, const void *y, int64_t n, ValueType vt, hipStream_t s);        // x -= y
+hipError_t vdiff3(void *r, const void *x, const void *y, int64_t n, ValueType vt, hipStream_t s);  // r = x - y
+hipError_t vscale(void *x, double a, int64_t n, ValueType vt, hipStream_t s);             // x *= a
+hipError_t vprod(void *r, const void *x, const void *y, int64_t n, ValueType vt, hipStream_t s);  // r = x.*y
+hipError_t vinverse(void *x, int64_t n, ValueType vt, hipStream_t s);                      // x = 1/x
+hipError_t vfill(void *x, double v, int64_t n, ValueType vt, hipStream_t s);
+hipError_t vcopy(void *dst, const void *src, int64_t n, ValueType vt, hipStream_t s);
+hipError_t convert_d2f(float *dst, const double *src, int64_t n, hipStream_t s);
+hipError_t convert_f2d(double *dst, const float *src, int64_t n, hipStream_t s);
+hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
+// x += alpha*y and return ||x||^2 (fused; alpha from device memory when non-null)
+hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_dev, int64_t n,
+                      ValueType vt, double *out, const ReduceSlots &slots, hipStream_t s);
+
+// ---- geometric transfers on (2^L+1)^2 lexicographic grids --------------
+// mask: 8 entries (corners then edges) on the host; value 2 = Dirichlet.
+hipError_t restriction(void *coarse, int64_t n_coarse, const void *fine, int64_t n_fine,
+                       const uint64_t *mask, ValueType vt, hipStream_t s);
+hipError_t prolongation(void *fine, int64_t n_fine, const void *coarse, int64_t n_coarse,
+                        const uint64_t *mask, ValueType vt, hipStream_t s);
+// fine = P coarse (+ Dirichlet mask); x += alpha*fine fused when x non-null.
+hipError_t prolongation_axpy(void *fine, int64_t n_fine, const void *coarse, int64_t n_coarse,
+                             const uint64_t *mask, void *x, double alpha, ValueType vt, hipStream_t s);
+
+}  // namespace hipx
+}  // namespace honei

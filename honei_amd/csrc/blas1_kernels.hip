@@ -1,0 +1,202 @@
+// blas1_kernels.hip -- streaming vector kernels and single-pass reductions.
+//
+// Elementwise kernels are grid-stride loops over 64-wide wavefronts.
+// Reductions accumulate in double, combine per block in a fixed tree,
+// publish one partial per block and let the last-arriving block form the
+// total on the device (no host round trip, no per-call allocation -- unlike
+// backends/cuda/dot_product.cu:95-109, which copies grid*block partials to the
+// host).  The grid size depends only on n, so results are reproducible.
+
+#include "backend.hh"
+
+#pragma clang fp contract(off)
+
+namespace honei {
+namespace hipx {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int stream_grid(int64_t n) {
+    int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return (int)g;
+}
+inline int reduce_grid(int64_t n) {
+    int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+    if (g < 1) g = 1;
+    if (g > 1024) g = 1024;
+    return (int)g;
+}
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+template <typename V> __global__ void k_axpy(V *x, const V *y, V b, const double *b_dev, int64_t n) {
+    const V bb = b_dev ? (V)*b_dev : b;
+    GRID_STRIDE(i, n) x[i] += y[i] * bb;
+}
+template <typename V> __global__ void k_scaled_sum3(V *r, const V *x, const V *y, V b, int64_t n) {
+    GRID_STRIDE(i, n) r[i] = x[i] + y[i] * b;
+}
+template <typename V> __global__ void k_scaled_sum_v(V *a, const V *b, const V *c, int64_t n) {
+    GRID_STRIDE(i, n) a[i] += b[i] * c[i];
+}
+template <typename V> __global__ void k_sum(V *x, const V *y, int64_t n) { GRID_STRIDE(i, n) x[i] += y[i]; }
+template <typename V> __global__ void k_diff2(V *x, const V *y, int64_t n) { GRID_STRIDE(i, n) x[i] -= y[i]; }
+template <typename V> __global__ void k_diff3(V *r, const V *x, const V *y, int64_t n) {
+    GRID_STRIDE(i, n) r[i] = x[i] - y[i];
+}
+template <typename V> __global__ void k_scale(V *x, V a, int64_t n) { GRID_STRIDE(i, n) x[i] *= a; }
+template <typename V> __global__ void k_prod(V *r, const V *x, const V *y, int64_t n) {
+    GRID_STRIDE(i, n) r[i] = x[i] * y[i];
+}
+template <typename V> __global__ void k_inverse(V *x, int64_t n) { GRID_STRIDE(i, n) x[i] = V(1) / x[i]; }
+template <typename V> __global__ void k_fill(V *x, V v, int64_t n) { GRID_STRIDE(i, n) x[i] = v; }
+template <typename V> __global__ void k_copy(V *d, const V *s, int64_t n) { GRID_STRIDE(i, n) d[i] = s[i]; }
+__global__ void k_d2f(float *d, const double *s, int64_t n) { GRID_STRIDE(i, n) d[i] = (float)s[i]; }
+__global__ void k_f2d(double *d, const float *s, int64_t n) { GRID_STRIDE(i, n) d[i] = (double)s[i]; }
+__global__ void k_zero_bytes(unsigned char *p, size_t n) {
+    GRID_STRIDE(i, (int64_t)n) p[i] = 0;
+}
+__global__ void k_zero_words(uint32_t *p, size_t n) {
+    GRID_STRIDE(i, (int64_t)n) p[i] = 0u;
+}
+
+// MODE 0: sum x*y ; 1: sum x*x ; 2: x += a*y then sum x*x
+template <typename V, int MODE>
+__global__ __launch_bounds__(kBlock) void k_reduce(V *x, const V *y, V a, const double *a_dev, int64_t n,
+                                                   double *out, ReduceSlots slots) {
+    double acc = 0.0;
+    V aa = a;
+    if constexpr (MODE == 2) {
+        if (a_dev) aa = (V)*a_dev;
+    }
+    GRID_STRIDE(i, n) {
+        if constexpr (MODE == 0) {
+            acc += (double)x[i] * (double)y[i];
+        } else if constexpr (MODE == 1) {
+            acc += (double)x[i] * (double)x[i];
+        } else {
+            const V v = x[i] + y[i] * aa;
+            x[i] = v;
+            acc += (double)v * (double)v;
+        }
+    }
+    __shared__ double bsum;
+    const double b = block_sum(acc);
+    if (threadIdx.x == 0) bsum = b;
+    __syncthreads();
+    if (arrive_last(slots, &bsum, 1)) {
+        const double tot = finish_partial(slots, 1, 0);
+        if (threadIdx.x == 0) *out = tot;
+        reset_counter(slots);
+    }
+}
+
+template <typename F>
+hipError_t by_type(ValueType vt, F &&f) {
+    if (vt == VT_F32) return f(float(0));
+    if (vt == VT_F64) return f(double(0));
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t dot(const void *x, const void *y, int64_t n, ValueType vt, double *out, const ReduceSlots &sl,
+               hipStream_t s) {
+    return by_type(vt, [&](auto z) {
+        typedef decltype(z) V;
+        hipLaunchKernelGGL((k_reduce<V, 0>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
+                           V(0), (const double *)nullptr, n, out, sl);
+        return hipGetLastError();
+    });
+}
+hipError_t norm2(const void *x, int64_t n, ValueType vt, double *out, const ReduceSlots &sl, hipStream_t s) {
+    return by_type(vt, [&](auto z) {
+        typedef decltype(z) V;
+        hipLaunchKernelGGL((k_reduce<V, 1>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)x,
+                           V(0), (const double *)nullptr, n, out, sl);
+        return hipGetLastError();
+    });
+}
+hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_dev, int64_t n, ValueType vt,
+                      double *out, const ReduceSlots &sl, hipStream_t s) {
+    return by_type(vt, [&](auto z) {
+        typedef decltype(z) V;
+        hipLaunchKernelGGL((k_reduce<V, 2>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
+                           (V)alpha, alpha_dev, n, out, sl);
+        return hipGetLastError();
+    });
+}
+
+#define ELEMENTWISE(launch)                                              \
+    return by_type(vt, [&](auto z) {                                    \
+        typedef decltype(z) V;                                           \
+        (void)sizeof(V);                                                 \
+        launch;                                                          \
+        return hipGetLastError();                                        \
+    })
+
+hipError_t axpy(void *x, const void *y, double b, const double *b_dev, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_axpy<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
+                                   (V)b, b_dev, n));
+}
+hipError_t scaled_sum3(void *r, const void *x, const void *y, double b, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_scaled_sum3<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)r,
+                                   (const V *)x, (const V *)y, (V)b, n));
+}
+hipError_t scaled_sum_v(void *a, const void *b, const void *c, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_scaled_sum_v<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)a,
+                                   (const V *)b, (const V *)c, n));
+}
+hipError_t vsum(void *x, const void *y, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_sum<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y, n));
+}
+hipError_t vdiff2(void *x, const void *y, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_diff2<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y, n));
+}
+hipError_t vdiff3(void *r, const void *x, const void *y, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_diff3<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)r, (const V *)x,
+                                   (const V *)y, n));
+}
+hipError_t vscale(void *x, double a, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_scale<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (V)a, n));
+}
+hipError_t vprod(void *r, const void *x, const void *y, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_prod<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)r, (const V *)x,
+                                   (const V *)y, n));
+}
+hipError_t vinverse(void *x, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_inverse<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, n));
+}
+hipError_t vfill(void *x, double v, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_fill<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (V)v, n));
+}
+hipError_t vcopy(void *d, const void *src, int64_t n, ValueType vt, hipStream_t s) {
+    ELEMENTWISE(hipLaunchKernelGGL((k_copy<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)d,
+                                   (const V *)src, n));
+}
+hipError_t convert_d2f(float *d, const double *src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_d2f, dim3(stream_grid(n)), dim3(kBlock), 0, s, d, src, n);
+    return hipGetLastError();
+}
+hipError_t convert_f2d(double *d, const float *src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_f2d, dim3(stream_grid(n)), dim3(kBlock), 0, s, d, src, n);
+    return hipGetLastError();
+}
+hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if (((uintptr_t)p % 4 == 0) && bytes % 4 == 0) {
+        hipLaunchKernelGGL(k_zero_words, dim3(stream_grid((int64_t)bytes / 4)), dim3(kBlock), 0, s, (uint32_t *)p,
+                           bytes / 4);
+    } else {
+        hipLaunchKernelGGL(k_zero_bytes, dim3(stream_grid((int64_t)bytes)), dim3(kBlock), 0, s,
+                           (unsigned char *)p, bytes);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hipx
+}  // namespace honei

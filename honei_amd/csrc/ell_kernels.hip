@@ -1,0 +1,186 @@
+// ell_kernels.hip -- ELL row kernels for gfx950: SpMV, defect, fused
+// defect+norm and fused Jacobi sweeps.
+//
+// One row is served by T consecutive lanes of a 64-wide wavefront (T a power
+// of two, T <= 64, matching the matrix's `threads` layout).  Lane t of row r
+// walks slots r*T + t + k*stride for k < Arl[r]; consecutive lanes therefore
+// read consecutive addresses of each column-major ELL stream, which the
+// hardware coalesces into full 64-byte-line requests.  Ax/Aj are streamed with
+// non-temporal loads so that the gathered vector keeps L2/MALL.  Partial sums
+// of the T lanes are combined with xor-butterflies, which for lane 0 adds the
+// same pairs in the same order as the reference's shared-memory tree
+// (backends/cuda/product.cu:176-198).  Multiply and add stay separate
+// roundings (no FMA contraction) so that for T = 1 the result is
+// bit-identical to Product<tags::CPU> (la/product.hh:148-206).
+
+#include "backend.hh"
+
+#pragma clang fp contract(off)
+
+namespace honei {
+namespace hipx {
+namespace {
+
+template <typename VT> struct VecOf { typedef VT type; };
+template <> struct VecOf<bf16> { typedef float type; };
+
+struct KArgs {
+    const int32_t *Aj;
+    const void *Ax;
+    const uint32_t *Arl;
+    int64_t stride;
+    void *out;
+    const void *x;
+    const void *rhs;
+    const void *dinv;
+    void *acc;
+    double *norm2;
+    int64_t row_start, row_end;
+    ReduceSlots slots;
+};
+
+template <typename VT, int T, int MODE, bool NORM, bool ACC>
+__global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
+    typedef typename VecOf<VT>::type V;
+    const VT *__restrict__ Ax = (const VT *)a.Ax;
+    const int32_t *__restrict__ Aj = a.Aj;
+    const V *__restrict__ x = (const V *)a.x;
+    const V *__restrict__ rhs = (const V *)a.rhs;
+    const V *__restrict__ dinv = (const V *)a.dinv;
+
+    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = (int)(gid & (T - 1));
+    const int64_t row = a.row_start + gid / T;
+    const bool active = row < a.row_end;
+
+    V s = V(0);
+    if (active) {
+        const uint32_t len = a.Arl[row];
+        int64_t slot = row * T + lane;
+#pragma unroll 4
+        for (uint32_t k = 0; k < len; ++k) {
+            const int32_t j = __builtin_nontemporal_load(Aj + slot);
+            const V av = load_value_nt(Ax + slot);
+            V xv;
+            if constexpr (MODE == ELL_JACOBI0)
+                xv = dinv[j] * rhs[j];
+            else
+                xv = x[j];
+            s += av * xv;
+            slot += a.stride;
+        }
+    }
+    if constexpr (T > 1) {
+#pragma unroll
+        for (int off = T / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    }
+
+    double nval = 0.0;
+    if (active && lane == 0) {
+        if constexpr (MODE == ELL_PRODUCT) {
+            ((V *)a.out)[row - a.row_start] = s;
+        } else if constexpr (MODE == ELL_DEFECT) {
+            const V r = rhs[row] - s;
+            ((V *)a.out)[row] = r;
+            if constexpr (NORM) nval = (double)r * (double)r;
+        } else {
+            const V xin = (MODE == ELL_JACOBI0) ? dinv[row] * rhs[row] : x[row];
+            const V t = rhs[row] - s;
+            const V o = xin + t * dinv[row];
+            ((V *)a.out)[row] = o;
+            if constexpr (ACC) {
+                V *acc = (V *)a.acc;
+                acc[row] = acc[row] + o;
+            }
+        }
+    }
+    if constexpr (NORM) {
+        __shared__ double bsum;
+        const double v = block_sum(nval);
+        if (threadIdx.x == 0) bsum = v;
+        __syncthreads();
+        if (arrive_last(a.slots, &bsum, 1)) {
+            const double tot = finish_partial(a.slots, 1, 0);
+            if (threadIdx.x == 0) *a.norm2 = tot;
+            reset_counter(a.slots);
+        }
+    }
+}
+
+typedef void (*KernelFn)(KArgs);
+
+template <typename VT, int T>
+KernelFn pick_mode(EllMode mode, bool norm, bool acc) {
+    switch (mode) {
+        case ELL_PRODUCT: return ell_row_kernel<VT, T, ELL_PRODUCT, false, false>;
+        case ELL_DEFECT:
+            return norm ? ell_row_kernel<VT, T, ELL_DEFECT, true, false>
+                        : ell_row_kernel<VT, T, ELL_DEFECT, false, false>;
+        case ELL_JACOBI:
+            return acc ? ell_row_kernel<VT, T, ELL_JACOBI, false, true>
+                       : ell_row_kernel<VT, T, ELL_JACOBI, false, false>;
+        case ELL_JACOBI0:
+            return acc ? ell_row_kernel<VT, T, ELL_JACOBI0, false, true>
+                       : ell_row_kernel<VT, T, ELL_JACOBI0, false, false>;
+    }
+    return nullptr;
+}
+
+template <typename VT>
+KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc) {
+    switch (T) {
+        case 1: return pick_mode<VT, 1>(mode, norm, acc);
+        case 2: return pick_mode<VT, 2>(mode, norm, acc);
+        case 4: return pick_mode<VT, 4>(mode, norm, acc);
+        case 8: return pick_mode<VT, 8>(mode, norm, acc);
+        case 16: return pick_mode<VT, 16>(mode, norm, acc);
+        case 32: return pick_mode<VT, 32>(mode, norm, acc);
+        case 64: return pick_mode<VT, 64>(mode, norm, acc);
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slots, hipStream_t s) {
+    const int T = A.threads;
+    const bool norm = (e.mode == ELL_DEFECT) && e.norm2 != nullptr;
+    const bool acc = (e.mode == ELL_JACOBI || e.mode == ELL_JACOBI0) && e.acc != nullptr;
+    KernelFn fn = nullptr;
+    switch (A.vtype) {
+        case VT_F32: fn = pick_threads<float>(T, e.mode, norm, acc); break;
+        case VT_F64: fn = pick_threads<double>(T, e.mode, norm, acc); break;
+        case VT_BF16: fn = pick_threads<bf16>(T, e.mode, norm, acc); break;
+    }
+    if (!fn) return hipErrorInvalidValue;
+    const int64_t nrows = e.row_end - e.row_start;
+    int block = e.block > 0 ? e.block : 256;
+    if (block % kWave != 0 || block > 1024 || block < T) return hipErrorInvalidValue;
+    if (nrows <= 0) {
+        if (norm) return fill_bytes_zero(e.norm2, sizeof(double), s);
+        return hipSuccess;
+    }
+    const int64_t grid = (nrows * T + block - 1) / block;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    if (norm && grid > slots.capacity) return hipErrorInvalidValue;
+    KArgs k;
+    k.Aj = A.Aj;
+    k.Ax = A.Ax;
+    k.Arl = A.Arl;
+    k.stride = A.stride;
+    k.out = e.out;
+    k.x = e.x;
+    k.rhs = e.rhs;
+    k.dinv = e.dinv;
+    k.acc = e.acc;
+    k.norm2 = e.norm2;
+    k.row_start = e.row_start;
+    k.row_end = e.row_end;
+    k.slots = slots;
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
+    return hipGetLastError();
+}
+
+}  // namespace hipx
+}  // namespace honei

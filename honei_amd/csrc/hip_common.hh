@@ -1,0 +1,112 @@
+// hip_common.hh -- shared device/host helpers of the MI355X (gfx950) backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace honei {
+namespace hipx {
+
+constexpr int kWave = 64;       // CDNA wavefront width
+constexpr int kXcds = 8;        // MI355X: 8 XCDs, each with its own L2
+
+// bf16 storage (matrix values only; arithmetic is FP32).
+struct bf16 {
+    uint16_t bits;
+};
+
+__host__ __device__ inline float to_float(float v) { return v; }
+__host__ __device__ inline double to_double(double v) { return v; }
+__device__ inline float bf16_to_float(uint16_t b) {
+    return __uint_as_float(((uint32_t)b) << 16);
+}
+
+// Accumulation type of a value type.
+template <typename VT> struct AccOf { typedef VT type; };
+template <> struct AccOf<bf16> { typedef float type; };
+
+template <typename VT> __device__ inline typename AccOf<VT>::type load_value_nt(const VT *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <> __device__ inline float load_value_nt<bf16>(const bf16 *p) {
+    return bf16_to_float(__builtin_nontemporal_load(&p->bits));
+}
+
+// The hardware hands work-group b to XCD b % 8.  Give every XCD one
+// contiguous range of logical blocks instead, so rows that share x entries
+// (neighbouring grid rows) meet in the same L2.  Bijective for any count.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nblocks) {
+    const int64_t q = nblocks / kXcds, r = nblocks % kXcds;
+    const int64_t x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Block-wide sum (any blockDim multiple of 64, at most 1024); result valid in
+// every thread.  Fixed order, hence deterministic.
+template <typename T> __device__ inline T block_sum(T v) {
+    __shared__ T red[1024 / kWave];
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    const int nw = blockDim.x / kWave;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    T t = (threadIdx.x < (unsigned)nw) ? red[threadIdx.x] : T(0);
+    if (w == 0) t = wave_sum(t);
+    if (threadIdx.x == 0) red[0] = t;
+    __syncthreads();
+    t = red[0];
+    __syncthreads();
+    return t;
+}
+
+// Cross-block reduction slots: one partial per block plus an arrival counter
+// that the last block resets, so no memset is needed between calls.
+struct ReduceSlots {
+    double *partials;      // [capacity][lanes]
+    unsigned int *counter; // 1 counter
+    int capacity;          // max blocks
+};
+
+// Publish this block's partials (lanes values, thread i < lanes holds value
+// i) and return true in every thread of the block that arrived last; that
+// block may then read all partials with finish_partial().
+__device__ inline bool arrive_last(const ReduceSlots &s, const double *vals, int lanes) {
+    __shared__ bool last;
+    if (threadIdx.x < (unsigned)lanes)
+        __hip_atomic_store(&s.partials[(size_t)blockIdx.x * lanes + threadIdx.x], vals[threadIdx.x],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int prev =
+            __hip_atomic_fetch_add(s.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    return last;
+}
+
+// Inside the last block: deterministic sum of partial `lane` over all blocks.
+__device__ inline double finish_partial(const ReduceSlots &s, int lanes, int lane) {
+    double v = 0.0;
+    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+        v += __hip_atomic_load(&s.partials[(size_t)b * lanes + lane], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    return block_sum(v);
+}
+
+__device__ inline void reset_counter(const ReduceSlots &s) {
+    if (threadIdx.x == 0) __hip_atomic_store(s.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace hipx
+}  // namespace honei

@@ -1,0 +1,172 @@
+/*
+ * honei_hip.h -- C ABI of the MI355X (gfx950) backend for HONEI's ELL
+ * sparse-solve path.
+ *
+ * Plain pointers and sizes only.  Pointers named *_dev or documented as
+ * device pointers are device memory obtained from honei_hip_malloc (or any
+ * hipMalloc); `stream` is a hipStream_t passed as void* (NULL = default
+ * stream).  Every function returns an int status (HONEI_HIP_SUCCESS = 0);
+ * no C++ exception crosses this boundary.  honei_hip_last_error() describes
+ * the last failure on the calling thread.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the HONEI source tree).  Differences in argument
+ * convention are stated next to the declaration.
+ */
+#ifndef HONEI_HIP_H
+#define HONEI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status -------------------------------------------------------------- */
+#define HONEI_HIP_SUCCESS 0
+#define HONEI_HIP_ERROR_INVALID_ARGUMENT 1 /* bad pointer / size / layout       */
+#define HONEI_HIP_ERROR_SIZE_MISMATCH 2    /* VectorSizeDoesNotMatch analogue    */
+#define HONEI_HIP_ERROR_OUT_OF_MEMORY 3
+#define HONEI_HIP_ERROR_RUNTIME 4          /* HIP runtime / launch failure       */
+#define HONEI_HIP_ERROR_NOT_CONVERGED 5    /* iteration limit reached            */
+#define HONEI_HIP_ERROR_BUDGET 6           /* request exceeds the memory budget  */
+#define HONEI_HIP_ERROR_NO_DEVICE 7
+
+/* value/vector type codes used by the generic entry points */
+#define HONEI_HIP_F32 0
+#define HONEI_HIP_F64 1
+#define HONEI_HIP_BF16 2 /* matrix values only; vectors are f32 */
+
+const char *honei_hip_last_error(void);
+const char *honei_hip_version(void);
+
+/* ---- device and memory -------------------------------------------------
+ * Replace backends/cuda/multi_gpu.cu:23-71 (device count/select),
+ * backends/cuda/operations.cu:22-26 (cuda_thread_synchronize) and
+ * backends/cuda/transfer.cu:49-160 (cuda_malloc, cuda_upload, cuda_download,
+ * cuda_copy, cuda_free, cuda_fill_zero, cuda_convert_*).  The fill is a
+ * kernel, not hipMemset, so it may be captured in a hipGraph.            */
+int honei_hip_device_count(int *count);
+int honei_hip_init(int device);
+int honei_hip_shutdown(void);
+int honei_hip_malloc(void **ptr, size_t bytes);
+int honei_hip_free(void *ptr);
+int honei_hip_upload(void *dst_dev, const void *src_host, size_t bytes, void *stream);
+int honei_hip_download(void *dst_host, const void *src_dev, size_t bytes, void *stream);
+int honei_hip_copy(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
+int honei_hip_fill_zero(void *dst_dev, size_t bytes, void *stream);
+int honei_hip_synchronize(void *stream);
+int honei_hip_stream_create(void **stream);
+int honei_hip_stream_destroy(void *stream);
+/* HIP events on a given stream, for timing launches where they run.       */
+int honei_hip_event_create(void **event);
+int honei_hip_event_destroy(void *event);
+int honei_hip_event_record(void *event, void *stream);
+int honei_hip_event_elapsed_ms(float *ms, void *start, void *stop); /* waits for stop */
+int honei_hip_convert_double_float(float *dst_dev, const double *src_dev, size_t n, void *stream);
+int honei_hip_convert_float_double(double *dst_dev, const float *src_dev, size_t n, void *stream);
+
+/* ELL index upload.  The reference keeps Aj/Arl as 64-bit unsigned long on
+ * the device (la/sparse_matrix_ell.hh:110-116); this backend stores 32-bit
+ * indices.  Narrows on the host, checks every value is < bound (columns for
+ * Aj; pass UINT64_MAX to skip) and uploads.  Synchronous.                 */
+int honei_hip_upload_indices(void *dst_dev_u32, const uint64_t *src_host, size_t n, uint64_t bound);
+/* Matrix values stored as bf16 (round to nearest even) for the optional
+ * reduced-storage FP32 path.  Synchronous.                                 */
+int honei_hip_upload_values_bf16(void *dst_dev_bf16, const float *src_host, size_t n);
+
+/* ---- ELL operators -----------------------------------------------------
+ * Aj: int32 [ncpr*stride], Arl: uint32 [rows], Ax: values [ncpr*stride];
+ * threads (T) in {1,2,4,8,16,32,64} must equal the matrix layout's T.
+ * blocksize 0 = default (256), else a multiple of 64 in [T, 1024].
+ *
+ * Replaces cuda_product_smell_dv_{float,double}
+ * (backends/cuda/operations.hh:87-93): y[r - row_start] = (A x)[r] for
+ * r in [row_start, row_end), the reference's sub-range convention
+ * (backends/cuda/product.cu:194).  _bf16: Ax bf16, x/y float.            */
+int honei_hip_product_smell_dv_float(void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                     size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
+                                     size_t blocksize, size_t threads, void *stream);
+int honei_hip_product_smell_dv_double(void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                      size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
+                                      size_t blocksize, size_t threads, void *stream);
+int honei_hip_product_smell_dv_bf16(void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                    size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
+                                    size_t blocksize, size_t threads, void *stream);
+
+/* Replaces cuda_defect_smell_dv_{float,double}
+ * (backends/cuda/operations.hh:41-47): result[r] = rhs[r] - (A b)[r]; same
+ * argument order (rhs, result, Aj, Ax, Arl, b).                            */
+int honei_hip_defect_smell_dv_float(void *rhs, void *result, const void *Aj, const void *Ax, const void *Arl,
+                                    void *b, size_t row_start, size_t row_end, size_t num_cols_per_row,
+                                    size_t stride, size_t blocksize, size_t threads, void *stream);
+int honei_hip_defect_smell_dv_double(void *rhs, void *result, const void *Aj, const void *Ax, const void *Arl,
+                                     void *b, size_t row_start, size_t row_end, size_t num_cols_per_row,
+                                     size_t stride, size_t blocksize, size_t threads, void *stream);
+
+/* Fused defect + norm (no reference counterpart; the reference runs
+ * Defect then Norm, math/multigrid.hh:2495-2498): result = rhs - A x over
+ * all rows and *norm2_dev = ||result||^2 (double accumulation), in one pass. */
+int honei_hip_defect_norm_smell_dv(int vtype, void *rhs, void *result, const void *Aj, const void *Ax,
+                                   const void *Arl, void *x, size_t rows, size_t num_cols_per_row, size_t stride,
+                                   size_t threads, double *norm2_dev, void *stream);
+
+/* Fused Jacobi sweep (math/jacobi.hh:268-279 = Defect + ScaledSum in one
+ * pass): x_new = x + dinv .* (rhs - A x); when acc is non-NULL also
+ * acc += x_new.  x_new must not alias x.  With x == NULL the input iterate
+ * is dinv .* rhs formed on the fly (first sweep from zero,
+ * math/jacobi.hh:481-493, fused with the second).                          */
+int honei_hip_jacobi_smell_dv(int vtype, void *x_new, const void *x, const void *rhs, const void *dinv,
+                              const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                              size_t num_cols_per_row, size_t stride, size_t threads, void *acc, void *stream);
+
+/* ---- BLAS-1 --------------------------------------------------------------
+ * Reductions: cuda_dot_product_two_* and cuda_norm_l2_one_*
+ * (backends/cuda/operations.hh:60-75) return the value on the host after a
+ * device-side final sum (no partial-sum download); the _dev forms leave it
+ * in device memory (double) without synchronising.  norm_l2_one is the
+ * squared norm, as Norm<vnt_l_two, false>.                                 */
+int honei_hip_dot_product_two_float(float *result, const void *x, const void *y, size_t n, void *stream);
+int honei_hip_dot_product_two_double(double *result, const void *x, const void *y, size_t n, void *stream);
+int honei_hip_norm_l2_one_float(float *result, const void *x, size_t n, void *stream);
+int honei_hip_norm_l2_one_double(double *result, const void *x, size_t n, void *stream);
+int honei_hip_dot_product_dev(int vtype, double *result_dev, const void *x, const void *y, size_t n, void *stream);
+int honei_hip_norm_l2_dev(int vtype, double *result_dev, const void *x, size_t n, void *stream);
+/* Fused axpy + norm (new): x += alpha*y; *result_dev = ||x||^2.            */
+int honei_hip_axpy_norm_dev(int vtype, void *x, const void *y, double alpha, double *result_dev, size_t n,
+                            void *stream);
+
+/* Replaces cuda_scaled_sum_three_{float,double}_s (r = x + s*y),
+ * cuda_scaled_sum_three_{float,double} (x += y .* z),
+ * cuda_sum_two_*, cuda_difference_two_* / _three_*, cuda_scale_one_*,
+ * cuda_element_product_three_*, cuda_element_inverse_one_*
+ * (backends/cuda/operations.hh:55-69,119-128); plus the two-operand axpy
+ * x += s*y used by ScaledSum (la/scaled_sum.hh:73-88).                     */
+int honei_hip_scaled_sum_two(int vtype, void *x, const void *y, double s, size_t n, void *stream);
+int honei_hip_scaled_sum_three_s(int vtype, void *r, const void *x, const void *y, double s, size_t n,
+                                 void *stream);
+int honei_hip_scaled_sum_three(int vtype, void *x, const void *y, const void *z, size_t n, void *stream);
+int honei_hip_sum_two(int vtype, void *a, const void *b, size_t n, void *stream);
+int honei_hip_difference_two(int vtype, void *a, const void *b, size_t n, void *stream);
+int honei_hip_difference_three(int vtype, void *r, const void *a, const void *b, size_t n, void *stream);
+int honei_hip_scale_one(int vtype, void *x, double a, size_t n, void *stream);
+int honei_hip_element_product_three(int vtype, void *r, const void *a, const void *b, size_t n, void *stream);
+int honei_hip_element_inverse_one(int vtype, void *x, size_t n, void *stream);
+
+/* ---- geometric transfers -------------------------------------------------
+ * Replace cuda_restriction_* / cuda_prolongation_*
+ * (backends/cuda/operations.hh:101-111) on lexicographic (2^L+1)^2 grids.
+ * mask: 8 host values (corners BL, BR, TR, TL, then edges B, R, T, L;
+ * 2 = Dirichlet), as MGInfo::macro_border_mask (math/multigrid.hh:78).
+ * Results are bit-identical to the CPU path.                              */
+int honei_hip_restriction(int vtype, void *coarse, size_t size_coarse, const void *fine, size_t size_fine,
+                          const uint64_t *mask, void *stream);
+int honei_hip_prolongation(int vtype, void *fine, size_t size_fine, const void *coarse, size_t size_coarse,
+                           const uint64_t *mask, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HONEI_HIP_H */

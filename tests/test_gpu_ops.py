@@ -1,0 +1,289 @@
+"""GPU parity of the ELL operators and BLAS-1/transfer kernels, through the C ABI.
+
+Bar: bit-exact where the arithmetic order matches the CPU path (T = 1 ELL
+products/defects/Jacobi, elementwise ops, geometric transfers); for T > 1
+(tree reduction over lanes, as the reference's CUDA kernel) the error per
+row is bounded by 1e-6 (FP32) / 1e-13 (FP64) relative to sum_j |a_rj x_j|.
+Reductions accumulate in double and are compared to an exact (math.fsum)
+sum with a 1e-12 relative bound.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import honei_amd as H
+import oracle
+from tests.patterns import honei_pattern_csr, random_band_csr, uniform_pm1
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float32: 1e-6, np.float64: 1e-13}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    H.init(0)
+    yield
+
+
+def _abs_row_sums(A, x):
+    """sum_j |a_rj x_j| per row, in float64."""
+    Aa = oracle.Ell(A.rows, A.cols, A.stride, A.ncpr, A.threads, A.Aj, np.abs(A.Ax.astype(np.float64)), A.Arl)
+    return oracle.product(Aa, np.abs(x.astype(np.float64)))
+
+
+def _gpu_product(A, x, vtype=None, **kw):
+    D = H.DeviceEll.from_oracle(A, vtype)
+    dx = H.DeviceArray.from_host(x, D.vec_dtype)
+    dy = H.DeviceArray(A.rows, D.vec_dtype)
+    D.product(dy, dx, **kw)
+    return dy.get()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("threads", [1, 2, 4, 8, 16, 32, 64])
+def test_product_random_band(dtype, threads):
+    n = 20000
+    print(f"product random band n={n} T={threads} {np.dtype(dtype).name}", flush=True)
+    rp, col, val = random_band_csr(n, seed=21, max_len=27, band=4096, dtype=dtype)
+    A = oracle.ell_from_sm(n, n, rp, col, val, threads)
+    x = uniform_pm1(2, n).astype(dtype)
+    y = _gpu_product(A, x)
+    yo = oracle.product(A, x)
+    if threads == 1:
+        assert np.array_equal(y.view(np.uint8), yo.view(np.uint8))
+    else:
+        err = np.abs(y.astype(np.float64) - yo.astype(np.float64))
+        assert (err <= TOL[dtype] * _abs_row_sums(A, x) + 1e-30).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("side", [3, 33, 129, 513])
+def test_product_q1_bitexact(dtype, side):
+    print(f"product q1 side={side} {np.dtype(dtype).name}", flush=True)
+    A = oracle.q1_matrix(side, dtype)
+    x = uniform_pm1(1, side * side).astype(dtype)
+    assert np.array_equal(_gpu_product(A, x), oracle.product(A, x))
+
+
+@pytest.mark.parametrize("threads", [1, 4, 16])
+@pytest.mark.parametrize("size", [11, 22, 88, 352])
+def test_product_reference_pattern(size, threads):
+    """la/product_TEST.cc:786-803 (x = 4, x0 = 1, x1 = 2; size x size+3)."""
+    for dtype in (np.float32, np.float64):
+        rp, col, val = honei_pattern_csr(size, size + 3, dtype=dtype)
+        A = oracle.ell_from_sm(size, size + 3, rp, col, val, threads)
+        x = np.full(size + 3, 4, dtype)
+        x[0], x[1] = 1, 2
+        y = _gpu_product(A, x)
+        yo = oracle.product(A, x)
+        err = np.abs(y.astype(np.float64) - yo)
+        assert (err <= TOL[dtype] * _abs_row_sums(A, x) + 1e-30).all()
+
+
+def test_product_row_range_offsets_output():
+    """Sub-range convention of cuda_product_smell_dv_*: y[r - row_start]."""
+    A = oracle.q1_matrix(65)
+    x = uniform_pm1(3, A.rows)
+    D = H.DeviceEll.from_oracle(A)
+    dx = H.DeviceArray.from_host(x)
+    dy = H.DeviceArray(1000, np.float64)
+    D.product(dy, dx, row_start=1500, row_end=2500)
+    assert np.array_equal(dy.get(), oracle.product(A, x)[1500:2500])
+
+
+def test_product_empty_and_single_row():
+    A = oracle.q1_matrix(3)
+    D = H.DeviceEll.from_oracle(A)
+    dx = H.DeviceArray.from_host(np.ones(9))
+    dy = H.DeviceArray(9, np.float64)
+    dy.upload(np.full(9, 7.0))
+    D.product(dy, dx, row_start=4, row_end=4)   # empty range: no write
+    assert (dy.get() == 7.0).all()
+    D.product(dy, dx, row_start=4, row_end=5)
+    assert dy.get()[0] == pytest.approx(8 / 3 - 8 / 3, abs=1e-15)
+
+
+def test_product_bf16_values():
+    """BF16 storage of matrix values, FP32 accumulation: equal to the FP32
+    product of the bf16-rounded matrix."""
+    n = 30000
+    rp, col, val = random_band_csr(n, seed=5, max_len=27, band=2000, dtype=np.float32)
+    A = oracle.ell_from_sm(n, n, rp, col, val, 1)
+    x = uniform_pm1(6, n).astype(np.float32)
+    y = _gpu_product(A, x, vtype=H.BF16)
+    u = A.Ax.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    Ab = oracle.Ell(A.rows, A.cols, A.stride, A.ncpr, 1, A.Aj, r.astype(np.uint32).view(np.float32), A.Arl)
+    assert np.array_equal(y, oracle.product(Ab, x))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("threads", [1, 8])
+def test_defect_and_fused_norm(dtype, threads):
+    n = 50000
+    rp, col, val = random_band_csr(n, seed=8, max_len=12, band=300, dtype=dtype)
+    A = oracle.ell_from_sm(n, n, rp, col, val, threads)
+    x = uniform_pm1(1, n).astype(dtype)
+    b = uniform_pm1(2, n).astype(dtype)
+    D = H.DeviceEll.from_oracle(A)
+    dx, db = H.DeviceArray.from_host(x), H.DeviceArray.from_host(b)
+    dr = H.DeviceArray(n, dtype)
+    D.defect(dr, db, dx)
+    r = dr.get()
+    ro = oracle.defect(A, b, x)
+    if threads == 1:
+        assert np.array_equal(r, ro)
+    else:
+        assert (np.abs(r.astype(np.float64) - ro) <= TOL[dtype] * (_abs_row_sums(A, x) + np.abs(b)) + 1e-30).all()
+    dn = H.DeviceArray(1, np.float64)
+    dr2 = H.DeviceArray(n, dtype)
+    D.defect_norm(dr2, db, dx, dn)
+    assert np.array_equal(dr2.get(), r)
+    exact = math.fsum((r.astype(np.float64) ** 2).tolist())
+    assert dn.get()[0] == pytest.approx(exact, rel=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_jacobi_sweeps_match_oracle(dtype):
+    """Fused sweep == Defect then ScaledSum(x, t, dinv) (math/jacobi.hh:268-279);
+    JACOBI0 == ElementProduct first sweep followed by one sweep."""
+    side = 257
+    A = oracle.q1_matrix(side, dtype)
+    n = side * side
+    rhs = uniform_pm1(4, n).astype(dtype)
+    dinv = (dtype(0.7) / oracle.q1_bands(side, dtype)[4]).astype(dtype)
+    x = uniform_pm1(5, n).astype(dtype)
+    D = H.DeviceEll.from_oracle(A)
+    dx, drhs, ddinv = H.DeviceArray.from_host(x), H.DeviceArray.from_host(rhs), H.DeviceArray.from_host(dinv)
+    dxn = H.DeviceArray(n, dtype)
+    D.jacobi(dxn, dx, drhs, ddinv)
+    expect = oracle.scaled_sum_v(x, oracle.defect(A, rhs, x), dinv)
+    assert np.array_equal(dxn.get(), expect)
+    # from zero, two sweeps fused, accumulated into acc
+    acc0 = uniform_pm1(6, n).astype(dtype)
+    dacc = H.DeviceArray.from_host(acc0)
+    D.jacobi(dxn, None, drhs, ddinv, acc=dacc)
+    c1 = (dinv * rhs).astype(dtype)
+    c2 = oracle.scaled_sum_v(c1, oracle.defect(A, rhs, c1), dinv)
+    assert np.array_equal(dxn.get(), c2)
+    assert np.array_equal(dacc.get(), (acc0 + c2).astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 1000, 1 << 20, 3_000_001])
+def test_reductions(dtype, n):
+    vt = H.F32 if dtype == np.float32 else H.F64
+    x = uniform_pm1(10, n).astype(dtype)
+    y = uniform_pm1(11, n).astype(dtype)
+    dx, dy = H.DeviceArray.from_host(x), H.DeviceArray.from_host(y)
+    out = H.DeviceArray(1, np.float64)
+    H.call("honei_hip_dot_product_dev", vt, out.ptr, dx.ptr, dy.ptr, n, None)
+    ex = math.fsum((x.astype(np.float64) * y.astype(np.float64)).tolist())
+    assert out.get()[0] == pytest.approx(ex, rel=1e-12, abs=1e-300)
+    H.call("honei_hip_norm_l2_dev", vt, out.ptr, dx.ptr, n, None)
+    ex2 = math.fsum((x.astype(np.float64) ** 2).tolist())
+    assert out.get()[0] == pytest.approx(ex2, rel=1e-12, abs=1e-300)
+    # repeated calls reuse the arrival counter: results stay identical
+    v1 = out.get()[0]
+    for _ in range(3):
+        H.call("honei_hip_norm_l2_dev", vt, out.ptr, dx.ptr, n, None)
+        assert out.get()[0] == v1
+
+
+def test_host_reductions_and_axpy_norm():
+    n = 100003
+    x = uniform_pm1(1, n)
+    y = uniform_pm1(2, n)
+    dx, dy = H.DeviceArray.from_host(x), H.DeviceArray.from_host(y)
+    import ctypes as C
+    r = C.c_double()
+    H.call("honei_hip_dot_product_two_double", C.byref(r), dx.ptr, dy.ptr, n, None)
+    assert r.value == pytest.approx(math.fsum((x * y).tolist()), rel=1e-12)
+    rf = C.c_float()
+    dxf = H.DeviceArray.from_host(x.astype(np.float32))
+    H.call("honei_hip_norm_l2_one_float", C.byref(rf), dxf.ptr, n, None)
+    assert rf.value == pytest.approx(float(np.sum(x.astype(np.float32).astype(np.float64) ** 2)), rel=1e-6)
+    out = H.DeviceArray(1, np.float64)
+    H.call("honei_hip_axpy_norm_dev", H.F64, dx.ptr, dy.ptr, -0.25, out.ptr, n, None)
+    xn = x + y * -0.25
+    assert np.array_equal(dx.get(), xn)
+    assert out.get()[0] == pytest.approx(math.fsum((xn * xn).tolist()), rel=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_elementwise_bitexact(dtype):
+    vt = H.F32 if dtype == np.float32 else H.F64
+    n = 777777
+    x = uniform_pm1(1, n).astype(dtype)
+    y = uniform_pm1(2, n).astype(dtype)
+    z = (uniform_pm1(3, n) + 2).astype(dtype)
+    s = dtype(0.37)
+
+    def run(name, *arrays_scalar, out=0):
+        devs = [H.DeviceArray.from_host(a) if isinstance(a, np.ndarray) else a for a in arrays_scalar]
+        args = [d.ptr if isinstance(d, H.DeviceArray) else d for d in devs]
+        H.call(name, vt, *args, n, None)
+        return devs[out].get()
+
+    assert np.array_equal(run("honei_hip_scaled_sum_two", x.copy(), y, float(s)), x + y * s)
+    assert np.array_equal(run("honei_hip_scaled_sum_three_s", np.zeros_like(x), x, y, float(s)), x + y * s)
+    assert np.array_equal(run("honei_hip_scaled_sum_three", x.copy(), y, z), x + y * z)
+    assert np.array_equal(run("honei_hip_sum_two", x.copy(), y), x + y)
+    assert np.array_equal(run("honei_hip_difference_two", x.copy(), y), x - y)
+    assert np.array_equal(run("honei_hip_difference_three", np.zeros_like(x), x, y), x - y)
+    assert np.array_equal(run("honei_hip_scale_one", x.copy(), float(s)), x * s)
+    assert np.array_equal(run("honei_hip_element_product_three", np.zeros_like(x), x, y), x * y)
+    assert np.array_equal(run("honei_hip_element_inverse_one", z.copy()), (dtype(1) / z).astype(dtype))
+
+
+def test_convert_roundtrip():
+    n = 123457
+    x = uniform_pm1(9, n) * 1e3
+    dx = H.DeviceArray.from_host(x)
+    df = H.DeviceArray(n, np.float32)
+    dd = H.DeviceArray(n, np.float64)
+    H.call("honei_hip_convert_double_float", df.ptr, dx.ptr, n, None)
+    H.call("honei_hip_convert_float_double", dd.ptr, df.ptr, n, None)
+    assert np.array_equal(df.get(), x.astype(np.float32))
+    assert np.array_equal(dd.get(), x.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("level", [1, 2, 3, 5, 8, 10])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_transfers_bitexact(level, dtype):
+    vt = H.F32 if dtype == np.float32 else H.F64
+    sc, sf = 2 ** level + 1, 2 ** (level + 1) + 1
+    c = uniform_pm1(level, sc * sc).astype(dtype)
+    f = uniform_pm1(level + 50, sf * sf).astype(dtype)
+    for mask in [(2,) * 8, (2, 2, 2, 2, 2, 1, 2, 2), (1,) * 8, (2, 1, 2, 1, 1, 2, 1, 2)]:
+        m = np.asarray(mask, np.uint64)
+        dc, df = H.DeviceArray.from_host(c), H.DeviceArray(sf * sf, dtype)
+        H.call("honei_hip_prolongation", vt, df.ptr, sf * sf, dc.ptr, sc * sc, m.ctypes.data, None)
+        assert np.array_equal(df.get(), oracle.prolongation(c, mask))
+        df2, dc2 = H.DeviceArray.from_host(f), H.DeviceArray(sc * sc, dtype)
+        H.call("honei_hip_restriction", vt, dc2.ptr, sc * sc, df2.ptr, sf * sf, m.ctypes.data, None)
+        assert np.array_equal(dc2.get(), oracle.restriction(f, mask))
+
+
+def test_invalid_arguments_are_rejected():
+    A = oracle.q1_matrix(5)
+    D = H.DeviceEll.from_oracle(A)
+    dx = H.DeviceArray.from_host(np.ones(25))
+    dy = H.DeviceArray(25, np.float64)
+    with pytest.raises(H.HipError) as e:
+        H.call("honei_hip_product_smell_dv_double", dx.ptr, dy.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, 25,
+               D.ncpr, D.stride, 0, 3, None)
+    assert e.value.code == 1
+    with pytest.raises(H.HipError):
+        H.call("honei_hip_product_smell_dv_double", dx.ptr, dy.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, 25,
+               D.ncpr, 8, 0, 1, None)    # stride < rows
+    m = np.full(8, 2, np.uint64)
+    with pytest.raises(H.HipError) as e:
+        H.call("honei_hip_restriction", H.F64, dy.ptr, 24, dx.ptr, 25, m.ctypes.data, None)
+    assert e.value.code == 2
+    bad = np.array([0, 1, 99], np.uint64)
+    buf = H.DeviceArray(3, np.int32)
+    with pytest.raises(H.HipError):
+        H.call("honei_hip_upload_indices", buf.ptr, bad.ctypes.data, 3, 25)
