@@ -73,6 +73,18 @@ _SIGS = {
     "honei_hip_element_inverse_one": [_i, _p, _sz, _p],
     "honei_hip_restriction": [_i, _p, _sz, _p, _sz, _p, _p],
     "honei_hip_prolongation": [_i, _p, _sz, _p, _sz, _p, _p],
+    "honei_hip_q1_assemble": [_i, _sz, _p, _p, _p, _sz, _p, _p, _d, _p],
+    "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
+                           C.POINTER(_d), _p],
+    "honei_hip_mg_create": [C.POINTER(_p), _i, _sz, _sz],
+    "honei_hip_mg_destroy": [_p],
+    "honei_hip_mg_configure": [_p, _p],
+    "honei_hip_mg_set_level": [_p, _sz, _sz, _sz, _sz, _sz, _p, _p, _p, _p],
+    "honei_hip_mg_set_transfer": [_p, _sz, _i, _sz, _sz, _sz, _sz, _sz, _p, _p, _p],
+    "honei_hip_mg_set_outer": [_p, _sz, _sz, _sz, _sz, _p, _p, _p],
+    "honei_hip_mg_build_q1": [_p, _d, _p],
+    "honei_hip_mg_solve_fixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
+    "honei_hip_mg_solve_mixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_d), _p],
 }
 _RESTYPE = {"honei_hip_last_error": C.c_char_p, "honei_hip_version": C.c_char_p}
 
@@ -253,3 +265,90 @@ class DeviceEll:
         call("honei_hip_jacobi_smell_dv", self.vtype, x_new.ptr, None if x is None else x.ptr, rhs.ptr, dinv.ptr,
              self.Aj.ptr, self.Ax.ptr, self.Arl.ptr, self.rows, self.ncpr, self.stride, self.threads,
              None if acc is None else acc.ptr, stream)
+
+
+class MGConfig(C.Structure):
+    """honei_hip_mg_config (include/honei_hip.h)."""
+    _fields_ = [("n_max_iter", _sz), ("n_pre_smooth", _sz), ("n_post_smooth", _sz),
+                ("initial_zero", _i), ("convergence_check", _i), ("prolmat", _i), ("zero_inner_start", _i),
+                ("tolerance", _d), ("adapt_correction_factor", _d), ("mask", _u64 * 8), ("max_outer", _sz),
+                ("post_smooth_uses_pre", _i)]
+
+
+def mg_config(n_max_iter=4, n_pre=2, n_post=2, tolerance=1e-8, initial_zero=False, adapt=1.0,
+              prolmat=False, zero_inner_start=False, mask=(2,) * 8, max_outer=0, post_smooth_uses_pre=True):
+    c = MGConfig()
+    c.n_max_iter, c.n_pre_smooth, c.n_post_smooth = n_max_iter, n_pre, n_post
+    c.initial_zero, c.convergence_check, c.prolmat = int(initial_zero), 0, int(prolmat)
+    c.zero_inner_start = int(zero_inner_start)
+    c.tolerance, c.adapt_correction_factor = tolerance, adapt
+    for i in range(8):
+        c.mask[i] = mask[i]
+    c.max_outer = max_outer
+    c.post_smooth_uses_pre = int(post_smooth_uses_pre)
+    return c
+
+
+class Multigrid:
+    """Owning handle to a device-resident multigrid hierarchy."""
+
+    def __init__(self, inner_vtype, min_level, max_level):
+        p = _p()
+        call("honei_hip_mg_create", C.byref(p), inner_vtype, min_level, max_level)
+        self.h = p.value
+        self.inner = inner_vtype
+        self.min_level, self.max_level = min_level, max_level
+
+    def configure(self, cfg: MGConfig):
+        call("honei_hip_mg_configure", self.h, C.byref(cfg))
+
+    def build_q1(self, omega=0.7, stream=None):
+        call("honei_hip_mg_build_q1", self.h, omega, stream)
+
+    def set_level(self, level, A, dinv):
+        Ax = np.ascontiguousarray(A.Ax, np.float64 if self.inner == F64 else np.float32)
+        dv = np.ascontiguousarray(dinv, np.float64 if self.inner == F64 else np.float32)
+        call("honei_hip_mg_set_level", self.h, level, A.rows, A.stride, A.ncpr, A.threads,
+             np.ascontiguousarray(A.Aj, np.uint64).ctypes.data_as(_p), Ax.ctypes.data_as(_p),
+             np.ascontiguousarray(A.Arl, np.uint64).ctypes.data_as(_p), dv.ctypes.data_as(_p))
+
+    def set_transfer(self, level, kind, A):
+        Ax = np.ascontiguousarray(A.Ax, np.float64 if self.inner == F64 else np.float32)
+        call("honei_hip_mg_set_transfer", self.h, level, kind, A.rows, A.cols, A.stride, A.ncpr, A.threads,
+             np.ascontiguousarray(A.Aj, np.uint64).ctypes.data_as(_p), Ax.ctypes.data_as(_p),
+             np.ascontiguousarray(A.Arl, np.uint64).ctypes.data_as(_p))
+
+    def set_outer(self, A):
+        Ax = np.ascontiguousarray(A.Ax, np.float64)
+        call("honei_hip_mg_set_outer", self.h, A.rows, A.stride, A.ncpr, A.threads,
+             np.ascontiguousarray(A.Aj, np.uint64).ctypes.data_as(_p), Ax.ctypes.data_as(_p),
+             np.ascontiguousarray(A.Arl, np.uint64).ctypes.data_as(_p))
+
+    def solve_fixed(self, b: DeviceArray, x: DeviceArray, stream=None):
+        it, fd = _sz(), _d()
+        call("honei_hip_mg_solve_fixed", self.h, b.ptr, x.ptr, C.byref(it), C.byref(fd), stream)
+        return it.value, fd.value
+
+    def solve_mixed(self, b: DeviceArray, x: DeviceArray, stream=None, check=True):
+        o, i, rd = _sz(), _sz(), _d()
+        rc = lib().honei_hip_mg_solve_mixed(self.h, b.ptr, x.ptr, C.byref(o), C.byref(i), C.byref(rd), stream)
+        if rc != SUCCESS and (check or rc != 5):
+            raise HipError(rc, "honei_hip_mg_solve_mixed", last_error())
+        return o.value, i.value, rd.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().honei_hip_mg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def cg_solve(D: DeviceEll, b: DeviceArray, x: DeviceArray, max_iters=10000, eps=1e-8, stream=None, check=True):
+    used, fd = _sz(), _d()
+    rc = lib().honei_hip_cg_solve(D.vtype, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, D.rows, D.ncpr, D.stride, D.threads,
+                                  b.ptr, x.ptr, max_iters, eps, C.byref(used), C.byref(fd), stream)
+    if rc != SUCCESS and (check or rc != 5):
+        raise HipError(rc, "honei_hip_cg_solve", last_error())
+    return used.value, fd.value

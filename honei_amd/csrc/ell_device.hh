@@ -1,0 +1,38 @@
+// ell_device.hh -- device-side ELL row sum shared by the row kernels.
+#pragma once
+
+#include "hip_common.hh"
+
+namespace honei {
+namespace hipx {
+
+// Partial row sum of lane `lane` (0 <= lane < T) of row `row`: the lane walks
+// slots row*T + lane + k*stride for k < len.  Multiply and add are separate
+// roundings (callers compile with fp contract off).
+template <typename VT, int T>
+__device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restrict__ Aj, const VT *__restrict__ Ax,
+                                                        int64_t row, int lane, uint32_t len, int64_t stride,
+                                                        const typename VecOf<VT>::type *__restrict__ x) {
+    typedef typename VecOf<VT>::type V;
+    V s = V(0);
+    int64_t slot = row * T + lane;
+#pragma unroll 4
+    for (uint32_t k = 0; k < len; ++k) {
+        const int32_t j = __builtin_nontemporal_load(Aj + slot);
+        const V av = load_value_nt(Ax + slot);
+        s += av * x[j];
+        slot += stride;
+    }
+    return s;
+}
+
+template <int T, typename V> __device__ inline V lane_group_sum(V s) {
+    if constexpr (T > 1) {
+#pragma unroll
+        for (int off = T / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    }
+    return s;
+}
+
+}  // namespace hipx
+}  // namespace honei

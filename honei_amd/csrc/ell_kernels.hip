@@ -21,9 +21,6 @@ namespace honei {
 namespace hipx {
 namespace {
 
-template <typename VT> struct VecOf { typedef VT type; };
-template <> struct VecOf<bf16> { typedef float type; };
-
 struct KArgs {
     const int32_t *Aj;
     const void *Ax;

@@ -23,6 +23,10 @@ __device__ inline float bf16_to_float(uint16_t b) {
     return __uint_as_float(((uint32_t)b) << 16);
 }
 
+// Vector element type for a matrix value type (bf16 matrices act on f32).
+template <typename VT> struct VecOf { typedef VT type; };
+template <> struct VecOf<bf16> { typedef float type; };
+
 // Accumulation type of a value type.
 template <typename VT> struct AccOf { typedef VT type; };
 template <> struct AccOf<bf16> { typedef float type; };

@@ -1,0 +1,141 @@
+// q1_kernels.hip -- device assembly of the Q1 Poisson operator directly in
+// the ELL layout (T = 1), one thread per row slot.
+//
+// Constant coefficient: the operator of FillMatrix<POISSON, DIRICHLET_0>
+// (math/fill_matrix.hh:36-78) converted through SparseMatrix to ELL: boundary
+// rows carry a single 1 on the diagonal, inner rows 8/3 and eight -1/3 in
+// ascending column order (LL, LD, LU, DL, DD, DU, UL, UD, UU).  Values are
+// DT(8.0/3.0), DT(-1.0/3.0) as in the reference, so the arrays are
+// bit-identical to SparseMatrixELL<DT>(SparseMatrix<DT>(BandedMatrixQx)).
+//
+// Variable coefficient (not in the reference): kappa per cell, bilinear
+// element stiffness kappa/6 * [[4,-1,-2,-1],...] summed over the four cells
+// around a node; kappa == 1 gives the constant stencil above.
+
+#include "solvers.hh"
+
+#pragma clang fp contract(off)
+
+namespace honei {
+namespace hipx {
+namespace {
+
+template <typename VT, typename V>
+__device__ inline void put(VT *Ax, int64_t s, V v) { Ax[s] = (VT)v; }
+template <>
+__device__ inline void put<bf16, float>(bf16 *Ax, int64_t s, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    Ax[s].bits = (uint16_t)(u >> 16);
+}
+
+template <typename VT>
+__global__ void k_q1_assemble(int64_t side, int32_t *Aj, VT *Ax, uint32_t *Arl, int64_t stride,
+                              const double *coeff, typename VecOf<VT>::type *dinv, double omega) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t n = side * side;
+    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (row >= stride) return;
+    if (row >= n) {
+        for (int k = 0; k < 9; ++k) {
+            Aj[row + k * stride] = 0;
+            put<VT, V>(Ax, row + k * stride, V(0));
+        }
+        return;
+    }
+    const int64_t i = row / side, j = row % side;
+    const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
+    if (boundary) {
+        Aj[row] = (int32_t)row;
+        put<VT, V>(Ax, row, V(1.0));
+        for (int k = 1; k < 9; ++k) {
+            Aj[row + k * stride] = 0;
+            put<VT, V>(Ax, row + k * stride, V(0));
+        }
+        Arl[row] = 1;
+        if (dinv) {
+            const V d = V(1.0);
+            const V inv = V(1) / d;
+            dinv[row] = inv * (V)omega;
+        }
+        return;
+    }
+    V w[9];
+    if (!coeff) {
+        const V c = V(-1.0 / 3.0);
+        for (int k = 0; k < 9; ++k) w[k] = c;
+        w[4] = V(8.0 / 3.0);
+    } else {
+        const int64_t nc = side - 1;
+        const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
+        const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+        w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
+        w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
+        w[2] = (V)(-se / 3.0);              // (i-1, j+1)
+        w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
+        w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
+        w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
+        w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
+        w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
+        w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+    }
+    const int64_t cols[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
+                             row + 1,        row + side - 1, row + side, row + side + 1};
+    for (int k = 0; k < 9; ++k) {
+        Aj[row + k * stride] = (int32_t)cols[k];
+        put<VT, V>(Ax, row + k * stride, w[k]);
+    }
+    Arl[row] = 9;
+    if (dinv) {
+        V d = w[4];
+        if constexpr (sizeof(VT) == 2) {   // diagonal as stored (bf16-rounded)
+            uint32_t u = __float_as_uint(d);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            d = __uint_as_float((u >> 16) << 16);
+        }
+        const V inv = V(1) / d;
+        dinv[row] = inv * (V)omega;
+    }
+}
+
+// coarse cell (I, J) = mean of fine cells (2I..2I+1, 2J..2J+1)
+__global__ void k_coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells) {
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= nc_cells * nc_cells) return;
+    const int64_t I = id / nc_cells, J = id % nc_cells;
+    const double a = fine[(2 * I) * nf_cells + 2 * J], b = fine[(2 * I) * nf_cells + 2 * J + 1];
+    const double c = fine[(2 * I + 1) * nf_cells + 2 * J], d = fine[(2 * I + 1) * nf_cells + 2 * J + 1];
+    coarse[id] = 0.25 * (((a + b) + c) + d);
+}
+
+}  // namespace
+
+hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32_t *Arl, int64_t stride,
+                       const double *coeff, void *dinv, double omega, hipStream_t s) {
+    if (side < 2 || stride < side * side) return hipErrorInvalidValue;
+    const int grid = (int)((stride + 255) / 256);
+    switch (vt) {
+        case VT_F32:
+            hipLaunchKernelGGL(k_q1_assemble<float>, dim3(grid), dim3(256), 0, s, side, Aj, (float *)Ax, Arl, stride,
+                               coeff, (float *)dinv, omega);
+            break;
+        case VT_F64:
+            hipLaunchKernelGGL(k_q1_assemble<double>, dim3(grid), dim3(256), 0, s, side, Aj, (double *)Ax, Arl,
+                               stride, coeff, (double *)dinv, omega);
+            break;
+        case VT_BF16:
+            hipLaunchKernelGGL(k_q1_assemble<bf16>, dim3(grid), dim3(256), 0, s, side, Aj, (bf16 *)Ax, Arl, stride,
+                               coeff, (float *)dinv, omega);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s) {
+    hipLaunchKernelGGL(k_coarsen_coeff, dim3((int)((nc_cells * nc_cells + 255) / 256)), dim3(256), 0, s, fine,
+                       nf_cells, coarse, nc_cells);
+    return hipGetLastError();
+}
+
+}  // namespace hipx
+}  // namespace honei

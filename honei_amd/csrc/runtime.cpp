@@ -47,10 +47,31 @@ Workspace *workspace_for(hipStream_t s) {
         fill_bytes_zero(w->scalars, sizeof(double) * Workspace::kScalars, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         set_error("workspace initialisation failed");
+        (void)hipFree(w->slots.partials);
+        (void)hipFree(w->slots.counter);
+        (void)hipFree(w->scalars);
+        delete w;
         return nullptr;
     }
     g_ws[key] = w;
     return w;
+}
+
+void *Workspace::ensure_scratch(size_t bytes) {
+    if (bytes <= scratch_bytes && scratch) return scratch;
+    if (scratch) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(scratch);
+        scratch = nullptr;
+        scratch_bytes = 0;
+    }
+    if (hipMalloc(&scratch, bytes) != hipSuccess) {
+        set_error("workspace scratch allocation failed");
+        scratch = nullptr;
+        return nullptr;
+    }
+    scratch_bytes = bytes;
+    return scratch;
 }
 
 void release_workspaces() {
@@ -59,6 +80,7 @@ void release_workspaces() {
         (void)hipFree(kv.second->slots.partials);
         (void)hipFree(kv.second->slots.counter);
         (void)hipFree(kv.second->scalars);
+        if (kv.second->scratch) (void)hipFree(kv.second->scratch);
         delete kv.second;
     }
     g_ws.clear();

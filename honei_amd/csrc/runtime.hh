@@ -20,6 +20,11 @@ struct Workspace {
     static constexpr int kScalars = 256;
     static constexpr int kSlotCapacity = 1 << 17;   // blocks
     static constexpr int kSlotLanes = 4;
+    void *scratch = nullptr;     // grows on demand, never shrinks
+    size_t scratch_bytes = 0;
+    // Returns a buffer of at least `bytes` (reallocating only when larger
+    // than any earlier request), or nullptr on allocation failure.
+    void *ensure_scratch(size_t bytes);
 };
 
 // Returns nullptr (and records the error) when allocation fails.

@@ -1,0 +1,306 @@
+// solver_kernels.hip -- device pieces of the CG and multigrid solvers.
+//
+// CG (math/cg.hh:63-133) runs as three kernels per iteration whose scalars
+// (alpha, lambda, talpha, convergence flag, iteration count) live in device
+// memory: the host never waits inside an iteration.  Kernels of an
+// iteration that starts after convergence return at once, so the host may
+// enqueue iterations in chunks and check the flag once per chunk.
+//
+// The coarse-grid CG of the multigrid V-cycle
+// (math/conjugate_gradients.hh:154-201 driven by :587-623) runs inside one
+// work-group: the coarsest Q1 grids have 9..1089 unknowns, far too small to
+// amortise a launch per vector operation.
+
+#include "solvers.hh"
+
+#include "ell_device.hh"
+
+#pragma clang fp contract(off)
+
+namespace honei {
+namespace hipx {
+namespace {
+
+template <typename V> __device__ inline V eps_of();
+template <> __device__ inline float eps_of<float>() { return 1.1920928955078125e-07f; }
+template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }
+
+// v = A p and partial v.p; the last block forms lambda = alpha / v.p.
+template <typename VT, int T>
+__global__ __launch_bounds__(256) void k_cg_spmv_dot(EllView A, const typename VecOf<VT>::type *p,
+                                                     typename VecOf<VT>::type *v, CGScalars *sc, ReduceSlots sl) {
+    typedef typename VecOf<VT>::type V;
+    if (sc->done) return;
+    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = (int)(gid & (T - 1));
+    const int64_t row = gid / T;
+    const bool active = row < A.rows;
+    V s = V(0);
+    if (active) s = ell_lane_sum<VT, T>(A.Aj, (const VT *)A.Ax, row, lane, A.Arl[row], A.stride, p);
+    s = lane_group_sum<T>(s);
+    double part = 0.0;
+    if (active && lane == 0) {
+        v[row] = s;
+        part = (double)s * (double)p[row];
+    }
+    __shared__ double bs;
+    const double b = block_sum(part);
+    if (threadIdx.x == 0) bs = b;
+    __syncthreads();
+    if (arrive_last(sl, &bs, 1)) {
+        const double tot = finish_partial(sl, 1, 0);
+        if (threadIdx.x == 0) {
+            const V temp = (V)tot;
+            const V e = eps_of<V>();
+            const V alpha = (V)sc->alpha;
+            sc->lambda = (double)(alpha / ((temp < 0 ? -temp : temp) > e ? temp : e));
+        }
+        reset_counter(sl);
+    }
+}
+
+// x += lambda p ; r -= lambda v ; alpha = ||r||^2 ; convergence test.
+template <typename V>
+__global__ __launch_bounds__(256) void k_cg_update(V *x, const V *p, V *r, const V *v, int64_t n, CGScalars *sc,
+                                                   ReduceSlots sl) {
+    if (sc->done) return;
+    const V lambda = (V)sc->lambda;
+    const V mlambda = -lambda;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += p[i] * lambda;
+        const V ri = r[i] + v[i] * mlambda;
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    }
+    __shared__ double bs;
+    const double b = block_sum(acc);
+    if (threadIdx.x == 0) bs = b;
+    __syncthreads();
+    if (arrive_last(sl, &bs, 1)) {
+        const double tot = finish_partial(sl, 1, 0);
+        if (threadIdx.x == 0) {
+            const V e = eps_of<V>();
+            const V alpha_old = (V)sc->alpha;
+            const V alpha = (V)tot;
+            sc->alpha = (double)alpha;
+            sc->talpha = (double)(alpha / ((alpha_old < 0 ? -alpha_old : alpha_old) > e ? alpha_old : e));
+            sc->iters += 1;
+            const V current = sqrt(alpha);
+            sc->current = (double)current;
+            if (current < (V)sc->eps_rel * (V)sc->initial || current < (V)sc->eps_rel || sc->iters >= sc->max_iters)
+                sc->done = 1;
+        }
+        reset_counter(sl);
+    }
+}
+
+// p = p*talpha + r (Scale then Sum, math/cg.hh:119-120)
+template <typename V>
+__global__ __launch_bounds__(256) void k_cg_direction(V *p, const V *r, int64_t n, const CGScalars *sc) {
+    if (sc->done) return;
+    const V t = (V)sc->talpha;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const V q = p[i] * t;
+        p[i] = q + r[i];
+    }
+}
+
+template <typename V>
+__global__ void k_cg_init(CGScalars *sc, double eps_rel, int max_iters) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const V alpha = (V)sc->alpha;
+        sc->alpha = (double)alpha;
+        sc->initial = (double)sqrt(alpha);
+        sc->current = sc->initial;
+        sc->eps_rel = eps_rel;
+        sc->iters = 0;
+        sc->max_iters = max_iters;
+        sc->done = max_iters <= 0 ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Single-work-group CG with the konv_rad stopping rule.  T = 1 layout or any
+// T (row sums are formed serially per row, k-major then lane, like the CPU).
+// ---------------------------------------------------------------------------
+template <typename VT>
+__device__ inline typename VecOf<VT>::type serial_row(const EllView &A, int64_t row,
+                                                      const typename VecOf<VT>::type *x) {
+    typedef typename VecOf<VT>::type V;
+    const VT *Ax = (const VT *)A.Ax;
+    V s = V(0);
+    const uint32_t len = A.Arl[row];
+    for (uint32_t k = 0; k < len; ++k)
+        for (int t = 0; t < A.threads; ++t) {
+            const int64_t slot = row * A.threads + t + (int64_t)k * A.stride;
+            V av;
+            if constexpr (sizeof(VT) == 2)
+                av = bf16_to_float(((const uint16_t *)Ax)[slot]);
+            else
+                av = Ax[slot];
+            s += av * x[A.Aj[slot]];
+        }
+    return s;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_coarse_cg(EllView A, const typename VecOf<VT>::type *rhs,
+                                                   typename VecOf<VT>::type *x, typename VecOf<VT>::type *g,
+                                                   typename VecOf<VT>::type *u, typename VecOf<VT>::type *en,
+                                                   double konv, int max_iters, int *iters_out) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t n = A.rows;
+    const V e = eps_of<V>();
+    auto vnorm = [&](const V *a) -> V {
+        double acc = 0.0;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += (double)a[i] * (double)a[i];
+        return (V)block_sum(acc);
+    };
+    auto vdot = [&](const V *a, const V *b) -> V {
+        double acc = 0.0;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += (double)a[i] * (double)b[i];
+        return (V)block_sum(acc);
+    };
+    // x = 0 ; g = A x - rhs ; u = -g
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = V(0);
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const V gi = serial_row<VT>(A, i, x) - rhs[i];
+        g[i] = gi;
+        u[i] = gi * V(-1.);
+    }
+    __syncthreads();
+    V norm_x = vnorm(x);
+    V norm_x_last = V(norm_x - V(1));
+    int it = 0;
+    while ((double)(norm_x - norm_x_last) > konv && it < max_iters) {
+        V upper = vnorm(g);
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) en[i] = serial_row<VT>(A, i, u);
+        __syncthreads();
+        V lower = vdot(en, u);
+        const V f1 = (lower < 0 ? -lower : lower) >= e ? V(upper / lower) : V(upper / e);
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) en[i] = u[i] * f1 + x[i];
+        __syncthreads();
+        lower = upper;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) g[i] = serial_row<VT>(A, i, en) - rhs[i];
+        __syncthreads();
+        upper = vnorm(g);
+        const V f2 = (lower < 0 ? -lower : lower) >= e ? V(upper / lower) : V(upper / e);
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            u[i] = u[i] * f2 - g[i];
+            x[i] = en[i];
+        }
+        __syncthreads();
+        ++it;
+        norm_x_last = norm_x;
+        norm_x = vnorm(x);
+    }
+    if (threadIdx.x == 0 && iters_out) *iters_out = it;
+}
+
+// Multigrid helpers ---------------------------------------------------------
+__global__ void k_boundary_guess(double *guess, const double *b, int64_t n, int64_t side) {
+    // first grid row, last grid row, first grid column take b
+    // (math/multigrid.hh:2423-2440)
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool bd = i < side || i >= n - side || (i % side) == 0;
+        if (bd) guess[i] = b[i];
+    }
+}
+
+__global__ void k_acc_scaled_f32(double *result, const float *x, double scale, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        result[i] += (double)x[i] * scale;
+}
+
+__global__ void k_scale_to_f32(float *dst, const double *src, double inv, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)(src[i] * inv);
+}
+
+inline int grid_for(int64_t n, int per = 1024) {
+    int64_t g = (n + per - 1) / per;
+    return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
+template <typename VT>
+hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc, const ReduceSlots &sl,
+                     hipStream_t s) {
+    typedef typename VecOf<VT>::type V;
+    const int T = A.threads;
+    const int64_t grid = (A.rows * T + 255) / 256;
+    if (grid > sl.capacity) return hipErrorInvalidValue;
+#define CGK(TT)                                                                                              \
+    case TT:                                                                                                 \
+        hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)p,  \
+                           (V *)v, sc, sl);                                                                  \
+        break;
+    switch (T) {
+        CGK(1) CGK(2) CGK(4) CGK(8) CGK(16) CGK(32) CGK(64)
+        default: return hipErrorInvalidValue;
+    }
+#undef CGK
+    const int g2 = grid_for(A.rows, 2048);
+    hipLaunchKernelGGL((k_cg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)p, (V *)r, (const V *)v,
+                       A.rows, sc, sl);
+    hipLaunchKernelGGL((k_cg_direction<V>), dim3(grid_for(A.rows)), dim3(256), 0, s, (V *)p, (const V *)r, A.rows,
+                       (const CGScalars *)sc);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
+                        const ReduceSlots &sl, hipStream_t s) {
+    switch (A.vtype) {
+        case VT_F32: return cg_iter_t<float>(A, x, p, r, v, sc, sl, s);
+        case VT_F64: return cg_iter_t<double>(A, x, p, r, v, sc, sl, s);
+        case VT_BF16: return cg_iter_t<bf16>(A, x, p, r, v, sc, sl, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s) {
+    if (vt == VT_F64)
+        hipLaunchKernelGGL(k_cg_init<double>, dim3(1), dim3(64), 0, s, sc, eps_rel, max_iters);
+    else
+        hipLaunchKernelGGL(k_cg_init<float>, dim3(1), dim3(64), 0, s, sc, eps_rel, max_iters);
+    return hipGetLastError();
+}
+
+hipError_t coarse_cg(const EllView &A, const void *rhs, void *x, void *g, void *u, void *en, double konv,
+                     int max_iters, int *iters_dev, hipStream_t s) {
+    switch (A.vtype) {
+        case VT_F32:
+            hipLaunchKernelGGL(k_coarse_cg<float>, dim3(1), dim3(256), 0, s, A, (const float *)rhs, (float *)x,
+                               (float *)g, (float *)u, (float *)en, konv, max_iters, iters_dev);
+            break;
+        case VT_F64:
+            hipLaunchKernelGGL(k_coarse_cg<double>, dim3(1), dim3(256), 0, s, A, (const double *)rhs, (double *)x,
+                               (double *)g, (double *)u, (double *)en, konv, max_iters, iters_dev);
+            break;
+        case VT_BF16:
+            hipLaunchKernelGGL(k_coarse_cg<bf16>, dim3(1), dim3(256), 0, s, A, (const float *)rhs, (float *)x,
+                               (float *)g, (float *)u, (float *)en, konv, max_iters, iters_dev);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t boundary_guess(double *guess, const double *b, int64_t n, int64_t side, hipStream_t s) {
+    hipLaunchKernelGGL(k_boundary_guess, dim3(grid_for(n)), dim3(256), 0, s, guess, b, n, side);
+    return hipGetLastError();
+}
+hipError_t acc_scaled_f32(double *result, const float *x, double scale, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_acc_scaled_f32, dim3(grid_for(n)), dim3(256), 0, s, result, x, scale, n);
+    return hipGetLastError();
+}
+hipError_t scale_to_f32(float *dst, const double *src, double inv, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_to_f32, dim3(grid_for(n)), dim3(256), 0, s, dst, src, inv, n);
+    return hipGetLastError();
+}
+
+}  // namespace hipx
+}  // namespace honei

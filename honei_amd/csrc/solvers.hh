@@ -1,0 +1,119 @@
+// solvers.hh -- device solver building blocks and the host-side solver
+// drivers (CG, multigrid FIXED / MIXED) of the MI355X backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "backend.hh"
+
+namespace honei {
+namespace hipx {
+
+// Device-resident CG state (math/cg.hh:63-133 scalars).
+struct CGScalars {
+    double alpha, lambda, talpha, initial, current, eps_rel;
+    int done, iters, max_iters, pad;
+};
+
+hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s);
+hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
+                        const ReduceSlots &sl, hipStream_t s);
+hipError_t coarse_cg(const EllView &A, const void *rhs, void *x, void *g, void *u, void *en, double konv,
+                     int max_iters, int *iters_dev, hipStream_t s);
+hipError_t boundary_guess(double *guess, const double *b, int64_t n, int64_t side, hipStream_t s);
+hipError_t acc_scaled_f32(double *result, const float *x, double scale, int64_t n, hipStream_t s);
+hipError_t scale_to_f32(float *dst, const double *src, double inv, int64_t n, hipStream_t s);
+
+// Device Q1 assembly (math/fill_matrix.hh:36-78 + la/sparse_matrix.hh:221-300
+// + la/sparse_matrix_ell-impl.hh:75-122, T = 1): Dirichlet rows are identity,
+// inner rows the 9-point stencil, entries in ascending column order.  With
+// a per-cell coefficient (side-1)^2 array the stencil is the Q1 bilinear
+// element stiffness summed over the four adjacent cells (coefficient 1
+// reproduces 8/3 and -1/3).  dinv (optional) = omega / diag.
+hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32_t *Arl, int64_t stride,
+                       const double *coeff, void *dinv, double omega, hipStream_t s);
+
+hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Owned device ELL matrix
+// ---------------------------------------------------------------------------
+struct DevEll {
+    EllView v{};
+    int32_t *Aj = nullptr;
+    void *Ax = nullptr;
+    uint32_t *Arl = nullptr;
+    bool owned = false;
+    void release();
+};
+
+hipError_t dev_ell_alloc(DevEll &M, ValueType vt, int64_t rows, int64_t cols, int64_t stride, int64_t ncpr,
+                         int threads);
+
+size_t vt_size(ValueType vt);           // bytes per matrix value
+size_t vec_size(ValueType vt);          // bytes per vector element (bf16 -> 4)
+
+// ---------------------------------------------------------------------------
+// Multigrid (math/multigrid.hh: MGInfo 66-149, _multigrid_kernel 1936-2250,
+// FIXED driver 828-870, MIXED driver 2395-2536)
+// ---------------------------------------------------------------------------
+struct MGConfig {
+    int64_t n_max_iter = 4, n_pre_smooth = 2, n_post_smooth = 2;
+    int initial_zero = 0, convergence_check = 0, prolmat = 0, zero_inner_start = 0;
+    double tolerance = 1e-8, adapt_correction_factor = 1.0;
+    uint64_t mask[8] = {2, 2, 2, 2, 2, 2, 2, 2};
+    int64_t max_outer = 0;          // 0 = unbounded (reference); capped at 10000
+    int post_smooth_uses_pre = 1;   // reference: post-smoothing runs n_pre_smooth sweeps
+};
+
+struct MGLevel {
+    int64_t n = 0, side = 0;
+    DevEll A, P, R;                 // P: prolongation into this level; R: restriction from it
+    void *dinv = nullptr;           // omega / diag(A)
+    void *c = nullptr, *d = nullptr, *rhs = nullptr, *x = nullptr, *t = nullptr;
+};
+
+class MGHierarchy {
+  public:
+    MGHierarchy(ValueType inner, int64_t min_level, int64_t max_level);
+    ~MGHierarchy();
+
+    ValueType inner;                // matrix value type of the inner (V-cycle) levels
+    int64_t min_level, max_level;
+    std::vector<MGLevel> L;         // indexed by level
+    DevEll A_outer;                 // FP64 system for MIXED (optional)
+    MGConfig cfg;
+
+    hipError_t alloc_level_vectors(int64_t level, int64_t n);
+    hipError_t alloc_coarse_scratch();
+    // Build every level (and the FP64 outer system) on the device as the
+    // constant-coefficient Q1 Dirichlet problem on (2^l+1)^2 grids.
+    hipError_t build_q1(double omega, hipStream_t s);
+
+    // One _multigrid_kernel call on the inner levels; returns the number of
+    // V-cycles through *inner_iters.  rhs[max] and x[max] are inputs.
+    hipError_t run_kernel(hipStream_t s, int64_t *inner_iters, double *final_defect);
+    // Drivers; b and x are device vectors of the inner type (FIXED) or
+    // double (MIXED).
+    hipError_t solve_fixed(const void *b, void *x, hipStream_t s, int64_t *iters, double *final_defect);
+    hipError_t solve_mixed(const double *b, double *x, hipStream_t s, int64_t *outer, int64_t *inner,
+                           double *rel_defect, std::vector<double> *history);
+
+  private:
+    void *g_ = nullptr, *u_ = nullptr, *en_ = nullptr;   // coarse CG scratch
+    int *iters_dev_ = nullptr;
+    double *scal_ = nullptr;                            // device scalars
+    double *guess_ = nullptr, *d64_ = nullptr, *res64_ = nullptr;
+    int64_t n64_ = 0;
+    ReduceSlots slots_{};
+    hipError_t smooth_from_zero(int64_t l, const void *rhs, void *out, void *acc, hipStream_t s);
+    hipError_t sweeps(int64_t l, void *x, const void *rhs, int64_t count, hipStream_t s);
+    hipError_t defect(int64_t l, void *out, const void *rhs, const void *x, double *norm2, hipStream_t s);
+    hipError_t read_scalar(double *dst, const double *src, hipStream_t s);
+};
+
+}  // namespace hipx
+}  // namespace honei

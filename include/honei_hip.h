@@ -165,6 +165,73 @@ int honei_hip_restriction(int vtype, void *coarse, size_t size_coarse, const voi
 int honei_hip_prolongation(int vtype, void *fine, size_t size_fine, const void *coarse, size_t size_coarse,
                            const uint64_t *mask, void *stream);
 
+/* ---- device Q1 assembly ----------------------------------------------------
+ * Replaces the host chain FillMatrix<POISSON, DIRICHLET_0>
+ * (math/fill_matrix.hh:36-78) -> SparseMatrix(BandedMatrixQx)
+ * (la/sparse_matrix.hh:221-300) -> SparseMatrixELL (T = 1,
+ * la/sparse_matrix_ell-impl.hh:75-122), built directly in HBM: Aj int32 and
+ * Ax [9*stride], Arl uint32 [side^2], stride >= side^2 (reference: 32-aligned).
+ * coeff_dev NULL: the reference's constant stencil, bit-identical arrays.
+ * coeff_dev non-NULL: per-cell coefficient (double, (side-1)^2, row-major
+ * cells) -- the variable-coefficient Q1 operator, which the reference lacks.
+ * dinv_dev (optional, vector type) = omega / diag, as the reference tests
+ * build it (math/poisson_mg_fixed_ell_TEST.cc:707-714).                    */
+int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl, size_t stride,
+                          const double *coeff_dev, void *dinv_dev, double omega, void *stream);
+
+/* ---- CG --------------------------------------------------------------------
+ * CGSolver<Tag, methods::NONE>::value (math/cg.hh:63-133) on a device ELL
+ * matrix: x (device, vector type) is the start vector and the result.
+ * Scalars stay on the device; the host reads the convergence flag once per
+ * chunk of iterations.  Returns HONEI_HIP_ERROR_NOT_CONVERGED when max_iters
+ * is reached first (x and *used_iters are still written).                  */
+int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                       size_t num_cols_per_row, size_t stride, size_t threads, const void *b_dev, void *x_dev,
+                       size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
+                       void *stream);
+
+/* ---- multigrid ---------------------------------------------------------------
+ * A device-resident hierarchy plays the role of MGInfo
+ * (math/multigrid.hh:66-149); solve_fixed / solve_mixed are
+ * Multigrid<..., JAC, CYCLE::V, FIXED|MIXED>::value on ELL
+ * (math/multigrid.hh:828-870, 2395-2536) with the reference's control flow.
+ * Levels are numbered like the reference (level l = (2^l+1)^2 nodes for the
+ * geometric transfers).  Host arrays are uploaded (indices narrowed to 32
+ * bits); build_q1 assembles every level and the FP64 outer system on the
+ * device instead.                                                          */
+typedef struct honei_hip_mg_config {
+    size_t n_max_iter;            /* V-cycles per inner solve             */
+    size_t n_pre_smooth;          /* Jacobi sweeps (omega inside dinv)    */
+    size_t n_post_smooth;         /* used only if post_smooth_uses_pre==0 */
+    int initial_zero;
+    int convergence_check;
+    int prolmat;                  /* 1: transfers are ELL products P / R  */
+    int zero_inner_start;         /* 1: inner start vector 0 every outer step */
+    double tolerance;             /* relative, inner and outer            */
+    double adapt_correction_factor; /* 0: alpha = (d,c)/(Ac,c)            */
+    uint64_t mask[8];             /* macro_border_mask                    */
+    size_t max_outer;             /* 0: no limit besides a 10000 cap      */
+    int post_smooth_uses_pre;     /* 1: reference behaviour               */
+} honei_hip_mg_config;
+
+int honei_hip_mg_create(void **mg, int inner_vtype, size_t min_level, size_t max_level);
+int honei_hip_mg_destroy(void *mg);
+int honei_hip_mg_configure(void *mg, const honei_hip_mg_config *cfg);
+int honei_hip_mg_set_level(void *mg, size_t level, size_t rows, size_t stride, size_t num_cols_per_row,
+                           size_t threads, const uint64_t *Aj, const void *Ax, const uint64_t *Arl,
+                           const void *dinv);
+/* kind 0: prolongation matrix into `level`; kind 1: restriction from it.   */
+int honei_hip_mg_set_transfer(void *mg, size_t level, int kind, size_t rows, size_t cols, size_t stride,
+                              size_t num_cols_per_row, size_t threads, const uint64_t *Aj, const void *Ax,
+                              const uint64_t *Arl);
+int honei_hip_mg_set_outer(void *mg, size_t rows, size_t stride, size_t num_cols_per_row, size_t threads,
+                           const uint64_t *Aj, const double *Ax, const uint64_t *Arl);
+int honei_hip_mg_build_q1(void *mg, double omega, void *stream);
+int honei_hip_mg_solve_fixed(void *mg, const void *b_dev, void *x_dev, size_t *iters, double *final_defect,
+                             void *stream);
+int honei_hip_mg_solve_mixed(void *mg, const double *b_dev, double *x_dev, size_t *outer_iters,
+                             size_t *inner_iters, double *rel_defect, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

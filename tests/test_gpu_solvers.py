@@ -1,0 +1,234 @@
+"""GPU parity of the device Q1 assembly and the CG / multigrid solvers.
+
+Bars:
+  * Q1 assembly: index arrays and values bit-identical to the CPU path
+    (FillMatrix -> SparseMatrix -> SparseMatrixELL).
+  * Solvers: the GPU reductions accumulate in double in a tree while the CPU
+    path sums sequentially in the working precision, so iterates agree to
+    rounding, not bits.  Checked: the same iteration counts as the oracle
+    (reference control flow), FP64 residual below the requested relative
+    tolerance, and the solution within 1e-8 (FP64) / 1e-6 (mixed) relative
+    of the oracle's; final error against the manufactured solution x_true
+    is asserted as well.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import honei_amd as H
+import oracle
+from tests.patterns import uniform_pm1
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    H.init(0)
+    yield
+
+
+def _device_q1(side, vt, coeff=None, omega=0.7):
+    n = side * side
+    stride = 32 * ((n + 31) // 32)
+    Aj = H.DeviceArray(9 * stride, np.int32)
+    Ax = H.DeviceArray(9 * stride, {H.F32: np.float32, H.F64: np.float64, H.BF16: np.uint16}[vt])
+    Arl = H.DeviceArray(n, np.uint32)
+    dinv = H.DeviceArray(n, np.float64 if vt == H.F64 else np.float32)
+    dc = H.DeviceArray.from_host(coeff, np.float64) if coeff is not None else None
+    H.call("honei_hip_q1_assemble", vt, side, Aj.ptr, Ax.ptr, Arl.ptr, stride, None if dc is None else dc.ptr,
+           dinv.ptr, omega, None)
+    return Aj.get(), Ax.get(), Arl.get(), dinv.get(), stride
+
+
+@pytest.mark.parametrize("side", [3, 5, 33, 129, 1025])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_q1_assembly_bitexact(side, dtype):
+    vt = H.F32 if dtype == np.float32 else H.F64
+    Aj, Ax, Arl, dinv, stride = _device_q1(side, vt)
+    A = oracle.q1_matrix(side, dtype)
+    assert stride == A.stride and A.ncpr == (9 if side > 2 else 1)
+    assert np.array_equal(Aj.astype(np.uint64), A.Aj)
+    assert np.array_equal(Ax.view(np.uint8), A.Ax.view(np.uint8))
+    assert np.array_equal(Arl.astype(np.uint64), A.Arl)
+    H_ = oracle.q1_hierarchy(1, min_level=1, dtype=dtype) if side == 3 else None
+    d = oracle.q1_bands(side, dtype)[4]
+    expect = ((np.ones(side * side, dtype) / d).astype(dtype) * dtype(0.7)).astype(dtype)
+    assert np.array_equal(dinv, expect)
+
+
+def _numpy_variable_q1(side, kappa):
+    """Brute-force assembly from element matrices (bilinear Q1, unit cells)."""
+    n = side * side
+    K = np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 6.0
+    A = np.zeros((n, n))
+    for ci in range(side - 1):
+        for cj in range(side - 1):
+            nodes = [ci * side + cj, ci * side + cj + 1, (ci + 1) * side + cj + 1, (ci + 1) * side + cj]
+            A[np.ix_(nodes, nodes)] += kappa[ci * (side - 1) + cj] * K
+    for r in range(n):
+        i, j = divmod(r, side)
+        if i in (0, side - 1) or j in (0, side - 1):
+            A[r] = 0
+            A[r, r] = 1
+    return A
+
+
+@pytest.mark.parametrize("side", [3, 5, 9, 17])
+def test_q1_variable_coefficient_assembly(side):
+    """No reference counterpart: pinned by brute-force element assembly and by
+    kappa = 1 reproducing the reference's constant operator bit-for-bit."""
+    kappa = np.exp(uniform_pm1(side, (side - 1) ** 2))
+    Aj, Ax, Arl, dinv, stride = _device_q1(side, H.F64, coeff=kappa)
+    E = oracle.Ell(side * side, side * side, stride, 9, 1, Aj.astype(np.uint64), Ax, Arl.astype(np.uint64))
+    np.testing.assert_allclose(E.to_dense(), _numpy_variable_q1(side, kappa), rtol=1e-14, atol=1e-15)
+    Aj1, Ax1, Arl1, _, _ = _device_q1(side, H.F64, coeff=np.ones((side - 1) ** 2))
+    A = oracle.q1_matrix(side)
+    assert np.array_equal(Ax1, A.Ax) and np.array_equal(Aj1.astype(np.uint64), A.Aj)
+
+
+def _problem(side, seed):
+    A = oracle.q1_matrix(side)
+    xt = uniform_pm1(seed, side * side)
+    return A, xt, oracle.product(A, xt)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("side", [33, 129])
+def test_cg_matches_oracle(dtype, side):
+    A, xt, b = _problem(side, 1)
+    A = A.astype(dtype)
+    b = b.astype(dtype)
+    x0 = np.zeros_like(b)
+    eps = 1e-8 if dtype == np.float64 else 1e-5
+    xo, used_o = oracle.cg_solver(A, b, x0, 5000, eps)
+    D = H.DeviceEll.from_oracle(A)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray.from_host(x0)
+    used, fd = H.cg_solve(D, db, dx, 5000, eps)
+    x = dx.get()
+    assert abs(used - used_o) <= (1 if dtype == np.float64 else 3)
+    r0 = np.linalg.norm(b.astype(np.float64))
+    r = np.linalg.norm(oracle.defect(A.astype(np.float64), b.astype(np.float64), x.astype(np.float64)))
+    assert r < (eps * 1.5) * r0
+    tol = 1e-8 if dtype == np.float64 else 2e-4
+    assert np.abs(x - xo).max() <= tol * np.abs(xo).max()
+
+
+def test_cg_config1_fp64():
+    """Config-1 problem: Q1 129x129, b = A x_true, x0 with Dirichlet values,
+    FP64 CG to 1e-8 relative."""
+    A, xt, b = _problem(129, 1)
+    x0 = np.zeros_like(b)
+    s = 129
+    for i in range(s):
+        x0[i], x0[-1 - i], x0[i * s] = b[i], b[-1 - i], b[i * s]
+    D = H.DeviceEll.from_oracle(A)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray.from_host(x0)
+    used, fd = H.cg_solve(D, db, dx, 10000, 1e-8)
+    x = dx.get()
+    xo, used_o = oracle.cg_solver(A, b, x0, 10000, 1e-8)
+    assert abs(used - used_o) <= 1
+    r = oracle.defect(A, b, x)
+    assert np.linalg.norm(r) < 1e-8 * np.linalg.norm(oracle.defect(A, b, x0)) * 1.01
+    assert np.abs(x - xt).max() < 1e-5
+
+
+def test_cg_not_converged_reports():
+    A, xt, b = _problem(65, 2)
+    D = H.DeviceEll.from_oracle(A)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(A.rows, np.float64)
+    dx.zero()
+    with pytest.raises(H.HipError) as e:
+        H.cg_solve(D, db, dx, 3, 1e-12)
+    assert e.value.code == 5
+
+
+@pytest.mark.parametrize("level", [2, 4, 6, 8])
+def test_mg_fixed_fp64_matches_oracle(level):
+    side = 2 ** level + 1
+    A, xt, b = _problem(side, 3)
+    Ho = oracle.q1_hierarchy(level, dtype=np.float64)
+    cfg_o = oracle.mg_config(1, level, n_max_iter=50, tolerance=1e-10)
+    xo, it_o, _ = oracle.mg_fixed(Ho, cfg_o, b)
+    mg = H.Multigrid(H.F64, 1, level)
+    mg.configure(H.mg_config(n_max_iter=50, tolerance=1e-10))
+    mg.build_q1(0.7)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(side * side, np.float64)
+    dx.zero()
+    it, fd = mg.solve_fixed(db, dx)
+    x = dx.get()
+    assert it == it_o
+    assert np.abs(x - xo).max() <= 1e-9 * max(1.0, np.abs(xo).max())
+    assert np.abs(x - xt).max() < 1e-8
+
+
+@pytest.mark.parametrize("level,n_max_iter", [(3, 4), (4, 4), (5, 2), (6, 4), (8, 4)])
+def test_mg_mixed_reference_semantics(level, n_max_iter):
+    """Reference MIXED driver (inner start carried over between outer
+    steps): same outer/inner counts as the CPU path, converged FP64 result."""
+    side = 2 ** level + 1
+    A, xt, b = _problem(side, 3)
+    Ho = oracle.q1_hierarchy(level, dtype=np.float32)
+    cfg_o = oracle.mg_config(1, level, n_max_iter=n_max_iter, tolerance=1e-8)
+    xo, outer_o, inner_o, _ = oracle.mg_mixed(Ho, cfg_o, A, b)
+    mg = H.Multigrid(H.F32, 1, level)
+    mg.configure(H.mg_config(n_max_iter=n_max_iter, tolerance=1e-8))
+    mg.build_q1(0.7)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(side * side, np.float64)
+    dx.zero()
+    outer, inner, rd = mg.solve_mixed(db, dx)
+    x = dx.get()
+    assert (outer, inner) == (outer_o, inner_o)
+    assert rd < 1e-8
+    r = oracle.defect(A, b, x)
+    assert np.linalg.norm(r) < 1e-8 * np.linalg.norm(oracle.defect(A, b, np.zeros_like(b)) ) * 10
+    assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
+    assert np.abs(x - xt).max() < 1e-6
+
+
+def test_mg_mixed_prolmat_hierarchy_from_host():
+    """Hierarchy uploaded from host arrays with PROLMAT transfers (P and R =
+    P^T as ELL products) -- the operator-list style data path."""
+    level = 5
+    side = 2 ** level + 1
+    A, xt, b = _problem(side, 4)
+    Ho = oracle.q1_hierarchy(level, dtype=np.float32, use_prolmat=True)
+    cfg_o = oracle.mg_config(1, level, n_max_iter=4, tolerance=1e-8, prolmat=True)
+    xo, outer_o, inner_o, _ = oracle.mg_mixed(Ho, cfg_o, A, b)
+    mg = H.Multigrid(H.F32, 1, level)
+    mg.configure(H.mg_config(n_max_iter=4, tolerance=1e-8, prolmat=True))
+    for l in range(1, level + 1):
+        mg.set_level(l, Ho.A[l], Ho.dinv[l])
+        if l > 1:
+            mg.set_transfer(l, 0, Ho.prol[l])
+            mg.set_transfer(l, 1, Ho.res[l])
+    mg.set_outer(A)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(side * side, np.float64)
+    dx.zero()
+    outer, inner, rd = mg.solve_mixed(db, dx)
+    assert (outer, inner) == (outer_o, inner_o)
+    assert np.abs(dx.get() - xo).max() <= 1e-6 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("level", [9, 11])
+def test_mg_mixed_large_zero_start(level):
+    """Config-3 shape (level 11 = 2049^2, 4.2M unknowns): FP32 V(2,2) inner,
+    FP64 outer to 1e-10; error vs x_true and vs an FP64 MG solve."""
+    side = 2 ** level + 1
+    n = side * side
+    xt = uniform_pm1(3, n)
+    A64 = oracle.q1_matrix(side)
+    b = oracle.product(A64, xt)
+    mg = H.Multigrid(H.F32, 1, level)
+    mg.configure(H.mg_config(n_max_iter=1, tolerance=1e-10, zero_inner_start=True))
+    mg.build_q1(0.7)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(n, np.float64)
+    dx.zero()
+    outer, inner, rd = mg.solve_mixed(db, dx)
+    x = dx.get()
+    print(f"level {level}: outer {outer} inner {inner} rel defect {rd:.3e} err {np.abs(x - xt).max():.3e}")
+    assert rd < 1e-10
+    assert np.abs(x - xt).max() < 1e-8
+    r = oracle.defect(A64, b, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
