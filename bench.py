@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X backend for HONEI's ELL sparse-solve path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one mixed-precision multigrid Poisson solve (FP32 V(2,2) Jacobi
+inner cycles, FP64 outer defect correction) of the Q1 Dirichlet problem on
+2049 x 2049 nodes (4,198,401 unknowns) to a relative FP64 residual of 1e-8,
+with b = A x_true, x_true uniform in [-1, 1] (synthetic, seeded), matrices
+and hierarchy resident in HBM before timing starts.  `value` is solves/s;
+p50/p99 latency of a solve is reported beside it.  The same run measures
+the standalone ELL SpMV (Q1 2049^2 and the random 4M-row ELL, FP32/FP64)
+as a fraction of HBM peak, which gives `roofline`, and times the reference
+CPU path for `cpu_baseline`.
+
+Prints exactly one JSON line on stdout (progress goes to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import honei_amd as H  # noqa: E402
+from tests.patterns import uniform_pm1  # noqa: E402
+
+METRIC = "Poisson solves/s + p99 latency at 1e-8 rel. residual; ELL SpMV % of HBM peak"
+PEAK_GBS = 8000.0
+SIDE = 2049
+LEVEL = 11
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def device_q1(side, vt):
+    """Q1 operator assembled on the device (bit-identical to the reference's
+    FillMatrix -> SparseMatrix -> SparseMatrixELL chain)."""
+    n = side * side
+    stride = 32 * ((n + 31) // 32)
+
+    class E:
+        pass
+
+    D = E()
+    D.rows, D.cols, D.stride, D.ncpr, D.threads, D.vtype = n, n, stride, 9, 1, vt
+    D.Aj = H.DeviceArray(9 * stride, np.int32)
+    D.Ax = H.DeviceArray(9 * stride, np.float64 if vt == H.F64 else np.float32)
+    D.Arl = H.DeviceArray(n, np.uint32)
+    H.call("honei_hip_q1_assemble", vt, side, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, stride, None, None, 0.0, None)
+    D.nnz = 9 * (side - 2) ** 2 + 4 * (side - 1)
+    return D
+
+
+def device_random_ell(n, vt, seed=2, max_len=27, band=4096):
+    """Random ELL of the SpMV configuration generated in HBM (T = 1, rows
+    U{1..27} long, distinct columns within +-4096 of the diagonal, N(0,1))."""
+    stride = 32 * ((n + 31) // 32)
+
+    class E:
+        pass
+
+    D = E()
+    D.rows, D.cols, D.stride, D.ncpr, D.threads, D.vtype = n, n, stride, max_len, 1, vt
+    D.Aj = H.DeviceArray(max_len * stride, np.int32)
+    D.Ax = H.DeviceArray(max_len * stride, np.float64 if vt == H.F64 else np.float32)
+    D.Arl = H.DeviceArray(n, np.uint32)
+    H.call("honei_hip_random_ell", vt, n, max_len, band, seed, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, stride, None)
+    D.nnz = int(D.Arl.get().astype(np.int64).sum())
+    return D
+
+
+def time_spmv(D, reps, x, y, label):
+    """Average duration of one ELL SpMV launch (HIP events on the launch
+    stream) and algorithmic bytes per launch: nnz*(s_v+4) + n*(s_x+s_y)."""
+    fn = {H.F32: "honei_hip_product_smell_dv_float", H.F64: "honei_hip_product_smell_dv_double"}[D.vtype]
+    args = (x.ptr, y.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, D.rows, D.ncpr, D.stride, 0, D.threads, None)
+    for _ in range(3):
+        H.call(fn, *args)
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record()
+    for _ in range(reps):
+        H.call(fn, *args)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    sv = 8 if D.vtype == H.F64 else 4
+    alg = D.nnz * (sv + 4) + D.rows * 2 * sv
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"[spmv] {label}: {ms * 1e3:.1f} us/launch, {alg / 1e6:.1f} MB, {gbs:.0f} GB/s = {100 * gbs / PEAK_GBS:.1f}%")
+    return {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
+            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
+
+
+def read_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    p = os.path.join(ROOT, "profiles", "ell_spmv_q1_2049_f32_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def cpu_baseline(b_small, side_small, level_small, n_big):
+    """Reference CPU path on the host: Multigrid<CPU, CPU, NONE, JAC, V,
+    MIXED>::value as HONEI ships it (oracle/_ref), else the clean-room
+    restatement; timed on the 1025^2 system and scaled to 2049^2 by the
+    unknown count (the V-cycle count does not depend on the level)."""
+    try:
+        from oracle import ref
+        if ref.available():
+            t0 = time.perf_counter()
+            x, outer, inner, secs = ref.mg_mixed(level_small, b_small, np.zeros_like(b_small), n_max_iter=4,
+                                                 tol=1e-8, timed=True)
+            kind = "reference"
+        else:
+            raise ImportError
+    except Exception:
+        import oracle
+        A = oracle.q1_matrix(side_small)
+        Hh = oracle.q1_hierarchy(level_small, dtype=np.float32)
+        cfg = oracle.mg_config(1, level_small, n_max_iter=4, tolerance=1e-8)
+        t0 = time.perf_counter()
+        x, outer, inner, _ = oracle.mg_mixed(Hh, cfg, A, b_small)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    scale = n_big / (side_small * side_small)
+    per_solve = secs * scale
+    log(f"[cpu] {kind}: {secs:.2f} s for the {side_small}^2 solve ({outer} outer / {inner} inner cycles)")
+    return {"value": round(1.0 / per_solve, 5), "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": (f"one Multigrid<CPU,CPU,NONE,JAC,V,MIXED> solve (n_max_iter=4, V(2,2), tol 1e-8) of the "
+                       f"{side_small}x{side_small} Q1 system took {secs:.2f} s on 1 host core; scaled x{scale:.3f} "
+                       f"by unknowns to the {SIDE}x{SIDE} workload")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section (for counter runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path is single-GPU by design",
+              file=sys.stderr)
+        sys.exit(2)
+
+    H.init(0)
+    n = SIDE * SIDE
+
+    # ---- standalone SpMV (roofline kernel: Q1 2049^2, FP32, T=1) ----------
+    spmv = {}
+    D32 = device_q1(SIDE, H.F32)
+    x32 = H.DeviceArray.from_host(uniform_pm1(2, n).astype(np.float32))
+    y32 = H.DeviceArray(n, np.float32)
+    spmv["q1_2049_f32"] = time_spmv(D32, 50, x32, y32, "q1 2049^2 f32")
+    if args.spmv_only:
+        print(json.dumps({"spmv": spmv}))
+        return
+    D64 = device_q1(SIDE, H.F64)
+    x64 = H.DeviceArray.from_host(uniform_pm1(2, n))
+    y64 = H.DeviceArray(n, np.float64)
+    spmv["q1_2049_f64"] = time_spmv(D64, 50, x64, y64, "q1 2049^2 f64")
+    del D64
+    for vt, dt, key in ((H.F32, np.float32, "random_4m_f32"), (H.F64, np.float64, "random_4m_f64")):
+        R = device_random_ell(1 << 22, vt)
+        xr = H.DeviceArray.from_host(uniform_pm1(2, R.rows).astype(dt))
+        yr = H.DeviceArray(R.rows, dt)
+        spmv[key] = time_spmv(R, 30, xr, yr, key)
+        spmv[key]["nnz"] = R.nnz
+        del R, xr, yr
+
+    # ---- Poisson solves ---------------------------------------------------
+    xt = uniform_pm1(3, n)
+    b = np.empty(n)
+    db = H.DeviceArray(n, np.float64)
+    dxt = H.DeviceArray.from_host(xt)
+    A64 = device_q1(SIDE, H.F64)
+    H.call("honei_hip_product_smell_dv_double", dxt.ptr, db.ptr, A64.Aj.ptr, A64.Ax.ptr, A64.Arl.ptr, 0, n, 9,
+           A64.stride, 0, 1, None)
+    b[:] = db.get()
+    mg = H.Multigrid(H.F32, 1, LEVEL)
+    mg.configure(H.mg_config(n_max_iter=1, tolerance=1e-8, zero_inner_start=True))
+    mg.build_q1(0.7)
+    dx = H.DeviceArray(n, np.float64)
+    for _ in range(args.warmup):
+        dx.zero()
+        mg.solve_mixed(db, dx)
+    lat = []
+    outer = inner = 0
+    H.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        dx.zero()
+        outer, inner, rd = mg.solve_mixed(db, dx)
+        lat.append(time.perf_counter() - t0)
+    H.synchronize()
+    total = time.perf_counter() - t_start
+    x = dx.get()
+    err = float(np.abs(x - xt).max())
+    log(f"[solve] {args.steps} solves in {total:.3f} s; outer {outer} inner {inner} rel.res {rd:.2e} err {err:.2e}")
+
+    # plain FP64 GPU solve (pure FP64 multigrid) for comparison
+    mg64 = H.Multigrid(H.F64, 1, LEVEL)
+    mg64.configure(H.mg_config(n_max_iter=100, tolerance=1e-8))
+    mg64.build_q1(0.7)
+    dx64 = H.DeviceArray(n, np.float64)
+    dx64.zero()
+    mg64.solve_fixed(db, dx64)
+    t64 = []
+    for _ in range(5):
+        dx64.zero()
+        t0 = time.perf_counter()
+        it64, _fd = mg64.solve_fixed(db, dx64)
+        t64.append(time.perf_counter() - t0)
+    err64 = float(np.abs(dx64.get() - xt).max())
+    del mg64, dx64
+
+    lat_ms = np.array(lat) * 1e3
+    value = args.steps / total
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (b = A x_true, x_true ~ U[-1,1], seed 3; matrices assembled on the device)",
+        "config": {"workload": "q1_poisson_2049x2049_mixed_mg_to_1e-8", "rows": n, "levels": LEVEL,
+                   "inner": "FP32 V(2,2) damped Jacobi 0.7, CG on 3x3, 1 cycle per outer step",
+                   "outer": "FP64 defect correction", "rel_residual": 1e-8},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4),
+                       "p99": round(float(np.percentile(lat_ms, 99)), 4),
+                       "max": round(float(lat_ms.max()), 4)},
+        "solve": {"outer_iterations": outer, "inner_cycles": inner, "rel_residual": rd, "max_err_vs_x_true": err},
+        "gpu_fp64_mg": {"solves_per_s": round(1.0 / float(np.median(t64)), 3), "cycles": it64,
+                        "max_err_vs_x_true": err64},
+        "spmv": spmv,
+    }
+    r = spmv["q1_2049_f32"]
+    out["roofline"] = {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
+                       "frac": round(r["achieved_gbs"] / PEAK_GBS, 4), "traffic": read_traffic(),
+                       "kernel": "ell_row_kernel<float,1,ELL_PRODUCT> on Q1 2049^2 (configs[1]a, FP32)",
+                       "algorithmic_bytes_per_launch": r["algorithmic_bytes"]}
+    if not args.no_cpu:
+        side_s, level_s = 1025, 10
+        Aq = None
+        xs = uniform_pm1(3, side_s * side_s)
+        # b for the smaller system on the device as well
+        Ds = device_q1(side_s, H.F64)
+        dxs = H.DeviceArray.from_host(xs)
+        dbs = H.DeviceArray(side_s * side_s, np.float64)
+        H.call("honei_hip_product_smell_dv_double", dxs.ptr, dbs.ptr, Ds.Aj.ptr, Ds.Ax.ptr, Ds.Arl.ptr, 0,
+               side_s * side_s, 9, Ds.stride, 0, 1, None)
+        del Aq
+        out["cpu_baseline"] = cpu_baseline(dbs.get(), side_s, level_s, n)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -36,6 +36,8 @@ hipError_t scale_to_f32(float *dst, const double *src, double inv, int64_t n, hi
 hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32_t *Arl, int64_t stride,
                        const double *coeff, void *dinv, double omega, hipStream_t s);
 
+hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
+                      uint32_t *Arl, int64_t stride, hipStream_t s);
 hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s);
 
 // ---------------------------------------------------------------------------

@@ -153,10 +153,16 @@ def prolongation(coarse, mask=(2,) * 8):
     return f
 
 
-def mg_mixed(max_level, b, x0, n_max_iter=4, n_pre=2, tol=1e-8, adapt=1.0):
+def mg_mixed(max_level, b, x0, n_max_iter=4, n_pre=2, tol=1e-8, adapt=1.0, timed=False):
+    """Multigrid<CPU, CPU, NONE, JAC, V, MIXED>::value of the reference on the
+    Dirichlet Q1 problem; with timed=True also returns the seconds spent in
+    the solve call alone (hierarchy set-up excluded)."""
     b = np.ascontiguousarray(b, np.float64)
     x = np.array(x0, np.float64, copy=True)
     inner = _u64()
+    secs = C.c_double()
     outer = lib().ref_mg_mixed(_u64(max_level), _u64(n_max_iter), _u64(n_pre), C.c_double(tol),
-                               C.c_double(adapt), _ptr(b), _ptr(x), C.byref(inner))
+                               C.c_double(adapt), _ptr(b), _ptr(x), C.byref(inner), C.byref(secs))
+    if timed:
+        return x, outer, inner.value, secs.value
     return x, outer, inner.value

@@ -22,6 +22,7 @@
 #include <honei/math/restriction.hh>
 #include <honei/util/configuration.hh>
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -224,7 +225,7 @@ void *ref_prolmat_d(u64 fine_side, u64 threads) {
 // does (mask all-Dirichlet here, initial_zero=false, adapt factor given).
 // Returns outer iterations (parsed from the driver's own report).
 u64 ref_mg_mixed(u64 max_level, u64 n_max_iter, u64 n_pre, double tol, double adapt,
-                 const double *b, double *x, u64 *inner_total) {
+                 const double *b, double *x, u64 *inner_total, double *solve_seconds) {
     set_threads(1);
     MGInfo<float, SparseMatrixELL<float> > info;
     info.is_smoother = false;
@@ -273,8 +274,11 @@ u64 ref_mg_mixed(u64 max_level, u64 n_max_iter, u64 n_pre, double tol, double ad
     u64 outer = 0;
     {
         CoutCapture cap;
+        const auto t0 = std::chrono::steady_clock::now();
         Multigrid<tags::CPU, tags::CPU, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
             *A, rhs, res, (unsigned long)11, std::numeric_limits<double>::epsilon(), info);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (solve_seconds) *solve_seconds = std::chrono::duration<double>(t1 - t0).count();
         outer = cap.find_count("TN of outer iters: ");
         if (inner_total) *inner_total = cap.find_count("TN of inner iters: ");
     }

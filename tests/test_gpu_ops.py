@@ -287,3 +287,47 @@ def test_invalid_arguments_are_rejected():
     buf = H.DeviceArray(3, np.int32)
     with pytest.raises(H.HipError):
         H.call("honei_hip_upload_indices", buf.ptr, bad.ctypes.data, 3, 25)
+
+
+def _device_random_ell(n, vt, seed=2, max_len=27, band=4096):
+    stride = 32 * ((n + 31) // 32)
+    Aj = H.DeviceArray(max_len * stride, np.int32)
+    Ax = H.DeviceArray(max_len * stride, np.float64 if vt == H.F64 else np.float32)
+    Arl = H.DeviceArray(n, np.uint32)
+    H.call("honei_hip_random_ell", vt, n, max_len, band, seed, Aj.ptr, Ax.ptr, Arl.ptr, stride, None)
+    return oracle.Ell(n, n, stride, max_len, 1, Aj.get().astype(np.uint64), Ax.get(), Arl.get().astype(np.uint64))
+
+
+@pytest.mark.parametrize("n,band", [(1000, 4096), (100000, 4096), (1 << 22, 4096)])
+def test_random_ell_generator_structure(n, band):
+    """Rows U{1..27} long, strictly increasing columns inside the clipped
+    window, zero padding, non-zero values; same matrix for the same seed."""
+    print(f"random ELL n={n}", flush=True)
+    A = _device_random_ell(n, H.F64, band=band)
+    L = A.Arl.astype(np.int64)
+    assert L.min() >= 1 and L.max() <= 27
+    M = A.Aj.reshape(27, A.stride)[:, :n].T.astype(np.int64)
+    V = A.Ax.reshape(27, A.stride)[:, :n].T
+    m = np.arange(27)[None, :] < L[:, None]
+    assert ((np.diff(np.where(m, M, 1 << 40), axis=1) > 0) | ~m[:, 1:]).all()
+    r = np.arange(n)[:, None]
+    assert (np.where(m, np.abs(M - r), 0) <= band).all() and (M < n).all()
+    assert (np.where(m, V != 0, True)).all() and (np.where(~m, (V == 0) & (M == 0), True)).all()
+    if n >= 100000:
+        assert abs(L.mean() - 14.0) < 0.05 and abs(V[m].std() - 1.0) < 0.01
+    B = _device_random_ell(n, H.F64, band=band)
+    assert np.array_equal(A.Aj, B.Aj) and np.array_equal(A.Ax, B.Ax)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_full_size_spmv_configs(dtype):
+    """SpMV configurations at full size: Q1 2049^2 (4,198,401 rows) and the
+    random 4,194,304-row ELL; T = 1 results bit-identical to the CPU path."""
+    vt = H.F32 if dtype == np.float32 else H.F64
+    print(f"full-size spmv {np.dtype(dtype).name}", flush=True)
+    A = oracle.q1_matrix(2049, dtype)
+    x = uniform_pm1(2, A.rows).astype(dtype)
+    assert np.array_equal(_gpu_product(A, x), oracle.product(A, x))
+    R = _device_random_ell(1 << 22, vt)
+    xr = uniform_pm1(2, R.rows).astype(dtype)
+    assert np.array_equal(_gpu_product(R, xr), oracle.product(R, xr))
