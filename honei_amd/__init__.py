@@ -74,7 +74,7 @@ _SIGS = {
     "honei_hip_restriction": [_i, _p, _sz, _p, _sz, _p, _p],
     "honei_hip_prolongation": [_i, _p, _sz, _p, _sz, _p, _p],
     "honei_hip_q1_assemble": [_i, _sz, _p, _p, _p, _sz, _p, _p, _d, _p],
-    "honei_hip_random_ell": [_i, _sz, _sz, _sz, _u64, _p, _p, _p, _sz, _p],
+    "honei_hip_random_ell": [_i, _sz, _sz, _sz, _u64, _p, _p, _p, _sz, _sz, _p],
     "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
                            C.POINTER(_d), _p],
     "honei_hip_mg_create": [C.POINTER(_p), _i, _sz, _sz],

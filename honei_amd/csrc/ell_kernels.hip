@@ -33,9 +33,13 @@ struct KArgs {
     void *acc;
     double *norm2;
     int64_t row_start, row_end;
+    int64_t nlogical;
     ReduceSlots slots;
 };
 
+// One block serves `per_block` consecutive logical blocks of blockDim/T rows
+// (per_block = 1 unless the launch was capped, as it is for launches that
+// carry a reduction: few blocks keep the arrival counter cold).
 template <typename VT, int T, int MODE, bool NORM, bool ACC>
 __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     typedef typename VecOf<VT>::type V;
@@ -45,50 +49,56 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     const V *__restrict__ rhs = (const V *)a.rhs;
     const V *__restrict__ dinv = (const V *)a.dinv;
 
-    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
-    const int lane = (int)(gid & (T - 1));
-    const int64_t row = a.row_start + gid / T;
-    const bool active = row < a.row_end;
-
-    V s = V(0);
-    if (active) {
-        const uint32_t len = a.Arl[row];
-        int64_t slot = row * T + lane;
-#pragma unroll 4
-        for (uint32_t k = 0; k < len; ++k) {
-            const int32_t j = __builtin_nontemporal_load(Aj + slot);
-            const V av = load_value_nt(Ax + slot);
-            V xv;
-            if constexpr (MODE == ELL_JACOBI0)
-                xv = dinv[j] * rhs[j];
-            else
-                xv = x[j];
-            s += av * xv;
-            slot += a.stride;
-        }
-    }
-    if constexpr (T > 1) {
-#pragma unroll
-        for (int off = T / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    }
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t per = a.nlogical / gridDim.x, rem = a.nlogical % gridDim.x;
+    const int64_t first = p * per + (p < rem ? p : rem);
+    const int64_t last = first + per + (p < rem ? 1 : 0);
 
     double nval = 0.0;
-    if (active && lane == 0) {
-        if constexpr (MODE == ELL_PRODUCT) {
-            ((V *)a.out)[row - a.row_start] = s;
-        } else if constexpr (MODE == ELL_DEFECT) {
-            const V r = rhs[row] - s;
-            ((V *)a.out)[row] = r;
-            if constexpr (NORM) nval = (double)r * (double)r;
-        } else {
-            const V xin = (MODE == ELL_JACOBI0) ? dinv[row] * rhs[row] : x[row];
-            const V t = rhs[row] - s;
-            const V o = xin + t * dinv[row];
-            ((V *)a.out)[row] = o;
-            if constexpr (ACC) {
-                V *acc = (V *)a.acc;
-                acc[row] = acc[row] + o;
+    for (int64_t lb = first; lb < last; ++lb) {
+        const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
+        const int lane = (int)(gid & (T - 1));
+        const int64_t row = a.row_start + gid / T;
+        const bool active = row < a.row_end;
+
+        V s = V(0);
+        if (active) {
+            const uint32_t len = a.Arl[row];
+            int64_t slot = row * T + lane;
+#pragma unroll 4
+            for (uint32_t k = 0; k < len; ++k) {
+                const int32_t j = __builtin_nontemporal_load(Aj + slot);
+                const V av = load_value_nt(Ax + slot);
+                V xv;
+                if constexpr (MODE == ELL_JACOBI0)
+                    xv = dinv[j] * rhs[j];
+                else
+                    xv = x[j];
+                s += av * xv;
+                slot += a.stride;
+            }
+        }
+        if constexpr (T > 1) {
+#pragma unroll
+            for (int off = T / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        }
+
+        if (active && lane == 0) {
+            if constexpr (MODE == ELL_PRODUCT) {
+                ((V *)a.out)[row - a.row_start] = s;
+            } else if constexpr (MODE == ELL_DEFECT) {
+                const V r = rhs[row] - s;
+                ((V *)a.out)[row] = r;
+                if constexpr (NORM) nval += (double)r * (double)r;
+            } else {
+                const V xin = (MODE == ELL_JACOBI0) ? dinv[row] * rhs[row] : x[row];
+                const V t = rhs[row] - s;
+                const V o = xin + t * dinv[row];
+                ((V *)a.out)[row] = o;
+                if constexpr (ACC) {
+                    V *acc = (V *)a.acc;
+                    acc[row] = acc[row] + o;
+                }
             }
         }
     }
@@ -158,8 +168,11 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
         if (norm) return fill_bytes_zero(e.norm2, sizeof(double), s);
         return hipSuccess;
     }
-    const int64_t grid = (nrows * T + block - 1) / block;
-    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    const int64_t nlogical = (nrows * T + block - 1) / block;
+    if (nlogical > 0x7fffffff) return hipErrorInvalidValue;
+    // launches that reduce run at most kReduceBlocks blocks
+    constexpr int64_t kReduceBlocks = 2048;
+    const int64_t grid = norm ? (nlogical < kReduceBlocks ? nlogical : kReduceBlocks) : nlogical;
     if (norm && grid > slots.capacity) return hipErrorInvalidValue;
     KArgs k;
     k.Aj = A.Aj;
@@ -174,6 +187,7 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     k.norm2 = e.norm2;
     k.row_start = e.row_start;
     k.row_end = e.row_end;
+    k.nlogical = nlogical;
     k.slots = slots;
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
     return hipGetLastError();

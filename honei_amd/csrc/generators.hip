@@ -25,14 +25,16 @@ __device__ inline double u01(uint64_t &st) { return (double)(splitmix_next(st) >
 
 template <typename VT, int MAXW>
 __global__ void k_random_ell(int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, VT *Ax, uint32_t *Arl,
-                             int64_t stride) {
+                             int64_t stride, int T) {
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= stride) return;
-    if (r >= n) {
-        for (int k = 0; k < max_len; ++k) {
-            Aj[r + k * stride] = 0;
-            Ax[r + k * stride] = VT(0);
-        }
+    const int64_t nchunks = (max_len + T - 1) / T;
+    if (r * T >= stride) return;
+    if (r >= n) {   // padding slots beyond the last row
+        for (int64_t k = 0; k < nchunks; ++k)
+            for (int t = 0; t < T && r * T + t < stride; ++t) {
+                Aj[r * T + t + k * stride] = 0;
+                Ax[r * T + t + k * stride] = VT(0);
+            }
         return;
     }
     uint64_t st = seed ^ ((uint64_t)r * 0xD1B54A32D192ED03ull);
@@ -60,35 +62,39 @@ __global__ void k_random_ell(int64_t n, int max_len, int64_t band, uint64_t seed
         cols[pos] = c;
         ++have;
     }
-    for (int k = 0; k < max_len; ++k) {
-        if (k < have) {
+    // entry t of the row goes to slot (t % T) + r*T + (t / T)*stride
+    for (int t = 0; t < nchunks * T; ++t) {
+        const int64_t slot = (t % T) + r * T + (int64_t)(t / T) * stride;
+        if (t < have) {
             double v;
             do {
                 const double u1 = u01(st), u2 = u01(st);
                 v = sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
             } while (v == 0.0);
-            Aj[r + k * stride] = cols[k];
-            Ax[r + k * stride] = (VT)v;
+            Aj[slot] = cols[t];
+            Ax[slot] = (VT)v;
         } else {
-            Aj[r + k * stride] = 0;
-            Ax[r + k * stride] = VT(0);
+            Aj[slot] = 0;
+            Ax[slot] = VT(0);
         }
     }
-    Arl[r] = (uint32_t)have;
+    Arl[r] = (uint32_t)((have + T - 1) / T);
 }
 
 }  // namespace
 
 hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
-                      uint32_t *Arl, int64_t stride, hipStream_t s) {
-    if (max_len < 1 || max_len > 64 || stride < n || band < max_len) return hipErrorInvalidValue;
-    const int grid = (int)((stride + 255) / 256);
+                      uint32_t *Arl, int64_t stride, int threads, hipStream_t s) {
+    if (max_len < 1 || max_len > 64 || stride < n * threads || band < max_len || threads < 1 || threads > 64)
+        return hipErrorInvalidValue;
+    const int64_t slots_rows = (stride + threads - 1) / threads;
+    const int grid = (int)((slots_rows + 255) / 256);
     if (vt == VT_F32)
         hipLaunchKernelGGL((k_random_ell<float, 64>), dim3(grid), dim3(256), 0, s, n, max_len, band, seed, Aj,
-                           (float *)Ax, Arl, stride);
+                           (float *)Ax, Arl, stride, threads);
     else if (vt == VT_F64)
         hipLaunchKernelGGL((k_random_ell<double, 64>), dim3(grid), dim3(256), 0, s, n, max_len, band, seed, Aj,
-                           (double *)Ax, Arl, stride);
+                           (double *)Ax, Arl, stride, threads);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

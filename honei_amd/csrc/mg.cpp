@@ -229,14 +229,23 @@ hipError_t MGHierarchy::run_kernel(hipStream_t s, int64_t *inner_iters, double *
     const ValueType vv = inner == VT_F64 ? VT_F64 : VT_F32;
     MGLevel &top = L[(size_t)maxl];
     double *nrm = scal_;   // scal_[0]: squared defect norm
+    // Inside the MIXED driver the defect norms of the last allowed cycle are
+    // never read (the driver forms its own FP64 defect), so that cycle skips
+    // its final top-level defect and the host read; with one cycle per call
+    // and no convergence test the initial norm is not needed either.  The
+    // iterate x and the cycle count are unchanged by these omissions.
+    const bool lean = mixed_inner_ && !cfg.convergence_check;
+    const bool need_initial = !(lean && cfg.n_max_iter == 1);
+    // d[max] = rhs when the start vector is zero (no matvec), else rhs - A x
+    const void *top_d = top.d;
     if (cfg.initial_zero) {
-        HX(vcopy(top.d, top.rhs, top.n, vv, s));
-        HX(norm2(top.d, top.n, vv, nrm, slots_, s));
+        top_d = top.rhs;
+        if (need_initial) HX(norm2(top.rhs, top.n, vv, nrm, slots_, s));
     } else {
-        HX(defect(maxl, top.d, top.rhs, top.x, nrm, s));
+        HX(defect(maxl, top.d, top.rhs, top.x, need_initial ? nrm : nullptr, s));
     }
-    double h;
-    HX(read_scalar(&h, nrm, s));
+    double h = 0.0;
+    if (need_initial) HX(read_scalar(&h, nrm, s));
     // defect in the inner precision (Norm<l2,true,Tag> on Prec_)
     double defn = (inner == VT_F64) ? std::sqrt(h) : (double)std::sqrt((float)h);
     const double initial = defn;
@@ -244,13 +253,14 @@ hipError_t MGHierarchy::run_kernel(hipStream_t s, int64_t *inner_iters, double *
     if (!(cfg.convergence_check && defn <= cfg.tolerance)) {
         int64_t iter;
         for (iter = 1; iter <= cfg.n_max_iter; ++iter) {
+            const bool skip_final = lean && iter == cfg.n_max_iter;
             int64_t l = maxl;
             if (minl != maxl) {
                 bool started = true;
                 while (true) {
                     MGLevel &lv = L[(size_t)l];
                     if (started) {
-                        HX(smooth_from_zero(l, lv.d, lv.c, lv.x, s));
+                        HX(smooth_from_zero(l, (l == maxl) ? top_d : lv.d, lv.c, lv.x, s));
                     } else {
                         HX(smooth_from_zero(l, lv.rhs, lv.x, nullptr, s));
                     }
@@ -276,7 +286,7 @@ hipError_t MGHierarchy::run_kernel(hipStream_t s, int64_t *inner_iters, double *
             {
                 MGLevel &lm = L[(size_t)minl];
                 const double konv = inner == VT_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
-                const void *crhs = (minl == maxl) ? lm.d : lm.rhs;
+                const void *crhs = (minl == maxl) ? top_d : lm.rhs;
                 HX(coarse_cg(lm.A.v, crhs, lm.x, g_, u_, en_, konv, (int)(4 * lm.n + 64), iters_dev_, s));
                 if (minl == maxl) {
                     HX(defect(minl, lm.d, lm.rhs, lm.x, nrm, s));
@@ -329,9 +339,10 @@ hipError_t MGHierarchy::run_kernel(hipStream_t s, int64_t *inner_iters, double *
                 }
                 const int64_t post = cfg.post_smooth_uses_pre ? cfg.n_pre_smooth : cfg.n_post_smooth;
                 HX(sweeps(l, lv.x, lv.rhs, post, s));
-                HX(defect(l, lv.d, lv.rhs, lv.x, l == maxl ? nrm : nullptr, s));
+                if (!(skip_final && l == maxl)) HX(defect(l, lv.d, lv.rhs, lv.x, l == maxl ? nrm : nullptr, s));
                 if (l == maxl) break;
             }
+            if (skip_final) continue;   // loop ends: iter == n_max_iter
             HX(read_scalar(&h, nrm, s));
             defn = (inner == VT_F64) ? std::sqrt(h) : (double)std::sqrt((float)h);
             if (defn <= initial * cfg.tolerance) break;
@@ -401,7 +412,10 @@ hipError_t MGHierarchy::solve_mixed(const double *b, double *x, hipStream_t s, i
         HX(scale_to_f32((float *)top.rhs, d64_, inv, n, s));
         if (cfg.zero_inner_start) HX(fill_bytes_zero(top.x, (size_t)n * 4, s));
         int64_t it = 0;
-        HX(run_kernel(s, &it, nullptr));
+        mixed_inner_ = true;
+        hipError_t ek = run_kernel(s, &it, nullptr);
+        mixed_inner_ = false;
+        HX(ek);
         inner_total += it;
         ++outer;
         HX(acc_scaled_f32(res64_, (const float *)top.x, scale, n, s));

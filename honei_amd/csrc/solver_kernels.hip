@@ -25,24 +25,31 @@ template <typename V> __device__ inline V eps_of();
 template <> __device__ inline float eps_of<float>() { return 1.1920928955078125e-07f; }
 template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }
 
-// v = A p and partial v.p; the last block forms lambda = alpha / v.p.
+// v = A p and partial v.p; the last block forms lambda = alpha / v.p.  At
+// most 2048 blocks, each walking a contiguous range of 256/T-row chunks.
 template <typename VT, int T>
 __global__ __launch_bounds__(256) void k_cg_spmv_dot(EllView A, const typename VecOf<VT>::type *p,
-                                                     typename VecOf<VT>::type *v, CGScalars *sc, ReduceSlots sl) {
+                                                     typename VecOf<VT>::type *v, CGScalars *sc, ReduceSlots sl,
+                                                     int64_t nlogical) {
     typedef typename VecOf<VT>::type V;
     if (sc->done) return;
-    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
-    const int lane = (int)(gid & (T - 1));
-    const int64_t row = gid / T;
-    const bool active = row < A.rows;
-    V s = V(0);
-    if (active) s = ell_lane_sum<VT, T>(A.Aj, (const VT *)A.Ax, row, lane, A.Arl[row], A.stride, p);
-    s = lane_group_sum<T>(s);
+    const int64_t pb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t per = nlogical / gridDim.x, rem = nlogical % gridDim.x;
+    const int64_t first = pb * per + (pb < rem ? pb : rem);
+    const int64_t last = first + per + (pb < rem ? 1 : 0);
     double part = 0.0;
-    if (active && lane == 0) {
-        v[row] = s;
-        part = (double)s * (double)p[row];
+    for (int64_t lb = first; lb < last; ++lb) {
+        const int64_t gid = lb * (int64_t)blockDim.x + threadIdx.x;
+        const int lane = (int)(gid & (T - 1));
+        const int64_t row = gid / T;
+        const bool active = row < A.rows;
+        V s = V(0);
+        if (active) s = ell_lane_sum<VT, T>(A.Aj, (const VT *)A.Ax, row, lane, A.Arl[row], A.stride, p);
+        s = lane_group_sum<T>(s);
+        if (active && lane == 0) {
+            v[row] = s;
+            part += (double)s * (double)p[row];
+        }
     }
     __shared__ double bs;
     const double b = block_sum(part);
@@ -230,12 +237,13 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
                      hipStream_t s) {
     typedef typename VecOf<VT>::type V;
     const int T = A.threads;
-    const int64_t grid = (A.rows * T + 255) / 256;
+    const int64_t nlogical = (A.rows * T + 255) / 256;
+    const int64_t grid = nlogical < 2048 ? nlogical : 2048;
     if (grid > sl.capacity) return hipErrorInvalidValue;
 #define CGK(TT)                                                                                              \
     case TT:                                                                                                 \
         hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)p,  \
-                           (V *)v, sc, sl);                                                                  \
+                           (V *)v, sc, sl, nlogical);                                                        \
         break;
     switch (T) {
         CGK(1) CGK(2) CGK(4) CGK(8) CGK(16) CGK(32) CGK(64)

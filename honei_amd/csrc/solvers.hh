@@ -37,7 +37,7 @@ hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32
                        const double *coeff, void *dinv, double omega, hipStream_t s);
 
 hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
-                      uint32_t *Arl, int64_t stride, hipStream_t s);
+                      uint32_t *Arl, int64_t stride, int threads, hipStream_t s);
 hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s);
 
 // ---------------------------------------------------------------------------
@@ -110,6 +110,7 @@ class MGHierarchy {
     double *scal_ = nullptr;                            // device scalars
     double *guess_ = nullptr, *d64_ = nullptr, *res64_ = nullptr;
     int64_t n64_ = 0;
+    bool mixed_inner_ = false;      // run_kernel called by the MIXED driver
     ReduceSlots slots_{};
     hipError_t smooth_from_zero(int64_t l, const void *rhs, void *out, void *acc, hipStream_t s);
     hipError_t sweeps(int64_t l, void *x, const void *rhs, int64_t count, hipStream_t s);

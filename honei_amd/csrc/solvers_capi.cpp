@@ -138,13 +138,15 @@ int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl,
 }
 
 int honei_hip_random_ell(int vtype, size_t n, size_t max_len, size_t band, uint64_t seed, void *Aj, void *Ax,
-                         void *Arl, size_t stride, void *stream) {
+                         void *Arl, size_t stride, size_t threads, void *stream) {
     if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
     if (!Aj || !Ax || !Arl) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
-    if (n == 0 || n > (size_t)INT32_MAX || max_len < 1 || max_len > 64 || band < max_len || stride < n)
-        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad sizes (need 1<=max_len<=64, band>=max_len, stride>=n)");
+    if (n == 0 || n > (size_t)INT32_MAX || max_len < 1 || max_len > 64 || band < max_len || !pow2_threads(threads) ||
+        stride < n * threads)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT,
+                    "bad sizes (need 1<=max_len<=64, band>=max_len, threads a power of two <= 64, stride>=n*threads)");
     return check(random_ell((ValueType)vtype, (int64_t)n, (int)max_len, (int64_t)band, seed, (int32_t *)Aj, Ax,
-                            (uint32_t *)Arl, (int64_t)stride, S(stream)),
+                            (uint32_t *)Arl, (int64_t)stride, (int)threads, S(stream)),
                  "random_ell");
 }
 

@@ -183,11 +183,12 @@ int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl,
  * Random ELL of the SpMV benchmark configuration generated in HBM (no
  * reference counterpart; the reference reads matrices from files): T = 1,
  * row lengths U{1..max_len}, distinct sorted columns uniform in
- * [r-band, r+band] clipped to [0, n), values N(0,1); Aj/Ax hold
- * max_len*stride slots, zero padded.  Deterministic in (n, max_len, band,
- * seed).                                                                   */
+ * [r-band, r+band] clipped to [0, n), values N(0,1), in the T-interleaved
+ * layout of `threads` (stride >= n*threads; Aj/Ax hold
+ * ceil(max_len/threads)*stride slots, zero padded).  Deterministic in
+ * (n, max_len, band, seed); the row contents do not depend on threads.     */
 int honei_hip_random_ell(int vtype, size_t n, size_t max_len, size_t band, uint64_t seed, void *Aj, void *Ax,
-                         void *Arl, size_t stride, void *stream);
+                         void *Arl, size_t stride, size_t threads, void *stream);
 
 /* ---- CG --------------------------------------------------------------------
  * CGSolver<Tag, methods::NONE>::value (math/cg.hh:63-133) on a device ELL

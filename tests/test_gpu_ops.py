@@ -289,13 +289,32 @@ def test_invalid_arguments_are_rejected():
         H.call("honei_hip_upload_indices", buf.ptr, bad.ctypes.data, 3, 25)
 
 
-def _device_random_ell(n, vt, seed=2, max_len=27, band=4096):
-    stride = 32 * ((n + 31) // 32)
-    Aj = H.DeviceArray(max_len * stride, np.int32)
-    Ax = H.DeviceArray(max_len * stride, np.float64 if vt == H.F64 else np.float32)
+def _device_random_ell(n, vt, seed=2, max_len=27, band=4096, threads=1):
+    stride = 32 * ((n * threads + 31) // 32)
+    ncpr = (max_len + threads - 1) // threads
+    Aj = H.DeviceArray(ncpr * stride, np.int32)
+    Ax = H.DeviceArray(ncpr * stride, np.float64 if vt == H.F64 else np.float32)
     Arl = H.DeviceArray(n, np.uint32)
-    H.call("honei_hip_random_ell", vt, n, max_len, band, seed, Aj.ptr, Ax.ptr, Arl.ptr, stride, None)
-    return oracle.Ell(n, n, stride, max_len, 1, Aj.get().astype(np.uint64), Ax.get(), Arl.get().astype(np.uint64))
+    H.call("honei_hip_random_ell", vt, n, max_len, band, seed, Aj.ptr, Ax.ptr, Arl.ptr, stride, threads, None)
+    return oracle.Ell(n, n, stride, ncpr, threads, Aj.get().astype(np.uint64), Ax.get(),
+                      Arl.get().astype(np.uint64))
+
+
+@pytest.mark.parametrize("threads", [2, 4, 8, 32])
+def test_random_ell_threads_layout_matches_conversion(threads):
+    """The generator's T-interleaved layout equals SparseMatrixELL(T) of the
+    same rows (CPU conversion of the T = 1 matrix)."""
+    n = 3000
+    A1 = _device_random_ell(n, H.F64, band=500)
+    AT = _device_random_ell(n, H.F64, band=500, threads=threads)
+    rp, col, val = A1.to_csr()
+    C = oracle.ell_from_sm(n, n, rp, col, val, threads)
+    assert (C.stride, C.ncpr) == (AT.stride, AT.ncpr)
+    assert np.array_equal(C.Aj, AT.Aj) and np.array_equal(C.Ax, AT.Ax) and np.array_equal(C.Arl, AT.Arl)
+    x = uniform_pm1(1, n)
+    y = _gpu_product(AT, x)
+    err = np.abs(y - oracle.product(AT, x))
+    assert (err <= 1e-13 * _abs_row_sums(AT, x) + 1e-30).all()
 
 
 @pytest.mark.parametrize("n,band", [(1000, 4096), (100000, 4096), (1 << 22, 4096)])
