@@ -152,7 +152,10 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section (for counter runs)")
+    ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter runs: 5 launches of the roofline SpMV and 5 of a 2^26-float scale kernel "
+                         "(reads and writes 256 MiB each) that calibrates FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path is single-GPU by design",
@@ -161,6 +164,20 @@ def main():
 
     H.init(0)
     n = SIDE * SIDE
+    if args.pmc_probe:
+        D32 = device_q1(SIDE, H.F32)
+        x32 = H.DeviceArray.from_host(uniform_pm1(2, n).astype(np.float32))
+        y32 = H.DeviceArray(n, np.float32)
+        for _ in range(5):
+            H.call("honei_hip_product_smell_dv_float", x32.ptr, y32.ptr, D32.Aj.ptr, D32.Ax.ptr, D32.Arl.ptr, 0, n,
+                   9, D32.stride, 0, 1, None)
+        big = H.DeviceArray(1 << 26, np.float32)
+        big.zero()
+        for _ in range(5):
+            H.call("honei_hip_scale_one", H.F32, big.ptr, 1.0, 1 << 26, None)
+        H.synchronize()
+        print(json.dumps({"pmc_probe": "done"}))
+        return
 
     # ---- standalone SpMV (roofline kernel: Q1 2049^2, FP32, T=1) ----------
     spmv = {}

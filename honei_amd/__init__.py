@@ -37,6 +37,7 @@ _SIGS = {
     "honei_hip_download": [_p, _p, _sz, _p],
     "honei_hip_copy": [_p, _p, _sz, _p],
     "honei_hip_fill_zero": [_p, _sz, _p],
+    "honei_hip_fill": [_i, _p, _d, _sz, _p],
     "honei_hip_synchronize": [_p],
     "honei_hip_stream_create": [C.POINTER(_p)],
     "honei_hip_stream_destroy": [_p],
@@ -77,6 +78,7 @@ _SIGS = {
     "honei_hip_random_ell": [_i, _sz, _sz, _sz, _u64, _p, _p, _p, _sz, _sz, _p],
     "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
                            C.POINTER(_d), _p],
+    "honei_hip_cg_konv": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _d, _sz, C.POINTER(_sz), _p],
     "honei_hip_mg_create": [C.POINTER(_p), _i, _sz, _sz],
     "honei_hip_mg_destroy": [_p],
     "honei_hip_mg_configure": [_p, _p],

@@ -1,9 +1,9 @@
 // blas1_kernels.hip -- streaming vector kernels and single-pass reductions.
 //
 // Elementwise kernels are grid-stride loops over 64-wide wavefronts.
-// Reductions accumulate in double, combine per block in a fixed tree,
-// publish one partial per block and let the last-arriving block form the
-// total on the device (no host round trip, no per-call allocation -- unlike
+// Reductions accumulate in double, combine per block in a fixed tree, store
+// one partial per block, and a one-block finisher forms the total on the
+// device (no host round trip, no per-call allocation -- unlike
 // backends/cuda/dot_product.cu:95-109, which copies grid*block partials to the
 // host).  The grid size depends only on n, so results are reproducible.
 
@@ -84,15 +84,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce(V *x, const V *y, V a, const 
             acc += (double)v * (double)v;
         }
     }
-    __shared__ double bsum;
     const double b = block_sum(acc);
-    if (threadIdx.x == 0) bsum = b;
-    __syncthreads();
-    if (arrive_last(slots, &bsum, 1)) {
-        const double tot = finish_partial(slots, 1, 0);
-        if (threadIdx.x == 0) *out = tot;
-        reset_counter(slots);
-    }
+    if (threadIdx.x == 0) slots.partials[blockIdx.x] = b;
+}
+
+__global__ void k_finish_sum(const double *partials, int64_t nparts, double *out) {
+    const double t = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) *out = t;
 }
 
 template <typename F>
@@ -108,16 +106,20 @@ hipError_t dot(const void *x, const void *y, int64_t n, ValueType vt, double *ou
                hipStream_t s) {
     return by_type(vt, [&](auto z) {
         typedef decltype(z) V;
-        hipLaunchKernelGGL((k_reduce<V, 0>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
+        const int g = reduce_grid(n);
+        hipLaunchKernelGGL((k_reduce<V, 0>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)y,
                            V(0), (const double *)nullptr, n, out, sl);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
         return hipGetLastError();
     });
 }
 hipError_t norm2(const void *x, int64_t n, ValueType vt, double *out, const ReduceSlots &sl, hipStream_t s) {
     return by_type(vt, [&](auto z) {
         typedef decltype(z) V;
-        hipLaunchKernelGGL((k_reduce<V, 1>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)x,
+        const int g = reduce_grid(n);
+        hipLaunchKernelGGL((k_reduce<V, 1>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)x,
                            V(0), (const double *)nullptr, n, out, sl);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
         return hipGetLastError();
     });
 }
@@ -125,8 +127,10 @@ hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_
                       double *out, const ReduceSlots &sl, hipStream_t s) {
     return by_type(vt, [&](auto z) {
         typedef decltype(z) V;
-        hipLaunchKernelGGL((k_reduce<V, 2>), dim3(reduce_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
+        const int g = reduce_grid(n);
+        hipLaunchKernelGGL((k_reduce<V, 2>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)y,
                            (V)alpha, alpha_dev, n, out, sl);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
         return hipGetLastError();
     });
 }

@@ -183,6 +183,11 @@ int honei_hip_fill_zero(void *dst, size_t bytes, void *stream) {
     if (bytes && !dst) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
     return check(fill_bytes_zero(dst, bytes, S(stream)), "fill_zero");
 }
+int honei_hip_fill(int vt, void *x, double value, size_t n, void *stream) {
+    if (!valid_vtype(vt)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
+    if (n && !x) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
+    return n ? check(vfill(x, value, (int64_t)n, (ValueType)vt, S(stream)), "fill") : HONEI_HIP_SUCCESS;
+}
 int honei_hip_synchronize(void *stream) {
     return check(stream ? hipStreamSynchronize(S(stream)) : hipDeviceSynchronize(), "synchronize");
 }

@@ -103,16 +103,14 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
         }
     }
     if constexpr (NORM) {
-        __shared__ double bsum;
         const double v = block_sum(nval);
-        if (threadIdx.x == 0) bsum = v;
-        __syncthreads();
-        if (arrive_last(a.slots, &bsum, 1)) {
-            const double tot = finish_partial(a.slots, 1, 0);
-            if (threadIdx.x == 0) *a.norm2 = tot;
-            reset_counter(a.slots);
-        }
+        if (threadIdx.x == 0) a.slots.partials[blockIdx.x] = v;
     }
+}
+
+__global__ void k_finish_norm(const double *partials, int64_t nparts, double *out) {
+    const double t = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) *out = t;
 }
 
 typedef void (*KernelFn)(KArgs);
@@ -170,10 +168,8 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     }
     const int64_t nlogical = (nrows * T + block - 1) / block;
     if (nlogical > 0x7fffffff) return hipErrorInvalidValue;
-    // launches that reduce run at most kReduceBlocks blocks
-    constexpr int64_t kReduceBlocks = 2048;
-    const int64_t grid = norm ? (nlogical < kReduceBlocks ? nlogical : kReduceBlocks) : nlogical;
-    if (norm && grid > slots.capacity) return hipErrorInvalidValue;
+    // launches that reduce keep the block count within the partial slots
+    const int64_t grid = (norm && nlogical > slots.capacity) ? slots.capacity : nlogical;
     KArgs k;
     k.Aj = A.Aj;
     k.Ax = A.Ax;
@@ -190,6 +186,7 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     k.nlogical = nlogical;
     k.slots = slots;
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
+    if (norm) hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, (const double *)slots.partials, grid, e.norm2);
     return hipGetLastError();
 }
 

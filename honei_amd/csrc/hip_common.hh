@@ -71,43 +71,22 @@ template <typename T> __device__ inline T block_sum(T v) {
     return t;
 }
 
-// Cross-block reduction slots: one partial per block plus an arrival counter
-// that the last block resets, so no memset is needed between calls.
+// Cross-block reduction slots.  A reducing kernel stores one partial per
+// block with a plain store; a one-block finisher launched behind it sums the
+// partials in a fixed order.  (A last-block-arrives scheme would need an
+// agent-scope release per block, which on MI355X writes back the XCD's L2 --
+// far more expensive than the extra launch.)
 struct ReduceSlots {
     double *partials;      // [capacity][lanes]
-    unsigned int *counter; // 1 counter
+    unsigned int *counter; // unused by the partial/finisher scheme; kept zero
     int capacity;          // max blocks
 };
 
-// Publish this block's partials (lanes values, thread i < lanes holds value
-// i) and return true in every thread of the block that arrived last; that
-// block may then read all partials with finish_partial().
-__device__ inline bool arrive_last(const ReduceSlots &s, const double *vals, int lanes) {
-    __shared__ bool last;
-    if (threadIdx.x < (unsigned)lanes)
-        __hip_atomic_store(&s.partials[(size_t)blockIdx.x * lanes + threadIdx.x], vals[threadIdx.x],
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int prev =
-            __hip_atomic_fetch_add(s.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    return last;
-}
-
-// Inside the last block: deterministic sum of partial `lane` over all blocks.
-__device__ inline double finish_partial(const ReduceSlots &s, int lanes, int lane) {
+// Deterministic sum of partials[b*lanes + lane] over b < nparts by one block.
+__device__ inline double sum_partials(const double *partials, int64_t nparts, int lanes, int lane) {
     double v = 0.0;
-    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-        v += __hip_atomic_load(&s.partials[(size_t)b * lanes + lane], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    for (int64_t b = threadIdx.x; b < nparts; b += blockDim.x) v += partials[b * lanes + lane];
     return block_sum(v);
-}
-
-__device__ inline void reset_counter(const ReduceSlots &s) {
-    if (threadIdx.x == 0) __hip_atomic_store(s.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

@@ -51,19 +51,20 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot(EllView A, const typename V
             part += (double)s * (double)p[row];
         }
     }
-    __shared__ double bs;
     const double b = block_sum(part);
-    if (threadIdx.x == 0) bs = b;
-    __syncthreads();
-    if (arrive_last(sl, &bs, 1)) {
-        const double tot = finish_partial(sl, 1, 0);
-        if (threadIdx.x == 0) {
-            const V temp = (V)tot;
-            const V e = eps_of<V>();
-            const V alpha = (V)sc->alpha;
-            sc->lambda = (double)(alpha / ((temp < 0 ? -temp : temp) > e ? temp : e));
-        }
-        reset_counter(sl);
+    if (threadIdx.x == 0) sl.partials[blockIdx.x] = b;
+}
+
+// lambda = alpha / (v.p) from the partials of k_cg_spmv_dot (one block)
+template <typename V>
+__global__ void k_cg_lambda(const double *partials, int64_t nparts, CGScalars *sc) {
+    if (sc->done) return;
+    const double tot = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) {
+        const V temp = (V)tot;
+        const V e = eps_of<V>();
+        const V alpha = (V)sc->alpha;
+        sc->lambda = (double)(alpha / ((temp < 0 ? -temp : temp) > e ? temp : e));
     }
 }
 
@@ -81,25 +82,26 @@ __global__ __launch_bounds__(256) void k_cg_update(V *x, const V *p, V *r, const
         r[i] = ri;
         acc += (double)ri * (double)ri;
     }
-    __shared__ double bs;
     const double b = block_sum(acc);
-    if (threadIdx.x == 0) bs = b;
-    __syncthreads();
-    if (arrive_last(sl, &bs, 1)) {
-        const double tot = finish_partial(sl, 1, 0);
-        if (threadIdx.x == 0) {
-            const V e = eps_of<V>();
-            const V alpha_old = (V)sc->alpha;
-            const V alpha = (V)tot;
-            sc->alpha = (double)alpha;
-            sc->talpha = (double)(alpha / ((alpha_old < 0 ? -alpha_old : alpha_old) > e ? alpha_old : e));
-            sc->iters += 1;
-            const V current = sqrt(alpha);
-            sc->current = (double)current;
-            if (current < (V)sc->eps_rel * (V)sc->initial || current < (V)sc->eps_rel || sc->iters >= sc->max_iters)
-                sc->done = 1;
-        }
-        reset_counter(sl);
+    if (threadIdx.x == 0) sl.partials[blockIdx.x] = b;
+}
+
+// alpha = ||r||^2, talpha, iteration count and convergence test (one block)
+template <typename V>
+__global__ void k_cg_alpha(const double *partials, int64_t nparts, CGScalars *sc) {
+    if (sc->done) return;
+    const double tot = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) {
+        const V e = eps_of<V>();
+        const V alpha_old = (V)sc->alpha;
+        const V alpha = (V)tot;
+        sc->alpha = (double)alpha;
+        sc->talpha = (double)(alpha / ((alpha_old < 0 ? -alpha_old : alpha_old) > e ? alpha_old : e));
+        sc->iters += 1;
+        const V current = sqrt(alpha);
+        sc->current = (double)current;
+        if (current < (V)sc->eps_rel * (V)sc->initial || current < (V)sc->eps_rel || sc->iters >= sc->max_iters)
+            sc->done = 1;
     }
 }
 
@@ -238,8 +240,7 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
     typedef typename VecOf<VT>::type V;
     const int T = A.threads;
     const int64_t nlogical = (A.rows * T + 255) / 256;
-    const int64_t grid = nlogical < 2048 ? nlogical : 2048;
-    if (grid > sl.capacity) return hipErrorInvalidValue;
+    const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
 #define CGK(TT)                                                                                              \
     case TT:                                                                                                 \
         hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)p,  \
@@ -250,9 +251,11 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
         default: return hipErrorInvalidValue;
     }
 #undef CGK
+    hipLaunchKernelGGL((k_cg_lambda<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     const int g2 = grid_for(A.rows, 2048);
     hipLaunchKernelGGL((k_cg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)p, (V *)r, (const V *)v,
                        A.rows, sc, sl);
+    hipLaunchKernelGGL((k_cg_alpha<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g2, sc);
     hipLaunchKernelGGL((k_cg_direction<V>), dim3(grid_for(A.rows)), dim3(256), 0, s, (V *)p, (const V *)r, A.rows,
                        (const CGScalars *)sc);
     return hipGetLastError();

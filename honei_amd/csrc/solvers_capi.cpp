@@ -124,6 +124,44 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     return HONEI_HIP_SUCCESS;
 }
 
+int honei_hip_cg_konv(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                      size_t num_cols_per_row, size_t stride, size_t threads, const void *b, void *x,
+                      double konv_rad, size_t max_iters, size_t *iters, void *stream) {
+    if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64 && vtype != HONEI_HIP_BF16)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
+    if (!Aj || !Ax || !Arl || !b || !x) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
+    if (!pow2_threads(threads) || rows * threads > stride)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad ELL layout");
+    if (max_iters > 0x7fffffff) max_iters = 0x7fffffff;
+    Workspace *w = workspace_for(S(stream));
+    if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    const size_t es = vec_size((ValueType)vtype);
+    const size_t vbytes = ((rows * es + 255) / 256) * 256;
+    char *base = (char *)w->ensure_scratch(3 * vbytes + 256);
+    if (!base) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    int *it_dev = (int *)(base + 3 * vbytes);
+    EllView A;
+    A.rows = (int64_t)rows;
+    A.cols = (int64_t)rows;
+    A.stride = (int64_t)stride;
+    A.ncpr = (int64_t)num_cols_per_row;
+    A.threads = (int)threads;
+    A.Aj = (const int32_t *)Aj;
+    A.Ax = Ax;
+    A.Arl = (const uint32_t *)Arl;
+    A.vtype = (ValueType)vtype;
+    int st = check(coarse_cg(A, b, x, base, base + vbytes, base + 2 * vbytes, konv_rad, (int)max_iters, it_dev,
+                             S(stream)),
+                   "cg_konv");
+    if (st) return st;
+    int h = 0;
+    if ((st = check(hipMemcpyAsync(&h, it_dev, sizeof(int), hipMemcpyDeviceToHost, S(stream)), "cg_konv readback")))
+        return st;
+    if ((st = check(hipStreamSynchronize(S(stream)), "cg_konv sync"))) return st;
+    if (iters) *iters = (size_t)h;
+    return HONEI_HIP_SUCCESS;
+}
+
 // ---- Q1 assembly -------------------------------------------------------------
 int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl, size_t stride,
                           const double *coeff, void *dinv, double omega, void *stream) {

@@ -56,6 +56,7 @@ int honei_hip_upload(void *dst_dev, const void *src_host, size_t bytes, void *st
 int honei_hip_download(void *dst_host, const void *src_dev, size_t bytes, void *stream);
 int honei_hip_copy(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
 int honei_hip_fill_zero(void *dst_dev, size_t bytes, void *stream);
+int honei_hip_fill(int vtype, void *x_dev, double value, size_t n, void *stream); /* fill<Tag>(x, value) */
 int honei_hip_synchronize(void *stream);
 int honei_hip_stream_create(void **stream);
 int honei_hip_stream_destroy(void *stream);
@@ -200,6 +201,15 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
                        size_t num_cols_per_row, size_t stride, size_t threads, const void *b_dev, void *x_dev,
                        size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
                        void *stream);
+
+/* ConjugateGradients<Tag, methods::NONE>::value(A, b, x, konv_rad)
+ * (math/conjugate_gradients.hh:154-201, 587-623), the multigrid coarse
+ * solver: x = 0, iterate until ||x||^2 grows by no more than konv_rad.
+ * Runs inside one work-group (for the small coarse grids it serves);
+ * max_iters bounds the loop (the reference has no bound).                   */
+int honei_hip_cg_konv(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                      size_t num_cols_per_row, size_t stride, size_t threads, const void *b_dev, void *x_dev,
+                      double konv_rad, size_t max_iters, size_t *iters, void *stream);
 
 /* ---- multigrid ---------------------------------------------------------------
  * A device-resident hierarchy plays the role of MGInfo
