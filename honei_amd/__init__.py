@@ -86,6 +86,7 @@ _SIGS = {
     "honei_hip_mg_set_transfer": [_p, _sz, _i, _sz, _sz, _sz, _sz, _sz, _p, _p, _p],
     "honei_hip_mg_set_outer": [_p, _sz, _sz, _sz, _sz, _p, _p, _p],
     "honei_hip_mg_build_q1": [_p, _d, _p],
+    "honei_hip_mg_build_q1_coeff": [_p, _d, _p, _i, _p],
     "honei_hip_mg_solve_fixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
     "honei_hip_mg_solve_mixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_d), _p],
 }
@@ -307,6 +308,9 @@ class Multigrid:
 
     def build_q1(self, omega=0.7, stream=None):
         call("honei_hip_mg_build_q1", self.h, omega, stream)
+
+    def build_q1_coeff(self, coeff: "DeviceArray", omega=0.7, averaging=0, stream=None):
+        call("honei_hip_mg_build_q1_coeff", self.h, omega, coeff.ptr, averaging, stream)
 
     def set_level(self, level, A, dinv):
         Ax = np.ascontiguousarray(A.Ax, np.float64 if self.inner == F64 else np.float32)

@@ -140,6 +140,42 @@ hipError_t MGHierarchy::build_q1(double omega, hipStream_t s) {
     return hipStreamSynchronize(s);
 }
 
+hipError_t MGHierarchy::build_q1_coeff(double omega, const double *coeff, int averaging, hipStream_t s) {
+    // per-level coefficient arrays, finest first
+    std::vector<double *> kap((size_t)max_level + 1, nullptr);
+    hipError_t err = hipSuccess;
+    for (int64_t l = max_level; l >= min_level && err == hipSuccess; --l) {
+        const int64_t cells = (1ll << l);
+        if (l == max_level) {
+            kap[(size_t)l] = (double *)coeff;
+            continue;
+        }
+        err = hipMalloc((void **)&kap[(size_t)l], sizeof(double) * (size_t)(cells * cells));
+        if (err == hipSuccess) err = coarsen_coeff(kap[(size_t)l + 1], 2 * cells, kap[(size_t)l], cells, averaging, s);
+    }
+    for (int64_t l = min_level; l <= max_level && err == hipSuccess; ++l) {
+        const int64_t side = (1ll << l) + 1, n = side * side;
+        const int64_t stride = 32 * ((n + 31) / 32);
+        err = alloc_level_vectors(l, n);
+        if (err == hipSuccess) err = dev_ell_alloc(L[(size_t)l].A, inner, n, n, stride, side > 2 ? 9 : 1, 1);
+        if (err == hipSuccess)
+            err = q1_assemble(inner, side, L[(size_t)l].A.Aj, L[(size_t)l].A.Ax, L[(size_t)l].A.Arl, stride,
+                              kap[(size_t)l], L[(size_t)l].dinv, omega, s);
+    }
+    if (err == hipSuccess) {
+        const int64_t side = (1ll << max_level) + 1, n = side * side;
+        const int64_t stride = 32 * ((n + 31) / 32);
+        err = dev_ell_alloc(A_outer, VT_F64, n, n, stride, 9, 1);
+        if (err == hipSuccess)
+            err = q1_assemble(VT_F64, side, A_outer.Aj, A_outer.Ax, A_outer.Arl, stride, coeff, nullptr, 0.0, s);
+    }
+    if (err == hipSuccess) err = alloc_coarse_scratch();
+    (void)hipStreamSynchronize(s);
+    for (int64_t l = min_level; l < max_level; ++l)
+        if (kap[(size_t)l]) (void)hipFree(kap[(size_t)l]);
+    return err;
+}
+
 hipError_t MGHierarchy::read_scalar(double *dst, const double *src, hipStream_t s) {
     HX(hipMemcpyAsync(dst, src, sizeof(double), hipMemcpyDeviceToHost, s));
     return hipStreamSynchronize(s);

@@ -98,14 +98,23 @@ __global__ void k_q1_assemble(int64_t side, int32_t *Aj, VT *Ax, uint32_t *Arl, 
     }
 }
 
-// coarse cell (I, J) = mean of fine cells (2I..2I+1, 2J..2J+1)
-__global__ void k_coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells) {
+// coarse cell (I, J) = mean of fine cells (2I..2I+1, 2J..2J+1):
+// averaging 0 arithmetic, 1 geometric, 2 harmonic
+__global__ void k_coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells,
+                                int averaging) {
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (id >= nc_cells * nc_cells) return;
     const int64_t I = id / nc_cells, J = id % nc_cells;
     const double a = fine[(2 * I) * nf_cells + 2 * J], b = fine[(2 * I) * nf_cells + 2 * J + 1];
     const double c = fine[(2 * I + 1) * nf_cells + 2 * J], d = fine[(2 * I + 1) * nf_cells + 2 * J + 1];
-    coarse[id] = 0.25 * (((a + b) + c) + d);
+    double m;
+    if (averaging == 1)
+        m = exp(0.25 * (((log(a) + log(b)) + log(c)) + log(d)));
+    else if (averaging == 2)
+        m = 4.0 / (((1.0 / a + 1.0 / b) + 1.0 / c) + 1.0 / d);
+    else
+        m = 0.25 * (((a + b) + c) + d);
+    coarse[id] = m;
 }
 
 }  // namespace
@@ -131,9 +140,10 @@ hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32
     return hipGetLastError();
 }
 
-hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s) {
+hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, int averaging,
+                         hipStream_t s) {
     hipLaunchKernelGGL(k_coarsen_coeff, dim3((int)((nc_cells * nc_cells + 255) / 256)), dim3(256), 0, s, fine,
-                       nf_cells, coarse, nc_cells);
+                       nf_cells, coarse, nc_cells, averaging);
     return hipGetLastError();
 }
 

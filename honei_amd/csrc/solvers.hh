@@ -38,7 +38,9 @@ hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32
 
 hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
                       uint32_t *Arl, int64_t stride, int threads, hipStream_t s);
-hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, hipStream_t s);
+// averaging: 0 arithmetic, 1 geometric, 2 harmonic mean of the 4 fine cells
+hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, int averaging,
+                         hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Owned device ELL matrix
@@ -94,6 +96,10 @@ class MGHierarchy {
     // Build every level (and the FP64 outer system) on the device as the
     // constant-coefficient Q1 Dirichlet problem on (2^l+1)^2 grids.
     hipError_t build_q1(double omega, hipStream_t s);
+    // Same with a per-cell coefficient on the finest grid ((2^max-1)^2
+    // doubles, device memory); coarse levels use the mean of the four fine
+    // cells they cover (rediscretisation).
+    hipError_t build_q1_coeff(double omega, const double *coeff_dev, int averaging, hipStream_t s);
 
     // One _multigrid_kernel call on the inner levels; returns the number of
     // V-cycles through *inner_iters.  rhs[max] and x[max] are inputs.

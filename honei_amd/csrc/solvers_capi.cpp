@@ -262,6 +262,12 @@ int honei_hip_mg_build_q1(void *mg, double omega, void *stream) {
     return check(((MGHierarchy *)mg)->build_q1(omega, S(stream)), "mg_build_q1");
 }
 
+int honei_hip_mg_build_q1_coeff(void *mg, double omega, const double *coeff, int averaging, void *stream) {
+    if (!mg || !coeff) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    if (averaging < 0 || averaging > 2) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "averaging must be 0, 1 or 2");
+    return check(((MGHierarchy *)mg)->build_q1_coeff(omega, coeff, averaging, S(stream)), "mg_build_q1_coeff");
+}
+
 int honei_hip_mg_solve_fixed(void *mg, const void *b, void *x, size_t *iters, double *final_defect, void *stream) {
     if (!mg || !b || !x) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
     MGHierarchy *h = (MGHierarchy *)mg;

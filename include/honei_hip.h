@@ -248,6 +248,11 @@ int honei_hip_mg_set_transfer(void *mg, size_t level, int kind, size_t rows, siz
 int honei_hip_mg_set_outer(void *mg, size_t rows, size_t stride, size_t num_cols_per_row, size_t threads,
                            const uint64_t *Aj, const double *Ax, const uint64_t *Arl);
 int honei_hip_mg_build_q1(void *mg, double omega, void *stream);
+/* Variable-coefficient Q1 hierarchy (no reference counterpart): coeff_dev
+ * holds (2^max_level)^2 per-cell values (double, row-major); coarser levels
+ * average the four fine cells (averaging 0 arithmetic, 1 geometric,
+ * 2 harmonic) and are rediscretised.                                        */
+int honei_hip_mg_build_q1_coeff(void *mg, double omega, const double *coeff_dev, int averaging, void *stream);
 int honei_hip_mg_solve_fixed(void *mg, const void *b_dev, void *x_dev, size_t *iters, double *final_defect,
                              void *stream);
 int honei_hip_mg_solve_mixed(void *mg, const double *b_dev, double *x_dev, size_t *outer_iters,
