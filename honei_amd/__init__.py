@@ -74,7 +74,7 @@ _SIGS = {
     "honei_hip_element_inverse_one": [_i, _p, _sz, _p],
     "honei_hip_restriction": [_i, _p, _sz, _p, _sz, _p, _p],
     "honei_hip_prolongation": [_i, _p, _sz, _p, _sz, _p, _p],
-    "honei_hip_q1_assemble": [_i, _sz, _p, _p, _p, _sz, _p, _p, _d, _p],
+    "honei_hip_q1_assemble": [_i, _sz, _p, _p, _p, _sz, _p, _p, _d, _sz, _p],
     "honei_hip_random_ell": [_i, _sz, _sz, _sz, _u64, _p, _p, _p, _sz, _sz, _p],
     "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
                            C.POINTER(_d), _p],
@@ -89,6 +89,22 @@ _SIGS = {
     "honei_hip_mg_build_q1_coeff": [_p, _d, _p, _i, _p],
     "honei_hip_mg_solve_fixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
     "honei_hip_mg_solve_mixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_d), _p],
+    "honei_hip_fill_uniform": [_p, _sz, _d, _d, _u64, _p],
+    "honei_hip_fill_lognormal": [_p, _sz, _d, _u64, _p],
+    "honei_hip_batch_config_default": [_p],
+    "honei_hip_batch_create": [C.POINTER(_p), _p, _p, C.c_int32, _p, _u64],
+    "honei_hip_batch_destroy": [_p],
+    "honei_hip_batch_info": [_p, C.POINTER(C.c_int64), C.POINTER(_u64), C.POINTER(C.c_int64)],
+    "honei_hip_batch_slot": [_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64)],
+    "honei_hip_batch_load": [_p, C.c_int64, _p, _p],
+    "honei_hip_batch_activate": [_p, C.c_int32],
+    "honei_hip_batch_step": [_p],
+    "honei_hip_batch_poll": [_p, _p],
+    "honei_hip_batch_result": [_p, C.c_int64, _p],
+    "honei_hip_batch_run": [_p, C.POINTER(C.c_int64)],
+    "honei_hip_batch_shape": [_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(_d),
+                              C.POINTER(C.c_int64)],
+    "honei_hip_serve_run": [_p, _p, C.c_int64, _p, _p],
 }
 _RESTYPE = {"honei_hip_last_error": C.c_char_p, "honei_hip_version": C.c_char_p}
 
@@ -348,6 +364,155 @@ class Multigrid:
             if self.h:
                 lib().honei_hip_mg_destroy(self.h)
                 self.h = None
+        except Exception:
+            pass
+
+
+class BatchConfig(C.Structure):
+    """honei_hip_batch_config (include/honei_hip.h)."""
+    _fields_ = [("inner_vtype", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", _d),
+                ("tolerance", _d), ("max_outer", C.c_int32), ("averaging", C.c_int32),
+                ("iters_per_launch", C.c_int32), ("autotune", C.c_int32)]
+
+
+class SlotInfo(C.Structure):
+    """honei_hip_slot_info (include/honei_hip.h)."""
+    _fields_ = [("initial", _d), ("norm", _d), ("rel", _d), ("state", C.c_int32), ("iters", C.c_int32),
+                ("status", C.c_int32), ("level", C.c_int32)]
+
+
+def batch_config(**kw):
+    c = BatchConfig()
+    call("honei_hip_batch_config_default", C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise TypeError(f"unknown batch option {k}")
+        setattr(c, k, v)
+    return c
+
+
+class Request(C.Structure):
+    """honei_hip_request (include/honei_hip.h)."""
+    _fields_ = [("arrival_s", _d), ("level", C.c_int32), ("pad", C.c_int32), ("coeff_seed", _u64),
+                ("rhs_seed", _u64), ("x_dev", _p)]
+
+
+class RequestResult(C.Structure):
+    """honei_hip_request_result (include/honei_hip.h)."""
+    _fields_ = [("admit_s", _d), ("done_s", _d), ("rel", _d), ("iters", C.c_int32), ("status", C.c_int32),
+                ("slot", C.c_int64)]
+
+
+class ServeStats(C.Structure):
+    """honei_hip_serve_stats (include/honei_hip.h)."""
+    _fields_ = [("completed", C.c_int64), ("not_converged", C.c_int64), ("rejected", C.c_int64),
+                ("launches", C.c_int64), ("max_queue", C.c_int64), ("wall_s", _d), ("solves_per_s", _d),
+                ("p50_ms", _d), ("p99_ms", _d), ("max_ms", _d), ("mean_ms", _d), ("device_bytes", _u64)]
+
+
+class ServeConfig(C.Structure):
+    """honei_hip_serve_config (include/honei_hip.h)."""
+    _fields_ = [("batch", BatchConfig), ("nclass", C.c_int32), ("max_slots_per_class", C.c_int32),
+                ("levels", C.c_int64 * 16), ("counts", C.c_int64 * 16), ("budget_bytes", _u64), ("sigma", _d)]
+
+
+def serve(levels, requests, counts=None, budget_bytes=0, sigma=1.0, max_slots_per_class=256, **batch_kw):
+    """Run the continuous-batching loop over `requests` (list of
+    (arrival_s, level, coeff_seed, rhs_seed[, DeviceArray x])).  Returns
+    (list of RequestResult, ServeStats)."""
+    cfg = ServeConfig()
+    cfg.batch = batch_config(**batch_kw)
+    cfg.nclass = len(levels)
+    cfg.max_slots_per_class = max_slots_per_class
+    for i, lv in enumerate(levels):
+        cfg.levels[i] = lv
+        cfg.counts[i] = 0 if counts is None else counts[i]
+    cfg.budget_bytes = budget_bytes
+    cfg.sigma = sigma
+    rq = (Request * max(1, len(requests)))()
+    for i, r in enumerate(requests):
+        rq[i].arrival_s, rq[i].level, rq[i].coeff_seed, rq[i].rhs_seed = r[0], r[1], r[2], r[3]
+        rq[i].x_dev = r[4].ptr if len(r) > 4 and r[4] is not None else None
+    res = (RequestResult * max(1, len(requests)))()
+    st = ServeStats()
+    call("honei_hip_serve_run", C.byref(cfg), rq, len(requests), res, C.byref(st))
+    return list(res)[:len(requests)], st
+
+
+def fill_uniform(arr: "DeviceArray", lo, hi, seed, stream=None):
+    call("honei_hip_fill_uniform", arr.ptr, arr.n, lo, hi, seed, stream)
+
+
+def fill_lognormal(arr: "DeviceArray", sigma, seed, stream=None):
+    call("honei_hip_fill_lognormal", arr.ptr, arr.n, sigma, seed, stream)
+
+
+class Batch:
+    """Owning handle to a batched multigrid pool (honei_hip_batch_*)."""
+
+    def __init__(self, levels, counts, cfg: BatchConfig = None, budget_bytes=0):
+        cfg = cfg if cfg is not None else batch_config()
+        lv = (C.c_int64 * len(levels))(*levels)
+        ct = (C.c_int64 * len(counts))(*counts)
+        p = _p()
+        call("honei_hip_batch_create", C.byref(p), lv, ct, len(levels), C.byref(cfg), budget_bytes)
+        self.h = p.value
+        n, b = C.c_int64(), _u64()
+        call("honei_hip_batch_info", self.h, C.byref(n), C.byref(b), None)
+        self.nslots, self.device_bytes = n.value, b.value
+        self.levels, self.counts = list(levels), list(counts)
+
+    def slot(self, cls, j):
+        g = C.c_int64()
+        call("honei_hip_batch_slot", self.h, cls, j, C.byref(g))
+        return g.value
+
+    def load(self, slot, coeff: "DeviceArray", b: "DeviceArray"):
+        call("honei_hip_batch_load", self.h, slot, coeff.ptr, b.ptr)
+
+    def activate(self, wait=True):
+        call("honei_hip_batch_activate", self.h, 1 if wait else 0)
+
+    def step(self):
+        call("honei_hip_batch_step", self.h)
+
+    def poll(self):
+        info = (SlotInfo * self.nslots)()
+        call("honei_hip_batch_poll", self.h, info)
+        return info
+
+    def result(self, slot, x: "DeviceArray"):
+        call("honei_hip_batch_result", self.h, slot, x.ptr)
+
+    def run(self):
+        k = C.c_int64()
+        call("honei_hip_batch_run", self.h, C.byref(k))
+        return k.value
+
+    def shape(self, level):
+        t, b = C.c_int32(), C.c_int32()
+        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), None, None)
+        return t.value, b.value
+
+    def tuning(self, level):
+        """(threads, block, measured sweep us, tuning-batch slots)."""
+        t, b, us, ns = C.c_int32(), C.c_int32(), _d(), C.c_int64()
+        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), C.byref(us), C.byref(ns))
+        return t.value, b.value, us.value, ns.value
+
+    def launches(self):
+        k = C.c_int64()
+        call("honei_hip_batch_info", self.h, None, None, C.byref(k))
+        return k.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().honei_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

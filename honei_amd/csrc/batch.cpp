@@ -1,0 +1,516 @@
+// batch.cpp -- host side of the batched multigrid solver: pool layout,
+// launch-shape autotuning, per-slot loading, graph capture and the outer
+// iteration loop.
+
+#include "batch.hh"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+
+namespace honei {
+namespace hipx {
+
+namespace {
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// Launch shapes chosen by the autotuner, per (device, level, value type), for
+// the lifetime of the process: every batch of a process uses one shape per
+// level, so a system's result does not depend on the other systems in its
+// batch.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int64_t, int>, LaunchShape> g_tuned;
+
+struct Level0 {
+    int64_t side, n, stride, ncpr, ax_ld, vld, bps;
+};
+
+Level0 level_geometry(int64_t level, int T, int block) {
+    Level0 g;
+    g.side = (1ll << level) + 1;
+    g.n = g.side * g.side;
+    g.stride = 32 * ((g.n * T + 31) / 32);
+    g.ncpr = (9 + T - 1) / T;
+    g.ax_ld = g.ncpr * g.stride;
+    g.vld = round_up(g.n, 64);
+    g.bps = (g.n * T + block - 1) / block;
+    return g;
+}
+
+}  // namespace
+
+BatchMG::~BatchMG() { release(); }
+
+void BatchMG::release() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    if (s_main_) (void)hipStreamSynchronize(s_main_);
+    if (s_load_) (void)hipStreamSynchronize(s_load_);
+    for (hipEvent_t e : load_ev_)
+        if (e) (void)hipEventDestroy(e);
+    load_ev_.clear();
+    for (void *p : allocs_) (void)hipFree(p);
+    allocs_.clear();
+    if (sc_host_) (void)hipHostFree(sc_host_);
+    sc_host_ = nullptr;
+    if (s_main_) (void)hipStreamDestroy(s_main_);
+    if (s_load_) (void)hipStreamDestroy(s_load_);
+    s_main_ = s_load_ = nullptr;
+    bytes_ = 0;
+}
+
+hipError_t BatchMG::alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    HX(hipMalloc(p, bytes));
+    allocs_.push_back(*p);
+    bytes_ += bytes;
+    return fill_bytes_zero(*p, bytes, s_main_);
+}
+
+int64_t BatchMG::slot(int cls, int64_t j) const {
+    if (cls < 0 || cls >= (int)class_of_.size()) return -1;
+    const BClass &c = C_[(size_t)class_of_[(size_t)cls]];
+    if (j < 0 || j >= c.count) return -1;
+    return c.first + j;
+}
+
+int64_t BatchMG::slot_level(int64_t g) const {
+    for (const BClass &c : C_)
+        if (g >= c.first && g < c.first + c.count) return c.level;
+    return -1;
+}
+
+// Time one batched Jacobi sweep for each (T, block) candidate on a synthetic
+// batch of the level's shape (constant-coefficient values; timing does not
+// depend on them) and keep the fastest.
+hipError_t BatchMG::tune_level(BLevel &lv) {
+    const ValueType vt = (ValueType)cfg_.inner_vtype;
+    int dev = 0;
+    HX(hipGetDevice(&dev));
+    const auto key = std::make_tuple(dev, lv.level, (int)vt);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            lv.T = it->second.T;
+            lv.block = it->second.block;
+            return hipSuccess;
+        }
+    }
+    const int64_t n = lv.n;
+    const int64_t ns = std::max<int64_t>(1, std::min<int64_t>(256, (int64_t(1) << 22) / n));
+    LaunchShape best;
+    best.us = 1e30;
+    hipEvent_t e0, e1;
+    HX(hipEventCreate(&e0));
+    HX(hipEventCreate(&e1));
+    hipError_t err = hipSuccess;
+    const size_t vs = vt_size(vt);
+    for (int T : {1, 2, 4}) {
+        const Level0 g = level_geometry(lv.level, T, 256);
+        void *Aj = nullptr, *Arl = nullptr, *Ax = nullptr, *vec = nullptr, *scd = nullptr, *lst = nullptr;
+        const size_t vbytes = (size_t)(ns * g.vld) * 4;
+        err = hipMalloc(&Aj, (size_t)g.ax_ld * 4);
+        if (err == hipSuccess) err = hipMalloc(&Arl, (size_t)n * 4);
+        if (err == hipSuccess) err = hipMalloc(&Ax, (size_t)(ns * g.ax_ld) * vs);
+        if (err == hipSuccess) err = hipMalloc(&vec, 4 * vbytes);
+        if (err == hipSuccess) err = hipMalloc(&scd, (size_t)ns * sizeof(SlotScalars));
+        if (err == hipSuccess) err = hipMalloc(&lst, (size_t)(2 * ns + 2) * sizeof(int32_t));
+        if (err == hipSuccess) err = fill_bytes_zero(vec, 4 * vbytes, s_main_);
+        if (err == hipSuccess)
+            err = q1_assemble(vt, g.side, (int32_t *)Aj, Ax, (uint32_t *)Arl, g.stride, nullptr, vec, 0.8, s_main_, T);
+        for (int64_t k = 1; k < ns && err == hipSuccess; ++k)
+            err = hipMemcpyAsync((char *)Ax + (size_t)(k * g.ax_ld) * vs, Ax, (size_t)g.ax_ld * vs,
+                                 hipMemcpyDeviceToDevice, s_main_);
+        if (err == hipSuccess) err = b_fill_state((SlotScalars *)scd, ns, SLOT_ACTIVE, s_main_);
+        if (err == hipSuccess) {
+            const int64_t bnd[2] = {0, ns};
+            err = b_compact((const SlotScalars *)scd, ns, bnd, 2, (int32_t *)lst, (int32_t *)lst + ns, s_main_);
+        }
+        const BWork w{(const int32_t *)lst, (const int32_t *)lst + ns, 0, 1, 0};
+        for (int block : {64, 128, 256, 512, 1024}) {
+            if (err != hipSuccess) break;
+            if (block < T) continue;
+            BMat m{(const int32_t *)Aj, (const uint32_t *)Arl, Ax, g.stride, g.ax_ld, n, g.vld,
+                   (n * T + block - 1) / block};
+            float *dinv = (float *)vec, *x = dinv + ns * g.vld, *rhs = x + ns * g.vld, *out = rhs + ns * g.vld;
+            const int reps = n >= 65536 ? 10 : 30;
+            for (int r = 0; r < 3 && err == hipSuccess; ++r)
+                err = b_ell(vt, T, block, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
+            if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
+            for (int r = 0; r < reps && err == hipSuccess; ++r)
+                err = b_ell(vt, T, block, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
+            if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            const double us = 1000.0 * ms / reps;
+            if (err == hipSuccess && us < best.us) {
+                best.us = us;
+                best.T = T;
+                best.block = block;
+            }
+        }
+        (void)hipStreamSynchronize(s_main_);
+        for (void *p : {Aj, Arl, Ax, vec, scd, lst})
+            if (p) (void)hipFree(p);
+        if (err != hipSuccess) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HX(err);
+    lv.T = best.T;
+    lv.block = best.block;
+    best.slots = ns;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[key] = best;
+    return hipSuccess;
+}
+
+bool batch_tuned_shape(int dev, int64_t level, int vt, LaunchShape *out) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(dev, level, vt));
+    if (it == g_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+uint64_t BatchMG::plan_bytes() const {
+    uint64_t b = 0;
+    const size_t vs = vt_size((ValueType)cfg_.inner_vtype);
+    for (int64_t l = 1; l <= max_level_; ++l) {
+        const BLevel &lv = L_[(size_t)l];
+        b += (uint64_t)lv.ax_ld * 4 + (uint64_t)lv.n * 4;                          // Aj, Arl
+        b += (uint64_t)(lv.cap * lv.ax_ld) * vs;                                    // Ax
+        b += (uint64_t)(lv.cap * lv.vld) * 4 * (l == 1 ? 8 : 5);                    // vectors
+    }
+    for (const BClass &c : C_) {
+        b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4;
+        b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * 3;
+        b += (uint64_t)(c.count * c.bps) * 8;
+    }
+    int64_t cells = 0;
+    for (int64_t l = 1; l <= max_level_; ++l) cells += ((1ll << l)) * ((1ll << l));
+    b += (uint64_t)cells * 8 + (uint64_t)nslots_ * sizeof(SlotScalars);
+    return b;
+}
+
+hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int nclass, const BatchConfig &cfg,
+                           uint64_t budget_bytes) {
+    release();
+    over_budget_ = false;
+    cfg_ = cfg;
+    if (nclass < 1 || nclass > kMaxClasses) return hipErrorInvalidValue;
+    if (cfg.inner_vtype != VT_F32 && cfg.inner_vtype != VT_BF16) return hipErrorInvalidValue;
+    if (cfg.n_pre < 2 || cfg.n_pre > 16 || cfg.n_post < 0 || cfg.n_post > 16) return hipErrorInvalidValue;
+    if (cfg.iters_per_launch < 1 || cfg.iters_per_launch > 64 || cfg.max_outer < 1) return hipErrorInvalidValue;
+    if (cfg.averaging < 0 || cfg.averaging > 2) return hipErrorInvalidValue;
+    std::vector<int> order((size_t)nclass);
+    for (int c = 0; c < nclass; ++c) {
+        if (levels[c] < 2 || levels[c] > 13 || counts[c] < 1 || counts[c] > 65535) return hipErrorInvalidValue;
+        order[(size_t)c] = c;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return levels[a] > levels[b]; });
+    class_of_.assign((size_t)nclass, 0);
+    C_.assign((size_t)nclass, BClass());
+    int64_t first = 0;
+    for (int k = 0; k < nclass; ++k) {
+        const int c = order[(size_t)k];
+        class_of_[(size_t)c] = k;
+        BClass &bc = C_[(size_t)k];
+        const Level0 g = level_geometry(levels[c], 1, 256);
+        bc.level = levels[c];
+        bc.side = g.side;
+        bc.n = g.n;
+        bc.first = first;
+        bc.count = counts[c];
+        bc.vld = g.vld;
+        bc.stride = g.stride;
+        bc.ax_ld = g.ax_ld;
+        bc.bps = g.bps;
+        first += counts[c];
+    }
+    nslots_ = first;
+    if (nslots_ > 65535) return hipErrorInvalidValue;
+    max_level_ = C_[0].level;
+
+    HX(hipStreamCreateWithFlags(&s_main_, hipStreamNonBlocking));
+    HX(hipStreamCreateWithFlags(&s_load_, hipStreamNonBlocking));
+
+    L_.assign((size_t)max_level_ + 1, BLevel());
+    for (int64_t l = 1; l <= max_level_; ++l) {
+        BLevel &lv = L_[(size_t)l];
+        lv.level = l;
+        lv.side = (1ll << l) + 1;
+        lv.n = lv.side * lv.side;
+        for (const BClass &c : C_)
+            if (c.level >= l) lv.cap += c.count;
+        lv.T = 1;
+        lv.block = 256;
+        if (cfg_.autotune && l > 1) HX(tune_level(lv));
+        const Level0 g = level_geometry(l, lv.T, lv.block);
+        lv.stride = g.stride;
+        lv.ncpr = g.ncpr;
+        lv.ax_ld = g.ax_ld;
+        lv.vld = g.vld;
+    }
+    if (budget_bytes && plan_bytes() > budget_bytes) {
+        over_budget_ = true;
+        return hipErrorOutOfMemory;
+    }
+
+    const ValueType vt = (ValueType)cfg_.inner_vtype;
+    const size_t vs = vt_size(vt);
+    for (int64_t l = 1; l <= max_level_; ++l) {
+        BLevel &lv = L_[(size_t)l];
+        HX(alloc((void **)&lv.Aj, (size_t)lv.ax_ld * 4));
+        HX(alloc((void **)&lv.Arl, (size_t)lv.n * 4));
+        HX(alloc(&lv.Ax, (size_t)(lv.cap * lv.ax_ld) * vs));
+        const size_t vb = (size_t)(lv.cap * lv.vld) * 4;
+        for (float **p : {&lv.dinv, &lv.rhs, &lv.x, &lv.d, &lv.t}) HX(alloc((void **)p, vb));
+        if (l == 1)
+            for (float **p : {&lv.g, &lv.u, &lv.en}) HX(alloc((void **)p, vb));
+        // shared index arrays (values of slot 0 are placeholders until a load)
+        HX(q1_assemble(vt, lv.side, lv.Aj, lv.Ax, lv.Arl, lv.stride, nullptr, lv.dinv, cfg_.omega, s_main_, lv.T));
+    }
+    for (BClass &c : C_) {
+        HX(alloc((void **)&c.Aj, (size_t)c.ax_ld * 4));
+        HX(alloc((void **)&c.Arl, (size_t)c.n * 4));
+        HX(alloc((void **)&c.Ax, (size_t)(c.count * c.ax_ld) * 8));
+        for (double **p : {&c.b, &c.x, &c.d}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
+        HX(alloc((void **)&c.partials, (size_t)(c.count * c.bps) * 8));
+        HX(q1_assemble(VT_F64, c.side, c.Aj, c.Ax, c.Arl, c.stride, nullptr, nullptr, 0.0, s_main_, 1));
+    }
+    kappa_off_.assign((size_t)max_level_ + 2, 0);
+    for (int64_t l = 1; l <= max_level_; ++l) kappa_off_[(size_t)l + 1] = kappa_off_[(size_t)l] + (1ll << (2 * l));
+    HX(alloc((void **)&kappa_, (size_t)kappa_off_[(size_t)max_level_ + 1] * 8));
+    HX(alloc((void **)&sc_, (size_t)nslots_ * sizeof(SlotScalars)));
+    HX(alloc((void **)&list_, (size_t)nslots_ * sizeof(int32_t)));
+    HX(alloc((void **)&bounds_, (size_t)(kMaxClasses + 1) * sizeof(int32_t)));
+    boundaries_.clear();
+    for (const BClass &c : C_) boundaries_.push_back(c.first);
+    boundaries_.push_back(nslots_);
+    HX(b_fill_state(sc_, nslots_, SLOT_IDLE, s_main_));
+    HX(hipHostMalloc((void **)&sc_host_, (size_t)nslots_ * sizeof(SlotScalars), hipHostMallocDefault));
+    host_state_.assign((size_t)nslots_, 0);
+    load_ev_.assign((size_t)nslots_, nullptr);
+    for (auto &e : load_ev_) HX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HX(hipStreamSynchronize(s_main_));
+    return capture();
+}
+
+BWork BatchMG::level_work(int64_t level) const {
+    int32_t hi = 0;
+    while (hi < (int32_t)C_.size() && C_[(size_t)hi].level >= level) ++hi;
+    return BWork{list_, bounds_, 0, hi, 0};
+}
+
+BWork BatchMG::class_work(size_t k) const {
+    return BWork{list_, bounds_, (int32_t)k, (int32_t)k + 1, C_[k].first};
+}
+
+BWork BatchMG::all_work() const { return BWork{list_, bounds_, 0, (int32_t)C_.size(), 0}; }
+
+hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
+    const ValueType vt = (ValueType)cfg_.inner_vtype;
+    auto mat = [&](const BLevel &lv) {
+        return BMat{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, (lv.n * lv.T + lv.block - 1) / lv.block};
+    };
+    for (int64_t l = max_level_; l >= 2; --l) {
+        BLevel &lv = L_[(size_t)l];
+        const BMat m = mat(lv);
+        const BWork w = level_work(l);
+        // x = n_pre Jacobi sweeps from zero (first two fused), ending in x
+        const int rest = cfg_.n_pre - 2;
+        float *src = (rest % 2 == 0) ? lv.x : lv.t;
+        HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI0, m, w, lv.cap, src, nullptr, lv.rhs, lv.dinv, s));
+        for (int k = 0; k < rest; ++k) {
+            float *dst = (src == lv.x) ? lv.t : lv.x;
+            HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
+            src = dst;
+        }
+        HX(b_ell(vt, lv.T, lv.block, ELL_DEFECT, m, w, lv.cap, lv.d, lv.x, lv.rhs, nullptr, s));
+        HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
+    }
+    HX(b_coarse_cg(vt, L_[1], level_work(1), s));
+    for (int64_t l = 2; l <= max_level_; ++l) {
+        BLevel &lv = L_[(size_t)l];
+        const BMat m = mat(lv);
+        const BWork w = level_work(l);
+        HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, s));
+        float *src = lv.x, *dst = lv.t;
+        for (int k = 0; k < cfg_.n_post; ++k) {
+            HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
+            std::swap(src, dst);
+        }
+        if (src != lv.x) HX(b_copy(lv, w, lv.x, src, s));
+    }
+    return hipSuccess;
+}
+
+hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
+    const int nb = (int)boundaries_.size();
+    // slots running or admitted since the last launch
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, s));
+    BCheckTab tab{};
+    tab.nclass = (int32_t)C_.size();
+    for (size_t k = 0; k < C_.size(); ++k) {
+        const BClass &c = C_[k];
+        const BMat m{c.Aj, c.Arl, c.Ax, c.stride, c.ax_ld, c.n, c.vld, c.bps};
+        HX(b_outer_defect(m, class_work(k), c.count, c.d, c.x, c.b, c.partials, s));
+        tab.first[k] = c.first;
+        tab.bps[k] = c.bps;
+        tab.partials[k] = c.partials;
+    }
+    HX(b_check(tab, all_work(), nslots_, sc_, cfg_.tolerance, cfg_.max_outer, s));
+    // slots still running after the check
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, s));
+    for (size_t k = 0; k < C_.size(); ++k) {
+        const BLevel &top = L_[(size_t)C_[k].level];
+        HX(b_scale_to_f32(C_[k], class_work(k), top.rhs, top.vld, sc_, s));
+    }
+    HX(enqueue_vcycle(s));
+    for (size_t k = 0; k < C_.size(); ++k) {
+        const BLevel &top = L_[(size_t)C_[k].level];
+        HX(b_acc(C_[k], class_work(k), top.x, top.vld, sc_, s));
+    }
+    return hipSuccess;
+}
+
+hipError_t BatchMG::capture() {
+    HX(hipStreamBeginCapture(s_main_, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int i = 0; i < cfg_.iters_per_launch && err == hipSuccess; ++i) err = enqueue_iteration(s_main_);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(s_main_, &g);
+    if (err != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return err;
+    }
+    HX(e2);
+    graph_ = g;
+    return hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+}
+
+hipError_t BatchMG::load(int64_t g, const double *coeff, const double *b) {
+    if (!coeff || !b) return hipErrorInvalidValue;
+    return load_impl(g, coeff, b, 0, 0.0, 0);
+}
+
+hipError_t BatchMG::load_seeded(int64_t g, uint64_t coeff_seed, double sigma, uint64_t rhs_seed) {
+    return load_impl(g, nullptr, nullptr, coeff_seed, sigma, rhs_seed);
+}
+
+// coeff/b null: kappa = exp(sigma N(0,1)) from coeff_seed, b uniform in
+// [-1, 1) from rhs_seed, drawn on the device into the slot directly.
+hipError_t BatchMG::load_impl(int64_t g, const double *coeff, const double *b, uint64_t coeff_seed, double sigma,
+                              uint64_t rhs_seed) {
+    if (g < 0 || g >= nslots_) return hipErrorInvalidValue;
+    if (host_state_[(size_t)g] != 0) return hipErrorInvalidValue;
+    const BClass *cp = nullptr;
+    for (const BClass &c : C_)
+        if (g >= c.first && g < c.first + c.count) cp = &c;
+    const BClass &c = *cp;
+    const int64_t ls = g - c.first, Lc = c.level;
+    const ValueType vt = (ValueType)cfg_.inner_vtype;
+    const size_t vs = vt_size(vt);
+    // The load runs beside the graph launches: the slot is idle (its last
+    // graph work finished before the poll that retired it, and result()
+    // copies synchronously), and idle slots are touched by no kernel.
+    hipStream_t s = s_load_;
+    const int64_t cells_top = (1ll << Lc) * (1ll << Lc);
+    double *kap = kappa_ + kappa_off_[(size_t)Lc];
+    double *bx = c.b + ls * c.vld, *xx = c.x + ls * c.vld;
+    if (coeff) {
+        HX(hipMemcpyAsync(kap, coeff, (size_t)cells_top * 8, hipMemcpyDeviceToDevice, s));
+        HX(hipMemcpyAsync(bx, b, (size_t)c.n * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        HX(fill_lognormal(kap, cells_top, sigma, coeff_seed, s));
+        HX(fill_uniform(bx, c.n, -1.0, 1.0, rhs_seed, s));
+    }
+    for (int64_t l = Lc - 1; l >= 1; --l)
+        HX(coarsen_coeff(kappa_ + kappa_off_[(size_t)l + 1], 1ll << (l + 1), kappa_ + kappa_off_[(size_t)l], 1ll << l,
+                         cfg_.averaging, s));
+    for (int64_t l = Lc; l >= 1; --l) {
+        BLevel &lv = L_[(size_t)l];
+        HX(q1_assemble(vt, lv.side, nullptr, (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs, nullptr, lv.stride,
+                       kappa_ + kappa_off_[(size_t)l], lv.dinv + g * lv.vld, cfg_.omega, s, lv.T));
+    }
+    HX(q1_assemble(VT_F64, c.side, nullptr, c.Ax + ls * c.ax_ld, nullptr, c.stride, kap, nullptr, 0.0, s, 1));
+    HX(fill_bytes_zero(xx, (size_t)c.n * 8, s));
+    HX(boundary_guess(xx, bx, c.n, c.side, s));
+    HX(hipEventRecord(load_ev_[(size_t)g], s));
+    host_state_[(size_t)g] = 1;
+    loading_.push_back(g);
+    return hipSuccess;
+}
+
+hipError_t BatchMG::activate_ready(bool wait) {
+    std::vector<int64_t> still;
+    for (int64_t g : loading_) {
+        hipError_t q = wait ? hipEventSynchronize(load_ev_[(size_t)g]) : hipEventQuery(load_ev_[(size_t)g]);
+        if (q == hipErrorNotReady) {
+            still.push_back(g);
+            continue;
+        }
+        HX(q);
+        HX(b_set_state(sc_, g, SLOT_NEW, s_main_));
+        host_state_[(size_t)g] = 2;
+        ++running_;
+    }
+    loading_.swap(still);
+    return hipSuccess;
+}
+
+hipError_t BatchMG::step() {
+    HX(activate_ready(false));
+    HX(hipGraphLaunch(exec_, s_main_));
+    ++launches_;
+    return hipSuccess;
+}
+
+hipError_t BatchMG::poll(std::vector<SlotScalars> &out, std::vector<int64_t> *retired) {
+    HX(hipMemcpyAsync(sc_host_, sc_, (size_t)nslots_ * sizeof(SlotScalars), hipMemcpyDeviceToHost, s_main_));
+    HX(hipStreamSynchronize(s_main_));
+    out.assign(sc_host_, sc_host_ + nslots_);
+    if (retired) retired->clear();
+    for (int64_t g = 0; g < nslots_; ++g) {
+        if (host_state_[(size_t)g] == 2 && sc_host_[g].state == SLOT_IDLE) {
+            host_state_[(size_t)g] = 0;
+            --running_;
+            if (retired) retired->push_back(g);
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t BatchMG::result(int64_t g, double *dst) {
+    if (g < 0 || g >= nslots_ || !dst) return hipErrorInvalidValue;
+    for (const BClass &c : C_)
+        if (g >= c.first && g < c.first + c.count) {
+            HX(hipMemcpyAsync(dst, c.x + (g - c.first) * c.vld, (size_t)c.n * 8, hipMemcpyDeviceToDevice, s_main_));
+            return hipStreamSynchronize(s_main_);
+        }
+    return hipErrorInvalidValue;
+}
+
+hipError_t BatchMG::run_to_completion(int64_t *launches) {
+    HX(activate_ready(true));
+    int64_t k = 0;
+    std::vector<SlotScalars> st;
+    while (running_ > 0) {
+        HX(step());
+        ++k;
+        HX(poll(st));
+    }
+    if (launches) *launches = k;
+    return hipSuccess;
+}
+
+}  // namespace hipx
+}  // namespace honei

@@ -1,0 +1,243 @@
+// batch.hh -- many independent Q1 systems of uneven sizes solved together by
+// mixed-precision multigrid defect correction (FP32 or BF16-stored V-cycle
+// inside, FP64 defect / update / norm outside), one hipGraph launch per
+// group of outer iterations.
+//
+// Slots.  The pool is a list of classes; class c holds count_c slots whose
+// finest grid is level L_c (side 2^L_c + 1).  Classes are kept in descending
+// level order, so global slot g = first_c + j is present on every level
+// l <= L_c, and the slots present on level l are exactly the prefix
+// [0, cap_l) with cap_l = sum of count_c over classes with L_c >= l.  Each
+// level therefore stores its batch as [cap_l][...] arrays indexed by the
+// global slot; index arrays (Aj, Arl) are shared by all slots of a level, and
+// values (Ax, dinv) and vectors are per slot.  A V-cycle of the batch runs the
+// same launches as a single-system V-cycle, each over the whole prefix: a
+// slot whose finest level is l enters the cycle at l with its scaled outer
+// defect as right-hand side; deeper slots enter with the restricted defect.
+//
+// Per-slot state lives on the device (SlotScalars): every kernel returns at
+// once for idle slots, and a check kernel at the start of each outer
+// iteration forms each slot's FP64 defect norm, decides convergence and
+// retires the slot.  Work admitted while others run is assembled on a
+// separate stream into an idle slot and activated between graph launches.
+//
+// Numerics per slot follow MGHierarchy::solve_mixed with n_max_iter = 1,
+// initial_zero, zero_inner_start and no inner convergence check (the lean
+// inner cycle), so with threads-per-row 1 a slot's result is bit-identical
+// to that single-system solve of the same system.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "solvers.hh"
+
+namespace honei {
+namespace hipx {
+
+enum SlotState : int32_t { SLOT_IDLE = 0, SLOT_ACTIVE = 1, SLOT_NEW = 2 };
+enum SlotStatus : int32_t { SLOT_OK = 0, SLOT_PENDING = -1, SLOT_NOT_CONVERGED = 5 };
+
+struct SlotScalars {
+    double initial;   // FP64 defect norm at the start vector
+    double norm;      // latest FP64 defect norm
+    double scale;     // inner right-hand side scaling of the current cycle
+    double inv;       // 1 / scale
+    double rel;       // norm / initial
+    int32_t state;    // SlotState
+    int32_t iters;    // outer iterations (V-cycles) applied
+    int32_t status;   // SlotStatus
+    int32_t pad;
+};
+
+constexpr int kMaxClasses = 16;
+
+struct BatchConfig {
+    int32_t inner_vtype = VT_F32;    // VT_F32 or VT_BF16 matrix values (vectors FP32)
+    int32_t n_pre = 2, n_post = 2;   // Jacobi sweeps (n_pre >= 2)
+    double omega = 0.8;              // Jacobi damping
+    double tolerance = 1e-8;         // relative FP64 defect
+    int32_t max_outer = 200;
+    int32_t averaging = 1;           // coefficient coarsening: 0 arith, 1 geometric, 2 harmonic
+    int32_t iters_per_launch = 4;    // outer iterations per graph launch
+    int32_t autotune = 1;            // 0: threads-per-row 1, block 256 everywhere
+};
+
+// Inner level l of the pool: slots [0, cap) present.
+struct BLevel {
+    int64_t level = 0, side = 0, n = 0, cap = 0, vld = 0;
+    int T = 1, block = 256;
+    int64_t stride = 0, ncpr = 0, ax_ld = 0;
+    int32_t *Aj = nullptr;
+    uint32_t *Arl = nullptr;
+    void *Ax = nullptr;
+    float *dinv = nullptr, *rhs = nullptr, *x = nullptr, *d = nullptr, *t = nullptr;
+    float *g = nullptr, *u = nullptr, *en = nullptr;   // coarse CG scratch (level 1 only)
+};
+
+// Class c: slots [first, first + count), finest level `level`, FP64 outer data.
+struct BClass {
+    int64_t level = 0, side = 0, n = 0, first = 0, count = 0, vld = 0;
+    int64_t stride = 0, ax_ld = 0, bps = 0;
+    int32_t *Aj = nullptr;
+    uint32_t *Arl = nullptr;
+    double *Ax = nullptr, *b = nullptr, *x = nullptr, *d = nullptr, *partials = nullptr;
+};
+
+// Threads-per-row / block-size choice for one level (autotuned, cached per
+// level and value type for the process).
+struct LaunchShape {
+    int T = 1, block = 256;
+    double us = 0.0;       // measured time of one batched Jacobi sweep
+    int64_t slots = 0;     // slots of the tuning batch
+};
+
+// Batched kernels (batch_kernels.hip).
+//
+// A launch covers the active slots list[a], a in [bounds[lo], bounds[hi]):
+// `list` holds the non-idle slots in ascending order and bounds[k] counts the
+// active slots below class boundary k (boundary k = first slot of class k,
+// boundary nclass = nslots), both written on the device by b_compact.  Every
+// batched kernel runs on a fixed grid and walks the active slots' row blocks
+// in contiguous chunks, so idle slots cost nothing and the launch
+// configuration never changes (graph-capturable).
+struct BWork {
+    const int32_t *list;
+    const int32_t *bounds;
+    int32_t lo, hi;
+    int64_t base;   // slot of local index 0 in the launch's arrays
+};
+
+struct BBoundaries {
+    int32_t n;
+    int64_t v[kMaxClasses + 1];
+};
+
+struct BMat {
+    const int32_t *Aj;
+    const uint32_t *Arl;
+    const void *Ax;
+    int64_t stride, ax_ld, n, vld, bps;
+};
+
+class BatchMG {
+  public:
+    BatchMG() = default;
+    ~BatchMG();
+    BatchMG(const BatchMG &) = delete;
+    BatchMG &operator=(const BatchMG &) = delete;
+
+    // levels[c] (2..13) with counts[c] >= 1; any order (sorted internally,
+    // class_of maps the caller's class index to the internal one).
+    hipError_t create(const int64_t *levels, const int64_t *counts, int nclass, const BatchConfig &cfg,
+                      uint64_t budget_bytes);
+    uint64_t bytes() const { return bytes_; }
+    bool over_budget() const { return over_budget_; }
+    // Device bytes the pool needs (after launch shapes are chosen).
+    uint64_t plan_bytes() const;
+    int64_t nslots() const { return nslots_; }
+    int64_t slot(int cls, int64_t j) const;   // -1 if out of range
+    int64_t slot_level(int64_t g) const;
+
+    // Assemble the hierarchy of slot g from a per-cell coefficient
+    // ((side-1)^2 doubles, device) and copy the right-hand side (side^2
+    // doubles, device); start vector 0 with boundary rows set to b.  Runs on
+    // the load stream; the slot joins the next launch after the work is done.
+    hipError_t load(int64_t g, const double *coeff_dev, const double *b_dev);
+    // Same with the coefficient exp(sigma N(0,1)) and b uniform in [-1, 1)
+    // drawn on the device from the seeds (synthetic requests).
+    hipError_t load_seeded(int64_t g, uint64_t coeff_seed, double sigma, uint64_t rhs_seed);
+    // Activate finished loads, then one graph launch (iters_per_launch outer
+    // iterations of every active slot).  Non-blocking.
+    hipError_t step();
+    // Wait for the launched work and copy every slot's scalars to host;
+    // `retired` (optional) receives the slots that finished since the last poll.
+    hipError_t poll(std::vector<SlotScalars> &out, std::vector<int64_t> *retired = nullptr);
+    // x of slot g (FP64, side^2) into dst (device); synchronous.
+    hipError_t result(int64_t g, double *dst);
+    // Queue finished loads for the next launch (wait: block until every load
+    // issued so far is done).
+    hipError_t activate_ready(bool wait);
+    // Load-free driver: activate everything loaded and step until no slot is
+    // active.  launches: graph launches issued.
+    hipError_t run_to_completion(int64_t *launches);
+    // Slot is loading or running (host view, updated by poll).
+    bool busy(int64_t g) const { return host_state_[(size_t)g] != 0; }
+    bool loading(int64_t g) const { return host_state_[(size_t)g] == 1; }
+    int64_t running() const { return running_; }
+    int64_t loading_count() const { return (int64_t)loading_.size(); }
+    int64_t launches() const { return launches_; }
+    const std::vector<BLevel> &levels() const { return L_; }
+    const BatchConfig &config() const { return cfg_; }
+
+  private:
+    BatchConfig cfg_;
+    std::vector<BLevel> L_;          // indexed by level (0 unused)
+    std::vector<BClass> C_;          // descending level
+    std::vector<int> class_of_;      // caller class -> internal class
+    int64_t nslots_ = 0, max_level_ = 0;
+    uint64_t bytes_ = 0;
+    std::vector<void *> allocs_;
+    SlotScalars *sc_ = nullptr;
+    SlotScalars *sc_host_ = nullptr;   // pinned
+    int32_t *list_ = nullptr;          // active slots (device, b_compact)
+    int32_t *bounds_ = nullptr;        // active counts below each class boundary
+    std::vector<int64_t> boundaries_;  // class firsts, then nslots
+    double *kappa_ = nullptr;          // coefficient scratch of one load, all levels
+    std::vector<int64_t> kappa_off_;
+    hipStream_t s_main_ = nullptr, s_load_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    std::vector<int> host_state_;      // 0 free, 1 loading, 2 running
+    std::vector<int64_t> loading_;
+    int64_t running_ = 0, launches_ = 0;
+    bool over_budget_ = false;
+    std::vector<hipEvent_t> load_ev_;
+
+    hipError_t alloc(void **p, size_t bytes);
+    hipError_t load_impl(int64_t g, const double *coeff, const double *b, uint64_t coeff_seed, double sigma,
+                         uint64_t rhs_seed);
+    hipError_t tune_level(BLevel &lv);
+    hipError_t capture();
+    hipError_t enqueue_iteration(hipStream_t s);
+    hipError_t enqueue_vcycle(hipStream_t s);
+    BWork level_work(int64_t level) const;
+    BWork class_work(size_t k) const;
+    BWork all_work() const;
+    void release();
+};
+
+// Shape chosen for (device, level, value type), if the autotuner ran.
+bool batch_tuned_shape(int dev, int64_t level, int vt, LaunchShape *out);
+
+// grid for up to `items` logical blocks
+int batch_grid(int64_t items);
+
+hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
+                     int32_t *bounds, hipStream_t s);
+hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
+                 void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
+hipError_t b_outer_defect(const BMat &m, const BWork &w, int64_t max_slots, double *d, const double *x,
+                          const double *b, double *partials, hipStream_t s);
+struct BCheckTab {
+    int32_t nclass;
+    int64_t first[kMaxClasses], bps[kMaxClasses];
+    const double *partials[kMaxClasses];
+};
+hipError_t b_check(const BCheckTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc, double tol,
+                   int32_t max_outer, hipStream_t s);
+hipError_t b_scale_to_f32(const BClass &c, const BWork &w, float *rhs_top, int64_t vld_top, const SlotScalars *sc,
+                          hipStream_t s);
+hipError_t b_acc(const BClass &c, const BWork &w, const float *x_top, int64_t vld_top, const SlotScalars *sc,
+                 hipStream_t s);
+hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
+hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
+hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);
+hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s);
+hipError_t b_set_state(SlotScalars *sc, int64_t g, int32_t state, hipStream_t s);
+hipError_t b_fill_state(SlotScalars *sc, int64_t nslots, int32_t state, hipStream_t s);
+
+}  // namespace hipx
+}  // namespace honei

@@ -1,0 +1,422 @@
+// batch_kernels.hip -- kernels of the batched multigrid solver.
+//
+// Every batched launch walks the active slots list[a], a in
+// [bounds[lo], bounds[hi]) (written by k_b_compact), on a fixed grid: the
+// launch's logical blocks (row blocks of the active slots, slot-major) are
+// split into contiguous chunks, one per work-group, with the XCD-aware
+// work-group order, so the row blocks of one slot stay on one XCD's L2 and
+// idle slots cost nothing.
+//
+// Row sums, transfers and the coarse CG use the same device code as the
+// single-system kernels (ell_device.hh, transfer_device.hh,
+// solver_device.hh), so a slot computes what the single-system solver
+// computes.  The FP64 outer defect+norm also reproduces the single-system
+// reduction order: 256-row blocks, partial b of a slot holding logical row
+// block xcd_block(b, blocks per slot), partials summed by one 1024-thread
+// block.
+
+#include "batch.hh"
+#include "ell_device.hh"
+#include "solver_device.hh"
+#include "transfer_device.hh"
+
+#pragma clang fp contract(off)
+
+namespace honei {
+namespace hipx {
+namespace {
+
+constexpr int kMaxGrid = 8192;
+
+// This work-group's contiguous share [first, last) of `total` logical blocks.
+__device__ inline void my_chunk(int64_t total, int64_t &first, int64_t &last) {
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t per = total / gridDim.x, rem = total % gridDim.x;
+    first = p * per + (p < rem ? p : rem);
+    last = first + per + (p < rem ? 1 : 0);
+}
+
+__device__ inline int64_t active_first(const BWork &w) { return w.bounds[w.lo]; }
+__device__ inline int64_t active_count(const BWork &w) { return w.bounds[w.hi] - w.bounds[w.lo]; }
+
+// list of non-idle slots (ascending) and, for each boundary k, the number of
+// them below boundaries[k]; one 1024-thread block.
+__global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64_t nslots, BBoundaries bd,
+                                                     int32_t *list, int32_t *bounds) {
+    __shared__ int32_t pre[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (nslots + 1023) / 1024;
+    const int64_t s0 = t * chunk, s1 = (s0 + chunk < nslots) ? s0 + chunk : nslots;
+    int32_t cnt = 0;
+    for (int64_t g = s0; g < s1; ++g) cnt += sc[g].state != SLOT_IDLE;
+    pre[t] = cnt;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele) over 1024 counts
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = t >= off ? pre[t - off] : 0;
+        __syncthreads();
+        pre[t] += v;
+        __syncthreads();
+    }
+    int32_t pos = pre[t] - cnt;
+    for (int64_t g = s0; g < s1; ++g)
+        if (sc[g].state != SLOT_IDLE) list[pos++] = (int32_t)g;
+    if (t < bd.n) {
+        const int64_t b = bd.v[t];
+        const int64_t owner = chunk > 0 ? b / chunk : 0;
+        int32_t c = (owner > 0 && owner <= 1023) ? pre[owner - 1] : (owner > 1023 ? pre[1023] : 0);
+        if (owner <= 1023)
+            for (int64_t g = owner * chunk; g < b && g < nslots; ++g) c += sc[g].state != SLOT_IDLE;
+        bounds[t] = c;
+    }
+}
+
+// One logical block = blockDim/T rows of one active slot.
+template <typename VT, int T, int MODE>
+__global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, const void *x, const void *rhs,
+                                                const void *dinv) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t a0 = active_first(w);
+    int64_t first, last;
+    my_chunk(active_count(w) * m.bps, first, last);
+    for (int64_t lb = first; lb < last; ++lb) {
+        const int64_t ls = w.list[a0 + lb / m.bps] - w.base, rb = lb % m.bps;
+        const VT *__restrict__ Ax = (const VT *)m.Ax + ls * m.ax_ld;
+        const int64_t vo = ls * m.vld;
+        const V *__restrict__ xs = (const V *)x + vo;
+        const V *__restrict__ rs = (const V *)rhs + vo;
+        const V *__restrict__ ds = (const V *)dinv + vo;
+        V *__restrict__ os = (V *)out + vo;
+        const int64_t gid = rb * (int64_t)blockDim.x + threadIdx.x;
+        const int lane = (int)(gid & (T - 1));
+        const int64_t row = gid / T;
+        const bool active = row < m.n;
+        V s = V(0);
+        if (active) {
+            if constexpr (MODE == ELL_JACOBI0)
+                s = ell_lane_sum<VT, T, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, rs, ds);
+            else
+                s = ell_lane_sum<VT, T>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, xs);
+        }
+        s = lane_group_sum<T>(s);
+        if (active && lane == 0) {
+            if constexpr (MODE == ELL_DEFECT) {
+                os[row] = rs[row] - s;
+            } else {
+                const V xin = (MODE == ELL_JACOBI0) ? ds[row] * rs[row] : xs[row];
+                const V t = rs[row] - s;
+                os[row] = xin + t * ds[row];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_outer_defect(BMat m, BWork w, double *d, const double *x,
+                                                        const double *b, double *partials) {
+    const int64_t a0 = active_first(w);
+    int64_t first, last;
+    my_chunk(active_count(w) * m.bps, first, last);
+    for (int64_t lb = first; lb < last; ++lb) {
+        const int64_t ls = w.list[a0 + lb / m.bps] - w.base, hb = lb % m.bps;
+        const int64_t rb = xcd_block(hb, m.bps);
+        const double *__restrict__ Ax = (const double *)m.Ax + ls * m.ax_ld;
+        const int64_t vo = ls * m.vld;
+        const int64_t row = rb * 256 + threadIdx.x;
+        double nval = 0.0;
+        if (row < m.n) {
+            const double s = ell_lane_sum<double, 1>(m.Aj, Ax, row, 0, m.Arl[row], m.stride, x + vo);
+            const double r = b[vo + row] - s;
+            d[vo + row] = r;
+            nval = r * r;
+        }
+        const double v = block_sum(nval);
+        if (threadIdx.x == 0) partials[ls * m.bps + hb] = v;
+    }
+}
+
+// One 1024-thread block per active slot (grid-stride): norm from the
+// partials, then the outer iteration's decision (MGHierarchy::solve_mixed:
+// initial norm, scale 1 for the first cycle, stop when norm < tol *
+// initial, give up at max_outer or on a non-finite norm, else scale = norm).
+__global__ __launch_bounds__(1024) void k_b_check(BCheckTab tab, BWork w, SlotScalars *sc, double tol,
+                                                  int32_t max_outer) {
+    const int64_t a0 = active_first(w), na = active_count(w);
+    for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
+        const int64_t g = w.list[a0 + a];
+        int c = 0;
+        while (c + 1 < tab.nclass && g >= tab.first[c + 1]) ++c;
+        const int64_t ls = g - tab.first[c];
+        const double h = sum_partials(tab.partials[c] + ls * tab.bps[c], tab.bps[c], 1, 0);
+        if (threadIdx.x == 0) {
+            SlotScalars s = sc[g];
+            const double nrm = sqrt(h);
+            s.norm = nrm;
+            if (s.state == SLOT_NEW) {
+                s.initial = nrm;
+                s.iters = 0;
+                s.rel = 1.0;
+                if (nrm == 0.0) {             // b - A x0 = 0: x0 is the solution
+                    s.rel = 0.0;
+                    s.state = SLOT_IDLE;
+                    s.status = SLOT_OK;
+                } else if (!isfinite(nrm)) {
+                    s.state = SLOT_IDLE;
+                    s.status = SLOT_NOT_CONVERGED;
+                } else {
+                    s.scale = 1.0;
+                    s.inv = 1.0;
+                    s.iters = 1;
+                    s.state = SLOT_ACTIVE;
+                }
+            } else {
+                s.rel = nrm / s.initial;
+                if (nrm < tol * s.initial) {
+                    s.state = SLOT_IDLE;
+                    s.status = SLOT_OK;
+                } else if (s.iters >= max_outer || !isfinite(nrm)) {
+                    s.state = SLOT_IDLE;
+                    s.status = SLOT_NOT_CONVERGED;
+                } else {
+                    s.scale = nrm;
+                    s.inv = 1.0 / nrm;
+                    s.iters += 1;
+                }
+            }
+            sc[g] = s;
+        }
+        __syncthreads();   // sum_partials reuses shared memory in the next round
+    }
+}
+
+// Elementwise launches: logical block = 256 consecutive entries of one slot.
+#define FOR_SLOT_ENTRIES(w, n, BODY)                                              \
+    do {                                                                          \
+        const int64_t a0_ = active_first(w);                                     \
+        const int64_t bpe_ = ((n) + 255) / 256;                                   \
+        int64_t first_, last_;                                                    \
+        my_chunk(active_count(w) * bpe_, first_, last_);                          \
+        for (int64_t lb_ = first_; lb_ < last_; ++lb_) {                          \
+            const int64_t g = w.list[a0_ + lb_ / bpe_];                           \
+            const int64_t i = (lb_ % bpe_) * 256 + threadIdx.x;                   \
+            if (i < (n)) {                                                        \
+                BODY;                                                             \
+            }                                                                     \
+        }                                                                         \
+    } while (0)
+
+// rhs_top[g] = (float)(d[ls] * inv[g])
+__global__ void k_b_scale(BWork w, const double *d, int64_t vld, int64_t n, float *rhs, int64_t vld_top,
+                          const SlotScalars *sc) {
+    FOR_SLOT_ENTRIES(w, n, {
+        const int64_t ls = g - w.base;
+        rhs[g * vld_top + i] = (float)(d[ls * vld + i] * sc[g].inv);
+    });
+}
+
+// x[ls] = (first cycle ? 0 : x[ls]) + (double)x_top[g] * scale[g]
+__global__ void k_b_acc(BWork w, double *x, int64_t vld, int64_t n, const float *xt, int64_t vld_top,
+                        const SlotScalars *sc) {
+    FOR_SLOT_ENTRIES(w, n, {
+        const int64_t ls = g - w.base;
+        const double base = sc[g].iters == 1 ? 0.0 : x[ls * vld + i];
+        x[ls * vld + i] = base + (double)xt[g * vld_top + i] * sc[g].scale;
+    });
+}
+
+// rhs_c[g] = R d_f[g] for active slots present on the fine level
+__global__ void k_b_restrict(BWork w, float *rhs_c, int64_t vld_c, const float *d_f, int64_t vld_f, int64_t m,
+                             Mask8 mk) {
+    const int64_t nc = (m + 1) * (m + 1);
+    FOR_SLOT_ENTRIES(w, nc, restrict_node<float>(rhs_c + g * vld_c, d_f + g * vld_f, m, mk, i));
+}
+
+// x_f[g] += P x_c[g]
+__global__ void k_b_prolong_axpy(BWork w, float *x_f, int64_t vld_f, const float *x_c, int64_t vld_c, int64_t m,
+                                 Mask8 mk) {
+    const int64_t nf = (2 * m + 1) * (2 * m + 1);
+    FOR_SLOT_ENTRIES(w, nf, (prolong_node<float, true>(nullptr, x_c + g * vld_c, m, mk, x_f + g * vld_f, 1.0f, i)));
+}
+
+__global__ void k_b_copy(BWork w, float *dst, const float *src, int64_t vld, int64_t n) {
+    FOR_SLOT_ENTRIES(w, n, dst[g * vld + i] = src[g * vld + i]);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t ax_ld, int64_t vld,
+                                                     const float *rhs, float *x, float *g, float *u, float *en,
+                                                     double konv, int max_iters) {
+    const int64_t a0 = active_first(w), na = active_count(w);
+    const void *Ax0 = A.Ax;
+    for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
+        const int64_t s = w.list[a0 + a];
+        A.Ax = (const VT *)Ax0 + s * ax_ld;
+        const int64_t o = s * vld;
+        (void)coarse_cg_block<VT>(A, rhs + o, x + o, g + o, u + o, en + o, konv, max_iters);
+        __syncthreads();
+    }
+}
+
+__global__ void k_b_set_state(SlotScalars *sc, int64_t g, int32_t state) {
+    if (threadIdx.x == 0) {
+        SlotScalars s = {};
+        s.state = state;
+        s.status = SLOT_PENDING;
+        s.scale = 1.0;
+        s.inv = 1.0;
+        sc[g] = s;
+    }
+}
+
+__global__ void k_b_fill_state(SlotScalars *sc, int64_t n, int32_t state) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g < n) {
+        SlotScalars s = {};
+        s.state = state;
+        s.status = SLOT_PENDING;
+        s.scale = 1.0;
+        s.inv = 1.0;
+        sc[g] = s;
+    }
+}
+
+typedef void (*BEllFn)(BMat, BWork, void *, const void *, const void *, const void *);
+
+template <typename VT, int T>
+BEllFn pick_mode(EllMode mode) {
+    switch (mode) {
+        case ELL_DEFECT: return k_b_ell<VT, T, ELL_DEFECT>;
+        case ELL_JACOBI: return k_b_ell<VT, T, ELL_JACOBI>;
+        case ELL_JACOBI0: return k_b_ell<VT, T, ELL_JACOBI0>;
+        default: return nullptr;
+    }
+}
+
+template <typename VT>
+BEllFn pick_threads(int T, EllMode mode) {
+    switch (T) {
+        case 1: return pick_mode<VT, 1>(mode);
+        case 2: return pick_mode<VT, 2>(mode);
+        case 4: return pick_mode<VT, 4>(mode);
+        case 8: return pick_mode<VT, 8>(mode);
+        default: return nullptr;
+    }
+}
+
+int entry_grid(int64_t max_slots, int64_t n) { return batch_grid(max_slots * ((n + 255) / 256)); }
+
+}  // namespace
+
+int batch_grid(int64_t items) { return (int)(items < 1 ? 1 : (items > kMaxGrid ? kMaxGrid : items)); }
+
+hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
+                     int32_t *bounds, hipStream_t s) {
+    if (nb < 1 || nb > kMaxClasses + 1 || nslots < 1) return hipErrorInvalidValue;
+    BBoundaries bd{};
+    bd.n = nb;
+    for (int k = 0; k < nb; ++k) bd.v[k] = boundaries[k];
+    hipLaunchKernelGGL(k_b_compact, dim3(1), dim3(1024), 0, s, sc, nslots, bd, list, bounds);
+    return hipGetLastError();
+}
+
+hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
+                 void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s) {
+    BEllFn fn = nullptr;
+    if (vt == VT_F32) fn = pick_threads<float>(T, mode);
+    else if (vt == VT_BF16) fn = pick_threads<bf16>(T, mode);
+    if (!fn || block % kWave || block > 1024 || block < T) return hipErrorInvalidValue;
+    if (m.bps != (m.n * T + block - 1) / block) return hipErrorInvalidValue;
+    if (max_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fn, dim3(batch_grid(max_slots * m.bps)), dim3(block), 0, s, m, w, out, x, rhs, dinv);
+    return hipGetLastError();
+}
+
+hipError_t b_outer_defect(const BMat &m, const BWork &w, int64_t max_slots, double *d, const double *x,
+                          const double *b, double *partials, hipStream_t s) {
+    if (m.bps != (m.n + 255) / 256) return hipErrorInvalidValue;
+    if (max_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_outer_defect, dim3(batch_grid(max_slots * m.bps)), dim3(256), 0, s, m, w, d, x, b,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t b_check(const BCheckTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc, double tol,
+                   int32_t max_outer, hipStream_t s) {
+    if (max_slots <= 0) return hipSuccess;
+    const int grid = (int)(max_slots < 1024 ? max_slots : 1024);
+    hipLaunchKernelGGL(k_b_check, dim3(grid), dim3(1024), 0, s, tab, w, sc, tol, max_outer);
+    return hipGetLastError();
+}
+
+hipError_t b_scale_to_f32(const BClass &c, const BWork &w, float *rhs_top, int64_t vld_top, const SlotScalars *sc,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_b_scale, dim3(entry_grid(c.count, c.n)), dim3(256), 0, s, w, (const double *)c.d, c.vld,
+                       c.n, rhs_top, vld_top, sc);
+    return hipGetLastError();
+}
+
+hipError_t b_acc(const BClass &c, const BWork &w, const float *x_top, int64_t vld_top, const SlotScalars *sc,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_b_acc, dim3(entry_grid(c.count, c.n)), dim3(256), 0, s, w, c.x, c.vld, c.n, x_top, vld_top,
+                       sc);
+    return hipGetLastError();
+}
+
+hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
+    const int64_t m = coarse.side - 1;
+    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_b_restrict, dim3(entry_grid(fine.cap, coarse.n)), dim3(256), 0, s, w, coarse.rhs,
+                       coarse.vld, (const float *)fine.d, fine.vld, m, pack_mask(nullptr));
+    return hipGetLastError();
+}
+
+hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
+    const int64_t m = coarse.side - 1;
+    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_b_prolong_axpy, dim3(entry_grid(fine.cap, fine.n)), dim3(256), 0, s, w, fine.x, fine.vld,
+                       (const float *)coarse.x, coarse.vld, m, pack_mask(nullptr));
+    return hipGetLastError();
+}
+
+hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_copy, dim3(entry_grid(lv.cap, lv.n)), dim3(256), 0, s, w, dst, src, lv.vld, lv.n);
+    return hipGetLastError();
+}
+
+hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s) {
+    EllView A{};
+    A.rows = lv.n;
+    A.cols = lv.n;
+    A.stride = lv.stride;
+    A.ncpr = lv.ncpr;
+    A.threads = lv.T;
+    A.Aj = lv.Aj;
+    A.Ax = lv.Ax;
+    A.Arl = lv.Arl;
+    A.vtype = vt;
+    const double konv = 1.1920928955078125e-07;   // FP32 epsilon, as the single-system coarse solve
+    const int max_iters = (int)(4 * lv.n + 64);
+    const int grid = (int)(lv.cap < 2048 ? lv.cap : 2048);
+    if (vt == VT_F32)
+        hipLaunchKernelGGL(k_b_coarse_cg<float>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
+                           (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
+    else if (vt == VT_BF16)
+        hipLaunchKernelGGL(k_b_coarse_cg<bf16>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
+                           (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t b_set_state(SlotScalars *sc, int64_t g, int32_t state, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_set_state, dim3(1), dim3(64), 0, s, sc, g, state);
+    return hipGetLastError();
+}
+
+hipError_t b_fill_state(SlotScalars *sc, int64_t nslots, int32_t state, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_fill_state, dim3(ceil_div(nslots, 256)), dim3(256), 0, s, sc, nslots, state);
+    return hipGetLastError();
+}
+
+}  // namespace hipx
+}  // namespace honei

@@ -1,18 +1,25 @@
 // ell_device.hh -- device-side ELL row sum shared by the row kernels.
 #pragma once
 
+// Multiply and add stay separate roundings in everything that follows (the
+// row sums and transfers reproduce the CPU path bit for bit).
+#pragma clang fp contract(off)
+
 #include "hip_common.hh"
 
 namespace honei {
 namespace hipx {
 
 // Partial row sum of lane `lane` (0 <= lane < T) of row `row`: the lane walks
-// slots row*T + lane + k*stride for k < len.  Multiply and add are separate
-// roundings (callers compile with fp contract off).
-template <typename VT, int T>
+// slots row*T + lane + k*stride for k < len and adds Ax * x[Aj].  With
+// SCALED the gathered operand is dinv[j] * x[j] instead (first Jacobi sweep
+// from a zero start, where the iterate is dinv .* rhs).  Multiply and add are
+// separate roundings (callers compile with fp contract off).
+template <typename VT, int T, bool SCALED = false>
 __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restrict__ Aj, const VT *__restrict__ Ax,
                                                         int64_t row, int lane, uint32_t len, int64_t stride,
-                                                        const typename VecOf<VT>::type *__restrict__ x) {
+                                                        const typename VecOf<VT>::type *__restrict__ x,
+                                                        const typename VecOf<VT>::type *__restrict__ dinv = nullptr) {
     typedef typename VecOf<VT>::type V;
     V s = V(0);
     int64_t slot = row * T + lane;
@@ -20,7 +27,12 @@ __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restric
     for (uint32_t k = 0; k < len; ++k) {
         const int32_t j = __builtin_nontemporal_load(Aj + slot);
         const V av = load_value_nt(Ax + slot);
-        s += av * x[j];
+        V xv;
+        if constexpr (SCALED)
+            xv = dinv[j] * x[j];
+        else
+            xv = x[j];
+        s += av * xv;
         slot += stride;
     }
     return s;

@@ -14,6 +14,7 @@
 // bit-identical to Product<tags::CPU> (la/product.hh:148-206).
 
 #include "backend.hh"
+#include "ell_device.hh"
 
 #pragma clang fp contract(off)
 
@@ -63,25 +64,12 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
 
         V s = V(0);
         if (active) {
-            const uint32_t len = a.Arl[row];
-            int64_t slot = row * T + lane;
-#pragma unroll 4
-            for (uint32_t k = 0; k < len; ++k) {
-                const int32_t j = __builtin_nontemporal_load(Aj + slot);
-                const V av = load_value_nt(Ax + slot);
-                V xv;
-                if constexpr (MODE == ELL_JACOBI0)
-                    xv = dinv[j] * rhs[j];
-                else
-                    xv = x[j];
-                s += av * xv;
-                slot += a.stride;
-            }
+            if constexpr (MODE == ELL_JACOBI0)
+                s = ell_lane_sum<VT, T, true>(Aj, Ax, row, lane, a.Arl[row], a.stride, rhs, dinv);
+            else
+                s = ell_lane_sum<VT, T>(Aj, Ax, row, lane, a.Arl[row], a.stride, x);
         }
-        if constexpr (T > 1) {
-#pragma unroll
-            for (int off = T / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        }
+        s = lane_group_sum<T>(s);
 
         if (active && lane == 0) {
             if constexpr (MODE == ELL_PRODUCT) {

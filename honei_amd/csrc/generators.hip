@@ -7,8 +7,12 @@
 // 75-122): slot r + k*stride, zero padding, stride 32-aligned by the caller.
 // Draws come from a SplitMix64 stream per row keyed by (seed, r), so the
 // matrix depends only on (n, max_len, band, seed).
+//
+// Fields for the Poisson workloads: uniform right-hand sides and lognormal
+// cell coefficients, element i drawn from position i of one SplitMix64
+// sequence, so any element can be generated independently.
 
-#include "backend.hh"
+#include "solvers.hh"
 
 namespace honei {
 namespace hipx {
@@ -81,6 +85,40 @@ __global__ void k_random_ell(int64_t n, int max_len, int64_t band, uint64_t seed
     Arl[r] = (uint32_t)((have + T - 1) / T);
 }
 
+// Output number i+1 of the SplitMix64 sequence started at `seed` (the same
+// draw the Python helper splitmix64(seed, n)[i] gives).
+__device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ inline double u01_at(uint64_t seed, uint64_t i) {
+    return (double)(splitmix_at(seed, i) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// dst[i] = lo + (hi - lo) * U_i
+__global__ void k_fill_uniform(double *dst, int64_t n, double lo, double hi, uint64_t seed) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = lo + (hi - lo) * u01_at(seed, (uint64_t)i);
+}
+
+// dst[i] = exp(sigma * z_i), z_i standard normal by Box-Muller from draws
+// 2i (u1 = 1 - U, never 0) and 2i+1.
+__global__ void k_fill_lognormal(double *dst, int64_t n, double sigma, uint64_t seed) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double u1 = 1.0 - u01_at(seed, 2 * (uint64_t)i);
+        const double u2 = u01_at(seed, 2 * (uint64_t)i + 1);
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        dst[i] = exp(sigma * z);
+    }
+}
+
+inline int field_grid(int64_t n) {
+    int64_t g = (n + 1023) / 1024;
+    return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
 }  // namespace
 
 hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
@@ -97,6 +135,18 @@ hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64
                            (double *)Ax, Arl, stride, threads);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t fill_uniform(double *dst, int64_t n, double lo, double hi, uint64_t seed, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_uniform, dim3(field_grid(n)), dim3(256), 0, s, dst, n, lo, hi, seed);
+    return hipGetLastError();
+}
+
+hipError_t fill_lognormal(double *dst, int64_t n, double sigma, uint64_t seed, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_lognormal, dim3(field_grid(n)), dim3(256), 0, s, dst, n, sigma, seed);
     return hipGetLastError();
 }
 

@@ -59,12 +59,6 @@ hipError_t dev_ell_alloc(DevEll &M, ValueType vt, int64_t rows, int64_t cols, in
     return hipSuccess;
 }
 
-#define HX(call)                                  \
-    do {                                          \
-        hipError_t _e = (call);                   \
-        if (_e != hipSuccess) return _e;          \
-    } while (0)
-
 MGHierarchy::MGHierarchy(ValueType in, int64_t minl, int64_t maxl) : inner(in), min_level(minl), max_level(maxl) {
     L.resize((size_t)maxl + 1);
 }

@@ -1,5 +1,5 @@
 // q1_kernels.hip -- device assembly of the Q1 Poisson operator directly in
-// the ELL layout (T = 1), one thread per row slot.
+// the ELL layout (any threads-per-row T), one thread per row.
 //
 // Constant coefficient: the operator of FillMatrix<POISSON, DIRICHLET_0>
 // (math/fill_matrix.hh:36-78) converted through SparseMatrix to ELL: boundary
@@ -29,65 +29,70 @@ __device__ inline void put<bf16, float>(bf16 *Ax, int64_t s, float v) {
     Ax[s].bits = (uint16_t)(u >> 16);
 }
 
+// One thread per row of the T-interleaved layout: entry t of the row goes to
+// slot (t % T) + row*T + (t / T)*stride, ncpr = ceil(9 / T) chunks, unused
+// slots (and the padding rows n <= row < stride/T) hold Aj = 0, Ax = 0.
+// Aj / Arl may be null to rewrite values only (shared index arrays).
 template <typename VT>
-__global__ void k_q1_assemble(int64_t side, int32_t *Aj, VT *Ax, uint32_t *Arl, int64_t stride,
+__global__ void k_q1_assemble(int64_t side, int T, int32_t *Aj, VT *Ax, uint32_t *Arl, int64_t stride,
                               const double *coeff, typename VecOf<VT>::type *dinv, double omega) {
     typedef typename VecOf<VT>::type V;
     const int64_t n = side * side;
     const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (row >= stride) return;
-    if (row >= n) {
-        for (int k = 0; k < 9; ++k) {
-            Aj[row + k * stride] = 0;
-            put<VT, V>(Ax, row + k * stride, V(0));
+    if (row * T >= stride) return;
+    const int ncpr = (9 + T - 1) / T;
+    int cnt = 0;
+    int64_t cols[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    V w[9] = {V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0)};
+    if (row < n) {
+        const int64_t i = row / side, j = row % side;
+        const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
+        if (boundary) {
+            cnt = 1;
+            cols[0] = row;
+            w[0] = V(1.0);
+        } else {
+            cnt = 9;
+            if (!coeff) {
+                const V c = V(-1.0 / 3.0);
+                for (int k = 0; k < 9; ++k) w[k] = c;
+                w[4] = V(8.0 / 3.0);
+            } else {
+                const int64_t nc = side - 1;
+                const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
+                const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+                w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
+                w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
+                w[2] = (V)(-se / 3.0);              // (i-1, j+1)
+                w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
+                w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
+                w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
+                w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
+                w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
+                w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+            }
+            const int64_t c0[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
+                                   row + 1,        row + side - 1, row + side, row + side + 1};
+            for (int k = 0; k < 9; ++k) cols[k] = c0[k];
         }
-        return;
     }
-    const int64_t i = row / side, j = row % side;
-    const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
-    if (boundary) {
-        Aj[row] = (int32_t)row;
-        put<VT, V>(Ax, row, V(1.0));
-        for (int k = 1; k < 9; ++k) {
-            Aj[row + k * stride] = 0;
-            put<VT, V>(Ax, row + k * stride, V(0));
-        }
-        Arl[row] = 1;
-        if (dinv) {
-            const V d = V(1.0);
-            const V inv = V(1) / d;
-            dinv[row] = inv * (V)omega;
-        }
-        return;
+    // entries at compile-time indices (no runtime-indexed register arrays)
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
+        const bool have = t < cnt;
+        if (Aj) Aj[slot] = have ? (int32_t)cols[t] : 0;
+        put<VT, V>(Ax, slot, have ? w[t] : V(0));
     }
-    V w[9];
-    if (!coeff) {
-        const V c = V(-1.0 / 3.0);
-        for (int k = 0; k < 9; ++k) w[k] = c;
-        w[4] = V(8.0 / 3.0);
-    } else {
-        const int64_t nc = side - 1;
-        const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
-        const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
-        w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
-        w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
-        w[2] = (V)(-se / 3.0);              // (i-1, j+1)
-        w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
-        w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
-        w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
-        w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
-        w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
-        w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+    for (int t = 9; t < T * ncpr; ++t) {   // padding lanes of the last chunk
+        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
+        if (Aj) Aj[slot] = 0;
+        put<VT, V>(Ax, slot, V(0));
     }
-    const int64_t cols[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
-                             row + 1,        row + side - 1, row + side, row + side + 1};
-    for (int k = 0; k < 9; ++k) {
-        Aj[row + k * stride] = (int32_t)cols[k];
-        put<VT, V>(Ax, row + k * stride, w[k]);
-    }
-    Arl[row] = 9;
+    if (row >= n) return;
+    if (Arl) Arl[row] = (uint32_t)((cnt + T - 1) / T);
     if (dinv) {
-        V d = w[4];
+        V d = (cnt == 1) ? w[0] : w[4];
         if constexpr (sizeof(VT) == 2) {   // diagonal as stored (bf16-rounded)
             uint32_t u = __float_as_uint(d);
             u += 0x7fffu + ((u >> 16) & 1u);
@@ -120,21 +125,24 @@ __global__ void k_coarsen_coeff(const double *fine, int64_t nf_cells, double *co
 }  // namespace
 
 hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32_t *Arl, int64_t stride,
-                       const double *coeff, void *dinv, double omega, hipStream_t s) {
-    if (side < 2 || stride < side * side) return hipErrorInvalidValue;
-    const int grid = (int)((stride + 255) / 256);
+                       const double *coeff, void *dinv, double omega, hipStream_t s, int threads) {
+    const int T = threads;
+    if (side < 2 || T < 1 || T > 32 || (T & (T - 1)) || stride < side * side * T || stride % T) {
+        return hipErrorInvalidValue;
+    }
+    const int grid = (int)((stride / T + 255) / 256);
     switch (vt) {
         case VT_F32:
-            hipLaunchKernelGGL(k_q1_assemble<float>, dim3(grid), dim3(256), 0, s, side, Aj, (float *)Ax, Arl, stride,
-                               coeff, (float *)dinv, omega);
+            hipLaunchKernelGGL(k_q1_assemble<float>, dim3(grid), dim3(256), 0, s, side, T, Aj, (float *)Ax, Arl,
+                               stride, coeff, (float *)dinv, omega);
             break;
         case VT_F64:
-            hipLaunchKernelGGL(k_q1_assemble<double>, dim3(grid), dim3(256), 0, s, side, Aj, (double *)Ax, Arl,
+            hipLaunchKernelGGL(k_q1_assemble<double>, dim3(grid), dim3(256), 0, s, side, T, Aj, (double *)Ax, Arl,
                                stride, coeff, (double *)dinv, omega);
             break;
         case VT_BF16:
-            hipLaunchKernelGGL(k_q1_assemble<bf16>, dim3(grid), dim3(256), 0, s, side, Aj, (bf16 *)Ax, Arl, stride,
-                               coeff, (float *)dinv, omega);
+            hipLaunchKernelGGL(k_q1_assemble<bf16>, dim3(grid), dim3(256), 0, s, side, T, Aj, (bf16 *)Ax, Arl,
+                               stride, coeff, (float *)dinv, omega);
             break;
     }
     return hipGetLastError();

@@ -1,0 +1,151 @@
+// serve.cpp -- continuous-batching request loop (see serve.hh).
+
+#include "serve.hh"
+
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <deque>
+#include <thread>
+
+namespace honei {
+namespace hipx {
+
+uint64_t slot_bytes(int64_t level, ValueType inner) {
+    const uint64_t vs = vt_size(inner);
+    uint64_t b = 0;
+    for (int64_t l = 1; l <= level; ++l) {
+        const uint64_t side = (1ull << l) + 1, n = side * side;
+        const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;
+        b += 9 * stride * vs + vld * 4 * (l == 1 ? 8 : 5);
+    }
+    const uint64_t side = (1ull << level) + 1, n = side * side;
+    const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;
+    b += 9 * stride * 8 + vld * 8 * 3 + (n + 255) / 256 * 8 + sizeof(SlotScalars) + 8;
+    return b;
+}
+
+hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n, ServeRecord *out,
+                     ServeStats *stats, bool *over_budget) {
+    using clock = std::chrono::steady_clock;
+    if (over_budget) *over_budget = false;
+    if (cfg.nclass < 1 || cfg.nclass > kMaxClasses || n < 0) return hipErrorInvalidValue;
+    // pool: explicit class sizes, or an equal share of the budget per class
+    // (matrix storage counted at the largest tuned layout, 12/9 of T = 1)
+    int64_t counts[kMaxClasses];
+    for (int c = 0; c < cfg.nclass; ++c) {
+        counts[c] = cfg.counts[c];
+        if (counts[c] > 0) continue;
+        if (!cfg.budget_bytes) return hipErrorInvalidValue;
+        const uint64_t per = slot_bytes(cfg.levels[c], (ValueType)cfg.batch.inner_vtype) * 4 / 3 + 1;
+        const uint64_t share = cfg.budget_bytes / (uint64_t)cfg.nclass;
+        counts[c] = std::max<int64_t>(1, std::min<int64_t>(cfg.max_slots_per_class, (int64_t)(share / per)));
+    }
+    BatchMG pool;
+    const hipError_t ce = pool.create(cfg.levels, counts, cfg.nclass, cfg.batch, cfg.budget_bytes);
+    if (ce != hipSuccess) {
+        if (over_budget) *over_budget = pool.over_budget();
+        return ce;
+    }
+    std::vector<std::deque<int64_t>> queue((size_t)cfg.nclass);
+    std::vector<std::vector<int64_t>> free_slots((size_t)cfg.nclass);
+    for (int c = 0; c < cfg.nclass; ++c)
+        for (int64_t j = counts[c] - 1; j >= 0; --j) free_slots[(size_t)c].push_back(pool.slot(c, j));
+    std::vector<int64_t> slot_req((size_t)pool.nslots(), -1);
+    std::vector<int> slot_class((size_t)pool.nslots(), 0);
+    for (int c = 0; c < cfg.nclass; ++c)
+        for (int64_t j = 0; j < counts[c]; ++j) slot_class[(size_t)pool.slot(c, j)] = c;
+    auto class_of = [&](int32_t level) {
+        for (int c = 0; c < cfg.nclass; ++c)
+            if (cfg.levels[c] == level) return c;
+        return -1;
+    };
+    for (int64_t r = 0; r < n; ++r) out[r] = ServeRecord();
+
+    ServeStats st;
+    st.device_bytes = pool.bytes();
+    std::vector<SlotScalars> sc;
+    std::vector<int64_t> retired;
+    const auto t0 = clock::now();
+    auto elapsed = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    int64_t next = 0, finished = 0, queued = 0;
+    while (finished < n) {
+        const double now = elapsed();
+        for (; next < n && reqs[next].arrival_s <= now; ++next) {
+            const int c = class_of(reqs[next].level);
+            if (c < 0) {
+                out[next].status = -2;   // no slot class for this size
+                ++st.rejected;
+                ++finished;
+                continue;
+            }
+            queue[(size_t)c].push_back(next);
+            ++queued;
+        }
+        st.max_queue = std::max(st.max_queue, queued);
+        for (int c = 0; c < cfg.nclass; ++c) {
+            auto &q = queue[(size_t)c];
+            auto &fs = free_slots[(size_t)c];
+            while (!q.empty() && !fs.empty()) {
+                const int64_t r = q.front(), g = fs.back();
+                q.pop_front();
+                fs.pop_back();
+                --queued;
+                HX(pool.load_seeded(g, reqs[r].coeff_seed, cfg.sigma, reqs[r].rhs_seed));
+                slot_req[(size_t)g] = r;
+                out[r].slot = g;
+                out[r].admit_s = elapsed();
+            }
+        }
+        if (pool.running() + pool.loading_count() == 0) {
+            if (next < n) {
+                const double wait = reqs[next].arrival_s - elapsed();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.002)));
+            }
+            continue;
+        }
+        if (pool.running() == 0) HX(pool.activate_ready(true));   // nothing to overlap with
+        HX(pool.step());
+        HX(pool.poll(sc, &retired));
+        const double t = elapsed();
+        for (int64_t g : retired) {
+            const int64_t r = slot_req[(size_t)g];
+            if (r < 0) continue;
+            const SlotScalars &s = sc[(size_t)g];
+            out[r].done_s = t;
+            out[r].iters = s.iters;
+            out[r].status = s.status;
+            out[r].rel = s.rel;
+            if (s.status == SLOT_OK) ++st.completed;
+            else ++st.not_converged;
+            if (reqs[r].x_dev) HX(pool.result(g, reqs[r].x_dev));
+            slot_req[(size_t)g] = -1;
+            free_slots[(size_t)slot_class[(size_t)g]].push_back(g);
+            ++finished;
+        }
+    }
+    st.wall_s = elapsed();
+    st.launches = pool.launches();
+    std::vector<double> lat;
+    for (int64_t r = 0; r < n; ++r)
+        if (out[r].status == SLOT_OK) lat.push_back(1000.0 * (out[r].done_s - reqs[r].arrival_s));
+    if (!lat.empty()) {
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) {
+            const size_t k = (size_t)std::min<double>((double)lat.size() - 1, std::ceil(p * (double)lat.size()) - 1);
+            return lat[k];
+        };
+        st.p50_ms = pct(0.50);
+        st.p99_ms = pct(0.99);
+        st.max_ms = lat.back();
+        double sum = 0.0;
+        for (double v : lat) sum += v;
+        st.mean_ms = sum / (double)lat.size();
+    }
+    st.solves_per_s = st.wall_s > 0 ? (double)st.completed / st.wall_s : 0.0;
+    if (stats) *stats = st;
+    return hipSuccess;
+}
+
+}  // namespace hipx
+}  // namespace honei

@@ -9,6 +9,13 @@
 
 #include "backend.hh"
 
+// Return the first failing HIP status of a host-side driver step.
+#define HX(call)                                  \
+    do {                                          \
+        hipError_t _e = (call);                   \
+        if (_e != hipSuccess) return _e;          \
+    } while (0)
+
 namespace honei {
 namespace hipx {
 
@@ -33,11 +40,16 @@ hipError_t scale_to_f32(float *dst, const double *src, double inv, int64_t n, hi
 // a per-cell coefficient (side-1)^2 array the stencil is the Q1 bilinear
 // element stiffness summed over the four adjacent cells (coefficient 1
 // reproduces 8/3 and -1/3).  dinv (optional) = omega / diag.
+// threads > 1 writes the T-interleaved layout (stride >= side^2 * T); null Aj
+// and Arl rewrite the values (and dinv) only.
 hipError_t q1_assemble(ValueType vt, int64_t side, int32_t *Aj, void *Ax, uint32_t *Arl, int64_t stride,
-                       const double *coeff, void *dinv, double omega, hipStream_t s);
+                       const double *coeff, void *dinv, double omega, hipStream_t s, int threads = 1);
 
 hipError_t random_ell(ValueType vt, int64_t n, int max_len, int64_t band, uint64_t seed, int32_t *Aj, void *Ax,
                       uint32_t *Arl, int64_t stride, int threads, hipStream_t s);
+// Seeded fields: uniform in [lo, hi), and exp(sigma * N(0,1)) (Box-Muller).
+hipError_t fill_uniform(double *dst, int64_t n, double lo, double hi, uint64_t seed, hipStream_t s);
+hipError_t fill_lognormal(double *dst, int64_t n, double sigma, uint64_t seed, hipStream_t s);
 // averaging: 0 arithmetic, 1 geometric, 2 harmonic mean of the 4 fine cells
 hipError_t coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells, int averaging,
                          hipStream_t s);

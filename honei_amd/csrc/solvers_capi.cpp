@@ -164,14 +164,16 @@ int honei_hip_cg_konv(int vtype, const void *Aj, const void *Ax, const void *Arl
 
 // ---- Q1 assembly -------------------------------------------------------------
 int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl, size_t stride,
-                          const double *coeff, void *dinv, double omega, void *stream) {
+                          const double *coeff, void *dinv, double omega, size_t threads, void *stream) {
     if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64 && vtype != HONEI_HIP_BF16)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
     if (!Aj || !Ax || !Arl) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
-    if (side < 2 || side > 46000 || stride < side * side)
+    if (threads < 1 || threads > 32 || (threads & (threads - 1)))
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "threads must be a power of two <= 32");
+    if (side < 2 || side > 46000 || stride < side * side * threads || stride % threads)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad side/stride");
     return check(q1_assemble((ValueType)vtype, (int64_t)side, (int32_t *)Aj, Ax, (uint32_t *)Arl,
-                             (int64_t)stride, coeff, dinv, omega, S(stream)),
+                             (int64_t)stride, coeff, dinv, omega, S(stream), (int)threads),
                  "q1_assemble");
 }
 

@@ -1,20 +1,9 @@
 // transfer_kernels.hip -- geometric multigrid transfers on lexicographic
-// (2^L+1)^2 Q1 grids, one thread per output node.
-//
-// Restriction is the transpose of bilinear interpolation (weights 1, 1/2,
-// 1/4; math/restriction.hh:40-172); prolongation is bilinear interpolation
-// (math/prolongation.hh:41-139).  Both finish with the Dirichlet mask of
-// math/apply_dirichlet_boundaries.hh:28-85 (8 flags: 4 corners, 4 edges;
-// 2 = Dirichlet -> node set to 0).
-//
-// The reference accumulates each coarse node from 9 fine values in the order
-// its sweep loops visit them.  Each thread here forms the same 9 terms, tags
-// each with the (sweep, sub-step) at which the reference adds it, sorts the
-// (at most 8) added terms by that tag with a small sorting network and sums
-// in that order, so the result is bit-identical to the CPU path rather than
-// merely close.
+// (2^L+1)^2 Q1 grids, one thread per output node (node arithmetic in
+// transfer_device.hh).
 
 #include "backend.hh"
+#include "transfer_device.hh"
 
 #pragma clang fp contract(off)
 
@@ -22,84 +11,10 @@ namespace honei {
 namespace hipx {
 namespace {
 
-struct Mask8 {
-    unsigned char m[8];
-};
-
-__device__ inline bool dirichlet_zero(int64_t r, int64_t c, int64_t M, const Mask8 &mk) {
-    int e;
-    if (r == 0 && c == 0) e = 0;
-    else if (r == 0 && c == M - 1) e = 1;
-    else if (r == M - 1 && c == M - 1) e = 2;
-    else if (r == M - 1 && c == 0) e = 3;
-    else if (r == 0) e = 4;
-    else if (c == M - 1) e = 5;
-    else if (r == M - 1) e = 6;
-    else if (c == 0) e = 7;
-    else return false;
-    return mk.m[e] == 2;
-}
-
-template <typename V>
-__device__ inline void cswap(int64_t &ka, V &va, int64_t &kb, V &vb) {
-    if (kb < ka) {
-        int64_t tk = ka; ka = kb; kb = tk;
-        V tv = va; va = vb; vb = tv;
-    }
-}
-
 template <typename V>
 __global__ void k_restriction(V *__restrict__ coarse, const V *__restrict__ fine, int64_t m, Mask8 mk) {
-    const int64_t nc = m + 1, nf = 2 * m + 1;
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (id >= nc * nc) return;
-    const int64_t r = id / nc, c = id % nc;
-    if (dirichlet_zero(r, c, nc, mk)) {
-        coarse[id] = V(0);
-        return;
-    }
-    auto F = [&](int64_t fr, int64_t fc) { return fine[fr * nf + fc]; };
-    const int64_t NONE = INT64_MAX;
-    // tag = sweep index i (1..m) * 8 + sub-step (A..H = 0..7)
-    int64_t k[8];
-    V v[8];
-    const V h = V(0.5), q = V(0.25);
-    // A_i (i=r): row i += .5 F(2i-1, 2c)
-    k[0] = r >= 1 ? r * 8 + 0 : NONE;           v[0] = r >= 1 ? h * F(2 * r - 1, 2 * c) : V(0);
-    // B_i (i=r+1): row i-1 += .5 F(2i-1, 2c)
-    k[1] = r + 1 <= m ? (r + 1) * 8 + 1 : NONE; v[1] = r + 1 <= m ? h * F(2 * r + 1, 2 * c) : V(0);
-    // C_i (i=c+1): col i-1 += .5 F(2r, 2i-1)
-    k[2] = c + 1 <= m ? (c + 1) * 8 + 2 : NONE; v[2] = c + 1 <= m ? h * F(2 * r, 2 * c + 1) : V(0);
-    // D_i (i=c): col i += .5 F(2r, 2i-1)
-    k[3] = c >= 1 ? c * 8 + 3 : NONE;           v[3] = c >= 1 ? h * F(2 * r, 2 * c - 1) : V(0);
-    // E_i (i=r+1, j=c<m): (i-1, j) += .25 F(2i-1, 2j+1)
-    k[4] = (r + 1 <= m && c < m) ? (r + 1) * 8 + 4 : NONE;
-    v[4] = (r + 1 <= m && c < m) ? q * F(2 * r + 1, 2 * c + 1) : V(0);
-    // F_i (i=r+1, j=c-1): (i-1, j+1) += .25 F(2i-1, 2j+1)
-    k[5] = (r + 1 <= m && c >= 1) ? (r + 1) * 8 + 5 : NONE;
-    v[5] = (r + 1 <= m && c >= 1) ? q * F(2 * r + 1, 2 * c - 1) : V(0);
-    // G_i (i=r, j=c<m): (i, j) += .25 F(2i-1, 2j+1)
-    k[6] = (r >= 1 && c < m) ? r * 8 + 6 : NONE;
-    v[6] = (r >= 1 && c < m) ? q * F(2 * r - 1, 2 * c + 1) : V(0);
-    // H_i (i=r, j=c-1): (i, j+1) += .25 F(2i-1, 2j+1)
-    k[7] = (r >= 1 && c >= 1) ? r * 8 + 7 : NONE;
-    v[7] = (r >= 1 && c >= 1) ? q * F(2 * r - 1, 2 * c - 1) : V(0);
-    // Batcher odd-even merge sort network for 8 keys (19 comparators)
-    cswap(k[0], v[0], k[1], v[1]); cswap(k[2], v[2], k[3], v[3]);
-    cswap(k[4], v[4], k[5], v[5]); cswap(k[6], v[6], k[7], v[7]);
-    cswap(k[0], v[0], k[2], v[2]); cswap(k[1], v[1], k[3], v[3]);
-    cswap(k[4], v[4], k[6], v[6]); cswap(k[5], v[5], k[7], v[7]);
-    cswap(k[1], v[1], k[2], v[2]); cswap(k[5], v[5], k[6], v[6]);
-    cswap(k[0], v[0], k[4], v[4]); cswap(k[1], v[1], k[5], v[5]);
-    cswap(k[2], v[2], k[6], v[6]); cswap(k[3], v[3], k[7], v[7]);
-    cswap(k[2], v[2], k[4], v[4]); cswap(k[3], v[3], k[5], v[5]);
-    cswap(k[1], v[1], k[2], v[2]); cswap(k[3], v[3], k[4], v[4]);
-    cswap(k[5], v[5], k[6], v[6]);
-    V s = F(2 * r, 2 * c);
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-        if (k[t] != NONE) s = s + v[t];
-    coarse[id] = s;
+    if (id < (m + 1) * (m + 1)) restrict_node(coarse, fine, m, mk, id);
 }
 
 // fine[R,C] from coarse, in the reference's accumulation order; optionally
@@ -107,30 +22,8 @@ __global__ void k_restriction(V *__restrict__ coarse, const V *__restrict__ fine
 template <typename V, bool AXPY>
 __global__ void k_prolongation(V *__restrict__ fine, const V *__restrict__ coarse, int64_t m, Mask8 mk,
                                V *__restrict__ x, V alpha) {
-    const int64_t nc = m + 1, nf = 2 * m + 1;
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (id >= nf * nf) return;
-    const int64_t R = id / nf, C = id % nf;
-    auto Cn = [&](int64_t cr, int64_t cc) { return coarse[cr * nc + cc]; };
-    V f;
-    if (dirichlet_zero(R, C, nf, mk)) {
-        f = V(0);
-    } else if ((R & 1) == 0 && (C & 1) == 0) {
-        f = Cn(R / 2, C / 2);
-    } else if ((R & 1) == 0) {
-        f = V(0) + Cn(R / 2, (C - 1) / 2) * V(0.5);
-        f = f + Cn(R / 2, (C + 1) / 2) * V(0.5);
-    } else if ((C & 1) == 0) {
-        f = V(0) + Cn((R - 1) / 2, C / 2) * V(0.5);
-        f = f + Cn((R + 1) / 2, C / 2) * V(0.5);
-    } else {
-        f = V(0) + Cn((R - 1) / 2, (C - 1) / 2) * V(0.25);
-        f = f + Cn((R - 1) / 2, (C + 1) / 2) * V(0.25);
-        f = f + Cn((R + 1) / 2, (C - 1) / 2) * V(0.25);
-        f = f + Cn((R + 1) / 2, (C + 1) / 2) * V(0.25);
-    }
-    fine[id] = f;
-    if constexpr (AXPY) x[id] = x[id] + f * alpha;
+    if (id < (2 * m + 1) * (2 * m + 1)) prolong_node<V, AXPY>(fine, coarse, m, mk, x, alpha, id);
 }
 
 bool side_of(int64_t n, int64_t &side) {
@@ -140,11 +33,6 @@ bool side_of(int64_t n, int64_t &side) {
     return s * s == n && s >= 3 && ((s - 1) & (s - 2)) == 0;
 }
 
-Mask8 pack(const uint64_t *mask) {
-    Mask8 m;
-    for (int i = 0; i < 8; ++i) m.m[i] = (unsigned char)(mask ? (mask[i] == 2 ? 2 : 1) : 2);
-    return m;
-}
 
 }  // namespace
 
@@ -156,10 +44,10 @@ hipError_t restriction(void *coarse, int64_t n_coarse, const void *fine, int64_t
     const int grid = ceil_div(n_coarse, 256);
     if (vt == VT_F32)
         hipLaunchKernelGGL(k_restriction<float>, dim3(grid), dim3(256), 0, s, (float *)coarse, (const float *)fine, m,
-                           pack(mask));
+                           pack_mask(mask));
     else if (vt == VT_F64)
         hipLaunchKernelGGL(k_restriction<double>, dim3(grid), dim3(256), 0, s, (double *)coarse, (const double *)fine,
-                           m, pack(mask));
+                           m, pack_mask(mask));
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -174,17 +62,17 @@ hipError_t prolongation_axpy(void *fine, int64_t n_fine, const void *coarse, int
     if (vt == VT_F32) {
         if (x)
             hipLaunchKernelGGL((k_prolongation<float, true>), dim3(grid), dim3(256), 0, s, (float *)fine,
-                               (const float *)coarse, m, pack(mask), (float *)x, (float)alpha);
+                               (const float *)coarse, m, pack_mask(mask), (float *)x, (float)alpha);
         else
             hipLaunchKernelGGL((k_prolongation<float, false>), dim3(grid), dim3(256), 0, s, (float *)fine,
-                               (const float *)coarse, m, pack(mask), (float *)nullptr, 0.f);
+                               (const float *)coarse, m, pack_mask(mask), (float *)nullptr, 0.f);
     } else if (vt == VT_F64) {
         if (x)
             hipLaunchKernelGGL((k_prolongation<double, true>), dim3(grid), dim3(256), 0, s, (double *)fine,
-                               (const double *)coarse, m, pack(mask), (double *)x, alpha);
+                               (const double *)coarse, m, pack_mask(mask), (double *)x, alpha);
         else
             hipLaunchKernelGGL((k_prolongation<double, false>), dim3(grid), dim3(256), 0, s, (double *)fine,
-                               (const double *)coarse, m, pack(mask), (double *)nullptr, 0.0);
+                               (const double *)coarse, m, pack_mask(mask), (double *)nullptr, 0.0);
     } else {
         return hipErrorInvalidValue;
     }

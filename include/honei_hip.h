@@ -169,16 +169,18 @@ int honei_hip_prolongation(int vtype, void *fine, size_t size_fine, const void *
 /* ---- device Q1 assembly ----------------------------------------------------
  * Replaces the host chain FillMatrix<POISSON, DIRICHLET_0>
  * (math/fill_matrix.hh:36-78) -> SparseMatrix(BandedMatrixQx)
- * (la/sparse_matrix.hh:221-300) -> SparseMatrixELL (T = 1,
- * la/sparse_matrix_ell-impl.hh:75-122), built directly in HBM: Aj int32 and
- * Ax [9*stride], Arl uint32 [side^2], stride >= side^2 (reference: 32-aligned).
+ * (la/sparse_matrix.hh:221-300) -> SparseMatrixELL with `threads` = T
+ * (la/sparse_matrix_ell-impl.hh:75-122), built directly in HBM: Aj int32 and
+ * Ax [ceil(9/T)*stride], Arl uint32 [side^2], stride >= side^2*T and a
+ * multiple of T (reference: 32*ceil(side^2*T/32)); T a power of two <= 32.
  * coeff_dev NULL: the reference's constant stencil, bit-identical arrays.
  * coeff_dev non-NULL: per-cell coefficient (double, (side-1)^2, row-major
  * cells) -- the variable-coefficient Q1 operator, which the reference lacks.
  * dinv_dev (optional, vector type) = omega / diag, as the reference tests
  * build it (math/poisson_mg_fixed_ell_TEST.cc:707-714).                    */
 int honei_hip_q1_assemble(int vtype, size_t side, void *Aj, void *Ax, void *Arl, size_t stride,
-                          const double *coeff_dev, void *dinv_dev, double omega, void *stream);
+                          const double *coeff_dev, void *dinv_dev, double omega, size_t threads,
+                          void *stream);
 
 /* ---- synthetic workloads -----------------------------------------------------
  * Random ELL of the SpMV benchmark configuration generated in HBM (no
@@ -257,6 +259,124 @@ int honei_hip_mg_solve_fixed(void *mg, const void *b_dev, void *x_dev, size_t *i
                              void *stream);
 int honei_hip_mg_solve_mixed(void *mg, const double *b_dev, double *x_dev, size_t *outer_iters,
                              size_t *inner_iters, double *rel_defect, void *stream);
+
+/* ---- seeded fields (synthetic Poisson workloads) ------------------------------
+ * No reference counterpart.  Element i is drawn from position i of the
+ * SplitMix64 sequence of `seed` (U = (u >> 11) * 2^-53):
+ * uniform: dst[i] = lo + (hi - lo) * U_i;
+ * lognormal: dst[i] = exp(sigma * z_i), z_i = sqrt(-2 ln(1 - U_2i)) * cos(2 pi U_2i+1). */
+int honei_hip_fill_uniform(double *dst_dev, size_t n, double lo, double hi, uint64_t seed, void *stream);
+int honei_hip_fill_lognormal(double *dst_dev, size_t n, double sigma, uint64_t seed, void *stream);
+
+/* ---- batched multigrid ----------------------------------------------------------
+ * No reference counterpart: the reference solves one system per call
+ * (Multigrid<...MIXED>::value, math/multigrid.hh:2395-2536).  A batch is a
+ * device-resident pool of slots in classes by finest level (side 2^L + 1,
+ * L in 2..13, at most 16 classes, 65535 slots); every slot holds a Q1
+ * hierarchy with a per-cell coefficient down to the 3x3 grid, solved by
+ * the MIXED defect correction (FP64 outer defect, update and norm; FP32 or
+ * BF16-stored V-cycle with n_pre / n_post damped Jacobi sweeps and the
+ * single-work-group CG on the coarsest grid).  One graph launch runs
+ * iters_per_launch outer iterations of every active slot; slots retire on
+ * the device when ||b - A x|| < tolerance * ||b - A x0||, where x0 is zero
+ * with the first row, last row and first column of the grid set to b.
+ * autotune: threads-per-row T in {1,2,4} and block size per level picked by
+ * timing (cached for the process); 0 uses T = 1, block 256, which makes each
+ * slot bit-identical to honei_hip_mg_solve_mixed with n_max_iter 1,
+ * initial_zero, zero_inner_start and post sweeps = pre sweeps.  The batch
+ * owns its streams; calls on one batch must not overlap.                    */
+typedef struct {
+    int32_t inner_vtype;       /* HONEI_HIP_F32 or HONEI_HIP_BF16 (matrix values) */
+    int32_t n_pre, n_post;     /* Jacobi sweeps, n_pre >= 2 */
+    double omega;              /* Jacobi damping */
+    double tolerance;          /* relative FP64 defect */
+    int32_t max_outer;         /* outer iterations before a slot gives up */
+    int32_t averaging;         /* coefficient coarsening: 0 arithmetic, 1 geometric, 2 harmonic */
+    int32_t iters_per_launch;  /* outer iterations per graph launch (1..64) */
+    int32_t autotune;
+} honei_hip_batch_config;
+
+typedef struct {
+    double initial;            /* ||b - A x0|| */
+    double norm;               /* latest ||b - A x|| (FP64) */
+    double rel;                /* norm / initial */
+    int32_t state;             /* 0 idle, 1 running, 2 admitted (first check pending), 3 loading */
+    int32_t iters;             /* outer iterations applied */
+    int32_t status;            /* 0 converged, 5 not converged, -1 pending */
+    int32_t level;             /* finest level of the slot */
+} honei_hip_slot_info;
+
+int honei_hip_batch_config_default(honei_hip_batch_config *cfg);
+/* levels[c], counts[c] for c < nclass; budget_bytes 0 = no limit, else the
+ * pool (matrices, hierarchies, vectors) must fit: HONEI_HIP_ERROR_BUDGET.   */
+int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *counts, int32_t nclass,
+                           const honei_hip_batch_config *cfg, uint64_t budget_bytes);
+int honei_hip_batch_destroy(void *batch);
+int honei_hip_batch_info(void *batch, int64_t *nslots, uint64_t *device_bytes, int64_t *launches);
+/* Global slot index of the j-th slot of class `cls` (caller's class order). */
+int honei_hip_batch_slot(void *batch, int32_t cls, int64_t j, int64_t *slot);
+/* Assemble slot `slot` (idle, result already taken) from coeff_dev
+ * ((side-1)^2 doubles) and b_dev (side^2 doubles); asynchronous, on the
+ * batch's load stream.  The slot joins at the next step after the copy. */
+int honei_hip_batch_load(void *batch, int64_t slot, const double *coeff_dev, const double *b_dev);
+/* Hand loads that have finished (wait != 0: all loads issued so far, blocking)
+ * to the next launch; honei_hip_batch_step does this without waiting.     */
+int honei_hip_batch_activate(void *batch, int32_t wait);
+int honei_hip_batch_step(void *batch);                          /* one graph launch, asynchronous */
+int honei_hip_batch_poll(void *batch, honei_hip_slot_info *info); /* waits; info[nslots] */
+int honei_hip_batch_result(void *batch, int64_t slot, double *x_dev);
+/* Step until every loaded slot has retired; *launches = graph launches. */
+int honei_hip_batch_run(void *batch, int64_t *launches);
+/* Launch shape of an inner level (threads per row, block size) and, when the
+ * autotuner chose it, the measured time of one batched Jacobi sweep over its
+ * tuning batch (microseconds, else 0) and that batch's slot count.         */
+int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, double *sweep_us,
+                          int64_t *tune_slots);
+
+/* ---- serving loop -------------------------------------------------------------------
+ * No reference counterpart.  Continuous batching over a batch pool: requests
+ * (Q1 systems, side 2^level + 1, coefficient exp(sigma N(0,1)) per cell and
+ * b uniform in [-1, 1), drawn on the device from the request's seeds) arrive
+ * at arrival_s seconds of wall clock after the call starts; each waits in a
+ * FIFO for a free slot of its size class, is assembled on the load stream
+ * while admitted work keeps iterating, and joins the next graph launch.
+ * Class sizes counts[c] = 0 are derived from budget_bytes (equal share per
+ * class, at most max_slots_per_class); the whole pool must fit the budget
+ * (HONEI_HIP_ERROR_BUDGET).  results[r].status: 0 converged, 5 not
+ * converged, -2 no class for the request's level.  Latency percentiles are
+ * over converged requests, from arrival to the poll that saw completion.   */
+typedef struct {
+    double arrival_s;
+    int32_t level;
+    int32_t pad;
+    uint64_t coeff_seed, rhs_seed;
+    double *x_dev;             /* optional: FP64 solution (side^2) copied here */
+} honei_hip_request;
+
+typedef struct {
+    double admit_s, done_s, rel;
+    int32_t iters, status;
+    int64_t slot;
+} honei_hip_request_result;
+
+typedef struct {
+    int64_t completed, not_converged, rejected, launches, max_queue;
+    double wall_s, solves_per_s, p50_ms, p99_ms, max_ms, mean_ms;
+    uint64_t device_bytes;
+} honei_hip_serve_stats;
+
+typedef struct {
+    honei_hip_batch_config batch;
+    int32_t nclass;
+    int32_t max_slots_per_class;
+    int64_t levels[16];
+    int64_t counts[16];
+    uint64_t budget_bytes;
+    double sigma;
+} honei_hip_serve_config;
+
+int honei_hip_serve_run(const honei_hip_serve_config *cfg, const honei_hip_request *reqs, int64_t nreq,
+                        honei_hip_request_result *results, honei_hip_serve_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -289,6 +289,32 @@ def banded_to_csr(bands):
     return rp, col, val
 
 
+def q1_coeff_bands(side, kappa, dtype=np.float64):
+    """Variable-coefficient Q1 operator as 9 bands (no reference counterpart;
+    bilinear element stiffness per cell, Dirichlet rows identity)."""
+    kappa = np.ascontiguousarray(kappa, np.float64)
+    assert kappa.size == (side - 1) ** 2
+    bands = np.zeros(9 * side * side, dtype)
+    getattr(lib(), f"oracle_q1_coeff_bands_{_sfx(dtype)}")(_u64(side), _ptr(kappa), _ptr(bands))
+    return bands.reshape(9, side * side)
+
+
+def q1_coeff_matrix(side, kappa, dtype=np.float64, threads=1) -> Ell:
+    rp, col, val = banded_to_csr(q1_coeff_bands(side, kappa, dtype))
+    n = side * side
+    return ell_from_sm(n, n, rp, col, val, threads)
+
+
+def coarsen_coeff(kappa, nf, averaging=1):
+    """Cells of the next coarser grid: mean of 2x2 fine cells (0 arithmetic,
+    1 geometric, 2 harmonic)."""
+    kappa = np.ascontiguousarray(kappa, np.float64)
+    nc = nf // 2
+    out = np.zeros(nc * nc)
+    lib().oracle_coarsen_coeff(_ptr(kappa), _u64(nf), _ptr(out), _u64(nc), C.c_int(averaging))
+    return out
+
+
 def q1_matrix(side, dtype=np.float64, threads=1, bc=0) -> Ell:
     rp, col, val = banded_to_csr(q1_bands(side, dtype, bc))
     n = side * side
@@ -427,6 +453,24 @@ class MGHierarchy:
                 arr[l].prol = self.prol[l].c_struct()
                 arr[l].res = self.res[l].c_struct()
         return arr
+
+
+def q1_coeff_hierarchy(max_level, kappa, averaging=1, omega=0.8, dtype=np.float32, min_level=1):
+    """Rediscretised variable-coefficient hierarchy: level l has (2^l+1)^2
+    nodes and the coefficient coarsened level by level; dinv = omega/diag."""
+    H = MGHierarchy()
+    k = np.ascontiguousarray(kappa, np.float64)
+    for l in range(max_level, min_level - 1, -1):
+        side = 2 ** l + 1
+        bands = q1_coeff_bands(side, k, dtype)
+        rp, col, val = banded_to_csr(bands)
+        n = side * side
+        H.A[l] = ell_from_sm(n, n, rp, col, val)
+        d = bands[4].astype(dtype)
+        H.dinv[l] = ((np.ones(n, dtype) / d).astype(dtype) * dtype(omega)).astype(dtype)
+        if l > min_level:
+            k = coarsen_coeff(k, 2 ** l, averaging)
+    return H
 
 
 def q1_hierarchy(max_level, min_level=1, dtype=np.float32, omega=0.7, use_prolmat=False, threads=1):

@@ -257,6 +257,54 @@ void q1_bands(u64 N, DT *bands, int bc) {
     }
 }
 
+// Variable-coefficient Q1 operator (no reference counterpart; the device
+// assembly in honei_amd/csrc/q1_kernels.hip follows the same formulas):
+// kappa per cell ((side-1)^2, row-major), bilinear element stiffness
+// kappa/6 * [[4,-1,-2,-1],[-1,4,-1,-2],[-2,-1,4,-1],[-1,-2,-1,4]] summed over
+// the four cells around node (i, j) -- cells sw=(i-1,j-1), se=(i-1,j),
+// nw=(i,j-1), ne=(i,j); Dirichlet rows identity.  kappa = 1 gives the
+// constant stencil of q1_bands (8/3, -1/3).
+template <typename DT>
+void q1_coeff_bands(u64 side, const double *kappa, DT *bands) {
+    const u64 N = side * side, nc = side - 1;
+    DT *b[9];
+    for (int k = 0; k < 9; ++k) b[k] = bands + (u64)k * N;
+    for (u64 r = 0; r < N; ++r) {
+        const u64 i = r / side, j = r % side;
+        for (int k = 0; k < 9; ++k) b[k][r] = DT(0);
+        if (i == 0 || i == side - 1 || j == 0 || j == side - 1) {
+            b[DD][r] = DT(1.0);
+            continue;
+        }
+        const double sw = kappa[(i - 1) * nc + (j - 1)], se = kappa[(i - 1) * nc + j];
+        const double nw = kappa[i * nc + (j - 1)], ne = kappa[i * nc + j];
+        b[LL][r] = (DT)(-sw / 3.0);
+        b[LD][r] = (DT)(-(sw + se) / 6.0);
+        b[LU][r] = (DT)(-se / 3.0);
+        b[DL][r] = (DT)(-(sw + nw) / 6.0);
+        b[DD][r] = (DT)(2.0 * (sw + se + nw + ne) / 3.0);
+        b[DU][r] = (DT)(-(se + ne) / 6.0);
+        b[UL][r] = (DT)(-nw / 3.0);
+        b[UD][r] = (DT)(-(nw + ne) / 6.0);
+        b[UU][r] = (DT)(-ne / 3.0);
+    }
+}
+
+// Coarse cell (I, J) from fine cells (2I..2I+1, 2J..2J+1): averaging 0
+// arithmetic, 1 geometric, 2 harmonic mean (rediscretised hierarchies).
+void coarsen_coeff(const double *fine, u64 nf, double *coarse, u64 nc, int averaging) {
+    for (u64 I = 0; I < nc; ++I)
+        for (u64 J = 0; J < nc; ++J) {
+            const double a = fine[(2 * I) * nf + 2 * J], b = fine[(2 * I) * nf + 2 * J + 1];
+            const double c = fine[(2 * I + 1) * nf + 2 * J], d = fine[(2 * I + 1) * nf + 2 * J + 1];
+            double m;
+            if (averaging == 1) m = std::exp(0.25 * (((std::log(a) + std::log(b)) + std::log(c)) + std::log(d)));
+            else if (averaging == 2) m = 4.0 / (((1.0 / a + 1.0 / b) + 1.0 / c) + 1.0 / d);
+            else m = 0.25 * (((a + b) + c) + d);
+            coarse[I * nc + J] = m;
+        }
+}
+
 // Each band k contributes entry (i, i + off_k) for i in [lo_k, hi_k), only when
 // non-zero; rows come out sorted by column because bands are visited in
 // ascending offset order.
@@ -757,6 +805,11 @@ void oracle_q1_bands_f(u64 N, float *bands, int bc) { q1_bands(N, bands, bc); }
 void oracle_q1_bands_d(u64 N, double *bands, int bc) { q1_bands(N, bands, bc); }
 u64 oracle_banded_q1_to_csr_f(u64 N, const float *bands, u64 *rp, u64 *col, float *val) { return banded_q1_to_csr(N, bands, rp, col, val); }
 u64 oracle_banded_q1_to_csr_d(u64 N, const double *bands, u64 *rp, u64 *col, double *val) { return banded_q1_to_csr(N, bands, rp, col, val); }
+void oracle_q1_coeff_bands_f(u64 side, const double *kappa, float *bands) { q1_coeff_bands(side, kappa, bands); }
+void oracle_q1_coeff_bands_d(u64 side, const double *kappa, double *bands) { q1_coeff_bands(side, kappa, bands); }
+void oracle_coarsen_coeff(const double *fine, u64 nf, double *coarse, u64 nc, int averaging) {
+    coarsen_coeff(fine, nf, coarse, nc, averaging);
+}
 
 // ---- transfers ---------------------------------------------------------------------
 void oracle_restriction_f(float *c, u64 nc, const float *f, u64 nf, const u64 *mask) { restriction(c, nc, f, nf, mask); }

@@ -87,3 +87,24 @@ def random_band_csr(n: int, seed: int, max_len: int = 27, band: int = 4096, dtyp
     vals = rng.standard_normal(nnz).astype(dtype)
     vals[vals == 0] = dtype(1.0)
     return rp, cols.astype(np.uint64), vals
+
+
+def lognormal_cells(seed: int, cells: int, sigma: float = 1.0) -> np.ndarray:
+    """exp(sigma * z), z ~ N(0,1) by Box-Muller from draws 2i (u1 = 1 - U)
+    and 2i+1 -- the device generator's formula (transcendentals may differ
+    from the device's in the last bit)."""
+    u = uniform01(seed, 2 * cells)
+    z = np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(6.283185307179586 * u[1::2])
+    return np.exp(sigma * z)
+
+
+def poisson_system_seeds(base: int, i: int):
+    """(coefficient seed, right-hand-side seed) of system i of a workload."""
+    return (base << 32) + 2 * i, (base << 32) + 2 * i + 1
+
+
+def batch_levels(nsys: int, seed: int, kmin: int = 5, kmax: int = 9) -> np.ndarray:
+    """Finest level k of each system, uniform over {kmin..kmax} (side 2^k+1,
+    log-uniform over the valid geometric-multigrid sides)."""
+    u = uniform01(seed, nsys)
+    return kmin + np.minimum((u * (kmax - kmin + 1)).astype(np.int64), kmax - kmin)

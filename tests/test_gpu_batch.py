@@ -1,0 +1,226 @@
+"""GPU tests of the batched multigrid solver (honei_hip_batch_*).
+
+The batch has no reference counterpart; its per-system numerics are those of
+the single-system MIXED driver (math/multigrid.hh:2395-2536 as restated by
+MGHierarchy::solve_mixed), so the bars are:
+  * threads-per-row 1 (autotune off): every slot's solution, outer iteration
+    count and relative defect are bit-identical to the single-system solve
+    of the same system;
+  * a system's result does not depend on the other systems in the batch or
+    on slot reuse (bit-identical alone and in the mix, with autotuning on);
+  * each solution's FP64 residual, computed on the host by the oracle with
+    the device-assembled FP64 operator, is below the tolerance.
+"""
+import numpy as np
+import pytest
+
+import honei_amd as H
+import oracle
+from tests.patterns import lognormal_cells, poisson_system_seeds, uniform01, uniform_pm1
+
+pytestmark = pytest.mark.gpu
+
+OMEGA, AVG, TOL = 0.8, 1, 1e-8
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    H.init(0)
+    yield
+
+
+def _system(level, i, base=11):
+    side = 2 ** level + 1
+    cs, bs = poisson_system_seeds(base, i)
+    return lognormal_cells(cs, (side - 1) ** 2), uniform_pm1(bs, side * side)
+
+
+def _fp64_operator(side, kappa):
+    n = side * side
+    stride = 32 * ((n + 31) // 32)
+    Aj = H.DeviceArray(9 * stride, np.int32)
+    Ax = H.DeviceArray(9 * stride, np.float64)
+    Arl = H.DeviceArray(n, np.uint32)
+    dk = H.DeviceArray.from_host(kappa, np.float64)
+    H.call("honei_hip_q1_assemble", H.F64, side, Aj.ptr, Ax.ptr, Arl.ptr, stride, dk.ptr, None, 0.0, 1, None)
+    return oracle.Ell(n, n, stride, 9, 1, Aj.get().astype(np.uint64), Ax.get(), Arl.get().astype(np.uint64))
+
+
+def _single(level, kappa, b, inner=H.F32):
+    n = (2 ** level + 1) ** 2
+    mg = H.Multigrid(inner, 1, level)
+    mg.configure(H.mg_config(n_max_iter=1, tolerance=TOL, zero_inner_start=True, initial_zero=True,
+                             max_outer=200))
+    mg.build_q1_coeff(H.DeviceArray.from_host(kappa, np.float64), OMEGA, AVG)
+    db, dx = H.DeviceArray.from_host(b), H.DeviceArray(n, np.float64)
+    dx.zero()
+    outer, _, rd = mg.solve_mixed(db, dx, check=False)
+    return outer, rd, dx.get()
+
+
+def _run_batch(levels, counts, systems, **cfg):
+    """systems: {(class, j): (kappa, b)}; returns {(class, j): (info, x)}."""
+    B = H.Batch(levels, counts, H.batch_config(**cfg))
+    keep = []
+    for (c, j), (kappa, b) in systems.items():
+        dk, db = H.DeviceArray.from_host(kappa, np.float64), H.DeviceArray.from_host(b)
+        keep += [dk, db]
+        B.load(B.slot(c, j), dk, db)
+    launches = B.run()
+    info = B.poll()
+    out = {}
+    for (c, j), (kappa, b) in systems.items():
+        g = B.slot(c, j)
+        x = H.DeviceArray(len(b), np.float64)
+        B.result(g, x)
+        i = info[g]
+        out[(c, j)] = ((i.state, i.iters, i.status, i.rel, i.level), x.get())
+    return out, launches, B
+
+
+def test_fill_fields_match_host_generators():
+    n = 10007
+    u = H.DeviceArray(n, np.float64)
+    H.fill_uniform(u, -1.0, 1.0, 1234)
+    assert np.array_equal(u.get(), uniform_pm1(1234, n))
+    k = H.DeviceArray(n, np.float64)
+    H.fill_lognormal(k, 1.0, 99)
+    np.testing.assert_allclose(k.get(), lognormal_cells(99, n), rtol=1e-13)
+
+
+def test_batch_bitexact_with_single_system_solves():
+    """Autotune off (T = 1, block 256): each slot equals the single-system
+    MIXED solve bit for bit -- outer iterations, relative defect, x."""
+    levels, counts = [3, 5, 4], [2, 2, 3]
+    systems = {(c, j): _system(levels[c], 10 * c + j) for c in range(3) for j in range(counts[c])}
+    res, launches, B = _run_batch(levels, counts, systems, autotune=0, iters_per_launch=3)
+    assert B.shape(5) == (1, 256)
+    for (c, j), ((state, iters, status, rel, level), x) in res.items():
+        assert (state, status, level) == (0, 0, levels[c])
+        outer, rd, xs = _single(levels[c], *systems[(c, j)])
+        assert iters == outer and rel == rd, (c, j, iters, outer, rel, rd)
+        assert np.array_equal(x, xs), (c, j, np.abs(x - xs).max())
+    assert launches == -(-max(r[0][1] + 1 for r in res.values()) // 3)
+
+
+def test_batch_residual_fp64_against_oracle():
+    """Every slot's FP64 residual, recomputed on the host, is < 1e-8 relative."""
+    levels, counts = [6, 4], [2, 2]
+    systems = {(c, j): _system(levels[c], 100 + 10 * c + j) for c in range(2) for j in range(counts[c])}
+    res, _, _ = _run_batch(levels, counts, systems)
+    for (c, j), ((state, iters, status, rel, level), x) in res.items():
+        side = 2 ** levels[c] + 1
+        kappa, b = systems[(c, j)]
+        A = _fp64_operator(side, kappa)
+        guess = np.zeros_like(b)
+        bd = np.zeros(side * side, bool)
+        bd[:side] = bd[-side:] = True
+        bd[::side] = True
+        guess[bd] = b[bd]
+        r0 = np.linalg.norm(oracle.defect(A, b, guess))
+        r = np.linalg.norm(oracle.defect(A, b, x))
+        assert status == 0 and rel < TOL
+        assert r / r0 < 1.001 * TOL, (c, j, r / r0)
+
+
+def test_batch_alone_equals_in_the_mix():
+    """Continuous batching invariant: a system's result is the same solved
+    alone and solved in a mixed batch (autotuned launch shapes)."""
+    target = _system(5, 7)
+    alone, _, _ = _run_batch([5], [1], {(0, 0): target})
+    others = {(c, j): _system(lv, 200 + 10 * c + j) for c, lv in enumerate([6, 5, 3]) for j in range(2)}
+    others[(1, 1)] = target
+    mix, _, _ = _run_batch([6, 5, 3], [2, 2, 2], others)
+    (a_info, a_x), (m_info, m_x) = alone[(0, 0)], mix[(1, 1)]
+    assert a_info == m_info
+    assert np.array_equal(a_x, m_x)
+
+
+def test_batch_slot_reuse_and_late_admission():
+    """A slot reloaded after its first system retired, while another slot is
+    still running, solves the new system exactly as a fresh batch does."""
+    sys_a, sys_b, sys_c = _system(4, 1), _system(4, 2), _system(6, 3)
+    B = H.Batch([4, 6], [1, 1], H.batch_config(iters_per_launch=1))
+    ga, gc = B.slot(0, 0), B.slot(1, 0)
+    keep = [H.DeviceArray.from_host(a, np.float64) for a in (*sys_a, *sys_c)]
+    B.load(ga, keep[0], keep[1])
+    B.load(gc, keep[2], keep[3])
+    while True:   # until slot a has been admitted and retired
+        B.step()
+        info = B.poll()
+        if info[ga].status != -1:
+            break
+    assert info[gc].state != 0, "the level-6 system should still be running"
+    xa = H.DeviceArray(17 * 17, np.float64)
+    B.result(ga, xa)
+    kb = [H.DeviceArray.from_host(a, np.float64) for a in sys_b]
+    B.load(ga, kb[0], kb[1])
+    B.run()
+    info = B.poll()
+    xb = H.DeviceArray(17 * 17, np.float64)
+    B.result(ga, xb)
+    fresh_a, _, _ = _run_batch([4], [1], {(0, 0): sys_a}, iters_per_launch=1)
+    fresh_b, _, _ = _run_batch([4], [1], {(0, 0): sys_b}, iters_per_launch=1)
+    assert np.array_equal(xa.get(), fresh_a[(0, 0)][1])
+    assert np.array_equal(xb.get(), fresh_b[(0, 0)][1])
+    assert (info[ga].iters, info[ga].rel) == fresh_b[(0, 0)][0][1:4:2]
+
+
+def test_batch_not_converged_and_budget():
+    res, _, _ = _run_batch([5], [1], {(0, 0): _system(5, 9)}, max_outer=2, iters_per_launch=4)
+    (state, iters, status, rel, level), _ = res[(0, 0)]
+    assert (state, iters, status) == (0, 2, 5) and rel > TOL
+    with pytest.raises(H.HipError) as e:
+        H.Batch([9], [4], H.batch_config(), budget_bytes=1 << 20)
+    assert e.value.code == 6
+    with pytest.raises(H.HipError) as e:
+        H.Batch([1], [1], H.batch_config())
+    assert e.value.code == 1
+
+
+def test_batch_bf16_inner_converges():
+    """BF16 matrix values in the V-cycle (FP32 vectors and arithmetic): the
+    FP64 outer loop still reaches 1e-8."""
+    levels, counts = [6, 5], [1, 2]
+    systems = {(c, j): _system(levels[c], 300 + 10 * c + j) for c in range(2) for j in range(counts[c])}
+    res, _, _ = _run_batch(levels, counts, systems, inner_vtype=H.BF16)
+    f32, _, _ = _run_batch(levels, counts, systems)
+    for key, ((state, iters, status, rel, level), x) in res.items():
+        assert status == 0 and rel < TOL
+        x32 = f32[key][1]
+        assert np.abs(x - x32).max() < 1e-5 * max(1.0, np.abs(x32).max())
+
+
+def test_serve_same_alone_and_in_the_mix():
+    """Continuous batching: a request's solution is bit-identical when served
+    alone and when it arrives while other requests of several sizes are
+    already running (and slots are reused)."""
+    levels = [4, 5, 6]
+    target = (0.0, 5, 777, 778)
+    xa = H.DeviceArray(33 * 33, np.float64)
+    res, st = H.serve(levels, [target + (xa,)], counts=[2, 2, 2])
+    assert res[0].status == 0 and st.completed == 1
+    mix = []
+    for i in range(12):
+        lv = levels[i % 3]
+        mix.append((0.002 * i, lv, 1000 + 2 * i, 1001 + 2 * i))
+    xm = H.DeviceArray(33 * 33, np.float64)
+    mix.insert(7, (0.002 * 6.5, 5, 777, 778, xm))
+    res, st = H.serve(levels, mix, counts=[2, 2, 2])
+    assert st.completed == len(mix) and st.rejected == 0 and st.not_converged == 0
+    for r, q in zip(res, mix):
+        assert r.status == 0 and r.rel < TOL
+        assert q[0] <= r.admit_s <= r.done_s
+    assert np.array_equal(xa.get(), xm.get())
+    assert st.p50_ms <= st.p99_ms <= st.max_ms
+
+
+def test_serve_budget_and_unknown_size():
+    with pytest.raises(H.HipError) as e:
+        H.serve([9], [(0.0, 9, 1, 2)], counts=[8], budget_bytes=64 << 20)
+    assert e.value.code == 6
+    res, st = H.serve([4], [(0.0, 4, 1, 2), (0.0, 7, 3, 4), (0.001, 4, 5, 6)], budget_bytes=1 << 30,
+                      max_slots_per_class=1)
+    assert [r.status for r in res] == [0, -2, 0]
+    assert st.completed == 2 and st.rejected == 1
+    assert res[2].admit_s >= res[0].done_s - 1e-9   # one slot: the second level-4 request waits for it

@@ -29,16 +29,17 @@ def _init():
     yield
 
 
-def _device_q1(side, vt, coeff=None, omega=0.7):
+def _device_q1(side, vt, coeff=None, omega=0.7, threads=1):
     n = side * side
-    stride = 32 * ((n + 31) // 32)
-    Aj = H.DeviceArray(9 * stride, np.int32)
-    Ax = H.DeviceArray(9 * stride, {H.F32: np.float32, H.F64: np.float64, H.BF16: np.uint16}[vt])
+    stride = 32 * ((n * threads + 31) // 32)
+    ncpr = (9 + threads - 1) // threads
+    Aj = H.DeviceArray(ncpr * stride, np.int32)
+    Ax = H.DeviceArray(ncpr * stride, {H.F32: np.float32, H.F64: np.float64, H.BF16: np.uint16}[vt])
     Arl = H.DeviceArray(n, np.uint32)
     dinv = H.DeviceArray(n, np.float64 if vt == H.F64 else np.float32)
     dc = H.DeviceArray.from_host(coeff, np.float64) if coeff is not None else None
     H.call("honei_hip_q1_assemble", vt, side, Aj.ptr, Ax.ptr, Arl.ptr, stride, None if dc is None else dc.ptr,
-           dinv.ptr, omega, None)
+           dinv.ptr, omega, threads, None)
     return Aj.get(), Ax.get(), Arl.get(), dinv.get(), stride
 
 
@@ -52,10 +53,21 @@ def test_q1_assembly_bitexact(side, dtype):
     assert np.array_equal(Aj.astype(np.uint64), A.Aj)
     assert np.array_equal(Ax.view(np.uint8), A.Ax.view(np.uint8))
     assert np.array_equal(Arl.astype(np.uint64), A.Arl)
-    H_ = oracle.q1_hierarchy(1, min_level=1, dtype=dtype) if side == 3 else None
     d = oracle.q1_bands(side, dtype)[4]
     expect = ((np.ones(side * side, dtype) / d).astype(dtype) * dtype(0.7)).astype(dtype)
     assert np.array_equal(dinv, expect)
+
+
+@pytest.mark.parametrize("side", [3, 17, 65])
+@pytest.mark.parametrize("threads", [2, 4, 8])
+def test_q1_assembly_threads_layout_bitexact(side, threads):
+    """T-interleaved layout: the same arrays as SparseMatrixELL(SparseMatrix, T)."""
+    Aj, Ax, Arl, _, stride = _device_q1(side, H.F32, threads=threads)
+    A = oracle.q1_matrix(side, np.float32, threads=threads)
+    assert stride == A.stride and len(Aj) == A.ncpr * A.stride
+    assert np.array_equal(Aj.astype(np.uint64), A.Aj)
+    assert np.array_equal(Ax.view(np.uint8), A.Ax.view(np.uint8))
+    assert np.array_equal(Arl.astype(np.uint64), A.Arl)
 
 
 def _numpy_variable_q1(side, kappa):
