@@ -89,6 +89,10 @@ _SIGS = {
     "honei_hip_mg_build_q1_coeff": [_p, _d, _p, _i, _p],
     "honei_hip_mg_solve_fixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
     "honei_hip_mg_solve_mixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_d), _p],
+    "honei_hip_product_bmdv_q1_float": [_p] * 11 + [_sz, _sz, _sz, _p],
+    "honei_hip_product_bmdv_q1_double": [_p] * 11 + [_sz, _sz, _sz, _p],
+    "honei_hip_defect_q1_float": [_p] * 12 + [_sz, _sz, _sz, _p],
+    "honei_hip_defect_q1_double": [_p] * 12 + [_sz, _sz, _sz, _p],
     "honei_hip_fill_uniform": [_p, _sz, _d, _d, _u64, _p],
     "honei_hip_fill_lognormal": [_p, _sz, _d, _u64, _p],
     "honei_hip_batch_config_default": [_p],

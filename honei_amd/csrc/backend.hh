@@ -78,6 +78,11 @@ hipError_t vcopy(void *dst, const void *src, int64_t n, ValueType vt, hipStream_
 hipError_t convert_d2f(float *dst, const double *src, int64_t n, hipStream_t s);
 hipError_t convert_f2d(double *dst, const float *src, int64_t n, hipStream_t s);
 hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
+
+// Banded Q1 product y = A x (rhs null) or defect y = rhs - A x; bands[9] =
+// LL, LD, LU, DL, DD, DU, UL, UD, UU device arrays of n = m*m values.
+hipError_t q1_banded(ValueType vt, const void *const *bands, const void *x, const void *rhs, void *y, int64_t n,
+                     int64_t m, int block, hipStream_t s);
 // x += alpha*y and return ||x||^2 (fused; alpha from device memory when non-null)
 hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_dev, int64_t n,
                       ValueType vt, double *out, const ReduceSlots &slots, hipStream_t s);

@@ -53,6 +53,8 @@ void BatchMG::release() {
     for (hipEvent_t e : load_ev_)
         if (e) (void)hipEventDestroy(e);
     load_ev_.clear();
+    if (null_ev_) (void)hipEventDestroy(null_ev_);
+    null_ev_ = nullptr;
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
     if (sc_host_) (void)hipHostFree(sc_host_);
@@ -241,6 +243,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
 
     HX(hipStreamCreateWithFlags(&s_main_, hipStreamNonBlocking));
     HX(hipStreamCreateWithFlags(&s_load_, hipStreamNonBlocking));
+    HX(hipEventCreateWithFlags(&null_ev_, hipEventDisableTiming));
 
     L_.assign((size_t)max_level_ + 1, BLevel());
     for (int64_t l = 1; l <= max_level_; ++l) {
@@ -423,6 +426,10 @@ hipError_t BatchMG::load_impl(int64_t g, const double *coeff, const double *b, u
     // graph work finished before the poll that retired it, and result()
     // copies synchronously), and idle slots are touched by no kernel.
     hipStream_t s = s_load_;
+    if (coeff) {   // caller data produced on the legacy default stream comes first
+        HX(hipEventRecord(null_ev_, nullptr));
+        HX(hipStreamWaitEvent(s, null_ev_, 0));
+    }
     const int64_t cells_top = (1ll << Lc) * (1ll << Lc);
     double *kap = kappa_ + kappa_off_[(size_t)Lc];
     double *bx = c.b + ls * c.vld, *xx = c.x + ls * c.vld;

@@ -195,6 +195,7 @@ class BatchMG {
     int64_t running_ = 0, launches_ = 0;
     bool over_budget_ = false;
     std::vector<hipEvent_t> load_ev_;
+    hipEvent_t null_ev_ = nullptr;
 
     hipError_t alloc(void **p, size_t bytes);
     hipError_t load_impl(int64_t g, const double *coeff, const double *b, uint64_t coeff_seed, double sigma,

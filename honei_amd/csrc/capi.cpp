@@ -286,6 +286,39 @@ PRODUCT_ENTRY(honei_hip_product_smell_dv_bf16, VT_BF16)
 DEFECT_ENTRY(honei_hip_defect_smell_dv_float, VT_F32)
 DEFECT_ENTRY(honei_hip_defect_smell_dv_double, VT_F64)
 
+// ---- banded Q1 ------------------------------------------------------------------
+static int banded_call(ValueType vt, void *rhs, void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul,
+                       void *ud, void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m, void *stream,
+                       const char *what) {
+    if (size == 0) return HONEI_HIP_SUCCESS;
+    if (!x || !y) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector");
+    if (m == 0 || m * m != size) return fail(HONEI_HIP_ERROR_SIZE_MISMATCH, "size must be m*m");
+    if (blocksize && (blocksize % 64 || blocksize > 256))
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "blocksize: 0 or a multiple of 64 up to 256");
+    const void *bands[9] = {ll, ld, lu, dl, dd, du, ul, ud, uu};
+    for (const void *b : bands)
+        if (!b) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null band");
+    return check(q1_banded(vt, bands, x, rhs, y, (int64_t)size, (int64_t)m, (int)blocksize, S(stream)), what);
+}
+
+#define BANDED_PRODUCT_ENTRY(NAME, VT)                                                                    \
+    int NAME(void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul, void *ud, void *uu,     \
+             void *x, void *y, size_t size, size_t blocksize, size_t m, void *stream) {                    \
+        return banded_call(VT, nullptr, ll, ld, lu, dl, dd, du, ul, ud, uu, x, y, size, blocksize, m, stream, \
+                           #NAME);                                                                         \
+    }
+#define BANDED_DEFECT_ENTRY(NAME, VT)                                                                     \
+    int NAME(void *rhs, void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul, void *ud,    \
+             void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m, void *stream) {          \
+        if (size && !rhs) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null rhs");                       \
+        return banded_call(VT, rhs, ll, ld, lu, dl, dd, du, ul, ud, uu, x, y, size, blocksize, m, stream,  \
+                           #NAME);                                                                         \
+    }
+BANDED_PRODUCT_ENTRY(honei_hip_product_bmdv_q1_float, VT_F32)
+BANDED_PRODUCT_ENTRY(honei_hip_product_bmdv_q1_double, VT_F64)
+BANDED_DEFECT_ENTRY(honei_hip_defect_q1_float, VT_F32)
+BANDED_DEFECT_ENTRY(honei_hip_defect_q1_double, VT_F64)
+
 int honei_hip_defect_norm_smell_dv(int vtype, void *rhs, void *result, const void *Aj, const void *Ax,
                                    const void *Arl, void *x, size_t rows, size_t ncpr, size_t stride, size_t threads,
                                    double *norm2_dev, void *stream) {

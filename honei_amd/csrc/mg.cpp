@@ -369,7 +369,10 @@ hipError_t MGHierarchy::run_kernel(hipStream_t s, int64_t *inner_iters, double *
                 }
                 const int64_t post = cfg.post_smooth_uses_pre ? cfg.n_pre_smooth : cfg.n_post_smooth;
                 HX(sweeps(l, lv.x, lv.rhs, post, s));
-                if (!(skip_final && l == maxl)) HX(defect(l, lv.d, lv.rhs, lv.x, l == maxl ? nrm : nullptr, s));
+                // The reference also forms the defect after post-smoothing on
+                // the lower levels; nothing reads it (the next down-leg
+                // recomputes it first), so only the top level's is formed.
+                if (l == maxl && !skip_final) HX(defect(l, lv.d, lv.rhs, lv.x, nrm, s));
                 if (l == maxl) break;
             }
             if (skip_final) continue;   // loop ends: iter == n_max_iter

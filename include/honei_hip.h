@@ -96,6 +96,28 @@ int honei_hip_product_smell_dv_bf16(void *x, void *y, const void *Aj, const void
                                     size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
                                     size_t blocksize, size_t threads, void *stream);
 
+/* ---- banded Q1 operators ---------------------------------------------------
+ * BandedMatrixQx<Q1Type, DT>: nine device bands of `size` = m*m values
+ * (LL, LD, LU, DL, DD, DU, UL, UD, UU: coefficients towards columns i-m-1,
+ * i-m, i-m+1, i-1, i, i+1, i+m-1, i+m, i+m+1).  blocksize 0 = 256, else a
+ * multiple of 64 up to 256.  Each row is summed in the order of
+ * Product<tags::CPU>::value(BandedMatrixQx, DenseVector) (la/product.hh:
+ * 371-400), so results are bit-identical to the CPU path.
+ * Replaces cuda_product_bmdv_q1_{float,double}
+ * (backends/cuda/operations.hh:77-85): y = A x, and cuda_defect_q1_{float,
+ * double} (backends/cuda/operations.hh:31-39): y = rhs - A x.            */
+int honei_hip_product_bmdv_q1_float(void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul, void *ud,
+                                    void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m, void *stream);
+int honei_hip_product_bmdv_q1_double(void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul, void *ud,
+                                     void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m,
+                                     void *stream);
+int honei_hip_defect_q1_float(void *rhs, void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul,
+                              void *ud, void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m,
+                              void *stream);
+int honei_hip_defect_q1_double(void *rhs, void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul,
+                               void *ud, void *uu, void *x, void *y, size_t size, size_t blocksize, size_t m,
+                               void *stream);
+
 /* Replaces cuda_defect_smell_dv_{float,double}
  * (backends/cuda/operations.hh:41-47): result[r] = rhs[r] - (A b)[r]; same
  * argument order (rhs, result, Aj, Ax, Arl, b).                            */

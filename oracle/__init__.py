@@ -289,6 +289,19 @@ def banded_to_csr(bands):
     return rp, col, val
 
 
+def banded_q1(bands, x, rhs=None):
+    """Product<tags::CPU>::value(BandedMatrixQx<Q1Type>, x) (la/product.hh:
+    371-400), or the defect rhs - A x when rhs is given."""
+    bands = np.ascontiguousarray(bands)
+    n = bands.shape[1]
+    x = np.ascontiguousarray(x, bands.dtype)
+    y = np.zeros(n, bands.dtype)
+    r = None if rhs is None else np.ascontiguousarray(rhs, bands.dtype)
+    getattr(lib(), f"oracle_banded_q1_{_sfx(bands.dtype)}")(_u64(n), _ptr(bands), None if r is None else _ptr(r),
+                                                             _ptr(x), _ptr(y))
+    return y
+
+
 def q1_coeff_bands(side, kappa, dtype=np.float64):
     """Variable-coefficient Q1 operator as 9 bands (no reference counterpart;
     bilinear element stiffness per cell, Dirichlet rows identity)."""

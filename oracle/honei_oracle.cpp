@@ -257,6 +257,29 @@ void q1_bands(u64 N, DT *bands, int bc) {
     }
 }
 
+// Product<tags::CPU>::value(BandedMatrixQx<Q1Type>, DenseVector)
+// (la/product.hh:371-400): DD first, then LL, LD, LU, DL, DU, UL, UD, UU,
+// each term guarded by its column range; Defect<tags::CPU> on the banded
+// matrix is rhs - product (math/defect.hh:194-199).
+template <typename DT>
+void banded_q1(u64 N, const DT *bands, const DT *rhs, const DT *x, DT *y) {
+    const DT *b[9];
+    for (int k = 0; k < 9; ++k) b[k] = bands + (u64)k * N;
+    const long n = (long)N, root = (long)std::sqrt((double)N);
+    for (long i = 0; i < n; ++i) {
+        DT r = b[DD][i] * x[i];
+        if (i - root - 1 >= 0) r += b[LL][i] * x[i - root - 1];
+        if (i - root >= 0) r += b[LD][i] * x[i - root];
+        if (i - root + 1 >= 0) r += b[LU][i] * x[i - root + 1];
+        if (i - 1 >= 0) r += b[DL][i] * x[i - 1];
+        if (i + 1 < n) r += b[DU][i] * x[i + 1];
+        if (i + root - 1 < n) r += b[UL][i] * x[i + root - 1];
+        if (i + root < n) r += b[UD][i] * x[i + root];
+        if (i + root + 1 < n) r += b[UU][i] * x[i + root + 1];
+        y[i] = rhs ? rhs[i] - r : r;
+    }
+}
+
 // Variable-coefficient Q1 operator (no reference counterpart; the device
 // assembly in honei_amd/csrc/q1_kernels.hip follows the same formulas):
 // kappa per cell ((side-1)^2, row-major), bilinear element stiffness
@@ -805,6 +828,12 @@ void oracle_q1_bands_f(u64 N, float *bands, int bc) { q1_bands(N, bands, bc); }
 void oracle_q1_bands_d(u64 N, double *bands, int bc) { q1_bands(N, bands, bc); }
 u64 oracle_banded_q1_to_csr_f(u64 N, const float *bands, u64 *rp, u64 *col, float *val) { return banded_q1_to_csr(N, bands, rp, col, val); }
 u64 oracle_banded_q1_to_csr_d(u64 N, const double *bands, u64 *rp, u64 *col, double *val) { return banded_q1_to_csr(N, bands, rp, col, val); }
+void oracle_banded_q1_f(u64 N, const float *bands, const float *rhs, const float *x, float *y) {
+    banded_q1(N, bands, rhs, x, y);
+}
+void oracle_banded_q1_d(u64 N, const double *bands, const double *rhs, const double *x, double *y) {
+    banded_q1(N, bands, rhs, x, y);
+}
 void oracle_q1_coeff_bands_f(u64 side, const double *kappa, float *bands) { q1_coeff_bands(side, kappa, bands); }
 void oracle_q1_coeff_bands_d(u64 side, const double *kappa, double *bands) { q1_coeff_bands(side, kappa, bands); }
 void oracle_coarsen_coeff(const double *fine, u64 nf, double *coarse, u64 nc, int averaging) {

@@ -153,6 +153,20 @@ def prolongation(coarse, mask=(2,) * 8):
     return f
 
 
+def banded_q1(bands, x, rhs=None):
+    """The reference's Product<tags::CPU>::value(BandedMatrixQx<Q1Type>, x), or
+    Defect<tags::CPU> (rhs - A x) when rhs is given; bands[9][n] in the order
+    LL, LD, LU, DL, DD, DU, UL, UD, UU."""
+    bands = np.ascontiguousarray(bands)
+    n = bands.shape[1]
+    s = "f" if bands.dtype == np.float32 else "d"
+    x = np.ascontiguousarray(x, bands.dtype)
+    y = np.zeros(n, bands.dtype)
+    r = None if rhs is None else np.ascontiguousarray(rhs, bands.dtype)
+    getattr(lib(), f"ref_banded_q1_{s}")(_u64(n), _ptr(bands), None if r is None else _ptr(r), _ptr(x), _ptr(y))
+    return y
+
+
 def mg_mixed(max_level, b, x0, n_max_iter=4, n_pre=2, tol=1e-8, adapt=1.0, timed=False):
     """Multigrid<CPU, CPU, NONE, JAC, V, MIXED>::value of the reference on the
     Dirichlet Q1 problem; with timed=True also returns the seconds spent in

@@ -7,6 +7,7 @@
 // honei_oracle.cpp and bench.py may time it as the CPU baseline.  No
 // reference source is copied here: this file only calls the reference's API.
 
+#include <honei/la/banded_matrix_qx.hh>
 #include <honei/la/dense_vector.hh>
 #include <honei/la/product.hh>
 #include <honei/la/sparse_matrix.hh>
@@ -90,6 +91,29 @@ struct CoutCapture {
 
 }  // namespace
 
+// ---- banded Q1: Product<tags::CPU>::value(BandedMatrixQx, DenseVector) and
+// Defect<tags::CPU> on it; bands[k*n + i], k = LL, LD, LU, DL, DD, DU, UL, UD, UU
+template <typename DT>
+static void banded_apply(u64 n, const DT *bands, const DT *rhs, const DT *x, DT *y) {
+    std::vector<DenseVector<DT>> b;
+    for (int k = 0; k < 9; ++k) {
+        DenseVector<DT> v(n);
+        for (u64 i = 0; i < n; ++i) v[i] = bands[(u64)k * n + i];
+        b.push_back(v);
+    }
+    BandedMatrixQx<Q1Type, DT> A(n, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8]);
+    DenseVector<DT> xv(n), yv(n, DT(0));
+    for (u64 i = 0; i < n; ++i) xv[i] = x[i];
+    if (rhs) {
+        DenseVector<DT> rv(n);
+        for (u64 i = 0; i < n; ++i) rv[i] = rhs[i];
+        Defect<tags::CPU>::value(yv, rv, A, xv);
+    } else {
+        Product<tags::CPU>::value(yv, A, xv);
+    }
+    for (u64 i = 0; i < n; ++i) y[i] = yv[i];
+}
+
 extern "C" {
 
 // ---- ELL handles ----------------------------------------------------------
@@ -133,6 +157,13 @@ void ref_ell_copy_d(void *h, u64 *Aj, double *Ax, u64 *Arl) {
 }
 void ref_ell_free_f(void *h) { delete (SparseMatrixELL<float> *)h; }
 void ref_ell_free_d(void *h) { delete (SparseMatrixELL<double> *)h; }
+
+void ref_banded_q1_f(u64 n, const float *bands, const float *rhs, const float *x, float *y) {
+    banded_apply(n, bands, rhs, x, y);
+}
+void ref_banded_q1_d(u64 n, const double *bands, const double *rhs, const double *x, double *y) {
+    banded_apply(n, bands, rhs, x, y);
+}
 
 // ---- operators on handles ---------------------------------------------------
 void ref_product_f(void *h, const float *x, float *y) {

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <map>
 #include <cstdlib>
 #include <functional>
 #include <sstream>
@@ -26,6 +27,7 @@
 #include "honei/la/scale.hh"
 #include "honei/la/scaled_sum.hh"
 #include "honei/la/sum.hh"
+#include "honei/math/batch_solver.hh"
 #include "honei/math/cg.hh"
 #include "honei/math/defect.hh"
 #include "honei/math/jacobi.hh"
@@ -386,6 +388,129 @@ void mg_solver_quad() {
     TEST_CHECK(err < 1e-5);
 }
 
+// ---- batched multigrid and the request loop (no reference counterpart) -------
+
+// Per-cell coefficient exp(sin(i) + cos(3 j)) (smooth, contrast ~e^2) and the
+// FP64 variable-coefficient Q1 operator assembled on the host from element
+// stiffness matrices (independent of the device assembly).
+static DenseVector<double> coefficient(unsigned long side, double shift) {
+    const unsigned long nc = side - 1;
+    DenseVector<double> k(nc * nc);
+    for (unsigned long i = 0; i < nc; ++i)
+        for (unsigned long j = 0; j < nc; ++j) k[i * nc + j] = std::exp(std::sin(0.7 * i + shift) + std::cos(3.1 * j));
+    return k;
+}
+
+static SparseMatrixELL<double> host_operator(unsigned long side, const DenseVector<double> &k) {
+    const double K[4][4] = {{4, -1, -2, -1}, {-1, 4, -1, -2}, {-2, -1, 4, -1}, {-1, -2, -1, 4}};
+    const unsigned long n = side * side, nc = side - 1;
+    std::vector<std::map<unsigned long, double>> rows(n);
+    for (unsigned long ci = 0; ci < nc; ++ci)
+        for (unsigned long cj = 0; cj < nc; ++cj) {
+            const unsigned long nd[4] = {ci * side + cj, ci * side + cj + 1, (ci + 1) * side + cj + 1,
+                                         (ci + 1) * side + cj};
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) rows[nd[a]][nd[b]] += k[ci * nc + cj] * K[a][b] / 6.0;
+        }
+    SparseMatrix<double> sm(n, n);
+    for (unsigned long r = 0; r < n; ++r) {
+        const unsigned long i = r / side, j = r % side;
+        if (i == 0 || j == 0 || i == side - 1 || j == side - 1) {
+            sm(r, r, 1.0);
+            continue;
+        }
+        for (auto &kv : rows[r]) sm(r, kv.first, kv.second);
+    }
+    return SparseMatrixELL<double>(sm);
+}
+
+static void batch_solver() {
+    const std::vector<unsigned long> levels = {3, 5, 4};
+    BatchConfig cfg;
+    cfg.autotune = 0;
+    BatchSolver<tags::GPU::HIP> solver(levels, {1, 2, 1}, cfg);
+    TEST_CHECK(solver.slots() == 4 && solver.device_bytes() > 0);
+    struct Sys {
+        unsigned long level, slot;
+        DenseVector<double> k, b;
+    };
+    std::vector<Sys> sys;
+    const unsigned long js[3] = {1, 2, 1};
+    for (unsigned c = 0; c < 3; ++c)
+        for (unsigned long j = 0; j < js[c]; ++j) {
+            const unsigned long side = (1ul << levels[c]) + 1;
+            DenseVector<double> b(side * side);
+            for (unsigned long i = 0; i < side * side; ++i) b[i] = std::sin(0.37 * i + c + j);
+            sys.push_back({levels[c], solver.slot(c, j), coefficient(side, 0.3 * c + j), b});
+        }
+    for (auto &s : sys) solver.load(s.slot, s.k, s.b);
+    const unsigned long launches = solver.run();
+    TEST_CHECK(launches > 0);
+    std::vector<SlotInfo> info = solver.poll();
+    for (auto &s : sys) {
+        const SlotInfo &in = info[s.slot];
+        TEST_CHECK(in.state == 0 && in.status == 0 && in.level == (int32_t)s.level && in.rel < 1e-8);
+        const unsigned long side = (1ul << s.level) + 1, n = side * side;
+        DenseVector<double> x(n);
+        solver.result(s.slot, x);
+        // FP64 residual with the host-assembled operator, against the start
+        // vector's (first row, last row, first column = b)
+        SparseMatrixELL<double> A = host_operator(side, s.k);
+        DenseVector<double> x0(n, 0.0);
+        for (unsigned long i = 0; i < n; ++i)
+            if (i < side || i >= n - side || i % side == 0) x0[i] = s.b[i];
+        auto rnorm = [&](const DenseVector<double> &v) {
+            double acc = 0.0;
+            for (unsigned long r = 0; r < n; ++r) {
+                double y = 0.0;
+                for (unsigned long c = 0; c < n; ++c) {
+                    const double a = A(r, c);
+                    if (a != 0.0) y += a * v[c];
+                }
+                acc += (s.b[r] - y) * (s.b[r] - y);
+            }
+            return std::sqrt(acc);
+        };
+        TEST_CHECK(rnorm(x) / rnorm(x0) < 1.001e-8);
+    }
+    // the same system alone in a fresh pool: bit-identical solution
+    BatchSolver<tags::GPU::HIP> alone({5}, {1}, cfg);
+    alone.load(alone.slot(0, 0), sys[1].k, sys[1].b);
+    alone.run();
+    DenseVector<double> xa(33 * 33), xm(33 * 33);
+    alone.result(alone.slot(0, 0), xa);
+    solver.result(sys[1].slot, xm);
+    for (unsigned long i = 0; i < 33 * 33; ++i) TEST_CHECK(xa[i] == xm[i]);
+}
+
+static void serve_loop() {
+    honei_hip_serve_config cfg{};
+    hip_check(honei_hip_batch_config_default(&cfg.batch), "config");
+    cfg.nclass = 2;
+    cfg.levels[0] = 4;
+    cfg.levels[1] = 5;
+    cfg.counts[0] = 2;
+    cfg.counts[1] = 1;
+    cfg.sigma = 1.0;
+    std::vector<honei_hip_request> reqs;
+    for (int i = 0; i < 9; ++i) {
+        honei_hip_request r{};
+        r.arrival_s = 0.001 * i;
+        r.level = (i % 3 == 2) ? 5 : 4;
+        r.coeff_seed = 100 + 2 * i;
+        r.rhs_seed = 101 + 2 * i;
+        reqs.push_back(r);
+    }
+    std::vector<honei_hip_request_result> res;
+    const honei_hip_serve_stats st = ServeLoop::run(cfg, reqs, res);
+    TEST_CHECK(st.completed == 9 && st.rejected == 0 && st.not_converged == 0);
+    for (size_t i = 0; i < res.size(); ++i) {
+        TEST_CHECK(res[i].status == 0 && res[i].rel < 1e-8);
+        TEST_CHECK(res[i].admit_s >= reqs[i].arrival_s && res[i].done_s >= res[i].admit_s);
+    }
+    TEST_CHECK(st.p50_ms <= st.p99_ms && st.p99_ms <= st.max_ms);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -413,6 +538,8 @@ int main(int argc, char **argv) {
         {"cg_solver<double>", cg_solver},
         {"multigrid_mixed", multigrid_mixed},
         {"mg_solver_quad_w_cycle", mg_solver_quad},
+        {"batch_solver", batch_solver},
+        {"serve_loop", serve_loop},
     };
     int failed = 0, run = 0;
     for (auto &t : tests) {

@@ -350,3 +350,40 @@ def test_full_size_spmv_configs(dtype):
     R = _device_random_ell(1 << 22, vt)
     xr = uniform_pm1(2, R.rows).astype(dtype)
     assert np.array_equal(_gpu_product(R, xr), oracle.product(R, xr))
+
+
+# ---- banded Q1 product / defect --------------------------------------------------------
+
+@pytest.mark.parametrize("side", [3, 33, 1025, 2049])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_banded_q1_bitexact(side, dtype):
+    """Product<tags::CPU>(BandedMatrixQx) order reproduced: bit-identical to
+    the oracle (itself bit-identical to the reference build); lognormal
+    coefficient bands at the smaller sizes, the Poisson bands at all."""
+    from tests.patterns import lognormal_cells
+    n = side * side
+    sfx = "float" if dtype == np.float32 else "double"
+    x = uniform_pm1(side, n).astype(dtype)
+    rhs = uniform_pm1(side + 1, n).astype(dtype)
+    cases = [oracle.q1_bands(side, dtype)]
+    if side <= 33:
+        cases.append(oracle.q1_coeff_bands(side, lognormal_cells(side, (side - 1) ** 2), dtype))
+    for bands in cases:
+        db = [H.DeviceArray.from_host(np.ascontiguousarray(bands[k]), dtype) for k in range(9)]
+        dx, dr = H.DeviceArray.from_host(x, dtype), H.DeviceArray.from_host(rhs, dtype)
+        dy = H.DeviceArray(n, dtype)
+        H.call(f"honei_hip_product_bmdv_q1_{sfx}", *[b.ptr for b in db], dx.ptr, dy.ptr, n, 0, side, None)
+        assert np.array_equal(dy.get().view(np.uint8), oracle.banded_q1(bands, x).view(np.uint8))
+        H.call(f"honei_hip_defect_q1_{sfx}", dr.ptr, *[b.ptr for b in db], dx.ptr, dy.ptr, n, 128, side, None)
+        assert np.array_equal(dy.get().view(np.uint8), oracle.banded_q1(bands, x, rhs).view(np.uint8))
+
+
+def test_banded_q1_invalid_arguments():
+    n = 25
+    d = [H.DeviceArray(n, np.float32) for _ in range(11)]
+    with pytest.raises(H.HipError) as e:
+        H.call("honei_hip_product_bmdv_q1_float", *[a.ptr for a in d], 24, 0, 5, None)
+    assert e.value.code == 2
+    with pytest.raises(H.HipError) as e:
+        H.call("honei_hip_product_bmdv_q1_float", *[a.ptr for a in d[:9]], d[9].ptr, d[10].ptr, n, 96, 5, None)
+    assert e.value.code == 1

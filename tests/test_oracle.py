@@ -442,3 +442,27 @@ def test_mg_mixed_lognormal_hierarchy_converges():
     bd[::side] = True
     r0 = np.linalg.norm(oracle.defect(A64, b, np.where(bd, b, 0.0)))
     assert np.linalg.norm(oracle.defect(A64, b, x)) / r0 < 1.001e-8
+
+
+# ---- banded Q1 product / defect (la/product.hh:371-400, math/defect.hh:194-199) ------
+
+@pytest.mark.parametrize("side", [3, 5, 33])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_banded_q1_vs_reference_and_ell(side, dtype):
+    from tests.patterns import lognormal_cells
+    x = uniform_pm1(side, side * side).astype(dtype)
+    rhs = uniform_pm1(side + 7, side * side).astype(dtype)
+    for bands in (oracle.q1_bands(side, dtype), oracle.q1_coeff_bands(side, lognormal_cells(side, (side - 1) ** 2),
+                                                                       dtype)):
+        y = oracle.banded_q1(bands, x)
+        r = oracle.banded_q1(bands, x, rhs)
+        if ref.available():
+            assert np.array_equal(y.view(np.uint8), ref.banded_q1(bands, x).view(np.uint8))
+            assert np.array_equal(r.view(np.uint8), ref.banded_q1(bands, x, rhs).view(np.uint8))
+        # same operator as the ELL conversion, different summation order
+        rp, col, val = oracle.banded_to_csr(bands)
+        E = oracle.ell_from_sm(side * side, side * side, rp, col, val)
+        ye = oracle.product(E, x)
+        tol = 4 * np.finfo(dtype).eps * (np.abs(bands).sum(0) * np.abs(x).max() + 1e-30)
+        assert np.all(np.abs(y - ye) <= tol)
+        assert np.array_equal(r, (rhs - y).astype(dtype))
