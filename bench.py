@@ -114,6 +114,35 @@ def time_spmv(D, reps, x, y, label):
             "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
 
 
+def time_banded_q1(side, dtype, reps=50):
+    """Banded Q1 product (BandedMatrixQx, no index stream): average launch
+    time and algorithmic bytes 9n*s_v + n*(s_x + s_y)."""
+    n = side * side
+    sv = np.dtype(dtype).itemsize
+    sfx = "float" if dtype == np.float32 else "double"
+    bands = [H.DeviceArray(n, dtype) for _ in range(9)]
+    for k, b in enumerate(bands):
+        b.zero()   # values do not change the traffic
+    x = H.DeviceArray.from_host(uniform_pm1(2, n).astype(dtype))
+    y = H.DeviceArray(n, dtype)
+    args = [b.ptr for b in bands] + [x.ptr, y.ptr, n, 0, side, None]
+    fn = f"honei_hip_product_bmdv_q1_{sfx}"
+    for _ in range(3):
+        H.call(fn, *args)
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record()
+    for _ in range(reps):
+        H.call(fn, *args)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    alg = 9 * n * sv + 2 * n * sv
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"[spmv] banded q1 {side}^2 {sfx}: {ms * 1e3:.1f} us/launch, {alg / 1e6:.1f} MB, {gbs:.0f} GB/s")
+    return {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
+            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
+
+
 def read_traffic():
     """HBM bytes per launch of the roofline kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
@@ -320,6 +349,8 @@ def main():
     y64 = H.DeviceArray(n, np.float64)
     spmv["q1_2049_f64"] = time_spmv(D64, 50, x64, y64, "q1 2049^2 f64")
     del D64, x64, y64
+    spmv["q1_2049_banded_f32"] = time_banded_q1(SIDE, np.float32)
+    spmv["q1_2049_banded_f64"] = time_banded_q1(SIDE, np.float64)
     for vt, dt, key in ((H.F32, np.float32, "random_4m_f32"), (H.F64, np.float64, "random_4m_f64")):
         R = device_random_ell(1 << 22, vt)
         xr = H.DeviceArray.from_host(uniform_pm1(2, R.rows).astype(dt))
