@@ -395,6 +395,20 @@ def main():
         log(f"[fp64] {n64} single-system FP64 solves in {t64:.3f} s ({n64 / t64:.1f} solves/s)")
     batch_kernel = batch_kernel_roofline(B, max(levels))
     del B
+    # the same step with BF16-stored V-cycle matrices (FP32 arithmetic)
+    Bb, slots_b = batched_pool(levels, inner_vtype=H.BF16)
+    batched_step(Bb, slots_b, kap, rhs)
+    tb = []
+    for _ in range(max(1, args.steps)):
+        sec, _done, _nl, info_b = batched_step(Bb, slots_b, kap, rhs)
+        tb.append(sec)
+    it_b = np.array([info_b[g].iters for g in slots_b])
+    bf16 = {"solves_per_s": round(len(levels) / float(np.median(tb)), 2),
+            "outer_iterations_median": int(np.median(it_b)), "outer_iterations_max": int(it_b.max()),
+            "max_rel_residual": float(max(info_b[g].rel for g in slots_b)),
+            "all_converged": bool(all(info_b[g].status == 0 for g in slots_b))}
+    log(f"[batch bf16] {bf16['solves_per_s']} solves/s, outer median {bf16['outer_iterations_median']}")
+    del Bb
 
     # ---- serving stream ---------------------------------------------------------------
     serving = None
@@ -436,7 +450,23 @@ def main():
         outer, inner, rd = mg.solve_mixed(db, dx)
         t1.append(time.perf_counter() - t0)
     err = float(np.abs(dx.get() - xt).max())
-    del mg, dx, db
+    del mg, dx
+    # the same system as a one-slot graph-launched pool (kappa = 1: the same operator)
+    P1 = H.Batch([LEVEL], [1], H.batch_config(omega=0.7, iters_per_launch=8))
+    ones = H.DeviceArray((SIDE - 1) ** 2, np.float64)
+    ones.upload(np.ones(ones.n))
+    xs = H.DeviceArray(n, np.float64)
+    tp = []
+    for k in range(6):
+        H.synchronize()
+        t0 = time.perf_counter()
+        P1.load(0, ones, db)
+        P1.run()
+        tp.append(time.perf_counter() - t0)
+    P1.result(0, xs)
+    info1 = P1.poll()[0]
+    err1 = float(np.abs(xs.get() - xt).max())
+    del P1, xs, ones, db
 
     out = {
         "metric": METRIC,
@@ -466,12 +496,19 @@ def main():
         "solve": {"outer_iterations_median": int(np.median(iters)), "outer_iterations_max": int(iters.max()),
                   "max_rel_residual": float(rels.max()), "all_converged": bool((status == 0).all()),
                   "graph_launches_per_step": round(launches / args.steps, 2)},
+        "bf16_inner": bf16,
         "fp64_single_gpu": fp64,
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
         "single_2049": {"solves_per_s": round(1.0 / float(np.median(t1)), 3), "outer_iterations": outer,
                         "rel_residual": rd, "max_err_vs_x_true": err,
-                        "what": "one Q1 2049^2 constant-coefficient system, b = A x_true, mixed MG to 1e-8"},
+                        "what": "one Q1 2049^2 constant-coefficient system, b = A x_true, mixed MG to 1e-8",
+                        "as_one_slot_pool": {"solves_per_s": round(1.0 / float(np.median(tp[1:])), 3),
+                                             "outer_iterations": info1.iters, "rel_residual": info1.rel,
+                                             "max_err_vs_x_true": err1,
+                                             "what": "same solve through a one-slot batch (graph launches, "
+                                                     "device-side convergence; includes the hierarchy "
+                                                     "assembly from the coefficient)"}},
         "spmv": spmv,
         "batch_kernel": batch_kernel,
     }

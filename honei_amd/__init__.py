@@ -89,6 +89,7 @@ _SIGS = {
     "honei_hip_mg_build_q1_coeff": [_p, _d, _p, _i, _p],
     "honei_hip_mg_solve_fixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
     "honei_hip_mg_solve_mixed": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_d), _p],
+    "honei_hip_mg_solve_pcg": [_p, _p, _p, C.POINTER(_sz), C.POINTER(_d), _p],
     "honei_hip_product_bmdv_q1_float": [_p] * 11 + [_sz, _sz, _sz, _p],
     "honei_hip_product_bmdv_q1_double": [_p] * 11 + [_sz, _sz, _sz, _p],
     "honei_hip_defect_q1_float": [_p] * 12 + [_sz, _sz, _sz, _p],
@@ -356,6 +357,14 @@ class Multigrid:
         call("honei_hip_mg_solve_fixed", self.h, b.ptr, x.ptr, C.byref(it), C.byref(fd), stream)
         return it.value, fd.value
 
+    def solve_pcg(self, b: DeviceArray, x: DeviceArray, stream=None, check=True):
+        """FP64 CG preconditioned by one V-cycle per iteration; (iterations, rel. defect)."""
+        i, rd = _sz(), _d()
+        rc = lib().honei_hip_mg_solve_pcg(self.h, b.ptr, x.ptr, C.byref(i), C.byref(rd), stream)
+        if rc != SUCCESS and (check or rc != 5):
+            raise HipError(rc, "honei_hip_mg_solve_pcg", last_error())
+        return i.value, rd.value
+
     def solve_mixed(self, b: DeviceArray, x: DeviceArray, stream=None, check=True):
         o, i, rd = _sz(), _sz(), _d()
         rc = lib().honei_hip_mg_solve_mixed(self.h, b.ptr, x.ptr, C.byref(o), C.byref(i), C.byref(rd), stream)
@@ -376,7 +385,7 @@ class BatchConfig(C.Structure):
     """honei_hip_batch_config (include/honei_hip.h)."""
     _fields_ = [("inner_vtype", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", _d),
                 ("tolerance", _d), ("max_outer", C.c_int32), ("averaging", C.c_int32),
-                ("iters_per_launch", C.c_int32), ("autotune", C.c_int32)]
+                ("iters_per_launch", C.c_int32), ("autotune", C.c_int32), ("outer_method", C.c_int32)]
 
 
 class SlotInfo(C.Structure):

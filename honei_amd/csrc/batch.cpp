@@ -132,7 +132,7 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
         if (err == hipSuccess) err = b_fill_state((SlotScalars *)scd, ns, SLOT_ACTIVE, s_main_);
         if (err == hipSuccess) {
             const int64_t bnd[2] = {0, ns};
-            err = b_compact((const SlotScalars *)scd, ns, bnd, 2, (int32_t *)lst, (int32_t *)lst + ns, s_main_);
+            err = b_compact((const SlotScalars *)scd, ns, bnd, 2, (int32_t *)lst, (int32_t *)lst + ns, false, s_main_);
         }
         const BWork w{(const int32_t *)lst, (const int32_t *)lst + ns, 0, 1, 0};
         for (int block : {64, 128, 256, 512, 1024}) {
@@ -193,7 +193,7 @@ uint64_t BatchMG::plan_bytes() const {
     }
     for (const BClass &c : C_) {
         b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4;
-        b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * 3;
+        b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
         b += (uint64_t)(c.count * c.bps) * 8;
     }
     int64_t cells = 0;
@@ -212,6 +212,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     if (cfg.n_pre < 2 || cfg.n_pre > 16 || cfg.n_post < 0 || cfg.n_post > 16) return hipErrorInvalidValue;
     if (cfg.iters_per_launch < 1 || cfg.iters_per_launch > 64 || cfg.max_outer < 1) return hipErrorInvalidValue;
     if (cfg.averaging < 0 || cfg.averaging > 2) return hipErrorInvalidValue;
+    if (cfg.outer != 0 && cfg.outer != 1) return hipErrorInvalidValue;
     std::vector<int> order((size_t)nclass);
     for (int c = 0; c < nclass; ++c) {
         if (levels[c] < 2 || levels[c] > 13 || counts[c] < 1 || counts[c] > 65535) return hipErrorInvalidValue;
@@ -286,6 +287,8 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
         HX(alloc((void **)&c.Arl, (size_t)c.n * 4));
         HX(alloc((void **)&c.Ax, (size_t)(c.count * c.ax_ld) * 8));
         for (double **p : {&c.b, &c.x, &c.d}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
+        if (cfg_.outer == 1)
+            for (double **p : {&c.p, &c.q}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
         HX(alloc((void **)&c.partials, (size_t)(c.count * c.bps) * 8));
         HX(q1_assemble(VT_F64, c.side, c.Aj, c.Ax, c.Arl, c.stride, nullptr, nullptr, 0.0, s_main_, 1));
     }
@@ -356,33 +359,62 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
-    const int nb = (int)boundaries_.size();
-    // slots running or admitted since the last launch
-    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, s));
-    BCheckTab tab{};
-    tab.nclass = (int32_t)C_.size();
+BClassTab BatchMG::class_table() const {
+    BClassTab t{};
+    t.nclass = (int32_t)C_.size();
     for (size_t k = 0; k < C_.size(); ++k) {
         const BClass &c = C_[k];
-        const BMat m{c.Aj, c.Arl, c.Ax, c.stride, c.ax_ld, c.n, c.vld, c.bps};
-        HX(b_outer_defect(m, class_work(k), c.count, c.d, c.x, c.b, c.partials, s));
-        tab.first[k] = c.first;
-        tab.bps[k] = c.bps;
-        tab.partials[k] = c.partials;
+        const BLevel &top = L_[(size_t)c.level];
+        BClassEntry &e = t.c[k];
+        e.first = c.first;
+        e.n = c.n;
+        e.vld = c.vld;
+        e.stride = c.stride;
+        e.ax_ld = c.ax_ld;
+        e.bps = c.bps;
+        e.vld_top = top.vld;
+        e.Aj = c.Aj;
+        e.Arl = c.Arl;
+        e.Ax = c.Ax;
+        e.b = c.b;
+        e.x = c.x;
+        e.r = c.d;
+        e.p = c.p;
+        e.q = c.q;
+        e.partials = c.partials;
+        e.rhs_top = top.rhs;
+        e.x_top = top.x;
     }
-    HX(b_check(tab, all_work(), nslots_, sc_, cfg_.tolerance, cfg_.max_outer, s));
-    // slots still running after the check
-    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, s));
-    for (size_t k = 0; k < C_.size(); ++k) {
-        const BLevel &top = L_[(size_t)C_[k].level];
-        HX(b_scale_to_f32(C_[k], class_work(k), top.rhs, top.vld, sc_, s));
-    }
+    return t;
+}
+
+// One outer iteration of every slot.  Defect correction (outer 0):
+//   list non-idle -> r = b - A x, |r| -> check -> list active -> rhs = r/|r|
+//   -> V-cycle -> x += |r| c.
+// Preconditioned CG (outer 1), z = |r| V(r/|r|):
+//   list non-idle -> r = b - A x for new / verifying slots -> check (recursive
+//   |r|) -> list active -> rhs = r/|r| -> V-cycle -> rho = r.z, beta ->
+//   p = z + beta p -> q = A p, p.q -> alpha -> x += alpha p, r -= alpha q, |r|.
+hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
+    const int nb = (int)boundaries_.size();
+    const BClassTab tab = class_table();
+    const BWork w = all_work();
+    int64_t blocks = 0;
+    for (const BClass &c : C_) blocks += c.count * c.bps;
+    const bool cg = cfg_.outer == 1;
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, false, s));
+    HX(b_outer(cg ? OP_DEFECT_SETTLE : OP_DEFECT_ALL, tab, w, blocks, sc_, s));
+    HX(b_finish(cg ? FIN_CHECK_CG : FIN_CHECK_DC, tab, w, nslots_, sc_, cfg_.tolerance, cfg_.max_outer, s));
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, true, s));
+    HX(b_outer(OP_SCALE, tab, w, blocks, sc_, s));
     HX(enqueue_vcycle(s));
-    for (size_t k = 0; k < C_.size(); ++k) {
-        const BLevel &top = L_[(size_t)C_[k].level];
-        HX(b_acc(C_[k], class_work(k), top.x, top.vld, sc_, s));
-    }
-    return hipSuccess;
+    if (!cg) return b_outer(OP_ACC, tab, w, blocks, sc_, s);
+    HX(b_outer(OP_DOT_RZ, tab, w, blocks, sc_, s));
+    HX(b_finish(FIN_BETA, tab, w, nslots_, sc_, 0.0, 0, s));
+    HX(b_outer(OP_PUPDATE, tab, w, blocks, sc_, s));
+    HX(b_outer(OP_SPMV_DOT, tab, w, blocks, sc_, s));
+    HX(b_finish(FIN_ALPHA, tab, w, nslots_, sc_, 0.0, 0, s));
+    return b_outer(OP_UPDATE, tab, w, blocks, sc_, s);
 }
 
 hipError_t BatchMG::capture() {

@@ -37,7 +37,9 @@
 namespace honei {
 namespace hipx {
 
-enum SlotState : int32_t { SLOT_IDLE = 0, SLOT_ACTIVE = 1, SLOT_NEW = 2 };
+// VERIFY: the recursive CG residual met the tolerance; the next iteration
+// recomputes b - A x before the slot retires.
+enum SlotState : int32_t { SLOT_IDLE = 0, SLOT_ACTIVE = 1, SLOT_NEW = 2, SLOT_VERIFY = 3 };
 enum SlotStatus : int32_t { SLOT_OK = 0, SLOT_PENDING = -1, SLOT_NOT_CONVERGED = 5 };
 
 struct SlotScalars {
@@ -46,6 +48,9 @@ struct SlotScalars {
     double scale;     // inner right-hand side scaling of the current cycle
     double inv;       // 1 / scale
     double rel;       // norm / initial
+    double rho;       // CG: r.z of the previous iteration (0: restart)
+    double beta;      // CG: direction update factor of this iteration
+    double alpha;     // CG: step length of this iteration
     int32_t state;    // SlotState
     int32_t iters;    // outer iterations (V-cycles) applied
     int32_t status;   // SlotStatus
@@ -63,6 +68,10 @@ struct BatchConfig {
     int32_t averaging = 1;           // coefficient coarsening: 0 arith, 1 geometric, 2 harmonic
     int32_t iters_per_launch = 4;    // outer iterations per graph launch
     int32_t autotune = 1;            // 0: threads-per-row 1, block 256 everywhere
+    // Outer iteration: 0 defect correction x += V(b - A x) (the reference's
+    // MIXED driver), 1 conjugate gradients in FP64 preconditioned by the
+    // V-cycle (robust where the rediscretised hierarchy over-corrects).
+    int32_t outer = 1;
 };
 
 // Inner level l of the pool: slots [0, cap) present.
@@ -84,6 +93,7 @@ struct BClass {
     int32_t *Aj = nullptr;
     uint32_t *Arl = nullptr;
     double *Ax = nullptr, *b = nullptr, *x = nullptr, *d = nullptr, *partials = nullptr;
+    double *p = nullptr, *q = nullptr;   // CG direction and A p
 };
 
 // Threads-per-row / block-size choice for one level (autotuned, cached per
@@ -109,6 +119,34 @@ struct BWork {
     int32_t lo, hi;
     int64_t base;   // slot of local index 0 in the launch's arrays
 };
+
+// Per-class data of the FP64 outer launches, which cover the active slots of
+// every class in one launch (logical block = 256 rows of one slot).
+struct BClassEntry {
+    int64_t first, n, vld, stride, ax_ld, bps, vld_top;
+    const int32_t *Aj;
+    const uint32_t *Arl;
+    const double *Ax, *b;
+    double *x, *r, *p, *q, *partials;
+    float *rhs_top;     // inner right-hand side of the class's finest level
+    const float *x_top; // inner solution of the class's finest level
+};
+struct BClassTab {
+    int32_t nclass;
+    BClassEntry c[kMaxClasses];
+};
+
+enum OuterOp : int32_t {
+    OP_DEFECT_ALL = 0,     // r = b - A x (+ |r|^2) for every listed slot
+    OP_DEFECT_SETTLE = 1,  // same, slots in ACTIVE state skipped (CG keeps its recursive residual)
+    OP_SCALE = 2,          // rhs_top = (float)(r / |r|)
+    OP_ACC = 3,            // x = (first cycle ? 0 : x) + |r| * x_top   (defect correction)
+    OP_DOT_RZ = 4,         // partial r.z, z = |r| * x_top
+    OP_PUPDATE = 5,        // p = z + beta p
+    OP_SPMV_DOT = 6,       // q = A p, partial p.q
+    OP_UPDATE = 7          // x += alpha p, r -= alpha q, partial |r|^2
+};
+enum FinishOp : int32_t { FIN_CHECK_DC = 0, FIN_CHECK_CG = 1, FIN_BETA = 2, FIN_ALPHA = 3 };
 
 struct BBoundaries {
     int32_t n;
@@ -207,6 +245,7 @@ class BatchMG {
     BWork level_work(int64_t level) const;
     BWork class_work(size_t k) const;
     BWork all_work() const;
+    BClassTab class_table() const;
     void release();
 };
 
@@ -216,23 +255,17 @@ bool batch_tuned_shape(int dev, int64_t level, int vt, LaunchShape *out);
 // grid for up to `items` logical blocks
 int batch_grid(int64_t items);
 
+// active_only: list ACTIVE slots; else every non-idle slot.
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
-                     int32_t *bounds, hipStream_t s);
+                     int32_t *bounds, bool active_only, hipStream_t s);
 hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
                  void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
-hipError_t b_outer_defect(const BMat &m, const BWork &w, int64_t max_slots, double *d, const double *x,
-                          const double *b, double *partials, hipStream_t s);
-struct BCheckTab {
-    int32_t nclass;
-    int64_t first[kMaxClasses], bps[kMaxClasses];
-    const double *partials[kMaxClasses];
-};
-hipError_t b_check(const BCheckTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc, double tol,
-                   int32_t max_outer, hipStream_t s);
-hipError_t b_scale_to_f32(const BClass &c, const BWork &w, float *rhs_top, int64_t vld_top, const SlotScalars *sc,
-                          hipStream_t s);
-hipError_t b_acc(const BClass &c, const BWork &w, const float *x_top, int64_t vld_top, const SlotScalars *sc,
-                 hipStream_t s);
+// FP64 outer work over the active slots of all classes (w: lo = 0, hi = nclass).
+hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
+                   hipStream_t s);
+// One block per listed slot: reduce its partials and update its scalars.
+hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
+                    double tol, int32_t max_outer, hipStream_t s);
 hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
 hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
 hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);

@@ -40,6 +40,7 @@ BatchConfig to_config(const honei_hip_batch_config &c) {
     b.averaging = c.averaging;
     b.iters_per_launch = c.iters_per_launch;
     b.autotune = c.autotune;
+    b.outer = c.outer_method;
     return b;
 }
 
@@ -69,6 +70,7 @@ int honei_hip_batch_config_default(honei_hip_batch_config *cfg) {
     cfg->averaging = b.averaging;
     cfg->iters_per_launch = b.iters_per_launch;
     cfg->autotune = b.autotune;
+    cfg->outer_method = b.outer;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -91,6 +93,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
     if (cfg->iters_per_launch < 1 || cfg->iters_per_launch > 64 || cfg->max_outer < 1)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "iters_per_launch 1..64, max_outer >= 1");
     if (cfg->averaging < 0 || cfg->averaging > 2) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "averaging 0..2");
+    if (cfg->outer_method != 0 && cfg->outer_method != 1)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "outer_method 0 or 1");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");
@@ -159,7 +163,7 @@ int honei_hip_batch_poll(void *batch, honei_hip_slot_info *info) {
         info[g].initial = s.initial;
         info[g].norm = s.norm;
         info[g].rel = s.rel;
-        info[g].state = b->loading(g) ? 3 : s.state;
+        info[g].state = b->loading(g) ? 3 : (s.state == SLOT_VERIFY ? SLOT_ACTIVE : s.state);
         info[g].iters = s.iters;
         info[g].status = s.status;
         info[g].level = (int32_t)b->slot_level(g);

@@ -41,14 +41,18 @@ __device__ inline int64_t active_count(const BWork &w) { return w.bounds[w.hi] -
 
 // list of non-idle slots (ascending) and, for each boundary k, the number of
 // them below boundaries[k]; one 1024-thread block.
+__device__ inline bool listed(const SlotScalars &s, bool active_only) {
+    return active_only ? s.state == SLOT_ACTIVE : s.state != SLOT_IDLE;
+}
+
 __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64_t nslots, BBoundaries bd,
-                                                     int32_t *list, int32_t *bounds) {
+                                                     int32_t *list, int32_t *bounds, bool active_only) {
     __shared__ int32_t pre[1024];
     const int t = threadIdx.x;
     const int64_t chunk = (nslots + 1023) / 1024;
     const int64_t s0 = t * chunk, s1 = (s0 + chunk < nslots) ? s0 + chunk : nslots;
     int32_t cnt = 0;
-    for (int64_t g = s0; g < s1; ++g) cnt += sc[g].state != SLOT_IDLE;
+    for (int64_t g = s0; g < s1; ++g) cnt += listed(sc[g], active_only);
     pre[t] = cnt;
     __syncthreads();
     // inclusive scan (Hillis-Steele) over 1024 counts
@@ -60,13 +64,13 @@ __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64
     }
     int32_t pos = pre[t] - cnt;
     for (int64_t g = s0; g < s1; ++g)
-        if (sc[g].state != SLOT_IDLE) list[pos++] = (int32_t)g;
+        if (listed(sc[g], active_only)) list[pos++] = (int32_t)g;
     if (t < bd.n) {
         const int64_t b = bd.v[t];
         const int64_t owner = chunk > 0 ? b / chunk : 0;
         int32_t c = (owner > 0 && owner <= 1023) ? pre[owner - 1] : (owner > 1023 ? pre[1023] : 0);
         if (owner <= 1023)
-            for (int64_t g = owner * chunk; g < b && g < nslots; ++g) c += sc[g].state != SLOT_IDLE;
+            for (int64_t g = owner * chunk; g < b && g < nslots; ++g) c += listed(sc[g], active_only);
         bounds[t] = c;
     }
 }
@@ -111,75 +115,159 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
     }
 }
 
-__global__ __launch_bounds__(256) void k_b_outer_defect(BMat m, BWork w, double *d, const double *x,
-                                                        const double *b, double *partials) {
-    const int64_t a0 = active_first(w);
-    int64_t first, last;
-    my_chunk(active_count(w) * m.bps, first, last);
-    for (int64_t lb = first; lb < last; ++lb) {
-        const int64_t ls = w.list[a0 + lb / m.bps] - w.base, hb = lb % m.bps;
-        const int64_t rb = xcd_block(hb, m.bps);
-        const double *__restrict__ Ax = (const double *)m.Ax + ls * m.ax_ld;
-        const int64_t vo = ls * m.vld;
-        const int64_t row = rb * 256 + threadIdx.x;
-        double nval = 0.0;
-        if (row < m.n) {
-            const double s = ell_lane_sum<double, 1>(m.Aj, Ax, row, 0, m.Arl[row], m.stride, x + vo);
-            const double r = b[vo + row] - s;
-            d[vo + row] = r;
-            nval = r * r;
+// ---- FP64 outer work, all classes in one launch ------------------------------
+// Logical block = 256 rows of one active slot; blocks ordered class by class,
+// slot by slot.  Partial index hb of a slot holds row block xcd_block(hb,
+// bps), the order a single-system launch of 256-row blocks uses.
+template <int OP>
+__global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const SlotScalars *sc) {
+    // per-class prefix of logical blocks, in LDS (indexed by the class)
+    __shared__ int64_t pre[kMaxClasses + 1];
+    if (threadIdx.x == 0) {
+        int64_t acc = 0;
+        pre[0] = 0;
+        for (int c = 0; c < tab.nclass; ++c) {
+            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * tab.c[c].bps;
+            pre[c + 1] = acc;
         }
-        const double v = block_sum(nval);
-        if (threadIdx.x == 0) partials[ls * m.bps + hb] = v;
+    }
+    __syncthreads();
+    int64_t first, last;
+    my_chunk(pre[tab.nclass], first, last);
+    int c = 0;
+    for (int64_t lb = first; lb < last; ++lb) {
+        while (lb >= pre[c + 1]) ++c;
+        const BClassEntry &e = tab.c[c];
+        const int64_t off = lb - pre[c];
+        const int64_t g = w.list[w.bounds[c] + off / e.bps];
+        const int64_t ls = g - e.first, hb = off % e.bps;
+        const SlotScalars &st = sc[g];
+        if (OP == OP_DEFECT_SETTLE && st.state == SLOT_ACTIVE) continue;   // uniform per block
+        const int64_t rb = xcd_block(hb, e.bps);
+        const int64_t row = rb * 256 + threadIdx.x;
+        const bool in = row < e.n;
+        const int64_t vo = ls * e.vld, vi = vo + row;
+        double part = 0.0;
+        if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE) {
+            if (in) {
+                const double s = ell_lane_sum<double, 1>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row], e.stride,
+                                                         e.x + vo);
+                const double r = e.b[vi] - s;
+                e.r[vi] = r;
+                part = r * r;
+            }
+        } else if constexpr (OP == OP_SCALE) {
+            if (in) e.rhs_top[g * e.vld_top + row] = (float)(e.r[vi] * st.inv);
+        } else if constexpr (OP == OP_ACC) {
+            if (in) {
+                const double base = st.iters == 1 ? 0.0 : e.x[vi];
+                e.x[vi] = base + (double)e.x_top[g * e.vld_top + row] * st.scale;
+            }
+        } else if constexpr (OP == OP_DOT_RZ) {
+            if (in) part = e.r[vi] * ((double)e.x_top[g * e.vld_top + row] * st.scale);
+        } else if constexpr (OP == OP_PUPDATE) {
+            if (in) {
+                const double z = (double)e.x_top[g * e.vld_top + row] * st.scale;
+                e.p[vi] = st.beta == 0.0 ? z : z + st.beta * e.p[vi];
+            }
+        } else if constexpr (OP == OP_SPMV_DOT) {
+            if (in) {
+                const double q = ell_lane_sum<double, 1>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row], e.stride,
+                                                         e.p + vo);
+                e.q[vi] = q;
+                part = e.p[vi] * q;
+            }
+        } else if constexpr (OP == OP_UPDATE) {
+            if (in) {
+                e.x[vi] = e.x[vi] + st.alpha * e.p[vi];
+                const double r = e.r[vi] - st.alpha * e.q[vi];
+                e.r[vi] = r;
+                part = r * r;
+            }
+        }
+        if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_DOT_RZ || OP == OP_SPMV_DOT ||
+                      OP == OP_UPDATE) {
+            const double v = block_sum(part);
+            if (threadIdx.x == 0) e.partials[ls * e.bps + hb] = v;
+        }
     }
 }
 
-// One 1024-thread block per active slot (grid-stride): norm from the
-// partials, then the outer iteration's decision (MGHierarchy::solve_mixed:
-// initial norm, scale 1 for the first cycle, stop when norm < tol *
-// initial, give up at max_outer or on a non-finite norm, else scale = norm).
-__global__ __launch_bounds__(1024) void k_b_check(BCheckTab tab, BWork w, SlotScalars *sc, double tol,
-                                                  int32_t max_outer) {
+// One 1024-thread block per listed slot (grid-stride): reduce the slot's
+// partials and update its scalars.
+//   FIN_CHECK_DC: MGHierarchy::solve_mixed's decision (initial norm, scale 1
+//     on the first cycle, stop when norm < tol * initial, give up at
+//     max_outer or on a non-finite norm, else scale = norm);
+//   FIN_CHECK_CG: the same test on CG's recursive residual, confirmed by a
+//     recomputed b - A x (VERIFY) before the slot retires;
+//   FIN_BETA: rho = r.z, beta = rho / rho_prev (0 after a (re)start);
+//   FIN_ALPHA: alpha = rho / p.Ap.
+template <int OP>
+__global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotScalars *sc, double tol,
+                                                   int32_t max_outer) {
     const int64_t a0 = active_first(w), na = active_count(w);
     for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
         const int64_t g = w.list[a0 + a];
         int c = 0;
-        while (c + 1 < tab.nclass && g >= tab.first[c + 1]) ++c;
-        const int64_t ls = g - tab.first[c];
-        const double h = sum_partials(tab.partials[c] + ls * tab.bps[c], tab.bps[c], 1, 0);
+        while (c + 1 < tab.nclass && g >= tab.c[c + 1].first) ++c;
+        const BClassEntry &e = tab.c[c];
+        const int64_t ls = g - e.first;
+        const double h = sum_partials(e.partials + ls * e.bps, e.bps, 1, 0);
         if (threadIdx.x == 0) {
             SlotScalars s = sc[g];
-            const double nrm = sqrt(h);
-            s.norm = nrm;
-            if (s.state == SLOT_NEW) {
-                s.initial = nrm;
-                s.iters = 0;
-                s.rel = 1.0;
-                if (nrm == 0.0) {             // b - A x0 = 0: x0 is the solution
-                    s.rel = 0.0;
-                    s.state = SLOT_IDLE;
-                    s.status = SLOT_OK;
-                } else if (!isfinite(nrm)) {
-                    s.state = SLOT_IDLE;
-                    s.status = SLOT_NOT_CONVERGED;
+            if constexpr (OP == FIN_CHECK_DC || OP == FIN_CHECK_CG) {
+                const double nrm = sqrt(h);
+                s.norm = nrm;
+                if (s.state == SLOT_NEW) {
+                    s.initial = nrm;
+                    s.iters = 0;
+                    s.rel = 1.0;
+                    s.rho = 0.0;
+                    if (nrm == 0.0) {             // b - A x0 = 0: x0 is the solution
+                        s.rel = 0.0;
+                        s.state = SLOT_IDLE;
+                        s.status = SLOT_OK;
+                    } else if (!isfinite(nrm)) {
+                        s.state = SLOT_IDLE;
+                        s.status = SLOT_NOT_CONVERGED;
+                    } else {
+                        s.scale = OP == FIN_CHECK_DC ? 1.0 : nrm;
+                        s.inv = OP == FIN_CHECK_DC ? 1.0 : 1.0 / nrm;
+                        s.iters = 1;
+                        s.state = SLOT_ACTIVE;
+                    }
                 } else {
-                    s.scale = 1.0;
-                    s.inv = 1.0;
-                    s.iters = 1;
-                    s.state = SLOT_ACTIVE;
+                    s.rel = nrm / s.initial;
+                    const bool met = nrm < tol * s.initial;
+                    if (met && (OP == FIN_CHECK_DC || s.state == SLOT_VERIFY)) {
+                        s.state = SLOT_IDLE;
+                        s.status = SLOT_OK;
+                    } else if (met) {
+                        s.state = SLOT_VERIFY;
+                    } else if (s.iters >= max_outer || !isfinite(nrm)) {
+                        s.state = SLOT_IDLE;
+                        s.status = SLOT_NOT_CONVERGED;
+                    } else {
+                        if (s.state == SLOT_VERIFY) {   // recursive residual drifted: restart
+                            s.state = SLOT_ACTIVE;
+                            s.rho = 0.0;
+                        }
+                        s.scale = nrm;
+                        s.inv = 1.0 / nrm;
+                        s.iters += 1;
+                    }
                 }
+            } else if constexpr (OP == FIN_BETA) {
+                s.beta = (s.rho > 0.0 && h > 0.0 && isfinite(h)) ? h / s.rho : 0.0;
+                s.rho = h;
             } else {
-                s.rel = nrm / s.initial;
-                if (nrm < tol * s.initial) {
-                    s.state = SLOT_IDLE;
-                    s.status = SLOT_OK;
-                } else if (s.iters >= max_outer || !isfinite(nrm)) {
+                const double ap = h;
+                if (ap > 0.0 && isfinite(ap) && s.rho > 0.0) {
+                    s.alpha = s.rho / ap;
+                } else {                           // breakdown: give up on this slot
+                    s.alpha = 0.0;
                     s.state = SLOT_IDLE;
                     s.status = SLOT_NOT_CONVERGED;
-                } else {
-                    s.scale = nrm;
-                    s.inv = 1.0 / nrm;
-                    s.iters += 1;
                 }
             }
             sc[g] = s;
@@ -188,7 +276,8 @@ __global__ __launch_bounds__(1024) void k_b_check(BCheckTab tab, BWork w, SlotSc
     }
 }
 
-// Elementwise launches: logical block = 256 consecutive entries of one slot.
+// Level-wise elementwise launches: logical block = 256 consecutive entries of
+// one active slot.
 #define FOR_SLOT_ENTRIES(w, n, BODY)                                              \
     do {                                                                          \
         const int64_t a0_ = active_first(w);                                     \
@@ -203,25 +292,6 @@ __global__ __launch_bounds__(1024) void k_b_check(BCheckTab tab, BWork w, SlotSc
             }                                                                     \
         }                                                                         \
     } while (0)
-
-// rhs_top[g] = (float)(d[ls] * inv[g])
-__global__ void k_b_scale(BWork w, const double *d, int64_t vld, int64_t n, float *rhs, int64_t vld_top,
-                          const SlotScalars *sc) {
-    FOR_SLOT_ENTRIES(w, n, {
-        const int64_t ls = g - w.base;
-        rhs[g * vld_top + i] = (float)(d[ls * vld + i] * sc[g].inv);
-    });
-}
-
-// x[ls] = (first cycle ? 0 : x[ls]) + (double)x_top[g] * scale[g]
-__global__ void k_b_acc(BWork w, double *x, int64_t vld, int64_t n, const float *xt, int64_t vld_top,
-                        const SlotScalars *sc) {
-    FOR_SLOT_ENTRIES(w, n, {
-        const int64_t ls = g - w.base;
-        const double base = sc[g].iters == 1 ? 0.0 : x[ls * vld + i];
-        x[ls * vld + i] = base + (double)xt[g * vld_top + i] * sc[g].scale;
-    });
-}
 
 // rhs_c[g] = R d_f[g] for active slots present on the fine level
 __global__ void k_b_restrict(BWork w, float *rhs_c, int64_t vld_c, const float *d_f, int64_t vld_f, int64_t m,
@@ -309,12 +379,12 @@ int entry_grid(int64_t max_slots, int64_t n) { return batch_grid(max_slots * ((n
 int batch_grid(int64_t items) { return (int)(items < 1 ? 1 : (items > kMaxGrid ? kMaxGrid : items)); }
 
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
-                     int32_t *bounds, hipStream_t s) {
+                     int32_t *bounds, bool active_only, hipStream_t s) {
     if (nb < 1 || nb > kMaxClasses + 1 || nslots < 1) return hipErrorInvalidValue;
     BBoundaries bd{};
     bd.n = nb;
     for (int k = 0; k < nb; ++k) bd.v[k] = boundaries[k];
-    hipLaunchKernelGGL(k_b_compact, dim3(1), dim3(1024), 0, s, sc, nslots, bd, list, bounds);
+    hipLaunchKernelGGL(k_b_compact, dim3(1), dim3(1024), 0, s, sc, nslots, bd, list, bounds, active_only);
     return hipGetLastError();
 }
 
@@ -330,34 +400,41 @@ hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, co
     return hipGetLastError();
 }
 
-hipError_t b_outer_defect(const BMat &m, const BWork &w, int64_t max_slots, double *d, const double *x,
-                          const double *b, double *partials, hipStream_t s) {
-    if (m.bps != (m.n + 255) / 256) return hipErrorInvalidValue;
+hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
+                   hipStream_t s) {
+    if (max_blocks <= 0) return hipSuccess;
+    const dim3 grid(batch_grid(max_blocks)), block(256);
+    switch (op) {
+        case OP_DEFECT_ALL: hipLaunchKernelGGL(k_b_outer<OP_DEFECT_ALL>, grid, block, 0, s, tab, w, sc); break;
+        case OP_DEFECT_SETTLE: hipLaunchKernelGGL(k_b_outer<OP_DEFECT_SETTLE>, grid, block, 0, s, tab, w, sc); break;
+        case OP_SCALE: hipLaunchKernelGGL(k_b_outer<OP_SCALE>, grid, block, 0, s, tab, w, sc); break;
+        case OP_ACC: hipLaunchKernelGGL(k_b_outer<OP_ACC>, grid, block, 0, s, tab, w, sc); break;
+        case OP_DOT_RZ: hipLaunchKernelGGL(k_b_outer<OP_DOT_RZ>, grid, block, 0, s, tab, w, sc); break;
+        case OP_PUPDATE: hipLaunchKernelGGL(k_b_outer<OP_PUPDATE>, grid, block, 0, s, tab, w, sc); break;
+        case OP_SPMV_DOT: hipLaunchKernelGGL(k_b_outer<OP_SPMV_DOT>, grid, block, 0, s, tab, w, sc); break;
+        case OP_UPDATE: hipLaunchKernelGGL(k_b_outer<OP_UPDATE>, grid, block, 0, s, tab, w, sc); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
+                    double tol, int32_t max_outer, hipStream_t s) {
     if (max_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_b_outer_defect, dim3(batch_grid(max_slots * m.bps)), dim3(256), 0, s, m, w, d, x, b,
-                       partials);
-    return hipGetLastError();
-}
-
-hipError_t b_check(const BCheckTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc, double tol,
-                   int32_t max_outer, hipStream_t s) {
-    if (max_slots <= 0) return hipSuccess;
-    const int grid = (int)(max_slots < 1024 ? max_slots : 1024);
-    hipLaunchKernelGGL(k_b_check, dim3(grid), dim3(1024), 0, s, tab, w, sc, tol, max_outer);
-    return hipGetLastError();
-}
-
-hipError_t b_scale_to_f32(const BClass &c, const BWork &w, float *rhs_top, int64_t vld_top, const SlotScalars *sc,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_b_scale, dim3(entry_grid(c.count, c.n)), dim3(256), 0, s, w, (const double *)c.d, c.vld,
-                       c.n, rhs_top, vld_top, sc);
-    return hipGetLastError();
-}
-
-hipError_t b_acc(const BClass &c, const BWork &w, const float *x_top, int64_t vld_top, const SlotScalars *sc,
-                 hipStream_t s) {
-    hipLaunchKernelGGL(k_b_acc, dim3(entry_grid(c.count, c.n)), dim3(256), 0, s, w, c.x, c.vld, c.n, x_top, vld_top,
-                       sc);
+    const dim3 grid((unsigned)(max_slots < 1024 ? max_slots : 1024)), block(1024);
+    switch (op) {
+        case FIN_CHECK_DC:
+            hipLaunchKernelGGL(k_b_finish<FIN_CHECK_DC>, grid, block, 0, s, tab, w, sc, tol, max_outer);
+            break;
+        case FIN_CHECK_CG:
+            hipLaunchKernelGGL(k_b_finish<FIN_CHECK_CG>, grid, block, 0, s, tab, w, sc, tol, max_outer);
+            break;
+        case FIN_BETA: hipLaunchKernelGGL(k_b_finish<FIN_BETA>, grid, block, 0, s, tab, w, sc, tol, max_outer); break;
+        case FIN_ALPHA:
+            hipLaunchKernelGGL(k_b_finish<FIN_ALPHA>, grid, block, 0, s, tab, w, sc, tol, max_outer);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

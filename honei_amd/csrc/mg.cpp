@@ -73,7 +73,7 @@ MGHierarchy::~MGHierarchy() {
     }
     A_outer.release();
     for (void *p : {g_, u_, en_, (void *)iters_dev_, (void *)scal_, (void *)guess_, (void *)d64_, (void *)res64_,
-                    (void *)slots_.partials, (void *)slots_.counter})
+                    (void *)p64_, (void *)q64_, (void *)slots_.partials, (void *)slots_.counter})
         if (p) (void)hipFree(p);
 }
 
@@ -403,13 +403,123 @@ hipError_t MGHierarchy::solve_fixed(const void *b, void *x, hipStream_t s, int64
     return hipStreamSynchronize(s);
 }
 
+hipError_t MGHierarchy::solve_pcg(const double *b, double *x, hipStream_t s, int64_t *iters_out,
+                                  double *rel_defect) {
+    MGLevel &top = L[(size_t)max_level];
+    if (!A_outer.Aj) return hipErrorInvalidValue;
+    const int64_t n = top.n;
+    const ValueType vv = inner == VT_F64 ? VT_F64 : VT_F32;
+    if (n64_ != n) {
+        for (double **p : {&guess_, &d64_, &res64_, &p64_, &q64_}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+            HX(hipMalloc((void **)p, (size_t)n * sizeof(double)));
+        }
+        n64_ = n;
+    }
+    double *r = d64_, *z = res64_, *p = p64_, *q = q64_;
+    double *sc = scal_ + 16;   // device scalars: |r|^2, r.z, p.q
+    // x0 with Dirichlet boundary values; r = b - A x0
+    HX(boundary_guess(x, b, n, top.side, s));
+    auto defect_norm = [&](double *h) -> hipError_t {
+        EllArgs a;
+        a.mode = ELL_DEFECT;
+        a.out = r;
+        a.x = x;
+        a.rhs = b;
+        a.norm2 = sc;
+        a.row_start = 0;
+        a.row_end = n;
+        HX(ell_apply(A_outer.v, a, slots_, s));
+        return read_scalar(h, sc, s);
+    };
+    double h = 0.0;
+    HX(defect_norm(&h));
+    const double initial = std::sqrt(h);
+    double nrm = initial, rho_prev = 0.0;
+    int64_t it = 0;
+    const int64_t cap = cfg.max_outer > 0 ? cfg.max_outer : 10000;
+    // the inner cycle runs lean: one V-cycle from zero, no inner norms
+    const MGConfig saved = cfg;
+    cfg.n_max_iter = 1;
+    cfg.initial_zero = 1;
+    cfg.convergence_check = 0;
+    hipError_t err = hipSuccess;
+    while (initial > 0.0 && it < cap && std::isfinite(nrm)) {
+        // z = |r| V(r / |r|)
+        const double inv = 1.0 / nrm;
+        if (vv == VT_F64) {
+            err = vcopy(top.rhs, r, n, VT_F64, s);
+            if (err == hipSuccess) err = vscale(top.rhs, inv, n, VT_F64, s);
+        } else {
+            err = scale_to_f32((float *)top.rhs, r, inv, n, s);
+        }
+        if (err == hipSuccess) err = fill_bytes_zero(top.x, (size_t)n * vec_size(inner), s);
+        if (err != hipSuccess) break;
+        int64_t inner_it = 0;
+        mixed_inner_ = true;
+        err = run_kernel(s, &inner_it, nullptr);
+        mixed_inner_ = false;
+        if (err != hipSuccess) break;
+        if (vv == VT_F64) {
+            err = vcopy(z, top.x, n, VT_F64, s);
+            if (err == hipSuccess) err = vscale(z, nrm, n, VT_F64, s);
+        } else {
+            err = fill_bytes_zero(z, (size_t)n * 8, s);
+            if (err == hipSuccess) err = acc_scaled_f32(z, (const float *)top.x, nrm, n, s);
+        }
+        ++it;
+        double rho = 0.0, sigma = 0.0;
+        if (err == hipSuccess) err = dot(r, z, n, VT_F64, sc + 1, slots_, s);
+        if (err == hipSuccess) err = read_scalar(&rho, sc + 1, s);
+        if (err != hipSuccess) break;
+        const double beta = (rho_prev > 0.0 && rho > 0.0) ? rho / rho_prev : 0.0;
+        err = beta == 0.0 ? vcopy(p, z, n, VT_F64, s) : scaled_sum3(p, z, p, beta, n, VT_F64, s);
+        if (err != hipSuccess) break;
+        {
+            EllArgs a;
+            a.mode = ELL_PRODUCT;
+            a.out = q;
+            a.x = p;
+            a.row_start = 0;
+            a.row_end = n;
+            err = ell_apply(A_outer.v, a, slots_, s);
+        }
+        if (err == hipSuccess) err = dot(p, q, n, VT_F64, sc + 2, slots_, s);
+        if (err == hipSuccess) err = read_scalar(&sigma, sc + 2, s);
+        if (err != hipSuccess) break;
+        if (!(sigma > 0.0) || !(rho > 0.0)) break;   // breakdown: not converged
+        const double alpha = rho / sigma;
+        err = axpy(x, p, alpha, nullptr, n, VT_F64, s);
+        if (err == hipSuccess) err = axpy_norm2(r, q, -alpha, nullptr, n, VT_F64, sc, slots_, s);
+        if (err == hipSuccess) err = read_scalar(&h, sc, s);
+        if (err != hipSuccess) break;
+        nrm = std::sqrt(h);
+        rho_prev = rho;
+        if (nrm < cfg.tolerance * initial) {   // confirm with the true residual
+            err = defect_norm(&h);
+            if (err != hipSuccess) break;
+            nrm = std::sqrt(h);
+            if (nrm < cfg.tolerance * initial) break;
+            rho_prev = 0.0;   // drifted: restart from the true residual
+        }
+    }
+    cfg = saved;
+    HX(err);
+    HX(hipStreamSynchronize(s));
+    if (iters_out) *iters_out = it;
+    const double rd = initial > 0 ? nrm / initial : 0.0;
+    if (rel_defect) *rel_defect = rd;
+    return (initial > 0.0 && !(nrm < cfg.tolerance * initial)) ? hipErrorNotReady : hipSuccess;
+}
+
 hipError_t MGHierarchy::solve_mixed(const double *b, double *x, hipStream_t s, int64_t *outer_out,
                                     int64_t *inner_out, double *rel_defect, std::vector<double> *history) {
     MGLevel &top = L[(size_t)max_level];
     if (inner == VT_F64 || !A_outer.Aj) return hipErrorInvalidValue;
     const int64_t n = top.n;
     if (n64_ != n) {
-        for (double **p : {&guess_, &d64_, &res64_}) {
+        for (double **p : {&guess_, &d64_, &res64_, &p64_, &q64_}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
             HX(hipMalloc((void **)p, (size_t)n * sizeof(double)));

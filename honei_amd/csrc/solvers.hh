@@ -119,6 +119,11 @@ class MGHierarchy {
     // Drivers; b and x are device vectors of the inner type (FIXED) or
     // double (MIXED).
     hipError_t solve_fixed(const void *b, void *x, hipStream_t s, int64_t *iters, double *final_defect);
+    // FP64 conjugate gradients on the outer system preconditioned by one
+    // lean inner V-cycle per iteration (z = |r| V(r/|r|)); start vector x
+    // with the MIXED driver's boundary values; converged when the recursive
+    // |r| < tol |r0| and a recomputed b - A x confirms it.
+    hipError_t solve_pcg(const double *b, double *x, hipStream_t s, int64_t *iters, double *rel_defect);
     hipError_t solve_mixed(const double *b, double *x, hipStream_t s, int64_t *outer, int64_t *inner,
                            double *rel_defect, std::vector<double> *history);
 
@@ -126,7 +131,7 @@ class MGHierarchy {
     void *g_ = nullptr, *u_ = nullptr, *en_ = nullptr;   // coarse CG scratch
     int *iters_dev_ = nullptr;
     double *scal_ = nullptr;                            // device scalars
-    double *guess_ = nullptr, *d64_ = nullptr, *res64_ = nullptr;
+    double *guess_ = nullptr, *d64_ = nullptr, *res64_ = nullptr, *p64_ = nullptr, *q64_ = nullptr;
     int64_t n64_ = 0;
     bool mixed_inner_ = false;      // run_kernel called by the MIXED driver
     ReduceSlots slots_{};

@@ -303,4 +303,19 @@ int honei_hip_mg_solve_mixed(void *mg, const double *b, double *x, size_t *outer
     return HONEI_HIP_SUCCESS;
 }
 
+int honei_hip_mg_solve_pcg(void *mg, const double *b, double *x, size_t *iters, double *rel_defect, void *stream) {
+    if (!mg || !b || !x) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    MGHierarchy *h = (MGHierarchy *)mg;
+    if (!h->A_outer.Aj) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "outer FP64 system not set");
+    for (int64_t l = h->min_level; l <= h->max_level; ++l)
+        if (!h->L[(size_t)l].A.Aj) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "hierarchy level missing");
+    int64_t it = 0;
+    double rd = 0;
+    const hipError_t e = h->solve_pcg(b, x, S(stream), &it, &rd);
+    if (iters) *iters = (size_t)it;
+    if (rel_defect) *rel_defect = rd;
+    if (e == hipErrorNotReady) return fail(HONEI_HIP_ERROR_NOT_CONVERGED, "preconditioned CG stopped before convergence");
+    return check(e, "mg_solve_pcg");
+}
+
 }  // extern "C"

@@ -281,6 +281,16 @@ int honei_hip_mg_solve_fixed(void *mg, const void *b_dev, void *x_dev, size_t *i
                              void *stream);
 int honei_hip_mg_solve_mixed(void *mg, const double *b_dev, double *x_dev, size_t *outer_iters,
                              size_t *inner_iters, double *rel_defect, void *stream);
+/* FP64 conjugate gradients on the outer system, preconditioned by one lean
+ * V-cycle of the hierarchy per iteration (z = |r| V(r/|r|); any inner type,
+ * FP64 inner = the plain FP64 solve).  No reference counterpart: the
+ * reference's PCG variants (math/conjugate_gradients.hh:884-913) use a
+ * Jacobi preconditioner.  x_dev: start vector (boundary values as the MIXED
+ * driver) and result; cfg.tolerance on |r|/|r0|, cfg.max_outer caps the
+ * iterations (0: 10000); converged only when a recomputed b - A x confirms.
+ * HONEI_HIP_ERROR_NOT_CONVERGED if the cap or a breakdown stops it.        */
+int honei_hip_mg_solve_pcg(void *mg, const double *b_dev, double *x_dev, size_t *iters, double *rel_defect,
+                           void *stream);
 
 /* ---- seeded fields (synthetic Poisson workloads) ------------------------------
  * No reference counterpart.  Element i is drawn from position i of the
@@ -298,14 +308,16 @@ int honei_hip_fill_lognormal(double *dst_dev, size_t n, double sigma, uint64_t s
  * hierarchy with a per-cell coefficient down to the 3x3 grid, solved by
  * the MIXED defect correction (FP64 outer defect, update and norm; FP32 or
  * BF16-stored V-cycle with n_pre / n_post damped Jacobi sweeps and the
- * single-work-group CG on the coarsest grid).  One graph launch runs
+ * single-work-group CG on the coarsest grid) in an FP64 outer iteration
+ * (defect correction or V-cycle-preconditioned CG).  One graph launch runs
  * iters_per_launch outer iterations of every active slot; slots retire on
  * the device when ||b - A x|| < tolerance * ||b - A x0||, where x0 is zero
  * with the first row, last row and first column of the grid set to b.
  * autotune: threads-per-row T in {1,2,4} and block size per level picked by
- * timing (cached for the process); 0 uses T = 1, block 256, which makes each
- * slot bit-identical to honei_hip_mg_solve_mixed with n_max_iter 1,
- * initial_zero, zero_inner_start and post sweeps = pre sweeps.  The batch
+ * timing (cached for the process); 0 uses T = 1, block 256, which with
+ * outer_method 0 makes each slot bit-identical to honei_hip_mg_solve_mixed
+ * with n_max_iter 1, initial_zero, zero_inner_start and post sweeps = pre
+ * sweeps.  The batch
  * owns its streams; calls on one batch must not overlap.                    */
 typedef struct {
     int32_t inner_vtype;       /* HONEI_HIP_F32 or HONEI_HIP_BF16 (matrix values) */
@@ -316,6 +328,9 @@ typedef struct {
     int32_t averaging;         /* coefficient coarsening: 0 arithmetic, 1 geometric, 2 harmonic */
     int32_t iters_per_launch;  /* outer iterations per graph launch (1..64) */
     int32_t autotune;
+    int32_t outer_method;      /* 0 defect correction x += V(b - A x) (the reference's MIXED
+                                  driver); 1 FP64 conjugate gradients preconditioned by the V-cycle,
+                                  converged residual confirmed by a recomputed b - A x (default) */
 } honei_hip_batch_config;
 
 typedef struct {

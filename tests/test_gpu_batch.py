@@ -89,11 +89,12 @@ def test_fill_fields_match_host_generators():
 
 
 def test_batch_bitexact_with_single_system_solves():
-    """Autotune off (T = 1, block 256): each slot equals the single-system
-    MIXED solve bit for bit -- outer iterations, relative defect, x."""
+    """Autotune off (T = 1, block 256), defect-correction outer loop: each
+    slot equals the single-system MIXED solve bit for bit -- outer
+    iterations, relative defect, x."""
     levels, counts = [3, 5, 4], [2, 2, 3]
     systems = {(c, j): _system(levels[c], 10 * c + j) for c in range(3) for j in range(counts[c])}
-    res, launches, B = _run_batch(levels, counts, systems, autotune=0, iters_per_launch=3)
+    res, launches, B = _run_batch(levels, counts, systems, autotune=0, iters_per_launch=3, outer_method=0)
     assert B.shape(5) == (1, 256)
     for (c, j), ((state, iters, status, rel, level), x) in res.items():
         assert (state, status, level) == (0, 0, levels[c])
@@ -224,3 +225,57 @@ def test_serve_budget_and_unknown_size():
     assert [r.status for r in res] == [0, -2, 0]
     assert st.completed == 2 and st.rejected == 1
     assert res[2].admit_s >= res[0].done_s - 1e-9   # one slot: the second level-4 request waits for it
+
+
+def test_batch_cg_matches_single_system_cg():
+    """Batched CG outer loop (default) against the single-system CG driver on
+    the same FP32 hierarchy: same iteration count within one, solutions
+    agree to the accuracy the tolerance implies."""
+    levels, counts = [6, 5], [1, 2]
+    systems = {(c, j): _system(levels[c], 400 + 10 * c + j) for c in range(2) for j in range(counts[c])}
+    res, _, _ = _run_batch(levels, counts, systems, autotune=0)
+    for (c, j), ((state, iters, status, rel, level), x) in res.items():
+        assert status == 0 and rel < TOL
+        L = levels[c]
+        n = (2 ** L + 1) ** 2
+        kappa, b = systems[(c, j)]
+        mg = H.Multigrid(H.F32, 1, L)
+        mg.configure(H.mg_config(tolerance=TOL, max_outer=200))
+        mg.build_q1_coeff(H.DeviceArray.from_host(kappa, np.float64), OMEGA, AVG)
+        db, dx = H.DeviceArray.from_host(b), H.DeviceArray(n, np.float64)
+        dx.zero()
+        it, rd = mg.solve_pcg(db, dx)
+        assert abs(it - iters) <= 1, (c, j, it, iters)
+        xs = dx.get()
+        assert np.abs(x - xs).max() <= 1e-5 * max(1.0, np.abs(xs).max())
+
+
+def test_batch_cg_converges_where_defect_correction_diverges():
+    """System 48 of the 256-system lognormal workload (seed 4, side 513): the
+    rediscretised hierarchy over-corrects and plain defect correction
+    diverges; the CG outer loop converges (FP64 residual recomputed)."""
+    side, n = 513, 513 * 513
+    cs, bs = poisson_system_seeds(4, 48)
+    k = H.DeviceArray((side - 1) ** 2, np.float64)
+    b = H.DeviceArray(n, np.float64)
+    H.fill_lognormal(k, 1.0, cs)
+    H.fill_uniform(b, -1.0, 1.0, bs)
+    H.synchronize()
+    out = {}
+    for method in (0, 1):
+        B = H.Batch([9], [1], H.batch_config(outer_method=method, max_outer=60))
+        B.load(0, k, b)
+        B.run()
+        x = H.DeviceArray(n, np.float64)
+        B.result(0, x)
+        out[method] = (B.poll()[0], x.get())
+    assert out[0][0].status == 5 and out[0][0].rel > 1.0
+    info, x = out[1]
+    assert info.status == 0 and info.rel < TOL and info.iters < 30
+    kappa, bh = k.get(), b.get()
+    A = _fp64_operator(side, kappa)
+    bd = np.zeros(n, bool)
+    bd[:side] = bd[-side:] = True
+    bd[::side] = True
+    r0 = np.linalg.norm(oracle.defect(A, bh, np.where(bd, bh, 0.0)))
+    assert np.linalg.norm(oracle.defect(A, bh, x)) / r0 < 1.001 * TOL
