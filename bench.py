@@ -7,9 +7,10 @@ One step = one batched solve: 256 independent Q1 Poisson systems, side
 2^k+1 with k uniform in {5..9} (33..513 nodes), a lognormal (sigma 1)
 per-cell diffusion coefficient and a right-hand side uniform in [-1, 1)
 (synthetic, seeded, drawn on the device), each solved to a relative FP64
-residual of 1e-8 by mixed-precision multigrid defect correction (FP32
-V(2,2) damped-Jacobi cycles, FP64 outer defect/update/norm) in graph
-launches over a device-resident slot pool.  The timed step covers the
+residual of 1e-8 by mixed-precision multigrid: FP64 conjugate gradients
+(defect, update, norms in FP64) preconditioned by one FP32 V(2,2)
+damped-Jacobi cycle per iteration, in graph launches over a
+device-resident slot pool.  The timed step covers the
 assembly of all 256 hierarchies from the resident coefficients and the
 solve.  `value` is solves/s; p50/p99 of the per-system completion time
 within a step are reported beside it.
@@ -191,8 +192,7 @@ def batched_step(B, slots, kap, rhs):
     completion seconds, launches, slot infos)."""
     launches0 = B.launches()
     t0 = time.perf_counter()
-    for g, k, b in zip(slots, kap, rhs):
-        B.load(g, k, b)
+    B.load_many(slots, kap, rhs)
     B.activate(True)
     done = np.full(len(slots), -1.0)
     while True:
@@ -208,26 +208,28 @@ def batched_step(B, slots, kap, rhs):
 
 
 def fp64_single_solves(levels, kap, rhs, limit_s=60.0):
-    """Plain single-system FP64 GPU multigrid (FIXED: FP64 V(2,2) cycles to
-    the same relative tolerance, same rediscretised hierarchy), one system at
-    a time; hierarchy build excluded from the time."""
-    total, cycles, n_done = 0.0, [], 0
+    """Plain single-system FP64 GPU solve: the same algorithm (CG
+    preconditioned by one V(2,2) cycle, same rediscretised hierarchy, same
+    tolerance) entirely in FP64, one system per call, host-driven; hierarchy
+    build excluded from the time."""
+    total, cycles, n_done, failed = 0.0, [], 0, 0
     for L, k, b in zip(levels, kap, rhs):
         mg = H.Multigrid(H.F64, 1, L)
-        mg.configure(H.mg_config(n_max_iter=200, tolerance=1e-8))
+        mg.configure(H.mg_config(tolerance=1e-8, max_outer=200))
         mg.build_q1_coeff(k, 0.8, 1)
         x = H.DeviceArray(b.n, np.float64)
         x.zero()
         H.synchronize()
         t0 = time.perf_counter()
-        it, _ = mg.solve_fixed(b, x)
+        it, rd = mg.solve_pcg(b, x, check=False)
         total += time.perf_counter() - t0
         cycles.append(it)
+        failed += not (rd < 1e-8)
         n_done += 1
         del mg
         if total > limit_s:
             break
-    return total, n_done, cycles
+    return total, n_done, cycles, failed
 
 
 def serving_requests(seconds, rate=SERVE_RATE, seed=SERVE_SEED):
@@ -307,6 +309,7 @@ def main():
                     help="length of the Poisson arrival stream of the serving run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the single-system FP64 comparison")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched step (for profiling)")
     ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 5 launches of the roofline SpMV and 5 of a 2^26-float scale kernel "
@@ -332,6 +335,19 @@ def main():
             H.call("honei_hip_scale_one", H.F32, big.ptr, 1.0, 1 << 26, None)
         H.synchronize()
         print(json.dumps({"pmc_probe": "done"}))
+        return
+
+    if args.batch_only:
+        levels, kap, rhs = batched_workload()
+        B, slots = batched_pool(levels)
+        for _ in range(args.warmup):
+            batched_step(B, slots, kap, rhs)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sec, done, nl, info = batched_step(B, slots, kap, rhs)
+        total = time.perf_counter() - t0
+        print(json.dumps({"batch_only": {"solves_per_s": round(len(levels) * args.steps / total, 2),
+                                         "steps": args.steps, "launches_last_step": nl}}))
         return
 
     # ---- standalone SpMV (roofline kernel: Q1 2049^2, FP32, T=1) ----------
@@ -388,10 +404,12 @@ def main():
     # ---- plain single-system FP64 GPU solve of the same systems -------------------
     fp64 = None
     if not args.no_fp64:
-        t64, n64, cyc = fp64_single_solves(levels, kap, rhs)
+        t64, n64, cyc, f64fail = fp64_single_solves(levels, kap, rhs)
         fp64 = {"solves_per_s": round(n64 / t64, 3), "systems": n64, "seconds": round(t64, 3),
-                "cycles_median": int(np.median(cyc)), "cycles_max": int(max(cyc)),
-                "what": "MGHierarchy FIXED FP64 V(2,2), one system per call, same hierarchy and tolerance"}
+                "iterations_median": int(np.median(cyc)), "iterations_max": int(max(cyc)),
+                "not_converged": f64fail,
+                "what": ("honei_hip_mg_solve_pcg with an FP64 hierarchy: FP64 CG preconditioned by an FP64 "
+                         "V(2,2) cycle, one system per call, host-driven, same tolerance")}
         log(f"[fp64] {n64} single-system FP64 solves in {t64:.3f} s ({n64 / t64:.1f} solves/s)")
     batch_kernel = batch_kernel_roofline(B, max(levels))
     del B
@@ -487,7 +505,8 @@ def main():
                    "sides": {str(2 ** L + 1): levels.count(L) for L in sorted(set(levels))},
                    "unknowns": int(sum((2 ** L + 1) ** 2 for L in levels)),
                    "inner": "FP32 V(2,2) damped Jacobi 0.8, coefficient coarsened by geometric mean, CG on 3x3",
-                   "outer": "FP64 defect correction, one V-cycle per outer step", "rel_residual": 1e-8,
+                   "outer": "FP64 conjugate gradients preconditioned by one V-cycle per iteration",
+                   "rel_residual": 1e-8,
                    "step": "assemble all hierarchies from resident coefficients + solve all systems"},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "p99": round(float(np.percentile(lat_ms, 99)), 3),

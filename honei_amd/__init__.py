@@ -102,6 +102,8 @@ _SIGS = {
     "honei_hip_batch_info": [_p, C.POINTER(C.c_int64), C.POINTER(_u64), C.POINTER(C.c_int64)],
     "honei_hip_batch_slot": [_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64)],
     "honei_hip_batch_load": [_p, C.c_int64, _p, _p],
+    "honei_hip_batch_load_many": [_p, C.c_int64, _p, _p, _p],
+    "honei_hip_batch_load_seeded": [_p, C.c_int64, _p, _p, _p, _d],
     "honei_hip_batch_activate": [_p, C.c_int32],
     "honei_hip_batch_step": [_p],
     "honei_hip_batch_poll": [_p, _p],
@@ -482,6 +484,18 @@ class Batch:
 
     def load(self, slot, coeff: "DeviceArray", b: "DeviceArray"):
         call("honei_hip_batch_load", self.h, slot, coeff.ptr, b.ptr)
+
+    def load_many(self, slots, coeffs, bs):
+        n = len(slots)
+        sl = (C.c_int64 * n)(*slots)
+        cp = (_p * n)(*[c.ptr for c in coeffs])
+        bp = (_p * n)(*[b.ptr for b in bs])
+        call("honei_hip_batch_load_many", self.h, n, sl, cp, bp)
+
+    def load_seeded(self, slots, coeff_seeds, rhs_seeds, sigma=1.0):
+        n = len(slots)
+        call("honei_hip_batch_load_seeded", self.h, n, (C.c_int64 * n)(*slots), (_u64 * n)(*coeff_seeds),
+             (_u64 * n)(*rhs_seeds), sigma)
 
     def activate(self, wait=True):
         call("honei_hip_batch_activate", self.h, 1 if wait else 0)

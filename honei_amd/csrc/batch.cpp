@@ -62,6 +62,15 @@ void BatchMG::release() {
     if (s_main_) (void)hipStreamDestroy(s_main_);
     if (s_load_) (void)hipStreamDestroy(s_load_);
     s_main_ = s_load_ = nullptr;
+    if (build_ev_) (void)hipEventDestroy(build_ev_);
+    build_ev_ = nullptr;
+    if (build_host_) (void)hipHostFree(build_host_);
+    build_host_ = nullptr;
+    if (build_dev_) (void)hipFree(build_dev_);
+    build_dev_ = nullptr;
+    if (build_kappa_) (void)hipFree(build_kappa_);
+    build_kappa_ = nullptr;
+    build_cap_ = build_kappa_cap_ = 0;
     bytes_ = 0;
 }
 
@@ -132,7 +141,8 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
         if (err == hipSuccess) err = b_fill_state((SlotScalars *)scd, ns, SLOT_ACTIVE, s_main_);
         if (err == hipSuccess) {
             const int64_t bnd[2] = {0, ns};
-            err = b_compact((const SlotScalars *)scd, ns, bnd, 2, (int32_t *)lst, (int32_t *)lst + ns, false, s_main_);
+            err = b_compact((const SlotScalars *)scd, ns, bnd, 2, (int32_t *)lst, (int32_t *)lst + ns, LIST_NONIDLE,
+                            s_main_);
         }
         const BWork w{(const int32_t *)lst, (const int32_t *)lst + ns, 0, 1, 0};
         for (int block : {64, 128, 256, 512, 1024}) {
@@ -196,9 +206,7 @@ uint64_t BatchMG::plan_bytes() const {
         b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
         b += (uint64_t)(c.count * c.bps) * 8;
     }
-    int64_t cells = 0;
-    for (int64_t l = 1; l <= max_level_; ++l) cells += ((1ll << l)) * ((1ll << l));
-    b += (uint64_t)cells * 8 + (uint64_t)nslots_ * sizeof(SlotScalars);
+    b += (uint64_t)nslots_ * sizeof(SlotScalars);
     return b;
 }
 
@@ -244,6 +252,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
 
     HX(hipStreamCreateWithFlags(&s_main_, hipStreamNonBlocking));
     HX(hipStreamCreateWithFlags(&s_load_, hipStreamNonBlocking));
+    HX(hipEventCreateWithFlags(&build_ev_, hipEventDisableTiming));
     HX(hipEventCreateWithFlags(&null_ev_, hipEventDisableTiming));
 
     L_.assign((size_t)max_level_ + 1, BLevel());
@@ -292,9 +301,6 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
         HX(alloc((void **)&c.partials, (size_t)(c.count * c.bps) * 8));
         HX(q1_assemble(VT_F64, c.side, c.Aj, c.Ax, c.Arl, c.stride, nullptr, nullptr, 0.0, s_main_, 1));
     }
-    kappa_off_.assign((size_t)max_level_ + 2, 0);
-    for (int64_t l = 1; l <= max_level_; ++l) kappa_off_[(size_t)l + 1] = kappa_off_[(size_t)l] + (1ll << (2 * l));
-    HX(alloc((void **)&kappa_, (size_t)kappa_off_[(size_t)max_level_ + 1] * 8));
     HX(alloc((void **)&sc_, (size_t)nslots_ * sizeof(SlotScalars)));
     HX(alloc((void **)&list_, (size_t)nslots_ * sizeof(int32_t)));
     HX(alloc((void **)&bounds_, (size_t)(kMaxClasses + 1) * sizeof(int32_t)));
@@ -392,8 +398,9 @@ BClassTab BatchMG::class_table() const {
 //   list non-idle -> r = b - A x, |r| -> check -> list active -> rhs = r/|r|
 //   -> V-cycle -> x += |r| c.
 // Preconditioned CG (outer 1), z = |r| V(r/|r|):
-//   list non-idle -> r = b - A x for new / verifying slots -> check (recursive
-//   |r|) -> list active -> rhs = r/|r| -> V-cycle -> rho = r.z, beta ->
+//   list new / verifying -> r = b - A x for them -> list non-idle -> check
+//   (recursive |r| for running slots) -> list active -> rhs = r/|r| ->
+//   V-cycle -> rho = r.z, beta ->
 //   p = z + beta p -> q = A p, p.q -> alpha -> x += alpha p, r -= alpha q, |r|.
 hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     const int nb = (int)boundaries_.size();
@@ -402,10 +409,14 @@ hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     int64_t blocks = 0;
     for (const BClass &c : C_) blocks += c.count * c.bps;
     const bool cg = cfg_.outer == 1;
-    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, false, s));
-    HX(b_outer(cg ? OP_DEFECT_SETTLE : OP_DEFECT_ALL, tab, w, blocks, sc_, s));
+    if (cg) {   // CG keeps its recursive residual: only new / verifying slots need b - A x
+        HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_UNSETTLED, s));
+        HX(b_outer(OP_DEFECT_SETTLE, tab, w, blocks, sc_, s));
+    }
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_NONIDLE, s));
+    if (!cg) HX(b_outer(OP_DEFECT_ALL, tab, w, blocks, sc_, s));
     HX(b_finish(cg ? FIN_CHECK_CG : FIN_CHECK_DC, tab, w, nslots_, sc_, cfg_.tolerance, cfg_.max_outer, s));
-    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, true, s));
+    HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_ACTIVE, s));
     HX(b_outer(OP_SCALE, tab, w, blocks, sc_, s));
     HX(enqueue_vcycle(s));
     if (!cg) return b_outer(OP_ACC, tab, w, blocks, sc_, s);
@@ -434,58 +445,110 @@ hipError_t BatchMG::capture() {
 
 hipError_t BatchMG::load(int64_t g, const double *coeff, const double *b) {
     if (!coeff || !b) return hipErrorInvalidValue;
-    return load_impl(g, coeff, b, 0, 0.0, 0);
+    return load_many(1, &g, &coeff, &b, nullptr, nullptr, 0.0);
 }
 
 hipError_t BatchMG::load_seeded(int64_t g, uint64_t coeff_seed, double sigma, uint64_t rhs_seed) {
-    return load_impl(g, nullptr, nullptr, coeff_seed, sigma, rhs_seed);
+    return load_many(1, &g, nullptr, nullptr, &coeff_seed, &rhs_seed, sigma);
 }
 
-// coeff/b null: kappa = exp(sigma N(0,1)) from coeff_seed, b uniform in
-// [-1, 1) from rhs_seed, drawn on the device into the slot directly.
-hipError_t BatchMG::load_impl(int64_t g, const double *coeff, const double *b, uint64_t coeff_seed, double sigma,
-                              uint64_t rhs_seed) {
-    if (g < 0 || g >= nslots_) return hipErrorInvalidValue;
-    if (host_state_[(size_t)g] != 0) return hipErrorInvalidValue;
-    const BClass *cp = nullptr;
-    for (const BClass &c : C_)
-        if (g >= c.first && g < c.first + c.count) cp = &c;
-    const BClass &c = *cp;
-    const int64_t ls = g - c.first, Lc = c.level;
+// The build runs on the load stream beside the graph launches: the slots are
+// idle (their last graph work finished before the poll that retired them,
+// and result() copies synchronously), and idle slots are touched by no
+// kernel.  Caller data produced on the legacy default stream comes first.
+hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *const *coeffs,
+                              const double *const *bs, const uint64_t *cseeds, const uint64_t *bseeds, double sigma) {
+    if (n <= 0) return hipSuccess;
+    if (n > 65535 || !slots) return hipErrorInvalidValue;
+    const bool seeded = coeffs == nullptr;
+    if (seeded ? (!cseeds || !bseeds) : !bs) return hipErrorInvalidValue;
+    std::vector<int64_t> cls((size_t)n);
+    int64_t kappa_need = 0;
+    int maxl = 1;
+    for (int64_t e = 0; e < n; ++e) {
+        const int64_t g = slots[e];
+        if (g < 0 || g >= nslots_ || host_state_[(size_t)g] != 0) return hipErrorInvalidValue;
+        for (int64_t f = 0; f < e; ++f)
+            if (slots[f] == g) return hipErrorInvalidValue;
+        if (!seeded && (!coeffs[e] || !bs[e])) return hipErrorInvalidValue;
+        size_t k = 0;
+        while (!(g >= C_[k].first && g < C_[k].first + C_[k].count)) ++k;
+        cls[(size_t)e] = (int64_t)k;
+        const int L = (int)C_[k].level;
+        maxl = std::max(maxl, L);
+        for (int l = 1; l <= L; ++l) kappa_need += (1ll << (2 * l));
+    }
+    hipStream_t s = s_load_;
+    // staging and scratch of the previous build must be free
+    HX(hipEventSynchronize(build_ev_));
+    if (n > build_cap_) {
+        if (build_host_) (void)hipHostFree(build_host_);
+        if (build_dev_) (void)hipFree(build_dev_);
+        build_host_ = nullptr;
+        build_dev_ = nullptr;
+        build_cap_ = 0;
+        HX(hipHostMalloc((void **)&build_host_, sizeof(BuildEntry) * (size_t)n, hipHostMallocDefault));
+        HX(hipMalloc((void **)&build_dev_, sizeof(BuildEntry) * (size_t)n));
+        build_cap_ = n;
+    }
+    if (kappa_need > build_kappa_cap_) {
+        if (build_kappa_) (void)hipFree(build_kappa_);
+        build_kappa_ = nullptr;
+        build_kappa_cap_ = 0;
+        HX(hipMalloc((void **)&build_kappa_, sizeof(double) * (size_t)kappa_need));
+        build_kappa_cap_ = kappa_need;
+    }
     const ValueType vt = (ValueType)cfg_.inner_vtype;
     const size_t vs = vt_size(vt);
-    // The load runs beside the graph launches: the slot is idle (its last
-    // graph work finished before the poll that retired it, and result()
-    // copies synchronously), and idle slots are touched by no kernel.
-    hipStream_t s = s_load_;
-    if (coeff) {   // caller data produced on the legacy default stream comes first
+    BuildParams P{};
+    P.entries = (int32_t)n;
+    P.averaging = cfg_.averaging;
+    P.vt = vt;
+    P.sigma = sigma;
+    P.omega = cfg_.omega;
+    for (int64_t l = 1; l <= max_level_; ++l) {
+        P.side[l] = L_[(size_t)l].side;
+        P.stride[l] = L_[(size_t)l].stride;
+        P.T[l] = L_[(size_t)l].T;
+    }
+    for (const BClass &c : C_) P.stride64[c.level] = c.stride;
+    int64_t koff = 0;
+    for (int64_t e = 0; e < n; ++e) {
+        const int64_t g = slots[e];
+        const BClass &c = C_[(size_t)cls[(size_t)e]];
+        const int64_t ls = g - c.first;
+        BuildEntry &b = build_host_[e];
+        b = BuildEntry{};
+        b.level = (int32_t)c.level;
+        b.seeded = seeded ? 1 : 0;
+        b.coeff_seed = seeded ? cseeds[e] : 0;
+        b.rhs_seed = seeded ? bseeds[e] : 0;
+        b.coeff_src = seeded ? nullptr : coeffs[e];
+        b.b_src = seeded ? nullptr : bs[e];
+        for (int64_t l = 1; l <= c.level; ++l) {
+            const BLevel &lv = L_[(size_t)l];
+            b.kappa[l] = build_kappa_ + koff;
+            koff += (1ll << (2 * l));
+            b.Ax[l] = (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs;
+            b.dinv[l] = lv.dinv + g * lv.vld;
+        }
+        b.Ax64 = c.Ax + ls * c.ax_ld;
+        b.b = c.b + ls * c.vld;
+        b.x = c.x + ls * c.vld;
+    }
+    if (!seeded) {
         HX(hipEventRecord(null_ev_, nullptr));
         HX(hipStreamWaitEvent(s, null_ev_, 0));
     }
-    const int64_t cells_top = (1ll << Lc) * (1ll << Lc);
-    double *kap = kappa_ + kappa_off_[(size_t)Lc];
-    double *bx = c.b + ls * c.vld, *xx = c.x + ls * c.vld;
-    if (coeff) {
-        HX(hipMemcpyAsync(kap, coeff, (size_t)cells_top * 8, hipMemcpyDeviceToDevice, s));
-        HX(hipMemcpyAsync(bx, b, (size_t)c.n * 8, hipMemcpyDeviceToDevice, s));
-    } else {
-        HX(fill_lognormal(kap, cells_top, sigma, coeff_seed, s));
-        HX(fill_uniform(bx, c.n, -1.0, 1.0, rhs_seed, s));
+    HX(hipMemcpyAsync(build_dev_, build_host_, sizeof(BuildEntry) * (size_t)n, hipMemcpyHostToDevice, s));
+    HX(b_build(build_dev_, P, maxl, s));
+    HX(hipEventRecord(build_ev_, s));
+    for (int64_t e = 0; e < n; ++e) {
+        const int64_t g = slots[e];
+        HX(hipEventRecord(load_ev_[(size_t)g], s));
+        host_state_[(size_t)g] = 1;
+        loading_.push_back(g);
     }
-    for (int64_t l = Lc - 1; l >= 1; --l)
-        HX(coarsen_coeff(kappa_ + kappa_off_[(size_t)l + 1], 1ll << (l + 1), kappa_ + kappa_off_[(size_t)l], 1ll << l,
-                         cfg_.averaging, s));
-    for (int64_t l = Lc; l >= 1; --l) {
-        BLevel &lv = L_[(size_t)l];
-        HX(q1_assemble(vt, lv.side, nullptr, (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs, nullptr, lv.stride,
-                       kappa_ + kappa_off_[(size_t)l], lv.dinv + g * lv.vld, cfg_.omega, s, lv.T));
-    }
-    HX(q1_assemble(VT_F64, c.side, nullptr, c.Ax + ls * c.ax_ld, nullptr, c.stride, kap, nullptr, 0.0, s, 1));
-    HX(fill_bytes_zero(xx, (size_t)c.n * 8, s));
-    HX(boundary_guess(xx, bx, c.n, c.side, s));
-    HX(hipEventRecord(load_ev_[(size_t)g], s));
-    host_state_[(size_t)g] = 1;
-    loading_.push_back(g);
     return hipSuccess;
 }
 

@@ -138,7 +138,7 @@ struct BClassTab {
 
 enum OuterOp : int32_t {
     OP_DEFECT_ALL = 0,     // r = b - A x (+ |r|^2) for every listed slot
-    OP_DEFECT_SETTLE = 1,  // same, slots in ACTIVE state skipped (CG keeps its recursive residual)
+    OP_DEFECT_SETTLE = 1,  // same op; CG lists only new / verifying slots for it
     OP_SCALE = 2,          // rhs_top = (float)(r / |r|)
     OP_ACC = 3,            // x = (first cycle ? 0 : x) + |r| * x_top   (defect correction)
     OP_DOT_RZ = 4,         // partial r.z, z = |r| * x_top
@@ -158,6 +158,29 @@ struct BMat {
     const uint32_t *Arl;
     const void *Ax;
     int64_t stride, ax_ld, n, vld, bps;
+};
+
+// ---- bulk hierarchy build (many slots per launch) ----------------------------
+constexpr int kMaxLevel = 13;
+
+// One slot to build: coefficient and right-hand side from device arrays or
+// drawn from seeds; every level's per-slot values, the FP64 outer operator,
+// b and the start vector are written.
+struct BuildEntry {
+    int32_t level;                // finest level of the slot
+    int32_t seeded;               // 1: coefficient exp(sigma N(0,1)), b ~ U[-1,1) from the seeds
+    uint64_t coeff_seed, rhs_seed;
+    const double *coeff_src, *b_src;
+    double *kappa[kMaxLevel + 1]; // coefficient of level l ((2^l)^2 cells), scratch; kappa[level] finest
+    void *Ax[kMaxLevel + 1];      // this slot's inner values per level
+    float *dinv[kMaxLevel + 1];   // this slot's omega / diag per level
+    double *Ax64, *b, *x;         // FP64 outer values, right-hand side, start vector
+};
+struct BuildParams {
+    int32_t entries, averaging, vt, pad;
+    double sigma, omega;
+    int64_t side[kMaxLevel + 1], stride[kMaxLevel + 1], stride64[kMaxLevel + 1];
+    int32_t T[kMaxLevel + 1];
 };
 
 class BatchMG {
@@ -187,6 +210,10 @@ class BatchMG {
     // Same with the coefficient exp(sigma N(0,1)) and b uniform in [-1, 1)
     // drawn on the device from the seeds (synthetic requests).
     hipError_t load_seeded(int64_t g, uint64_t coeff_seed, double sigma, uint64_t rhs_seed);
+    // Many slots in one bulk build (a few launches for all of them): from
+    // device arrays (coeffs/bs non-null) or from seeds (cseeds/bseeds).
+    hipError_t load_many(int64_t n, const int64_t *slots, const double *const *coeffs, const double *const *bs,
+                         const uint64_t *cseeds, const uint64_t *bseeds, double sigma);
     // Activate finished loads, then one graph launch (iters_per_launch outer
     // iterations of every active slot).  Non-blocking.
     hipError_t step();
@@ -223,8 +250,7 @@ class BatchMG {
     int32_t *list_ = nullptr;          // active slots (device, b_compact)
     int32_t *bounds_ = nullptr;        // active counts below each class boundary
     std::vector<int64_t> boundaries_;  // class firsts, then nslots
-    double *kappa_ = nullptr;          // coefficient scratch of one load, all levels
-    std::vector<int64_t> kappa_off_;
+
     hipStream_t s_main_ = nullptr, s_load_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
@@ -236,8 +262,12 @@ class BatchMG {
     hipEvent_t null_ev_ = nullptr;
 
     hipError_t alloc(void **p, size_t bytes);
-    hipError_t load_impl(int64_t g, const double *coeff, const double *b, uint64_t coeff_seed, double sigma,
-                         uint64_t rhs_seed);
+    // bulk build state: device entry table, pinned staging, coefficient scratch
+    BuildEntry *build_dev_ = nullptr, *build_host_ = nullptr;
+    int64_t build_cap_ = 0;
+    double *build_kappa_ = nullptr;
+    int64_t build_kappa_cap_ = 0;
+    hipEvent_t build_ev_ = nullptr;
     hipError_t tune_level(BLevel &lv);
     hipError_t capture();
     hipError_t enqueue_iteration(hipStream_t s);
@@ -255,9 +285,12 @@ bool batch_tuned_shape(int dev, int64_t level, int vt, LaunchShape *out);
 // grid for up to `items` logical blocks
 int batch_grid(int64_t items);
 
-// active_only: list ACTIVE slots; else every non-idle slot.
+hipError_t b_build(const BuildEntry *entries_dev, const BuildParams &P, int max_level, hipStream_t s);
+
+// Which slots a compaction lists.
+enum ListMode : int { LIST_NONIDLE = 0, LIST_ACTIVE = 1, LIST_UNSETTLED = 2 /* NEW or VERIFY */ };
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
-                     int32_t *bounds, bool active_only, hipStream_t s);
+                     int32_t *bounds, int mode, hipStream_t s);
 hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
                  void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
 // FP64 outer work over the active slots of all classes (w: lo = 0, hi = nclass).

@@ -142,6 +142,35 @@ int honei_hip_batch_load(void *batch, int64_t slot, const double *coeff_dev, con
     return check(b->load(slot, coeff_dev, b_dev), "batch_load");
 }
 
+static int check_slots(BatchMG *b, int64_t n, const int64_t *slots) {
+    if (n < 0 || n > 65535 || (n && !slots)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad slot list");
+    for (int64_t e = 0; e < n; ++e) {
+        if (slots[e] < 0 || slots[e] >= b->nslots()) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "slot out of range");
+        if (b->busy(slots[e])) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "slot is loading or running");
+        for (int64_t f = 0; f < e; ++f)
+            if (slots[f] == slots[e]) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "slot listed twice");
+    }
+    return HONEI_HIP_SUCCESS;
+}
+
+int honei_hip_batch_load_many(void *batch, int64_t n, const int64_t *slots, const double *const *coeff_dev,
+                              const double *const *b_dev) {
+    if (!batch || (n > 0 && (!coeff_dev || !b_dev))) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    BatchMG *b = (BatchMG *)batch;
+    if (int st = check_slots(b, n, slots)) return st;
+    for (int64_t e = 0; e < n; ++e)
+        if (!coeff_dev[e] || !b_dev[e]) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null coefficient or rhs");
+    return check(b->load_many(n, slots, coeff_dev, b_dev, nullptr, nullptr, 0.0), "batch_load_many");
+}
+
+int honei_hip_batch_load_seeded(void *batch, int64_t n, const int64_t *slots, const uint64_t *coeff_seeds,
+                                const uint64_t *rhs_seeds, double sigma) {
+    if (!batch || (n > 0 && (!coeff_seeds || !rhs_seeds))) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    BatchMG *b = (BatchMG *)batch;
+    if (int st = check_slots(b, n, slots)) return st;
+    return check(b->load_many(n, slots, nullptr, nullptr, coeff_seeds, rhs_seeds, sigma), "batch_load_seeded");
+}
+
 int honei_hip_batch_activate(void *batch, int32_t wait) {
     if (!batch) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "batch is null");
     return check(((BatchMG *)batch)->activate_ready(wait != 0), "batch_activate");

@@ -17,6 +17,7 @@
 
 #include "batch.hh"
 #include "ell_device.hh"
+#include "q1_device.hh"
 #include "solver_device.hh"
 #include "transfer_device.hh"
 
@@ -41,18 +42,20 @@ __device__ inline int64_t active_count(const BWork &w) { return w.bounds[w.hi] -
 
 // list of non-idle slots (ascending) and, for each boundary k, the number of
 // them below boundaries[k]; one 1024-thread block.
-__device__ inline bool listed(const SlotScalars &s, bool active_only) {
-    return active_only ? s.state == SLOT_ACTIVE : s.state != SLOT_IDLE;
+__device__ inline bool listed(const SlotScalars &s, int mode) {
+    if (mode == LIST_ACTIVE) return s.state == SLOT_ACTIVE;
+    if (mode == LIST_UNSETTLED) return s.state == SLOT_NEW || s.state == SLOT_VERIFY;
+    return s.state != SLOT_IDLE;
 }
 
 __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64_t nslots, BBoundaries bd,
-                                                     int32_t *list, int32_t *bounds, bool active_only) {
+                                                     int32_t *list, int32_t *bounds, int mode) {
     __shared__ int32_t pre[1024];
     const int t = threadIdx.x;
     const int64_t chunk = (nslots + 1023) / 1024;
     const int64_t s0 = t * chunk, s1 = (s0 + chunk < nslots) ? s0 + chunk : nslots;
     int32_t cnt = 0;
-    for (int64_t g = s0; g < s1; ++g) cnt += listed(sc[g], active_only);
+    for (int64_t g = s0; g < s1; ++g) cnt += listed(sc[g], mode);
     pre[t] = cnt;
     __syncthreads();
     // inclusive scan (Hillis-Steele) over 1024 counts
@@ -64,13 +67,13 @@ __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64
     }
     int32_t pos = pre[t] - cnt;
     for (int64_t g = s0; g < s1; ++g)
-        if (listed(sc[g], active_only)) list[pos++] = (int32_t)g;
+        if (listed(sc[g], mode)) list[pos++] = (int32_t)g;
     if (t < bd.n) {
         const int64_t b = bd.v[t];
         const int64_t owner = chunk > 0 ? b / chunk : 0;
         int32_t c = (owner > 0 && owner <= 1023) ? pre[owner - 1] : (owner > 1023 ? pre[1023] : 0);
         if (owner <= 1023)
-            for (int64_t g = owner * chunk; g < b && g < nslots; ++g) c += listed(sc[g], active_only);
+            for (int64_t g = owner * chunk; g < b && g < nslots; ++g) c += listed(sc[g], mode);
         bounds[t] = c;
     }
 }
@@ -98,9 +101,9 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
         V s = V(0);
         if (active) {
             if constexpr (MODE == ELL_JACOBI0)
-                s = ell_lane_sum<VT, T, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, rs, ds);
+                s = ell_lane_sum<VT, T, true, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, rs, ds);
             else
-                s = ell_lane_sum<VT, T>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, xs);
+                s = ell_lane_sum<VT, T, false, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, xs);
         }
         s = lane_group_sum<T>(s);
         if (active && lane == 0) {
@@ -142,7 +145,6 @@ __global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const S
         const int64_t g = w.list[w.bounds[c] + off / e.bps];
         const int64_t ls = g - e.first, hb = off % e.bps;
         const SlotScalars &st = sc[g];
-        if (OP == OP_DEFECT_SETTLE && st.state == SLOT_ACTIVE) continue;   // uniform per block
         const int64_t rb = xcd_block(hb, e.bps);
         const int64_t row = rb * 256 + threadIdx.x;
         const bool in = row < e.n;
@@ -150,8 +152,8 @@ __global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const S
         double part = 0.0;
         if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE) {
             if (in) {
-                const double s = ell_lane_sum<double, 1>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row], e.stride,
-                                                         e.x + vo);
+                const double s = ell_lane_sum<double, 1, false, true>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row],
+                                                                      e.stride, e.x + vo);
                 const double r = e.b[vi] - s;
                 e.r[vi] = r;
                 part = r * r;
@@ -172,8 +174,8 @@ __global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const S
             }
         } else if constexpr (OP == OP_SPMV_DOT) {
             if (in) {
-                const double q = ell_lane_sum<double, 1>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row], e.stride,
-                                                         e.p + vo);
+                const double q = ell_lane_sum<double, 1, false, true>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row],
+                                                                      e.stride, e.p + vo);
                 e.q[vi] = q;
                 part = e.p[vi] * q;
             }
@@ -349,6 +351,50 @@ __global__ void k_b_fill_state(SlotScalars *sc, int64_t n, int32_t state) {
     }
 }
 
+// ---- bulk hierarchy build: blockIdx.y = entry ------------------------------------
+// finest coefficient and b (copied or drawn), start vector x0 (b on the
+// first row, last row and first column, else 0)
+__global__ void k_bb_fields(const BuildEntry *entries, BuildParams P) {
+    const BuildEntry &e = entries[blockIdx.y];
+    const int64_t side = P.side[e.level], n = side * side, cells = (side - 1) * (side - 1);
+    double *kap = e.kappa[e.level];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < cells) kap[i] = e.seeded ? lognormal_at(P.sigma, e.coeff_seed, (uint64_t)i) : e.coeff_src[i];
+        const double bi = e.seeded ? uniform_at(-1.0, 1.0, e.rhs_seed, (uint64_t)i) : e.b_src[i];
+        e.b[i] = bi;
+        e.x[i] = guess_node(i, n, side) ? bi : 0.0;
+    }
+}
+
+// coefficient of level l from level l + 1, for entries finer than l
+__global__ void k_bb_coarsen(const BuildEntry *entries, BuildParams P, int l) {
+    const BuildEntry &e = entries[blockIdx.y];
+    if (e.level <= l) return;
+    const int64_t nc = P.side[l] - 1, cells = nc * nc;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < cells; id += (int64_t)gridDim.x * blockDim.x)
+        coarsen_cell(e.kappa[l + 1], 2 * nc, e.kappa[l], nc, P.averaging, id);
+}
+
+// inner values and dinv of level l (shared Aj / Arl untouched)
+template <typename VT>
+__global__ void k_bb_level(const BuildEntry *entries, BuildParams P, int l) {
+    const BuildEntry &e = entries[blockIdx.y];
+    if (e.level < l) return;
+    const int T = P.T[l];
+    const int64_t rows = P.stride[l] / T;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
+        q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], e.dinv[l], P.omega);
+}
+
+// FP64 outer values of the finest level
+__global__ void k_bb_outer(const BuildEntry *entries, BuildParams P) {
+    const BuildEntry &e = entries[blockIdx.y];
+    const int L = e.level;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < P.stride64[L];
+         r += (int64_t)gridDim.x * blockDim.x)
+        q1_row<double>(P.side[L], 1, r, nullptr, e.Ax64, nullptr, P.stride64[L], e.kappa[L], nullptr, 0.0);
+}
+
 typedef void (*BEllFn)(BMat, BWork, void *, const void *, const void *, const void *);
 
 template <typename VT, int T>
@@ -379,12 +425,12 @@ int entry_grid(int64_t max_slots, int64_t n) { return batch_grid(max_slots * ((n
 int batch_grid(int64_t items) { return (int)(items < 1 ? 1 : (items > kMaxGrid ? kMaxGrid : items)); }
 
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
-                     int32_t *bounds, bool active_only, hipStream_t s) {
+                     int32_t *bounds, int mode, hipStream_t s) {
     if (nb < 1 || nb > kMaxClasses + 1 || nslots < 1) return hipErrorInvalidValue;
     BBoundaries bd{};
     bd.n = nb;
     for (int k = 0; k < nb; ++k) bd.v[k] = boundaries[k];
-    hipLaunchKernelGGL(k_b_compact, dim3(1), dim3(1024), 0, s, sc, nslots, bd, list, bounds, active_only);
+    hipLaunchKernelGGL(k_b_compact, dim3(1), dim3(1024), 0, s, sc, nslots, bd, list, bounds, mode);
     return hipGetLastError();
 }
 
@@ -481,6 +527,32 @@ hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream
                            (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t b_build(const BuildEntry *entries, const BuildParams &P, int max_level, hipStream_t s) {
+    if (P.entries <= 0) return hipSuccess;
+    if (P.entries > 65535 || max_level < 1 || max_level > kMaxLevel) return hipErrorInvalidValue;
+    auto grid = [&](int64_t work) {
+        int64_t g = (work + 255) / 256;
+        return dim3((unsigned)(g < 1 ? 1 : (g > 2048 ? 2048 : g)), (unsigned)P.entries);
+    };
+    const int64_t nmax = P.side[max_level] * P.side[max_level];
+    hipLaunchKernelGGL(k_bb_fields, grid(nmax), dim3(256), 0, s, entries, P);
+    for (int l = max_level - 1; l >= 1; --l) {
+        const int64_t nc = P.side[l] - 1;
+        hipLaunchKernelGGL(k_bb_coarsen, grid(nc * nc), dim3(256), 0, s, entries, P, l);
+    }
+    for (int l = max_level; l >= 1; --l) {
+        const int64_t rows = P.stride[l] / P.T[l];
+        if (P.vt == VT_F32)
+            hipLaunchKernelGGL(k_bb_level<float>, grid(rows), dim3(256), 0, s, entries, P, l);
+        else if (P.vt == VT_BF16)
+            hipLaunchKernelGGL(k_bb_level<bf16>, grid(rows), dim3(256), 0, s, entries, P, l);
+        else
+            return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_bb_outer, grid(P.stride64[max_level]), dim3(256), 0, s, entries, P);
     return hipGetLastError();
 }
 

@@ -14,8 +14,10 @@ namespace hipx {
 // slots row*T + lane + k*stride for k < len and adds Ax * x[Aj].  With
 // SCALED the gathered operand is dinv[j] * x[j] instead (first Jacobi sweep
 // from a zero start, where the iterate is dinv .* rhs).  Multiply and add are
-// separate roundings (callers compile with fp contract off).
-template <typename VT, int T, bool SCALED = false>
+// separate roundings (callers compile with fp contract off).  Ax is streamed
+// with non-temporal loads; Aj too unless SHARED_AJ (one index array serving
+// many value arrays, as in a batch, where it should stay in L2 / MALL).
+template <typename VT, int T, bool SCALED = false, bool SHARED_AJ = false>
 __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restrict__ Aj, const VT *__restrict__ Ax,
                                                         int64_t row, int lane, uint32_t len, int64_t stride,
                                                         const typename VecOf<VT>::type *__restrict__ x,
@@ -25,7 +27,11 @@ __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restric
     int64_t slot = row * T + lane;
 #pragma unroll 4
     for (uint32_t k = 0; k < len; ++k) {
-        const int32_t j = __builtin_nontemporal_load(Aj + slot);
+        int32_t j;
+        if constexpr (SHARED_AJ)
+            j = Aj[slot];
+        else
+            j = __builtin_nontemporal_load(Aj + slot);
         const V av = load_value_nt(Ax + slot);
         V xv;
         if constexpr (SCALED)

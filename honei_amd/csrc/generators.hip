@@ -12,6 +12,7 @@
 // cell coefficients, element i drawn from position i of one SplitMix64
 // sequence, so any element can be generated independently.
 
+#include "q1_device.hh"
 #include "solvers.hh"
 
 namespace honei {
@@ -85,33 +86,16 @@ __global__ void k_random_ell(int64_t n, int max_len, int64_t band, uint64_t seed
     Arl[r] = (uint32_t)((have + T - 1) / T);
 }
 
-// Output number i+1 of the SplitMix64 sequence started at `seed` (the same
-// draw the Python helper splitmix64(seed, n)[i] gives).
-__device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t i) {
-    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-__device__ inline double u01_at(uint64_t seed, uint64_t i) {
-    return (double)(splitmix_at(seed, i) >> 11) * (1.0 / 9007199254740992.0);
-}
-
 // dst[i] = lo + (hi - lo) * U_i
 __global__ void k_fill_uniform(double *dst, int64_t n, double lo, double hi, uint64_t seed) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = lo + (hi - lo) * u01_at(seed, (uint64_t)i);
+        dst[i] = uniform_at(lo, hi, seed, (uint64_t)i);
 }
 
-// dst[i] = exp(sigma * z_i), z_i standard normal by Box-Muller from draws
-// 2i (u1 = 1 - U, never 0) and 2i+1.
+// dst[i] = exp(sigma * z_i) (lognormal_at)
 __global__ void k_fill_lognormal(double *dst, int64_t n, double sigma, uint64_t seed) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double u1 = 1.0 - u01_at(seed, 2 * (uint64_t)i);
-        const double u2 = u01_at(seed, 2 * (uint64_t)i + 1);
-        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-        dst[i] = exp(sigma * z);
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = lognormal_at(sigma, seed, (uint64_t)i);
 }
 
 inline int field_grid(int64_t n) {

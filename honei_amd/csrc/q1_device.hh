@@ -1,0 +1,150 @@
+// q1_device.hh -- per-row / per-cell device code of the Q1 hierarchy build,
+// shared by the single-system kernels (one system per launch) and the bulk
+// build of the batched solver (many systems per launch), so both produce the
+// same bits.
+#pragma once
+
+// Multiply and add stay separate roundings in everything that follows.
+#pragma clang fp contract(off)
+
+#include "hip_common.hh"
+
+namespace honei {
+namespace hipx {
+
+template <typename VT, typename V>
+__device__ inline void q1_put(VT *Ax, int64_t s, V v) { Ax[s] = (VT)v; }
+template <>
+__device__ inline void q1_put<bf16, float>(bf16 *Ax, int64_t s, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    Ax[s].bits = (uint16_t)(u >> 16);
+}
+
+// Row `row` of the Q1 operator on a side x side grid in the T-interleaved
+// ELL layout: entry t at slot (t % T) + row*T + (t / T)*stride, ncpr =
+// ceil(9 / T) chunks, unused slots (and padding rows n <= row < stride/T)
+// Aj = 0, Ax = 0.  Boundary rows: identity.  Inner rows: the constant
+// stencil (coeff null: DT(8/3), DT(-1/3), as FillMatrix<POISSON,
+// DIRICHLET_0>, math/fill_matrix.hh:36-78) or the bilinear element
+// stiffness summed over the four cells around the node (coeff[(side-1)^2]).
+// Aj / Arl null: values only.  dinv (optional) = omega / diag as stored.
+// Entries are written at compile-time indices (no runtime-indexed register
+// arrays).
+template <typename VT>
+__device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT *Ax, uint32_t *Arl, int64_t stride,
+                              const double *coeff, typename VecOf<VT>::type *dinv, double omega) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t n = side * side;
+    const int ncpr = (9 + T - 1) / T;
+    int cnt = 0;
+    int64_t cols[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    V w[9] = {V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0)};
+    if (row < n) {
+        const int64_t i = row / side, j = row % side;
+        const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
+        if (boundary) {
+            cnt = 1;
+            cols[0] = row;
+            w[0] = V(1.0);
+        } else {
+            cnt = 9;
+            if (!coeff) {
+                const V c = V(-1.0 / 3.0);
+                for (int k = 0; k < 9; ++k) w[k] = c;
+                w[4] = V(8.0 / 3.0);
+            } else {
+                const int64_t nc = side - 1;
+                const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
+                const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+                w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
+                w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
+                w[2] = (V)(-se / 3.0);              // (i-1, j+1)
+                w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
+                w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
+                w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
+                w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
+                w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
+                w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+            }
+            const int64_t c0[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
+                                   row + 1,        row + side - 1, row + side, row + side + 1};
+            for (int k = 0; k < 9; ++k) cols[k] = c0[k];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
+        const bool have = t < cnt;
+        if (Aj) Aj[slot] = have ? (int32_t)cols[t] : 0;
+        q1_put<VT, V>(Ax, slot, have ? w[t] : V(0));
+    }
+    for (int t = 9; t < T * ncpr; ++t) {   // padding lanes of the last chunk
+        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
+        if (Aj) Aj[slot] = 0;
+        q1_put<VT, V>(Ax, slot, V(0));
+    }
+    if (row >= n) return;
+    if (Arl) Arl[row] = (uint32_t)((cnt + T - 1) / T);
+    if (dinv) {
+        V d = (cnt == 1) ? w[0] : w[4];
+        if constexpr (sizeof(VT) == 2) {   // diagonal as stored (bf16-rounded)
+            uint32_t u = __float_as_uint(d);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            d = __uint_as_float((u >> 16) << 16);
+        }
+        const V inv = V(1) / d;
+        dinv[row] = inv * (V)omega;
+    }
+}
+
+// Coarse cell id = (I, J) of an nc x nc cell grid from fine cells (2I..2I+1,
+// 2J..2J+1) of an nf = 2 nc grid: averaging 0 arithmetic, 1 geometric,
+// 2 harmonic mean.
+__device__ inline void coarsen_cell(const double *fine, int64_t nf, double *coarse, int64_t nc, int averaging,
+                                    int64_t id) {
+    const int64_t I = id / nc, J = id % nc;
+    const double a = fine[(2 * I) * nf + 2 * J], b = fine[(2 * I) * nf + 2 * J + 1];
+    const double c = fine[(2 * I + 1) * nf + 2 * J], d = fine[(2 * I + 1) * nf + 2 * J + 1];
+    double m;
+    if (averaging == 1)
+        m = exp(0.25 * (((log(a) + log(b)) + log(c)) + log(d)));
+    else if (averaging == 2)
+        m = 4.0 / (((1.0 / a + 1.0 / b) + 1.0 / c) + 1.0 / d);
+    else
+        m = 0.25 * (((a + b) + c) + d);
+    coarse[id] = m;
+}
+
+// Output number i+1 of the SplitMix64 sequence started at `seed` (the same
+// draw the Python helper splitmix64(seed, n)[i] gives); U = (u >> 11) 2^-53.
+__device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ inline double u01_at(uint64_t seed, uint64_t i) {
+    return (double)(splitmix_at(seed, i) >> 11) * (1.0 / 9007199254740992.0);
+}
+// lo + (hi - lo) U_i
+__device__ inline double uniform_at(double lo, double hi, uint64_t seed, uint64_t i) {
+    return lo + (hi - lo) * u01_at(seed, i);
+}
+// exp(sigma z_i), z_i standard normal by Box-Muller from draws 2i (u1 = 1 -
+// U, never 0) and 2i+1
+__device__ inline double lognormal_at(double sigma, uint64_t seed, uint64_t i) {
+    const double u1 = 1.0 - u01_at(seed, 2 * i);
+    const double u2 = u01_at(seed, 2 * i + 1);
+    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    return exp(sigma * z);
+}
+
+// Start vector of the MIXED driver: first grid row, last grid row and first
+// grid column take b (math/multigrid.hh:2423-2440).
+__device__ inline bool guess_node(int64_t i, int64_t n, int64_t side) {
+    return i < side || i >= n - side || (i % side) == 0;
+}
+
+}  // namespace hipx
+}  // namespace honei

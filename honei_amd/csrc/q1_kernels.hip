@@ -12,6 +12,7 @@
 // element stiffness kappa/6 * [[4,-1,-2,-1],...] summed over the four cells
 // around a node; kappa == 1 gives the constant stencil above.
 
+#include "q1_device.hh"
 #include "solvers.hh"
 
 #pragma clang fp contract(off)
@@ -20,106 +21,20 @@ namespace honei {
 namespace hipx {
 namespace {
 
-template <typename VT, typename V>
-__device__ inline void put(VT *Ax, int64_t s, V v) { Ax[s] = (VT)v; }
-template <>
-__device__ inline void put<bf16, float>(bf16 *Ax, int64_t s, float v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    Ax[s].bits = (uint16_t)(u >> 16);
-}
-
-// One thread per row of the T-interleaved layout: entry t of the row goes to
-// slot (t % T) + row*T + (t / T)*stride, ncpr = ceil(9 / T) chunks, unused
-// slots (and the padding rows n <= row < stride/T) hold Aj = 0, Ax = 0.
-// Aj / Arl may be null to rewrite values only (shared index arrays).
+// One thread per row of the T-interleaved layout (rows up to stride/T cover
+// the padding); the row arithmetic is q1_row (q1_device.hh).
 template <typename VT>
 __global__ void k_q1_assemble(int64_t side, int T, int32_t *Aj, VT *Ax, uint32_t *Arl, int64_t stride,
                               const double *coeff, typename VecOf<VT>::type *dinv, double omega) {
-    typedef typename VecOf<VT>::type V;
-    const int64_t n = side * side;
     const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (row * T >= stride) return;
-    const int ncpr = (9 + T - 1) / T;
-    int cnt = 0;
-    int64_t cols[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    V w[9] = {V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0), V(0)};
-    if (row < n) {
-        const int64_t i = row / side, j = row % side;
-        const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
-        if (boundary) {
-            cnt = 1;
-            cols[0] = row;
-            w[0] = V(1.0);
-        } else {
-            cnt = 9;
-            if (!coeff) {
-                const V c = V(-1.0 / 3.0);
-                for (int k = 0; k < 9; ++k) w[k] = c;
-                w[4] = V(8.0 / 3.0);
-            } else {
-                const int64_t nc = side - 1;
-                const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
-                const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
-                w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
-                w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
-                w[2] = (V)(-se / 3.0);              // (i-1, j+1)
-                w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
-                w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
-                w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
-                w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
-                w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
-                w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
-            }
-            const int64_t c0[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
-                                   row + 1,        row + side - 1, row + side, row + side + 1};
-            for (int k = 0; k < 9; ++k) cols[k] = c0[k];
-        }
-    }
-    // entries at compile-time indices (no runtime-indexed register arrays)
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
-        const bool have = t < cnt;
-        if (Aj) Aj[slot] = have ? (int32_t)cols[t] : 0;
-        put<VT, V>(Ax, slot, have ? w[t] : V(0));
-    }
-    for (int t = 9; t < T * ncpr; ++t) {   // padding lanes of the last chunk
-        const int64_t slot = (t % T) + row * T + (int64_t)(t / T) * stride;
-        if (Aj) Aj[slot] = 0;
-        put<VT, V>(Ax, slot, V(0));
-    }
-    if (row >= n) return;
-    if (Arl) Arl[row] = (uint32_t)((cnt + T - 1) / T);
-    if (dinv) {
-        V d = (cnt == 1) ? w[0] : w[4];
-        if constexpr (sizeof(VT) == 2) {   // diagonal as stored (bf16-rounded)
-            uint32_t u = __float_as_uint(d);
-            u += 0x7fffu + ((u >> 16) & 1u);
-            d = __uint_as_float((u >> 16) << 16);
-        }
-        const V inv = V(1) / d;
-        dinv[row] = inv * (V)omega;
-    }
+    q1_row<VT>(side, T, row, Aj, Ax, Arl, stride, coeff, dinv, omega);
 }
 
-// coarse cell (I, J) = mean of fine cells (2I..2I+1, 2J..2J+1):
-// averaging 0 arithmetic, 1 geometric, 2 harmonic
 __global__ void k_coarsen_coeff(const double *fine, int64_t nf_cells, double *coarse, int64_t nc_cells,
                                 int averaging) {
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (id >= nc_cells * nc_cells) return;
-    const int64_t I = id / nc_cells, J = id % nc_cells;
-    const double a = fine[(2 * I) * nf_cells + 2 * J], b = fine[(2 * I) * nf_cells + 2 * J + 1];
-    const double c = fine[(2 * I + 1) * nf_cells + 2 * J], d = fine[(2 * I + 1) * nf_cells + 2 * J + 1];
-    double m;
-    if (averaging == 1)
-        m = exp(0.25 * (((log(a) + log(b)) + log(c)) + log(d)));
-    else if (averaging == 2)
-        m = 4.0 / (((1.0 / a + 1.0 / b) + 1.0 / c) + 1.0 / d);
-    else
-        m = 0.25 * (((a + b) + c) + d);
-    coarse[id] = m;
+    if (id < nc_cells * nc_cells) coarsen_cell(fine, nf_cells, coarse, nc_cells, averaging, id);
 }
 
 }  // namespace

@@ -65,7 +65,8 @@ hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n
     ServeStats st;
     st.device_bytes = pool.bytes();
     std::vector<SlotScalars> sc;
-    std::vector<int64_t> retired;
+    std::vector<int64_t> retired, adm_slots;
+    std::vector<uint64_t> adm_cs, adm_bs;
     const auto t0 = clock::now();
     auto elapsed = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
     int64_t next = 0, finished = 0, queued = 0;
@@ -83,6 +84,10 @@ hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n
             ++queued;
         }
         st.max_queue = std::max(st.max_queue, queued);
+        // admit into free slots, one bulk build for everything admitted now
+        adm_slots.clear();
+        adm_cs.clear();
+        adm_bs.clear();
         for (int c = 0; c < cfg.nclass; ++c) {
             auto &q = queue[(size_t)c];
             auto &fs = free_slots[(size_t)c];
@@ -91,12 +96,17 @@ hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n
                 q.pop_front();
                 fs.pop_back();
                 --queued;
-                HX(pool.load_seeded(g, reqs[r].coeff_seed, cfg.sigma, reqs[r].rhs_seed));
+                adm_slots.push_back(g);
+                adm_cs.push_back(reqs[r].coeff_seed);
+                adm_bs.push_back(reqs[r].rhs_seed);
                 slot_req[(size_t)g] = r;
                 out[r].slot = g;
                 out[r].admit_s = elapsed();
             }
         }
+        if (!adm_slots.empty())
+            HX(pool.load_many((int64_t)adm_slots.size(), adm_slots.data(), nullptr, nullptr, adm_cs.data(),
+                              adm_bs.data(), cfg.sigma));
         if (pool.running() + pool.loading_count() == 0) {
             if (next < n) {
                 const double wait = reqs[next].arrival_s - elapsed();

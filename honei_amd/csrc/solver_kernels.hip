@@ -14,6 +14,7 @@
 #include "solvers.hh"
 
 #include "ell_device.hh"
+#include "q1_device.hh"
 #include "solver_device.hh"
 
 #pragma clang fp contract(off)
@@ -140,10 +141,8 @@ __global__ __launch_bounds__(256) void k_coarse_cg(EllView A, const typename Vec
 __global__ void k_boundary_guess(double *guess, const double *b, int64_t n, int64_t side) {
     // first grid row, last grid row, first grid column take b
     // (math/multigrid.hh:2423-2440)
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const bool bd = i < side || i >= n - side || (i % side) == 0;
-        if (bd) guess[i] = b[i];
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (guess_node(i, n, side)) guess[i] = b[i];
 }
 
 __global__ void k_acc_scaled_f32(double *result, const float *x, double scale, int64_t n) {

@@ -356,6 +356,13 @@ int honei_hip_batch_slot(void *batch, int32_t cls, int64_t j, int64_t *slot);
  * ((side-1)^2 doubles) and b_dev (side^2 doubles); asynchronous, on the
  * batch's load stream.  The slot joins at the next step after the copy. */
 int honei_hip_batch_load(void *batch, int64_t slot, const double *coeff_dev, const double *b_dev);
+/* Many slots in one bulk build (a handful of launches for all of them):
+ * from device arrays (coeff_dev[i], b_dev[i]) or drawn from seeds
+ * (coefficient exp(sigma N(0,1)), b uniform in [-1, 1)).                 */
+int honei_hip_batch_load_many(void *batch, int64_t n, const int64_t *slots, const double *const *coeff_dev,
+                              const double *const *b_dev);
+int honei_hip_batch_load_seeded(void *batch, int64_t n, const int64_t *slots, const uint64_t *coeff_seeds,
+                                const uint64_t *rhs_seeds, double sigma);
 /* Hand loads that have finished (wait != 0: all loads issued so far, blocking)
  * to the next launch; honei_hip_batch_step does this without waiting.     */
 int honei_hip_batch_activate(void *batch, int32_t wait);

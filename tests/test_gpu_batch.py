@@ -279,3 +279,43 @@ def test_batch_cg_converges_where_defect_correction_diverges():
     bd[::side] = True
     r0 = np.linalg.norm(oracle.defect(A, bh, np.where(bd, bh, 0.0)))
     assert np.linalg.norm(oracle.defect(A, bh, x)) / r0 < 1.001 * TOL
+
+
+def test_bulk_loads_equal_single_loads():
+    """One bulk build of several slots (from device arrays, and from seeds)
+    gives the same solutions, bit for bit, as loading the slots one by one
+    from the same data."""
+    levels, counts = [5, 4], [2, 2]
+    seeds = [poisson_system_seeds(21, i) for i in range(4)]
+    data = []
+    for i, L in enumerate([5, 5, 4, 4]):
+        side = 2 ** L + 1
+        k = H.DeviceArray((side - 1) ** 2, np.float64)
+        b = H.DeviceArray(side * side, np.float64)
+        H.fill_lognormal(k, 1.0, seeds[i][0])
+        H.fill_uniform(b, -1.0, 1.0, seeds[i][1])
+        data.append((k, b))
+    H.synchronize()
+    outs = []
+    for mode in ("single", "many", "seeded"):
+        B = H.Batch(levels, counts, H.batch_config())
+        slots = [B.slot(0, 0), B.slot(0, 1), B.slot(1, 0), B.slot(1, 1)]
+        if mode == "single":
+            for g, (k, b) in zip(slots, data):
+                B.load(g, k, b)
+        elif mode == "many":
+            B.load_many(slots, [k for k, _ in data], [b for _, b in data])
+        else:
+            B.load_seeded(slots, [s[0] for s in seeds], [s[1] for s in seeds])
+        B.run()
+        info = B.poll()
+        xs = []
+        for g, (k, b) in zip(slots, data):
+            x = H.DeviceArray(b.n, np.float64)
+            B.result(g, x)
+            xs.append((info[g].iters, info[g].status, x.get()))
+        outs.append(xs)
+    for other in outs[1:]:
+        for (i0, s0, x0), (i1, s1, x1) in zip(outs[0], other):
+            assert (i0, s0) == (i1, s1) and s0 == 0
+            assert np.array_equal(x0, x1)
