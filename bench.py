@@ -250,19 +250,20 @@ def serving_requests(seconds, rate=SERVE_RATE, seed=SERVE_SEED):
 
 def batch_kernel_roofline(B, level):
     """Bandwidth of the batched Jacobi sweep on `level` from the autotuner's
-    own timing (tuning batch of `slots` systems)."""
-    T, block, us, ns = B.tuning(level)
+    own timing (tuning batch of `slots` systems sharing one index array)."""
+    T, block, group, us, ns = B.tuning(level)
     if us <= 0:
         return None
     side = 2 ** level + 1
     n = side * side
     nnz = 9 * (side - 2) ** 2 + 4 * (side - 1)
-    # values + indices + x, rhs, dinv, out per row
-    bytes_ = ns * (nnz * (4 + 4) + n * 4 * 4)
+    # per system: FP32 values + x, rhs, dinv, out per row; once: the shared indices and row lengths
+    bytes_ = ns * (nnz * 4 + n * 4 * 4) + nnz * 4 + n * 4
     gbs = bytes_ / (us * 1e-6) / 1e9
-    return {"kernel": f"k_b_ell<float,{T},JACOBI> level {level} ({side}^2), block {block}", "slots": ns,
+    return {"kernel": f"k_b_ell<float,{T},JACOBI,{group}> level {level} ({side}^2), block {block}", "slots": ns,
             "us": round(us, 2), "algorithmic_bytes": int(bytes_), "achieved_gbs": round(gbs, 1),
-            "frac": round(gbs / PEAK_GBS, 4)}
+            "frac": round(gbs / PEAK_GBS, 4),
+            "bytes_rule": "slots*(nnz*4 + n*16) + nnz*4 + n*4: the column indices are shared by all slots"}
 
 
 def cpu_baseline_batched(levels, budget_s=15.0):
@@ -354,7 +355,8 @@ def main():
             sec, done, nl, info = batched_step(B, slots, kap, rhs)
         total = time.perf_counter() - t0
         print(json.dumps({"batch_only": {"solves_per_s": round(len(levels) * args.steps / total, 2),
-                                         "steps": args.steps, "launches_last_step": nl}}))
+                                         "steps": args.steps, "launches_last_step": nl,
+                                         "shapes": {L: B.tuning(L) for L in range(2, max(levels) + 1)}}}))
         return
 
     if args.serve_only:

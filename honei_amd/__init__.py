@@ -109,8 +109,8 @@ _SIGS = {
     "honei_hip_batch_poll": [_p, _p],
     "honei_hip_batch_result": [_p, C.c_int64, _p],
     "honei_hip_batch_run": [_p, C.POINTER(C.c_int64)],
-    "honei_hip_batch_shape": [_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(_d),
-                              C.POINTER(C.c_int64)],
+    "honei_hip_batch_shape": [_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(_d), C.POINTER(C.c_int64)],
     "honei_hip_serve_run": [_p, _p, C.c_int64, _p, _p],
 }
 _RESTYPE = {"honei_hip_last_error": C.c_char_p, "honei_hip_version": C.c_char_p}
@@ -517,15 +517,16 @@ class Batch:
         return k.value
 
     def shape(self, level):
-        t, b = C.c_int32(), C.c_int32()
-        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), None, None)
-        return t.value, b.value
+        """(threads per row, block size, slots per thread) of an inner level."""
+        t, b, g = C.c_int32(), C.c_int32(), C.c_int32()
+        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), C.byref(g), None, None)
+        return t.value, b.value, g.value
 
     def tuning(self, level):
-        """(threads, block, measured sweep us, tuning-batch slots)."""
-        t, b, us, ns = C.c_int32(), C.c_int32(), _d(), C.c_int64()
-        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), C.byref(us), C.byref(ns))
-        return t.value, b.value, us.value, ns.value
+        """(threads, block, slots per thread, measured sweep us, tuning-batch slots)."""
+        t, b, g, us, ns = C.c_int32(), C.c_int32(), C.c_int32(), _d(), C.c_int64()
+        call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), C.byref(g), C.byref(us), C.byref(ns))
+        return t.value, b.value, g.value, us.value, ns.value
 
     def launches(self):
         k = C.c_int64()

@@ -110,6 +110,7 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
         if (it != g_tuned.end()) {
             lv.T = it->second.T;
             lv.block = it->second.block;
+            lv.group = it->second.group;
             return hipSuccess;
         }
     }
@@ -146,26 +147,29 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
         }
         const BWork w{(const int32_t *)lst, (const int32_t *)lst + ns, 0, 1, 0};
         for (int block : {64, 128, 256, 512, 1024}) {
-            if (err != hipSuccess) break;
-            if (block < T) continue;
-            BMat m{(const int32_t *)Aj, (const uint32_t *)Arl, Ax, g.stride, g.ax_ld, n, g.vld,
-                   (n * T + block - 1) / block};
-            float *dinv = (float *)vec, *x = dinv + ns * g.vld, *rhs = x + ns * g.vld, *out = rhs + ns * g.vld;
-            const int reps = n >= 65536 ? 10 : 30;
-            for (int r = 0; r < 3 && err == hipSuccess; ++r)
-                err = b_ell(vt, T, block, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
-            if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
-            for (int r = 0; r < reps && err == hipSuccess; ++r)
-                err = b_ell(vt, T, block, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
-            if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
-            if (err == hipSuccess) err = hipEventSynchronize(e1);
-            float ms = 0.f;
-            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-            const double us = 1000.0 * ms / reps;
-            if (err == hipSuccess && us < best.us) {
-                best.us = us;
-                best.T = T;
-                best.block = block;
+            for (int group : {1, 2, 4}) {
+                if (err != hipSuccess) break;
+                if (block < T || group > ns) continue;
+                BMat m{(const int32_t *)Aj, (const uint32_t *)Arl, Ax, g.stride, g.ax_ld, n, g.vld,
+                       (n * T + block - 1) / block};
+                float *dinv = (float *)vec, *x = dinv + ns * g.vld, *rhs = x + ns * g.vld, *out = rhs + ns * g.vld;
+                const int reps = n >= 65536 ? 10 : 30;
+                for (int r = 0; r < 3 && err == hipSuccess; ++r)
+                    err = b_ell(vt, T, block, group, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
+                if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
+                for (int r = 0; r < reps && err == hipSuccess; ++r)
+                    err = b_ell(vt, T, block, group, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
+                if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                const double us = 1000.0 * ms / reps;
+                if (err == hipSuccess && us < best.us) {
+                    best.us = us;
+                    best.T = T;
+                    best.block = block;
+                    best.group = group;
+                }
             }
         }
         (void)hipStreamSynchronize(s_main_);
@@ -178,6 +182,7 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
     HX(err);
     lv.T = best.T;
     lv.block = best.block;
+    lv.group = best.group;
     best.slots = ns;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tuned[key] = best;
@@ -265,6 +270,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
             if (c.level >= l) lv.cap += c.count;
         lv.T = 1;
         lv.block = 256;
+        lv.group = 1;
         if (cfg_.autotune && l > 1) HX(tune_level(lv));
         const Level0 g = level_geometry(l, lv.T, lv.block);
         lv.stride = g.stride;
@@ -340,13 +346,13 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         // x = n_pre Jacobi sweeps from zero (first two fused), ending in x
         const int rest = cfg_.n_pre - 2;
         float *src = (rest % 2 == 0) ? lv.x : lv.t;
-        HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI0, m, w, lv.cap, src, nullptr, lv.rhs, lv.dinv, s));
+        HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI0, m, w, lv.cap, src, nullptr, lv.rhs, lv.dinv, s));
         for (int k = 0; k < rest; ++k) {
             float *dst = (src == lv.x) ? lv.t : lv.x;
-            HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
+            HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
             src = dst;
         }
-        HX(b_ell(vt, lv.T, lv.block, ELL_DEFECT, m, w, lv.cap, lv.d, lv.x, lv.rhs, nullptr, s));
+        HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_DEFECT, m, w, lv.cap, lv.d, lv.x, lv.rhs, nullptr, s));
         HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
     }
     HX(b_coarse_cg(vt, L_[1], level_work(1), s));
@@ -357,7 +363,7 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, s));
         float *src = lv.x, *dst = lv.t;
         for (int k = 0; k < cfg_.n_post; ++k) {
-            HX(b_ell(vt, lv.T, lv.block, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
+            HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
             std::swap(src, dst);
         }
         if (src != lv.x) HX(b_copy(lv, w, lv.x, src, s));
@@ -553,7 +559,7 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
 }
 
 hipError_t BatchMG::activate_ready(bool wait) {
-    std::vector<int64_t> still;
+    std::vector<int64_t> still, ready;
     for (int64_t g : loading_) {
         hipError_t q = wait ? hipEventSynchronize(load_ev_[(size_t)g]) : hipEventQuery(load_ev_[(size_t)g]);
         if (q == hipErrorNotReady) {
@@ -561,10 +567,11 @@ hipError_t BatchMG::activate_ready(bool wait) {
             continue;
         }
         HX(q);
-        HX(b_set_state(sc_, g, SLOT_NEW, s_main_));
-        host_state_[(size_t)g] = 2;
-        ++running_;
+        ready.push_back(g);
     }
+    HX(b_set_states(sc_, ready.data(), (int64_t)ready.size(), SLOT_NEW, s_main_));
+    for (int64_t g : ready) host_state_[(size_t)g] = 2;
+    running_ += (int64_t)ready.size();
     loading_.swap(still);
     return hipSuccess;
 }

@@ -77,7 +77,7 @@ struct BatchConfig {
 // Inner level l of the pool: slots [0, cap) present.
 struct BLevel {
     int64_t level = 0, side = 0, n = 0, cap = 0, vld = 0;
-    int T = 1, block = 256;
+    int T = 1, block = 256, group = 1;   // lanes per row, work-group size, slots per thread
     int64_t stride = 0, ncpr = 0, ax_ld = 0;
     int32_t *Aj = nullptr;
     uint32_t *Arl = nullptr;
@@ -99,7 +99,7 @@ struct BClass {
 // Threads-per-row / block-size choice for one level (autotuned, cached per
 // level and value type for the process).
 struct LaunchShape {
-    int T = 1, block = 256;
+    int T = 1, block = 256, group = 1;
     double us = 0.0;       // measured time of one batched Jacobi sweep
     int64_t slots = 0;     // slots of the tuning batch
 };
@@ -291,8 +291,8 @@ hipError_t b_build(const BuildEntry *entries_dev, const BuildParams &P, int max_
 enum ListMode : int { LIST_NONIDLE = 0, LIST_ACTIVE = 1, LIST_UNSETTLED = 2 /* NEW or VERIFY */ };
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
                      int32_t *bounds, int mode, hipStream_t s);
-hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
-                 void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
+hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const BMat &m, const BWork &w,
+                 int64_t max_slots, void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
 // FP64 outer work over the active slots of all classes (w: lo = 0, hi = nclass).
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s);
@@ -303,7 +303,14 @@ hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, 
 hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
 hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);
 hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s);
-hipError_t b_set_state(SlotScalars *sc, int64_t g, int32_t state, hipStream_t s);
+// Slot ids of one state-setting launch (kernel argument).
+constexpr int kMaxSlotIds = 256;
+struct SlotIds {
+    int32_t n;
+    int32_t id[kMaxSlotIds];
+};
+// Fresh scalars in `state` for slots[0..n), one launch per 256 ids.
+hipError_t b_set_states(SlotScalars *sc, const int64_t *slots, int64_t n, int32_t state, hipStream_t s);
 hipError_t b_fill_state(SlotScalars *sc, int64_t nslots, int32_t state, hipStream_t s);
 
 }  // namespace hipx

@@ -212,14 +212,15 @@ int honei_hip_batch_run(void *batch, int64_t *launches) {
     return check(((BatchMG *)batch)->run_to_completion(launches), "batch_run");
 }
 
-int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, double *sweep_us,
-                          int64_t *tune_slots) {
+int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, int32_t *group,
+                          double *sweep_us, int64_t *tune_slots) {
     if (!batch) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "batch is null");
     BatchMG *b = (BatchMG *)batch;
     const auto &L = b->levels();
     if (level < 1 || level >= (int32_t)L.size()) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "level out of range");
     if (threads) *threads = L[(size_t)level].T;
     if (block) *block = L[(size_t)level].block;
+    if (group) *group = L[(size_t)level].group;
     LaunchShape sh;
     int dev = 0;
     const bool tuned = hipGetDevice(&dev) == hipSuccess &&

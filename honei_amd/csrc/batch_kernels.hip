@@ -78,41 +78,67 @@ __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64
     }
 }
 
-// One logical block = blockDim/T rows of one active slot.
-template <typename VT, int T, int MODE>
+// One logical block = blockDim/T rows of a group of G consecutive active
+// slots, all of one level: each index is loaded once for the group, each
+// slot's sums are the single-slot sums.  A short last group repeats its first
+// slot in the spare places and stores only its own slots.
+template <typename VT, int T, int MODE, int G>
 __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, const void *x, const void *rhs,
                                                 const void *dinv) {
     typedef typename VecOf<VT>::type V;
-    const int64_t a0 = active_first(w);
+    const int64_t a0 = active_first(w), na = active_count(w);
     int64_t first, last;
-    my_chunk(active_count(w) * m.bps, first, last);
+    my_chunk(((na + G - 1) / G) * m.bps, first, last);
+    if (first >= last) return;
+    int64_t grp = first / m.bps, rb = first - grp * m.bps;   // advanced incrementally below
     for (int64_t lb = first; lb < last; ++lb) {
-        const int64_t ls = w.list[a0 + lb / m.bps] - w.base, rb = lb % m.bps;
-        const VT *__restrict__ Ax = (const VT *)m.Ax + ls * m.ax_ld;
-        const int64_t vo = ls * m.vld;
-        const V *__restrict__ xs = (const V *)x + vo;
-        const V *__restrict__ rs = (const V *)rhs + vo;
-        const V *__restrict__ ds = (const V *)dinv + vo;
-        V *__restrict__ os = (V *)out + vo;
-        const int64_t gid = rb * (int64_t)blockDim.x + threadIdx.x;
+        const int64_t e0 = grp * G, rb_now = rb;
+        if (++rb == m.bps) {
+            rb = 0;
+            ++grp;
+        }
+        const int cnt = (int)((na - e0) < G ? (na - e0) : G);
+        const VT *ax[G];
+        const V *xs[G], *rs[G], *ds[G];
+        V *os[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int64_t ls = w.list[a0 + e0 + (g < cnt ? g : 0)] - w.base;
+            const int64_t vo = ls * m.vld;
+            ax[g] = (const VT *)m.Ax + ls * m.ax_ld;
+            xs[g] = (const V *)x + vo;
+            rs[g] = (const V *)rhs + vo;
+            ds[g] = (const V *)dinv + vo;
+            os[g] = (V *)out + vo;
+        }
+        const int64_t gid = rb_now * (int64_t)blockDim.x + threadIdx.x;
         const int lane = (int)(gid & (T - 1));
         const int64_t row = gid / T;
         const bool active = row < m.n;
-        V s = V(0);
+        V s[G];
         if (active) {
             if constexpr (MODE == ELL_JACOBI0)
-                s = ell_lane_sum<VT, T, true, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, rs, ds);
+                ell_lane_sum_group<VT, T, G, true>(m.Aj, ax, row, lane, m.Arl[row], m.stride, rs, ds, s);
             else
-                s = ell_lane_sum<VT, T, false, true>(m.Aj, Ax, row, lane, m.Arl[row], m.stride, xs);
+                ell_lane_sum_group<VT, T, G, false>(m.Aj, ax, row, lane, m.Arl[row], m.stride, xs, ds, s);
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) s[g] = V(0);
         }
-        s = lane_group_sum<T>(s);
+#pragma unroll
+        for (int g = 0; g < G; ++g) s[g] = lane_group_sum<T>(s[g]);
         if (active && lane == 0) {
-            if constexpr (MODE == ELL_DEFECT) {
-                os[row] = rs[row] - s;
-            } else {
-                const V xin = (MODE == ELL_JACOBI0) ? ds[row] * rs[row] : xs[row];
-                const V t = rs[row] - s;
-                os[row] = xin + t * ds[row];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g < cnt) {
+                    if constexpr (MODE == ELL_DEFECT) {
+                        os[g][row] = rs[g][row] - s[g];
+                    } else {
+                        const V xin = (MODE == ELL_JACOBI0) ? ds[g][row] * rs[g][row] : xs[g][row];
+                        const V t = rs[g][row] - s[g];
+                        os[g][row] = xin + t * ds[g][row];
+                    }
+                }
             }
         }
     }
@@ -137,15 +163,27 @@ __global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const S
     __syncthreads();
     int64_t first, last;
     my_chunk(pre[tab.nclass], first, last);
+    if (first >= last) return;
+    // (class, slot of the class list, row block) of `first`, then advanced incrementally
     int c = 0;
+    while (first >= pre[c + 1]) ++c;
+    int64_t si = (first - pre[c]) / tab.c[c].bps, hb = (first - pre[c]) - si * tab.c[c].bps;
     for (int64_t lb = first; lb < last; ++lb) {
-        while (lb >= pre[c + 1]) ++c;
+        if (lb == pre[c + 1]) {
+            while (lb >= pre[c + 1]) ++c;
+            si = 0;
+            hb = 0;
+        }
         const BClassEntry &e = tab.c[c];
-        const int64_t off = lb - pre[c];
-        const int64_t g = w.list[w.bounds[c] + off / e.bps];
-        const int64_t ls = g - e.first, hb = off % e.bps;
+        const int64_t g = w.list[w.bounds[c] + si];
+        const int64_t ls = g - e.first;
+        const int64_t hb_now = hb;
+        if (++hb == e.bps) {
+            hb = 0;
+            ++si;
+        }
         const SlotScalars &st = sc[g];
-        const int64_t rb = xcd_block(hb, e.bps);
+        const int64_t rb = xcd_block(hb_now, e.bps);
         const int64_t row = rb * 256 + threadIdx.x;
         const bool in = row < e.n;
         const int64_t vo = ls * e.vld, vi = vo + row;
@@ -190,7 +228,7 @@ __global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const S
         if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_DOT_RZ || OP == OP_SPMV_DOT ||
                       OP == OP_UPDATE) {
             const double v = block_sum(part);
-            if (threadIdx.x == 0) e.partials[ls * e.bps + hb] = v;
+            if (threadIdx.x == 0) e.partials[ls * e.bps + hb_now] = v;
         }
     }
 }
@@ -286,9 +324,14 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
         const int64_t bpe_ = ((n) + 255) / 256;                                   \
         int64_t first_, last_;                                                    \
         my_chunk(active_count(w) * bpe_, first_, last_);                          \
+        int64_t s_ = first_ / bpe_, b_ = first_ - s_ * bpe_;                      \
         for (int64_t lb_ = first_; lb_ < last_; ++lb_) {                          \
-            const int64_t g = w.list[a0_ + lb_ / bpe_];                           \
-            const int64_t i = (lb_ % bpe_) * 256 + threadIdx.x;                   \
+            const int64_t g = w.list[a0_ + s_];                                   \
+            const int64_t i = b_ * 256 + threadIdx.x;                             \
+            if (++b_ == bpe_) {                                                   \
+                b_ = 0;                                                           \
+                ++s_;                                                             \
+            }                                                                     \
             if (i < (n)) {                                                        \
                 BODY;                                                             \
             }                                                                     \
@@ -296,20 +339,20 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
     } while (0)
 
 // rhs_c[g] = R d_f[g] for active slots present on the fine level
-__global__ void k_b_restrict(BWork w, float *rhs_c, int64_t vld_c, const float *d_f, int64_t vld_f, int64_t m,
+__global__ __launch_bounds__(256) void k_b_restrict(BWork w, float *rhs_c, int64_t vld_c, const float *d_f, int64_t vld_f, int64_t m,
                              Mask8 mk) {
     const int64_t nc = (m + 1) * (m + 1);
     FOR_SLOT_ENTRIES(w, nc, restrict_node<float>(rhs_c + g * vld_c, d_f + g * vld_f, m, mk, i));
 }
 
 // x_f[g] += P x_c[g]
-__global__ void k_b_prolong_axpy(BWork w, float *x_f, int64_t vld_f, const float *x_c, int64_t vld_c, int64_t m,
+__global__ __launch_bounds__(256) void k_b_prolong_axpy(BWork w, float *x_f, int64_t vld_f, const float *x_c, int64_t vld_c, int64_t m,
                                  Mask8 mk) {
     const int64_t nf = (2 * m + 1) * (2 * m + 1);
     FOR_SLOT_ENTRIES(w, nf, (prolong_node<float, true>(nullptr, x_c + g * vld_c, m, mk, x_f + g * vld_f, 1.0f, i)));
 }
 
-__global__ void k_b_copy(BWork w, float *dst, const float *src, int64_t vld, int64_t n) {
+__global__ __launch_bounds__(256) void k_b_copy(BWork w, float *dst, const float *src, int64_t vld, int64_t n) {
     FOR_SLOT_ENTRIES(w, n, dst[g * vld + i] = src[g * vld + i]);
 }
 
@@ -328,14 +371,16 @@ __global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t
     }
 }
 
-__global__ void k_b_set_state(SlotScalars *sc, int64_t g, int32_t state) {
-    if (threadIdx.x == 0) {
+// Fresh scalars in `state` for the slots of one id list (passed by value).
+__global__ void k_b_set_states(SlotScalars *sc, SlotIds ids, int32_t state) {
+    const int k = (int)threadIdx.x;
+    if (k < ids.n) {
         SlotScalars s = {};
         s.state = state;
         s.status = SLOT_PENDING;
         s.scale = 1.0;
         s.inv = 1.0;
-        sc[g] = s;
+        sc[ids.id[k]] = s;
     }
 }
 
@@ -397,23 +442,33 @@ __global__ void k_bb_outer(const BuildEntry *entries, BuildParams P) {
 
 typedef void (*BEllFn)(BMat, BWork, void *, const void *, const void *, const void *);
 
-template <typename VT, int T>
+template <typename VT, int T, int G>
 BEllFn pick_mode(EllMode mode) {
     switch (mode) {
-        case ELL_DEFECT: return k_b_ell<VT, T, ELL_DEFECT>;
-        case ELL_JACOBI: return k_b_ell<VT, T, ELL_JACOBI>;
-        case ELL_JACOBI0: return k_b_ell<VT, T, ELL_JACOBI0>;
+        case ELL_DEFECT: return k_b_ell<VT, T, ELL_DEFECT, G>;
+        case ELL_JACOBI: return k_b_ell<VT, T, ELL_JACOBI, G>;
+        case ELL_JACOBI0: return k_b_ell<VT, T, ELL_JACOBI0, G>;
+        default: return nullptr;
+    }
+}
+
+template <typename VT, int T>
+BEllFn pick_group(int group, EllMode mode) {
+    switch (group) {
+        case 1: return pick_mode<VT, T, 1>(mode);
+        case 2: return pick_mode<VT, T, 2>(mode);
+        case 4: return pick_mode<VT, T, 4>(mode);
         default: return nullptr;
     }
 }
 
 template <typename VT>
-BEllFn pick_threads(int T, EllMode mode) {
+BEllFn pick_threads(int T, int group, EllMode mode) {
     switch (T) {
-        case 1: return pick_mode<VT, 1>(mode);
-        case 2: return pick_mode<VT, 2>(mode);
-        case 4: return pick_mode<VT, 4>(mode);
-        case 8: return pick_mode<VT, 8>(mode);
+        case 1: return pick_group<VT, 1>(group, mode);
+        case 2: return pick_group<VT, 2>(group, mode);
+        case 4: return pick_group<VT, 4>(group, mode);
+        case 8: return pick_group<VT, 8>(group, mode);
         default: return nullptr;
     }
 }
@@ -434,15 +489,16 @@ hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *bound
     return hipGetLastError();
 }
 
-hipError_t b_ell(ValueType vt, int T, int block, EllMode mode, const BMat &m, const BWork &w, int64_t max_slots,
-                 void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s) {
+hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const BMat &m, const BWork &w,
+                 int64_t max_slots, void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s) {
     BEllFn fn = nullptr;
-    if (vt == VT_F32) fn = pick_threads<float>(T, mode);
-    else if (vt == VT_BF16) fn = pick_threads<bf16>(T, mode);
+    if (vt == VT_F32) fn = pick_threads<float>(T, group, mode);
+    else if (vt == VT_BF16) fn = pick_threads<bf16>(T, group, mode);
     if (!fn || block % kWave || block > 1024 || block < T) return hipErrorInvalidValue;
     if (m.bps != (m.n * T + block - 1) / block) return hipErrorInvalidValue;
     if (max_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fn, dim3(batch_grid(max_slots * m.bps)), dim3(block), 0, s, m, w, out, x, rhs, dinv);
+    hipLaunchKernelGGL(fn, dim3(batch_grid(((max_slots + group - 1) / group) * m.bps)), dim3(block), 0, s, m, w, out,
+                       x, rhs, dinv);
     return hipGetLastError();
 }
 
@@ -556,9 +612,16 @@ hipError_t b_build(const BuildEntry *entries, const BuildParams &P, int max_leve
     return hipGetLastError();
 }
 
-hipError_t b_set_state(SlotScalars *sc, int64_t g, int32_t state, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_set_state, dim3(1), dim3(64), 0, s, sc, g, state);
-    return hipGetLastError();
+hipError_t b_set_states(SlotScalars *sc, const int64_t *slots, int64_t n, int32_t state, hipStream_t s) {
+    for (int64_t k0 = 0; k0 < n; k0 += kMaxSlotIds) {
+        SlotIds ids{};
+        ids.n = (int32_t)(n - k0 < kMaxSlotIds ? n - k0 : kMaxSlotIds);
+        for (int k = 0; k < ids.n; ++k) ids.id[k] = (int32_t)slots[k0 + k];
+        hipLaunchKernelGGL(k_b_set_states, dim3(1), dim3(kMaxSlotIds), 0, s, sc, ids, state);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t b_fill_state(SlotScalars *sc, int64_t nslots, int32_t state, hipStream_t s) {

@@ -25,19 +25,24 @@
 namespace honei {
 namespace hipx {
 
+// Bit e set: border part e is Dirichlet (its nodes are set to 0).
 struct Mask8 {
-    unsigned char m[8];
+    uint32_t bits;
 };
 
 // Host: the reference's macroBorderMask (8 values, 2 = Dirichlet) packed for
 // kernel arguments; a null mask means all Dirichlet.
 inline Mask8 pack_mask(const uint64_t *mask) {
-    Mask8 m;
-    for (int i = 0; i < 8; ++i) m.m[i] = (unsigned char)(mask ? (mask[i] == 2 ? 2 : 1) : 2);
+    Mask8 m{0u};
+    for (int i = 0; i < 8; ++i)
+        if (!mask || mask[i] == 2) m.bits |= 1u << i;
     return m;
 }
 
-__device__ inline bool dirichlet_zero(int64_t r, int64_t c, int64_t M, const Mask8 &mk) {
+// Grids handled by the transfers: node ids fit 32 bits (side <= 65535).
+constexpr int64_t kMaxTransferSide = 65535;
+
+__device__ inline bool dirichlet_zero(uint32_t r, uint32_t c, uint32_t M, const Mask8 &mk) {
     int e;
     if (r == 0 && c == 0) e = 0;
     else if (r == 0 && c == M - 1) e = 1;
@@ -48,13 +53,13 @@ __device__ inline bool dirichlet_zero(int64_t r, int64_t c, int64_t M, const Mas
     else if (r == M - 1) e = 6;
     else if (c == 0) e = 7;
     else return false;
-    return mk.m[e] == 2;
+    return (mk.bits >> e) & 1u;
 }
 
 template <typename V>
-__device__ inline void cswap(int64_t &ka, V &va, int64_t &kb, V &vb) {
+__device__ inline void cswap(int32_t &ka, V &va, int32_t &kb, V &vb) {
     if (kb < ka) {
-        int64_t tk = ka; ka = kb; kb = tk;
+        int32_t tk = ka; ka = kb; kb = tk;
         V tv = va; va = vb; vb = tv;
     }
 }
@@ -64,36 +69,37 @@ template <typename V>
 __device__ inline void restrict_node(V *__restrict__ coarse, const V *__restrict__ fine, int64_t m, const Mask8 &mk,
                                      int64_t id) {
     const int64_t nc = m + 1, nf = 2 * m + 1;
-    const int64_t r = id / nc, c = id % nc;
-    if (dirichlet_zero(r, c, nc, mk)) {
+    const int64_t r = (uint32_t)id / (uint32_t)nc, c = (uint32_t)id % (uint32_t)nc;
+    if (dirichlet_zero((uint32_t)r, (uint32_t)c, (uint32_t)nc, mk)) {
         coarse[id] = V(0);
         return;
     }
     auto F = [&](int64_t fr, int64_t fc) { return fine[fr * nf + fc]; };
-    const int64_t NONE = INT64_MAX;
+    const int32_t NONE = INT32_MAX;
+    const int32_t ri = (int32_t)r, ci = (int32_t)c;
     // tag = sweep index i (1..m) * 8 + sub-step (A..H = 0..7)
-    int64_t k[8];
+    int32_t k[8];
     V v[8];
     const V h = V(0.5), q = V(0.25);
     // A_i (i=r): row i += .5 F(2i-1, 2c)
-    k[0] = r >= 1 ? r * 8 + 0 : NONE;           v[0] = r >= 1 ? h * F(2 * r - 1, 2 * c) : V(0);
+    k[0] = r >= 1 ? ri * 8 + 0 : NONE;           v[0] = r >= 1 ? h * F(2 * r - 1, 2 * c) : V(0);
     // B_i (i=r+1): row i-1 += .5 F(2i-1, 2c)
-    k[1] = r + 1 <= m ? (r + 1) * 8 + 1 : NONE; v[1] = r + 1 <= m ? h * F(2 * r + 1, 2 * c) : V(0);
+    k[1] = r + 1 <= m ? (ri + 1) * 8 + 1 : NONE; v[1] = r + 1 <= m ? h * F(2 * r + 1, 2 * c) : V(0);
     // C_i (i=c+1): col i-1 += .5 F(2r, 2i-1)
-    k[2] = c + 1 <= m ? (c + 1) * 8 + 2 : NONE; v[2] = c + 1 <= m ? h * F(2 * r, 2 * c + 1) : V(0);
+    k[2] = c + 1 <= m ? (ci + 1) * 8 + 2 : NONE; v[2] = c + 1 <= m ? h * F(2 * r, 2 * c + 1) : V(0);
     // D_i (i=c): col i += .5 F(2r, 2i-1)
-    k[3] = c >= 1 ? c * 8 + 3 : NONE;           v[3] = c >= 1 ? h * F(2 * r, 2 * c - 1) : V(0);
+    k[3] = c >= 1 ? ci * 8 + 3 : NONE;           v[3] = c >= 1 ? h * F(2 * r, 2 * c - 1) : V(0);
     // E_i (i=r+1, j=c<m): (i-1, j) += .25 F(2i-1, 2j+1)
-    k[4] = (r + 1 <= m && c < m) ? (r + 1) * 8 + 4 : NONE;
+    k[4] = (r + 1 <= m && c < m) ? (ri + 1) * 8 + 4 : NONE;
     v[4] = (r + 1 <= m && c < m) ? q * F(2 * r + 1, 2 * c + 1) : V(0);
     // F_i (i=r+1, j=c-1): (i-1, j+1) += .25 F(2i-1, 2j+1)
-    k[5] = (r + 1 <= m && c >= 1) ? (r + 1) * 8 + 5 : NONE;
+    k[5] = (r + 1 <= m && c >= 1) ? (ri + 1) * 8 + 5 : NONE;
     v[5] = (r + 1 <= m && c >= 1) ? q * F(2 * r + 1, 2 * c - 1) : V(0);
     // G_i (i=r, j=c<m): (i, j) += .25 F(2i-1, 2j+1)
-    k[6] = (r >= 1 && c < m) ? r * 8 + 6 : NONE;
+    k[6] = (r >= 1 && c < m) ? ri * 8 + 6 : NONE;
     v[6] = (r >= 1 && c < m) ? q * F(2 * r - 1, 2 * c + 1) : V(0);
     // H_i (i=r, j=c-1): (i, j+1) += .25 F(2i-1, 2j+1)
-    k[7] = (r >= 1 && c >= 1) ? r * 8 + 7 : NONE;
+    k[7] = (r >= 1 && c >= 1) ? ri * 8 + 7 : NONE;
     v[7] = (r >= 1 && c >= 1) ? q * F(2 * r - 1, 2 * c - 1) : V(0);
     // Batcher odd-even merge sort network for 8 keys (19 comparators)
     cswap(k[0], v[0], k[1], v[1]); cswap(k[2], v[2], k[3], v[3]);
@@ -119,10 +125,10 @@ template <typename V, bool AXPY>
 __device__ inline void prolong_node(V *__restrict__ fine, const V *__restrict__ coarse, int64_t m, const Mask8 &mk,
                                     V *__restrict__ x, V alpha, int64_t id) {
     const int64_t nc = m + 1, nf = 2 * m + 1;
-    const int64_t R = id / nf, C = id % nf;
+    const int64_t R = (uint32_t)id / (uint32_t)nf, C = (uint32_t)id % (uint32_t)nf;
     auto Cn = [&](int64_t cr, int64_t cc) { return coarse[cr * nc + cc]; };
     V f;
-    if (dirichlet_zero(R, C, nf, mk)) {
+    if (dirichlet_zero((uint32_t)R, (uint32_t)C, (uint32_t)nf, mk)) {
         f = V(0);
     } else if ((R & 1) == 0 && (C & 1) == 0) {
         f = Cn(R / 2, C / 2);

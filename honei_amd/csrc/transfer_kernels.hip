@@ -12,7 +12,7 @@ namespace hipx {
 namespace {
 
 template <typename V>
-__global__ void k_restriction(V *__restrict__ coarse, const V *__restrict__ fine, int64_t m, Mask8 mk) {
+__global__ __launch_bounds__(256) void k_restriction(V *__restrict__ coarse, const V *__restrict__ fine, int64_t m, Mask8 mk) {
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (id < (m + 1) * (m + 1)) restrict_node(coarse, fine, m, mk, id);
 }
@@ -20,7 +20,7 @@ __global__ void k_restriction(V *__restrict__ coarse, const V *__restrict__ fine
 // fine[R,C] from coarse, in the reference's accumulation order; optionally
 // x[R,C] += alpha * fine[R,C] (the MG correction ScaledSum, fused).
 template <typename V, bool AXPY>
-__global__ void k_prolongation(V *__restrict__ fine, const V *__restrict__ coarse, int64_t m, Mask8 mk,
+__global__ __launch_bounds__(256) void k_prolongation(V *__restrict__ fine, const V *__restrict__ coarse, int64_t m, Mask8 mk,
                                V *__restrict__ x, V alpha) {
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (id < (2 * m + 1) * (2 * m + 1)) prolong_node<V, AXPY>(fine, coarse, m, mk, x, alpha, id);
@@ -30,7 +30,7 @@ bool side_of(int64_t n, int64_t &side) {
     int64_t s = 1;
     while (s * s < n) ++s;
     side = s;
-    return s * s == n && s >= 3 && ((s - 1) & (s - 2)) == 0;
+    return s * s == n && s >= 3 && s <= kMaxTransferSide && ((s - 1) & (s - 2)) == 0;
 }
 
 

@@ -371,11 +371,12 @@ int honei_hip_batch_poll(void *batch, honei_hip_slot_info *info); /* waits; info
 int honei_hip_batch_result(void *batch, int64_t slot, double *x_dev);
 /* Step until every loaded slot has retired; *launches = graph launches. */
 int honei_hip_batch_run(void *batch, int64_t *launches);
-/* Launch shape of an inner level (threads per row, block size) and, when the
- * autotuner chose it, the measured time of one batched Jacobi sweep over its
- * tuning batch (microseconds, else 0) and that batch's slot count.         */
-int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, double *sweep_us,
-                          int64_t *tune_slots);
+/* Launch shape of an inner level (threads per row, block size, slots per
+ * thread: one index load serves that many systems) and, when the autotuner
+ * chose it, the measured time of one batched Jacobi sweep over its tuning
+ * batch (microseconds, else 0) and that batch's slot count.                */
+int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, int32_t *group,
+                          double *sweep_us, int64_t *tune_slots);
 
 /* ---- serving loop -------------------------------------------------------------------
  * No reference counterpart.  Continuous batching over a batch pool: requests

@@ -95,7 +95,7 @@ def test_batch_bitexact_with_single_system_solves():
     levels, counts = [3, 5, 4], [2, 2, 3]
     systems = {(c, j): _system(levels[c], 10 * c + j) for c in range(3) for j in range(counts[c])}
     res, launches, B = _run_batch(levels, counts, systems, autotune=0, iters_per_launch=3, outer_method=0)
-    assert B.shape(5) == (1, 256)
+    assert B.shape(5) == (1, 256, 1)
     for (c, j), ((state, iters, status, rel, level), x) in res.items():
         assert (state, status, level) == (0, 0, levels[c])
         outer, rd, xs = _single(levels[c], *systems[(c, j)])
