@@ -209,7 +209,7 @@ uint64_t BatchMG::plan_bytes() const {
     for (const BClass &c : C_) {
         b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4;
         b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
-        b += (uint64_t)(c.count * c.bps) * 8;
+        b += (uint64_t)(c.count * 4 * c.bps) * 8;   // partials
     }
     b += (uint64_t)nslots_ * sizeof(SlotScalars);
     return b;
@@ -304,7 +304,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
         for (double **p : {&c.b, &c.x, &c.d}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
         if (cfg_.outer == 1)
             for (double **p : {&c.p, &c.q}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
-        HX(alloc((void **)&c.partials, (size_t)(c.count * c.bps) * 8));
+        HX(alloc((void **)&c.partials, (size_t)(c.count * 4 * c.bps) * 8));   // 64-row chunks
         HX(q1_assemble(VT_F64, c.side, c.Aj, c.Ax, c.Arl, c.stride, nullptr, nullptr, 0.0, s_main_, 1));
     }
     HX(alloc((void **)&sc_, (size_t)nslots_ * sizeof(SlotScalars)));
@@ -384,6 +384,8 @@ BClassTab BatchMG::class_table() const {
         e.stride = c.stride;
         e.ax_ld = c.ax_ld;
         e.bps = c.bps;
+        e.pst = 4 * c.bps;
+        e.nps = (c.n + 63) / 64;
         e.vld_top = top.vld;
         e.Aj = c.Aj;
         e.Arl = c.Arl;

@@ -124,6 +124,8 @@ struct BWork {
 // every class in one launch (logical block = 256 rows of one slot).
 struct BClassEntry {
     int64_t first, n, vld, stride, ax_ld, bps, vld_top;
+    int64_t pst;   // partial stride per slot: 4 bps
+    int64_t nps;   // partials per slot of the CG reductions: one per 64-row chunk
     const int32_t *Aj;
     const uint32_t *Arl;
     const double *Ax, *b;
@@ -282,8 +284,8 @@ class BatchMG {
 // Shape chosen for (device, level, value type), if the autotuner ran.
 bool batch_tuned_shape(int dev, int64_t level, int vt, LaunchShape *out);
 
-// grid for up to `items` logical blocks
-int batch_grid(int64_t items);
+// grid of `block`-thread work-groups for up to `items` logical blocks
+int batch_grid(int64_t items, int block);
 
 hipError_t b_build(const BuildEntry *entries_dev, const BuildParams &P, int max_level, hipStream_t s);
 
@@ -294,6 +296,7 @@ hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *bound
 hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const BMat &m, const BWork &w,
                  int64_t max_slots, void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
 // FP64 outer work over the active slots of all classes (w: lo = 0, hi = nclass).
+// max_blocks: 256-row blocks of all slots of all classes.
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s);
 // One block per listed slot: reduce its partials and update its scalars.

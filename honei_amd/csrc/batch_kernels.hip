@@ -27,14 +27,23 @@ namespace honei {
 namespace hipx {
 namespace {
 
-constexpr int kMaxGrid = 8192;
+// Launch grids hold about as many work-groups as the chip keeps resident
+// (256 CUs x 2048 threads); work-group p walks logical blocks p, p + grid,
+// p + 2 grid, ... with p = xcd_block(blockIdx.x, grid), so in each round the
+// grid covers one contiguous window of logical blocks and every XCD a
+// contiguous piece of it (few concurrent memory streams, neighbouring rows in
+// one L2).  Logical block counts stay below 2^31 (checked by the launchers),
+// so the per-block index arithmetic is 32-bit.
+constexpr int64_t kResidentThreads = 256 * 2048;
+constexpr int64_t kMaxLogicalBlocks = (int64_t(1) << 31) - 1;
 
-// This work-group's contiguous share [first, last) of `total` logical blocks.
-__device__ inline void my_chunk(int64_t total, int64_t &first, int64_t &last) {
-    const int64_t p = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t per = total / gridDim.x, rem = total % gridDim.x;
-    first = p * per + (p < rem ? p : rem);
-    last = first + per + (p < rem ? 1 : 0);
+__device__ inline int64_t first_block() { return xcd_block(blockIdx.x, gridDim.x); }
+
+// q = a / b, a - q b for 0 <= a < 2^31, 0 < b < 2^31
+__device__ inline void div32(int64_t a, int64_t b, int64_t &q, int64_t &r) {
+    const uint32_t qq = (uint32_t)a / (uint32_t)b;
+    q = qq;
+    r = a - (int64_t)qq * b;
 }
 
 __device__ inline int64_t active_first(const BWork &w) { return w.bounds[w.lo]; }
@@ -87,16 +96,11 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
                                                 const void *dinv) {
     typedef typename VecOf<VT>::type V;
     const int64_t a0 = active_first(w), na = active_count(w);
-    int64_t first, last;
-    my_chunk(((na + G - 1) / G) * m.bps, first, last);
-    if (first >= last) return;
-    int64_t grp = first / m.bps, rb = first - grp * m.bps;   // advanced incrementally below
-    for (int64_t lb = first; lb < last; ++lb) {
-        const int64_t e0 = grp * G, rb_now = rb;
-        if (++rb == m.bps) {
-            rb = 0;
-            ++grp;
-        }
+    const int64_t total = ((na + G - 1) / G) * m.bps;
+    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+        int64_t grp, rb_now;
+        div32(lb, m.bps, grp, rb_now);
+        const int64_t e0 = grp * G;
         const int cnt = (int)((na - e0) < G ? (na - e0) : G);
         const VT *ax[G];
         const V *xs[G], *rs[G], *ds[G];
@@ -145,90 +149,170 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
 }
 
 // ---- FP64 outer work, all classes in one launch ------------------------------
-// Logical block = 256 rows of one active slot; blocks ordered class by class,
-// slot by slot.  Partial index hb of a slot holds row block xcd_block(hb,
-// bps), the order a single-system launch of 256-row blocks uses.
-template <int OP>
-__global__ __launch_bounds__(256) void k_b_outer(BClassTab tab, BWork w, const SlotScalars *sc) {
-    // per-class prefix of logical blocks, in LDS (indexed by the class)
+// Both outer kernels order their logical blocks class by class (per-class
+// prefix in LDS, indexed by the class) and walk each slot's rows in order.
+// Reductions store partials per slot (stride pst) in one of two layouts:
+//   DC (the defect of defect correction): one partial per 256-row block,
+//     partial index b holding row block xcd_block(b, bps) of that slot alone
+//     summed by block_sum, the order a single-system launch of 256-row
+//     blocks uses, so the batched solve reproduces the single-system norm
+//     bit for bit;
+//   otherwise (the CG reductions): one partial per 64-row chunk, the
+//     wavefront sum of that chunk (nps = ceil(n / 64) partials per slot), so
+//     no launch waits at a work-group barrier.
+
+// Row sums (defect, A p): logical block = 256 rows of a group of G
+// consecutive listed slots of one class (a short last group repeats its
+// first slot in the spare places and stores only its own slots); each
+// column index is loaded once for the group.
+template <int OP, int G, bool DC>
+__global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, const SlotScalars *sc) {
+    static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_SPMV_DOT, "row-sum op");
     __shared__ int64_t pre[kMaxClasses + 1];
     if (threadIdx.x == 0) {
         int64_t acc = 0;
         pre[0] = 0;
         for (int c = 0; c < tab.nclass; ++c) {
-            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * tab.c[c].bps;
+            acc += (int64_t)((w.bounds[c + 1] - w.bounds[c] + G - 1) / G) * tab.c[c].bps;
             pre[c + 1] = acc;
         }
     }
     __syncthreads();
-    int64_t first, last;
-    my_chunk(pre[tab.nclass], first, last);
-    if (first >= last) return;
-    // (class, slot of the class list, row block) of `first`, then advanced incrementally
-    int c = 0;
-    while (first >= pre[c + 1]) ++c;
-    int64_t si = (first - pre[c]) / tab.c[c].bps, hb = (first - pre[c]) - si * tab.c[c].bps;
-    for (int64_t lb = first; lb < last; ++lb) {
-        if (lb == pre[c + 1]) {
-            while (lb >= pre[c + 1]) ++c;
-            si = 0;
-            hb = 0;
-        }
+    const int64_t total = pre[tab.nclass];
+    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+        int c = 0;
+        while (lb >= pre[c + 1]) ++c;
         const BClassEntry &e = tab.c[c];
+        int64_t gi, hb;   // group of the class list, row block
+        div32(lb - pre[c], e.bps, gi, hb);
+        const int64_t l0 = w.bounds[c] + gi * G;
+        const int cnt = (int)((w.bounds[c + 1] - l0) < G ? (w.bounds[c + 1] - l0) : G);
+        int64_t ls[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) ls[k] = w.list[l0 + (k < cnt ? k : 0)] - e.first;
+        const int64_t row = hb * 256 + threadIdx.x;
+        double part[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) part[k] = 0.0;
+        if (row < e.n) {
+            const double *ax[G], *xv[G], *unused[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                ax[k] = e.Ax + ls[k] * e.ax_ld;
+                xv[k] = (OP == OP_SPMV_DOT ? e.p : e.x) + ls[k] * e.vld;
+                unused[k] = nullptr;
+            }
+            double sum[G];
+            ell_lane_sum_group<double, 1, G, false>(e.Aj, ax, row, 0, e.Arl[row], e.stride, xv, unused, sum);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const int64_t vi = ls[k] * e.vld + row;
+                if constexpr (OP == OP_SPMV_DOT) {
+                    const double pv = e.p[vi];
+                    if (k < cnt) e.q[vi] = sum[k];
+                    part[k] = pv * sum[k];
+                } else {
+                    const double r = e.b[vi] - sum[k];
+                    if (k < cnt) e.r[vi] = r;
+                    part[k] = r * r;
+                }
+            }
+        }
+        if constexpr (DC) {
+            block_sum_first<G>(part);
+            if (threadIdx.x == 0) {
+                const int64_t pidx = xcd_unblock(hb, e.bps);
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    if (k < cnt) e.partials[ls[k] * e.pst + pidx] = part[k];
+            }
+        } else {
+            const int64_t chunk = hb * (256 / kWave) + threadIdx.x / kWave;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const double v = wave_sum(part[k]);
+                if (threadIdx.x % kWave == 0 && k < cnt && chunk < e.nps) e.partials[ls[k] * e.pst + chunk] = v;
+            }
+        }
+    }
+}
+
+// Element-wise ops: logical block = R consecutive 256-row blocks of one
+// listed slot; thread t takes row t of each, with every load of the R rows
+// issued before any use (rows past the end load the last row and are
+// dropped), so one slot lookup serves R rows and their loads overlap.
+template <int OP, int R>
+__global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, const SlotScalars *sc) {
+    static_assert(OP == OP_SCALE || OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_PUPDATE || OP == OP_UPDATE,
+                  "element-wise op");
+    constexpr bool REDUCE = OP == OP_DOT_RZ || OP == OP_UPDATE;
+    __shared__ int64_t pre[kMaxClasses + 1];
+    if (threadIdx.x == 0) {
+        int64_t acc = 0;
+        pre[0] = 0;
+        for (int c = 0; c < tab.nclass; ++c) {
+            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * ((tab.c[c].bps + R - 1) / R);
+            pre[c + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int64_t total = pre[tab.nclass];
+    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+        int c = 0;
+        while (lb >= pre[c + 1]) ++c;
+        const BClassEntry &e = tab.c[c];
+        int64_t si, sb;   // slot of the class list, group of R row blocks
+        div32(lb - pre[c], (e.bps + R - 1) / R, si, sb);
         const int64_t g = w.list[w.bounds[c] + si];
         const int64_t ls = g - e.first;
-        const int64_t hb_now = hb;
-        if (++hb == e.bps) {
-            hb = 0;
-            ++si;
-        }
         const SlotScalars &st = sc[g];
-        const int64_t rb = xcd_block(hb_now, e.bps);
-        const int64_t row = rb * 256 + threadIdx.x;
-        const bool in = row < e.n;
-        const int64_t vo = ls * e.vld, vi = vo + row;
-        double part = 0.0;
-        if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE) {
-            if (in) {
-                const double s = ell_lane_sum<double, 1, false, true>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row],
-                                                                      e.stride, e.x + vo);
-                const double r = e.b[vi] - s;
-                e.r[vi] = r;
-                part = r * r;
-            }
-        } else if constexpr (OP == OP_SCALE) {
-            if (in) e.rhs_top[g * e.vld_top + row] = (float)(e.r[vi] * st.inv);
-        } else if constexpr (OP == OP_ACC) {
-            if (in) {
-                const double base = st.iters == 1 ? 0.0 : e.x[vi];
-                e.x[vi] = base + (double)e.x_top[g * e.vld_top + row] * st.scale;
-            }
-        } else if constexpr (OP == OP_DOT_RZ) {
-            if (in) part = e.r[vi] * ((double)e.x_top[g * e.vld_top + row] * st.scale);
-        } else if constexpr (OP == OP_PUPDATE) {
-            if (in) {
-                const double z = (double)e.x_top[g * e.vld_top + row] * st.scale;
-                e.p[vi] = st.beta == 0.0 ? z : z + st.beta * e.p[vi];
-            }
-        } else if constexpr (OP == OP_SPMV_DOT) {
-            if (in) {
-                const double q = ell_lane_sum<double, 1, false, true>(e.Aj, e.Ax + ls * e.ax_ld, row, 0, e.Arl[row],
-                                                                      e.stride, e.p + vo);
-                e.q[vi] = q;
-                part = e.p[vi] * q;
-            }
-        } else if constexpr (OP == OP_UPDATE) {
-            if (in) {
-                e.x[vi] = e.x[vi] + st.alpha * e.p[vi];
-                const double r = e.r[vi] - st.alpha * e.q[vi];
-                e.r[vi] = r;
-                part = r * r;
+        const double *rs = e.r + ls * e.vld, *ps = e.p + ls * e.vld, *qs = e.q + ls * e.vld;
+        const double *xs = e.x + ls * e.vld;
+        const float *zs = e.x_top + g * e.vld_top;
+        int64_t row[R];
+        bool in[R];
+        double rv[R], zv[R], pv[R], qv[R], xo[R], part[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {   // every load first
+            row[j] = (sb * R + j) * 256 + threadIdx.x;
+            in[j] = row[j] < e.n;
+            const int64_t rc = in[j] ? row[j] : e.n - 1;
+            if constexpr (OP == OP_SCALE || OP == OP_DOT_RZ || OP == OP_UPDATE) rv[j] = rs[rc];
+            if constexpr (OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_PUPDATE) zv[j] = (double)zs[rc];
+            if constexpr (OP == OP_PUPDATE || OP == OP_UPDATE) pv[j] = ps[rc];
+            if constexpr (OP == OP_UPDATE) qv[j] = qs[rc];
+            if constexpr (OP == OP_ACC || OP == OP_UPDATE) xo[j] = xs[rc];
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            part[j] = 0.0;
+            const int64_t vi = ls * e.vld + row[j];
+            if constexpr (OP == OP_SCALE) {
+                if (in[j]) e.rhs_top[g * e.vld_top + row[j]] = (float)(rv[j] * st.inv);
+            } else if constexpr (OP == OP_ACC) {
+                const double base = st.iters == 1 ? 0.0 : xo[j];
+                if (in[j]) e.x[vi] = base + zv[j] * st.scale;
+            } else if constexpr (OP == OP_DOT_RZ) {
+                if (in[j]) part[j] = rv[j] * (zv[j] * st.scale);
+            } else if constexpr (OP == OP_PUPDATE) {
+                const double z = zv[j] * st.scale;
+                if (in[j]) e.p[vi] = st.beta == 0.0 ? z : z + st.beta * pv[j];
+            } else if constexpr (OP == OP_UPDATE) {
+                const double r = rv[j] - st.alpha * qv[j];
+                if (in[j]) {
+                    e.x[vi] = xo[j] + st.alpha * pv[j];
+                    e.r[vi] = r;
+                    part[j] = r * r;
+                }
             }
         }
-        if constexpr (OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_DOT_RZ || OP == OP_SPMV_DOT ||
-                      OP == OP_UPDATE) {
-            const double v = block_sum(part);
-            if (threadIdx.x == 0) e.partials[ls * e.bps + hb_now] = v;
+        if constexpr (REDUCE) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const double v = wave_sum(part[j]);
+                const int64_t chunk = (sb * R + j) * (256 / kWave) + threadIdx.x / kWave;
+                if (threadIdx.x % kWave == 0 && chunk < e.nps) e.partials[ls * e.pst + chunk] = v;
+            }
         }
     }
 }
@@ -252,7 +336,7 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
         while (c + 1 < tab.nclass && g >= tab.c[c + 1].first) ++c;
         const BClassEntry &e = tab.c[c];
         const int64_t ls = g - e.first;
-        const double h = sum_partials(e.partials + ls * e.bps, e.bps, 1, 0);
+        const double h = sum_partials(e.partials + ls * e.pst, OP == FIN_CHECK_DC ? e.bps : e.nps, 1, 0);
         if (threadIdx.x == 0) {
             SlotScalars s = sc[g];
             if constexpr (OP == FIN_CHECK_DC || OP == FIN_CHECK_CG) {
@@ -322,16 +406,12 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
     do {                                                                          \
         const int64_t a0_ = active_first(w);                                     \
         const int64_t bpe_ = ((n) + 255) / 256;                                   \
-        int64_t first_, last_;                                                    \
-        my_chunk(active_count(w) * bpe_, first_, last_);                          \
-        int64_t s_ = first_ / bpe_, b_ = first_ - s_ * bpe_;                      \
-        for (int64_t lb_ = first_; lb_ < last_; ++lb_) {                          \
+        const int64_t total_ = active_count(w) * bpe_;                            \
+        for (int64_t lb_ = first_block(); lb_ < total_; lb_ += gridDim.x) {      \
+            int64_t s_, b_;                                                       \
+            div32(lb_, bpe_, s_, b_);                                             \
             const int64_t g = w.list[a0_ + s_];                                   \
             const int64_t i = b_ * 256 + threadIdx.x;                             \
-            if (++b_ == bpe_) {                                                   \
-                b_ = 0;                                                           \
-                ++s_;                                                             \
-            }                                                                     \
             if (i < (n)) {                                                        \
                 BODY;                                                             \
             }                                                                     \
@@ -473,11 +553,15 @@ BEllFn pick_threads(int T, int group, EllMode mode) {
     }
 }
 
-int entry_grid(int64_t max_slots, int64_t n) { return batch_grid(max_slots * ((n + 255) / 256)); }
+int entry_grid(int64_t max_slots, int64_t n) { return batch_grid(max_slots * ((n + 255) / 256), 256); }
+bool entry_fits(int64_t max_slots, int64_t n) { return max_slots * ((n + 255) / 256) <= kMaxLogicalBlocks; }
 
 }  // namespace
 
-int batch_grid(int64_t items) { return (int)(items < 1 ? 1 : (items > kMaxGrid ? kMaxGrid : items)); }
+int batch_grid(int64_t items, int block) {
+    const int64_t cap = kResidentThreads / block;
+    return (int)(items < 1 ? 1 : (items > cap ? cap : items));
+}
 
 hipError_t b_compact(const SlotScalars *sc, int64_t nslots, const int64_t *boundaries, int nb, int32_t *list,
                      int32_t *bounds, int mode, hipStream_t s) {
@@ -497,24 +581,35 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
     if (!fn || block % kWave || block > 1024 || block < T) return hipErrorInvalidValue;
     if (m.bps != (m.n * T + block - 1) / block) return hipErrorInvalidValue;
     if (max_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fn, dim3(batch_grid(((max_slots + group - 1) / group) * m.bps)), dim3(block), 0, s, m, w, out,
-                       x, rhs, dinv);
+    const int64_t items = ((max_slots + group - 1) / group) * m.bps;
+    if (items > kMaxLogicalBlocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3(batch_grid(items, block)), dim3(block), 0, s, m, w, out, x, rhs, dinv);
     return hipGetLastError();
 }
+
+// Slots per logical block of the row-sum launches, 256-row blocks per
+// logical block of the element-wise launches.
+constexpr int kOuterRowGroup = 2, kOuterElemBlocks = 4;
 
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s) {
     if (max_blocks <= 0) return hipSuccess;
-    const dim3 grid(batch_grid(max_blocks)), block(256);
+    if (max_blocks > kMaxLogicalBlocks) return hipErrorInvalidValue;
+    constexpr int G = kOuterRowGroup, R = kOuterElemBlocks;
+    const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)), ge(batch_grid((max_blocks + R - 1) / R, 256));
     switch (op) {
-        case OP_DEFECT_ALL: hipLaunchKernelGGL(k_b_outer<OP_DEFECT_ALL>, grid, block, 0, s, tab, w, sc); break;
-        case OP_DEFECT_SETTLE: hipLaunchKernelGGL(k_b_outer<OP_DEFECT_SETTLE>, grid, block, 0, s, tab, w, sc); break;
-        case OP_SCALE: hipLaunchKernelGGL(k_b_outer<OP_SCALE>, grid, block, 0, s, tab, w, sc); break;
-        case OP_ACC: hipLaunchKernelGGL(k_b_outer<OP_ACC>, grid, block, 0, s, tab, w, sc); break;
-        case OP_DOT_RZ: hipLaunchKernelGGL(k_b_outer<OP_DOT_RZ>, grid, block, 0, s, tab, w, sc); break;
-        case OP_PUPDATE: hipLaunchKernelGGL(k_b_outer<OP_PUPDATE>, grid, block, 0, s, tab, w, sc); break;
-        case OP_SPMV_DOT: hipLaunchKernelGGL(k_b_outer<OP_SPMV_DOT>, grid, block, 0, s, tab, w, sc); break;
-        case OP_UPDATE: hipLaunchKernelGGL(k_b_outer<OP_UPDATE>, grid, block, 0, s, tab, w, sc); break;
+        case OP_DEFECT_ALL: hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_ALL, G, true>), gr, block, 0, s, tab, w, sc); break;
+        case OP_DEFECT_SETTLE:
+            hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false>), gr, block, 0, s, tab, w, sc);
+            break;
+        case OP_SPMV_DOT:
+            hipLaunchKernelGGL((k_b_outer_rows<OP_SPMV_DOT, G, false>), gr, block, 0, s, tab, w, sc);
+            break;
+        case OP_SCALE: hipLaunchKernelGGL((k_b_outer_elem<OP_SCALE, R>), ge, block, 0, s, tab, w, sc); break;
+        case OP_ACC: hipLaunchKernelGGL((k_b_outer_elem<OP_ACC, R>), ge, block, 0, s, tab, w, sc); break;
+        case OP_DOT_RZ: hipLaunchKernelGGL((k_b_outer_elem<OP_DOT_RZ, R>), ge, block, 0, s, tab, w, sc); break;
+        case OP_PUPDATE: hipLaunchKernelGGL((k_b_outer_elem<OP_PUPDATE, R>), ge, block, 0, s, tab, w, sc); break;
+        case OP_UPDATE: hipLaunchKernelGGL((k_b_outer_elem<OP_UPDATE, R>), ge, block, 0, s, tab, w, sc); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -542,7 +637,7 @@ hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t m
 
 hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
     const int64_t m = coarse.side - 1;
-    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap) return hipErrorInvalidValue;
+    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || !entry_fits(fine.cap, coarse.n)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_b_restrict, dim3(entry_grid(fine.cap, coarse.n)), dim3(256), 0, s, w, coarse.rhs,
                        coarse.vld, (const float *)fine.d, fine.vld, m, pack_mask(nullptr));
     return hipGetLastError();
@@ -550,13 +645,14 @@ hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, 
 
 hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
     const int64_t m = coarse.side - 1;
-    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap) return hipErrorInvalidValue;
+    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || !entry_fits(fine.cap, fine.n)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_b_prolong_axpy, dim3(entry_grid(fine.cap, fine.n)), dim3(256), 0, s, w, fine.x, fine.vld,
                        (const float *)coarse.x, coarse.vld, m, pack_mask(nullptr));
     return hipGetLastError();
 }
 
 hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s) {
+    if (!entry_fits(lv.cap, lv.n)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_b_copy, dim3(entry_grid(lv.cap, lv.n)), dim3(256), 0, s, w, dst, src, lv.vld, lv.n);
     return hipGetLastError();
 }

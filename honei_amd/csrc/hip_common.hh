@@ -47,6 +47,21 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Inverse of xcd_block: the b with xcd_block(b, nblocks) == rb.
+__device__ inline int64_t xcd_unblock(int64_t rb, int64_t nblocks) {
+    const int64_t q = nblocks / kXcds, r = nblocks % kXcds;
+    int64_t x, i;
+    if (rb < r * (q + 1)) {
+        x = rb / (q + 1);
+        i = rb - x * (q + 1);
+    } else {
+        const int64_t t = rb - r * (q + 1);
+        x = r + t / q;
+        i = t - (x - r) * q;
+    }
+    return i * kXcds + x;
+}
+
 template <typename T> __device__ inline T wave_sum(T v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -69,6 +84,29 @@ template <typename T> __device__ inline T block_sum(T v) {
     t = red[0];
     __syncthreads();
     return t;
+}
+
+// G block-wide sums at once, each in block_sum's order (so bit-identical to
+// it); the results are valid in thread 0 only.  Ends with a barrier, so the
+// next call may follow at once.
+template <int G, typename T> __device__ inline void block_sum_first(T (&v)[G]) {
+    __shared__ T red[G][1024 / kWave];
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    const int nw = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const T s = wave_sum(v[k]);
+        if (l == 0) red[k][w] = s;
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const T t = (threadIdx.x < (unsigned)nw) ? red[k][threadIdx.x] : T(0);
+            v[k] = wave_sum(t);
+        }
+    }
+    __syncthreads();
 }
 
 // Cross-block reduction slots.  A reducing kernel stores one partial per

@@ -11,7 +11,7 @@
 namespace honei {
 namespace hipx {
 
-uint64_t slot_bytes(int64_t level, ValueType inner) {
+uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method) {
     const uint64_t vs = vt_size(inner);
     uint64_t b = 0;
     for (int64_t l = 1; l <= level; ++l) {
@@ -21,7 +21,7 @@ uint64_t slot_bytes(int64_t level, ValueType inner) {
     }
     const uint64_t side = (1ull << level) + 1, n = side * side;
     const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;
-    b += 9 * stride * 8 + vld * 8 * 3 + (n + 255) / 256 * 8 + sizeof(SlotScalars) + 8;
+    b += 9 * stride * 8 + vld * 8 * (outer_method == 1 ? 5 : 3) + 4 * ((n + 255) / 256) * 8 + sizeof(SlotScalars) + 8;
     return b;
 }
 
@@ -37,7 +37,7 @@ hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n
         counts[c] = cfg.counts[c];
         if (counts[c] > 0) continue;
         if (!cfg.budget_bytes) return hipErrorInvalidValue;
-        const uint64_t per = slot_bytes(cfg.levels[c], (ValueType)cfg.batch.inner_vtype) * 4 / 3 + 1;
+        const uint64_t per = slot_bytes(cfg.levels[c], (ValueType)cfg.batch.inner_vtype, cfg.batch.outer) * 4 / 3 + 1;
         const uint64_t share = cfg.budget_bytes / (uint64_t)cfg.nclass;
         counts[c] = std::max<int64_t>(1, std::min<int64_t>(cfg.max_slots_per_class, (int64_t)(share / per)));
     }

@@ -49,7 +49,7 @@ struct ServeConfig {
 
 // Device bytes one slot of finest level `level` occupies (threads-per-row 1
 // layout; tuned layouts may need up to a third more matrix storage).
-uint64_t slot_bytes(int64_t level, ValueType inner);
+uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method);
 
 hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n, ServeRecord *out,
                      ServeStats *stats, bool *over_budget);
