@@ -116,6 +116,59 @@ def time_spmv(D, reps, x, y, label):
             "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
 
 
+def hbm_ceiling(reps=20):
+    """Measured copy ceilings: a device-to-device copy of 1 GiB and the
+    in-place scale kernel (read + write) over 2^28 floats, HIP events on the
+    launch stream; bytes moved = read + write."""
+    nf = 1 << 28
+    a = H.DeviceArray(nf, np.float32)
+    b = H.DeviceArray(nf, np.float32)
+    a.zero()
+    b.zero()
+    out = {}
+    for label, fn in (("d2d_copy", lambda: H.call("honei_hip_copy", b.ptr, a.ptr, nf * 4, None)),
+                      ("scale_kernel", lambda: H.call("honei_hip_scale_one", H.F32, a.ptr, 1.0, nf, None))):
+        for _ in range(3):
+            fn()
+        H.synchronize()
+        e0, e1 = H.Event(), H.Event()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        ms = e0.elapsed_ms(e1) / reps
+        gbs = 2 * nf * 4 / (ms * 1e-3) / 1e9
+        out[label] = {"us": round(ms * 1e3, 1), "gbs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
+        log(f"[hbm] {label}: {gbs:.0f} GB/s")
+    del a, b
+    return out
+
+
+def cpu_spmv_q1(side, reps=5):
+    """The oracle's C restatement of Product<tags::CPU::Generic> (one core)
+    and mc::Product (row blocks over all host cores) on the Q1 side^2 FP32
+    operator: median wall time per product."""
+    import oracle
+    A = oracle.q1_matrix(side, np.float32)
+    x = uniform_pm1(2, A.rows).astype(np.float32)
+    parts = os.cpu_count() or 1
+    res = {}
+    for label, fn in (("generic_1_core", lambda: oracle.product(A, x)),
+                      ("mc_all_cores", lambda: oracle.product_mc(A, x, parts))):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(ts))
+        res[label] = {"ms": round(ms, 3), "gbs": round(335347792 / (ms * 1e-3) / 1e9, 2)}
+    res["cores"] = parts
+    res["kind"] = "port"
+    log(f"[cpu spmv] 1 core {res['generic_1_core']['ms']} ms, {parts} cores {res['mc_all_cores']['ms']} ms")
+    return res
+
+
 def time_banded_q1(side, dtype, reps=50):
     """Banded Q1 product (BandedMatrixQx, no index stream): average launch
     time and algorithmic bytes 9n*s_v + n*(s_x + s_y)."""
@@ -391,6 +444,9 @@ def main():
         spmv[key] = time_spmv(R, 30, xr, yr, key)
         spmv[key]["nnz"] = R.nnz
         del R, xr, yr
+    spmv["hbm_ceiling"] = hbm_ceiling()
+    if not args.no_cpu:
+        spmv["cpu_q1_2049_f32"] = cpu_spmv_q1(SIDE)
 
     # ---- batched solve: the measured step ------------------------------------
     levels, kap, rhs = batched_workload()

@@ -121,10 +121,13 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
         const bool active = row < m.n;
         V s[G];
         if (active) {
+            // batch levels are Q1 operators: stored width q1_width(T) by construction
             if constexpr (MODE == ELL_JACOBI0)
-                ell_lane_sum_group<VT, T, G, true>(m.Aj, ax, row, lane, m.Arl[row], m.stride, rs, ds, s);
+                ell_lane_sum_group_w<VT, T, G, q1_width(T), true>(m.Aj, ax, row, lane, m.Arl[row], m.stride, rs, ds,
+                                                                  s);
             else
-                ell_lane_sum_group<VT, T, G, false>(m.Aj, ax, row, lane, m.Arl[row], m.stride, xs, ds, s);
+                ell_lane_sum_group_w<VT, T, G, q1_width(T), false>(m.Aj, ax, row, lane, m.Arl[row], m.stride, xs, ds,
+                                                                   s);
         } else {
 #pragma unroll
             for (int g = 0; g < G; ++g) s[g] = V(0);
@@ -203,7 +206,9 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
                 unused[k] = nullptr;
             }
             double sum[G];
-            ell_lane_sum_group<double, 1, G, false>(e.Aj, ax, row, 0, e.Arl[row], e.stride, xv, unused, sum);
+            // the FP64 outer operator: Q1, one lane per row, stored width 9
+            ell_lane_sum_group_w<double, 1, G, q1_width(1), false>(e.Aj, ax, row, 0, e.Arl[row], e.stride, xv, unused,
+                                                                   sum);
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const int64_t vi = ls[k] * e.vld + row;

@@ -77,6 +77,46 @@ __device__ inline void ell_lane_sum_group(const int32_t *__restrict__ Aj, const 
     }
 }
 
+// ell_lane_sum_group for a matrix whose stored width ncpr is the
+// compile-time NC: the lane's NC column indices are loaded first, then all
+// values and gathered x of the G arrays, so a row costs two dependent memory
+// round trips instead of one pair per unrolled step.  Slots k >= len are
+// read too (padding holds Aj = 0, Ax = 0, inside the ncpr*stride arrays) but
+// never added: each sum is the same sequence of roundings as ell_lane_sum.
+// As in ell_lane_sum_group every array is read (spare ones point at a live
+// one).
+template <typename VT, int T, int G, int NC, bool SCALED = false>
+__device__ inline void ell_lane_sum_group_w(const int32_t *__restrict__ Aj, const VT *const (&Ax)[G], int64_t row,
+                                            int lane, uint32_t len, int64_t stride,
+                                            const typename VecOf<VT>::type *const (&x)[G],
+                                            const typename VecOf<VT>::type *const (&dinv)[G],
+                                            typename VecOf<VT>::type (&s)[G]) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t slot0 = row * T + lane;
+    int32_t j[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) j[k] = Aj[slot0 + k * stride];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        V av[NC], xv[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            av[k] = load_value_nt(Ax[g] + slot0 + k * stride);
+            if constexpr (SCALED)
+                xv[k] = dinv[g][j[k]] * x[g][j[k]];
+            else
+                xv[k] = x[g][j[k]];
+        }
+        V t = V(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) t = ((uint32_t)k < len) ? t + av[k] * xv[k] : t;
+        s[g] = t;
+    }
+}
+
+// Stored width of the Q1 operator (9 entries per row) at T lanes per row.
+constexpr int q1_width(int T) { return (9 + T - 1) / T; }
+
 template <int T, typename V> __device__ inline V lane_group_sum(V s) {
     if constexpr (T > 1) {
 #pragma unroll
