@@ -117,17 +117,20 @@ def time_spmv(D, reps, x, y, label):
 
 
 def hbm_ceiling(reps=20):
-    """Measured copy ceilings: a device-to-device copy of 1 GiB and the
-    in-place scale kernel (read + write) over 2^28 floats, HIP events on the
-    launch stream; bytes moved = read + write."""
+    """Measured ceilings over 2^28 floats (1 GiB), HIP events on the launch
+    stream: a device-to-device copy and the in-place scale kernel (bytes =
+    read + write), and the squared-norm reduction (read only)."""
     nf = 1 << 28
     a = H.DeviceArray(nf, np.float32)
     b = H.DeviceArray(nf, np.float32)
+    r = H.DeviceArray(1, np.float64)
     a.zero()
     b.zero()
     out = {}
-    for label, fn in (("d2d_copy", lambda: H.call("honei_hip_copy", b.ptr, a.ptr, nf * 4, None)),
-                      ("scale_kernel", lambda: H.call("honei_hip_scale_one", H.F32, a.ptr, 1.0, nf, None))):
+    for label, fn, nbytes in (
+            ("d2d_copy", lambda: H.call("honei_hip_copy", b.ptr, a.ptr, nf * 4, None), 2 * nf * 4),
+            ("scale_kernel", lambda: H.call("honei_hip_scale_one", H.F32, a.ptr, 1.0, nf, None), 2 * nf * 4),
+            ("read_norm_kernel", lambda: H.call("honei_hip_norm_l2_dev", H.F32, r.ptr, a.ptr, nf, None), nf * 4)):
         for _ in range(3):
             fn()
         H.synchronize()
@@ -137,10 +140,10 @@ def hbm_ceiling(reps=20):
             fn()
         e1.record()
         ms = e0.elapsed_ms(e1) / reps
-        gbs = 2 * nf * 4 / (ms * 1e-3) / 1e9
+        gbs = nbytes / (ms * 1e-3) / 1e9
         out[label] = {"us": round(ms * 1e3, 1), "gbs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
         log(f"[hbm] {label}: {gbs:.0f} GB/s")
-    del a, b
+    del a, b, r
     return out
 
 

@@ -46,8 +46,10 @@ BatchMG::~BatchMG() { release(); }
 void BatchMG::release() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    exec_ = nullptr;
-    graph_ = nullptr;
+    if (exec1_) (void)hipGraphExecDestroy(exec1_);
+    if (graph1_) (void)hipGraphDestroy(graph1_);
+    exec_ = exec1_ = nullptr;
+    graph_ = graph1_ = nullptr;
     if (s_main_) (void)hipStreamSynchronize(s_main_);
     if (s_load_) (void)hipStreamSynchronize(s_load_);
     for (hipEvent_t e : load_ev_)
@@ -436,10 +438,10 @@ hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     return b_outer(OP_UPDATE, tab, w, blocks, sc_, s);
 }
 
-hipError_t BatchMG::capture() {
+hipError_t BatchMG::capture_iterations(int iters, hipGraph_t *graph, hipGraphExec_t *exec) {
     HX(hipStreamBeginCapture(s_main_, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
-    for (int i = 0; i < cfg_.iters_per_launch && err == hipSuccess; ++i) err = enqueue_iteration(s_main_);
+    for (int i = 0; i < iters && err == hipSuccess; ++i) err = enqueue_iteration(s_main_);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(s_main_, &g);
     if (err != hipSuccess) {
@@ -447,8 +449,18 @@ hipError_t BatchMG::capture() {
         return err;
     }
     HX(e2);
-    graph_ = g;
-    return hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+    *graph = g;
+    return hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+}
+
+// The full graph runs iters_per_launch iterations; a one-iteration graph
+// serves launches that only a few running slots need (the tail of a batch),
+// so finished work does not sit behind up to iters_per_launch - 1 empty
+// iterations.
+hipError_t BatchMG::capture() {
+    HX(capture_iterations(cfg_.iters_per_launch, &graph_, &exec_));
+    if (cfg_.iters_per_launch > 1) HX(capture_iterations(1, &graph1_, &exec1_));
+    return hipSuccess;
 }
 
 hipError_t BatchMG::load(int64_t g, const double *coeff, const double *b) {
@@ -580,7 +592,9 @@ hipError_t BatchMG::activate_ready(bool wait) {
 
 hipError_t BatchMG::step() {
     HX(activate_ready(false));
-    HX(hipGraphLaunch(exec_, s_main_));
+    // a tail: few slots still running and none waiting to join
+    const bool tail = exec1_ && loading_.empty() && running_ <= std::max<int64_t>(1, nslots_ / 32);
+    HX(hipGraphLaunch(tail ? exec1_ : exec_, s_main_));
     ++launches_;
     return hipSuccess;
 }

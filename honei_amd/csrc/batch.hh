@@ -254,8 +254,8 @@ class BatchMG {
     std::vector<int64_t> boundaries_;  // class firsts, then nslots
 
     hipStream_t s_main_ = nullptr, s_load_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
+    hipGraph_t graph_ = nullptr, graph1_ = nullptr;      // iters_per_launch / one iteration
+    hipGraphExec_t exec_ = nullptr, exec1_ = nullptr;
     std::vector<int> host_state_;      // 0 free, 1 loading, 2 running
     std::vector<int64_t> loading_;
     int64_t running_ = 0, launches_ = 0;
@@ -272,6 +272,7 @@ class BatchMG {
     hipEvent_t build_ev_ = nullptr;
     hipError_t tune_level(BLevel &lv);
     hipError_t capture();
+    hipError_t capture_iterations(int iters, hipGraph_t *graph, hipGraphExec_t *exec);
     hipError_t enqueue_iteration(hipStream_t s);
     hipError_t enqueue_vcycle(hipStream_t s);
     BWork level_work(int64_t level) const;
