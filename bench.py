@@ -172,6 +172,39 @@ def cpu_spmv_q1(side, reps=5):
     return res
 
 
+def time_spmv_staged(D, reps, x, y, label, block=1024, chunks=1):
+    """time_spmv for the product with each work-group's x window staged in
+    LDS (column windows computed once per matrix, outside the timed
+    launches)."""
+    rpb = block // D.threads * chunks
+    nb = (D.rows + rpb - 1) // rpb
+    ranges = H.DeviceArray(2 * nb, np.int32)
+    import ctypes
+    w = ctypes.c_int64()
+    H.call("honei_hip_ell_block_ranges", D.Aj.ptr, D.Arl.ptr, D.rows, D.ncpr, D.stride, D.threads, rpb, ranges.ptr,
+           ctypes.byref(w), None)
+    vs = 8 if D.vtype == H.F64 else 4
+    window = min(w.value, 98304 // vs)
+    args = (D.vtype, x.ptr, y.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, D.rows, D.ncpr, D.stride, D.threads, block,
+            chunks, ranges.ptr, nb, window, None)
+    for _ in range(3):
+        H.call("honei_hip_product_smell_dv_staged", *args)
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record()
+    for _ in range(reps):
+        H.call("honei_hip_product_smell_dv_staged", *args)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    alg = D.nnz * (vs + 4) + D.rows * 2 * vs
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"[spmv] {label} (x staged in LDS, window {window}): {ms * 1e3:.1f} us/launch, {gbs:.0f} GB/s = "
+        f"{100 * gbs / PEAK_GBS:.1f}%")
+    return {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
+            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "window": int(window), "block": block, "chunks": chunks,
+            "what": "honei_hip_product_smell_dv_staged, column windows per block computed once per matrix"}
+
+
 def time_banded_q1(side, dtype, reps=50):
     """Banded Q1 product (BandedMatrixQx, no index stream): average launch
     time and algorithmic bytes 9n*s_v + n*(s_x + s_y)."""
@@ -446,6 +479,7 @@ def main():
         yr = H.DeviceArray(R.rows, dt)
         spmv[key] = time_spmv(R, 30, xr, yr, key)
         spmv[key]["nnz"] = R.nnz
+        spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
         del R, xr, yr
     spmv["hbm_ceiling"] = hbm_ceiling()
     if not args.no_cpu:

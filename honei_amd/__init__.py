@@ -112,6 +112,9 @@ _SIGS = {
     "honei_hip_batch_shape": [_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                               C.POINTER(_d), C.POINTER(C.c_int64)],
     "honei_hip_serve_run": [_p, _p, C.c_int64, _p, _p],
+    "honei_hip_ell_block_ranges": [_p, _p, _sz, _sz, _sz, _sz, _sz, _p, C.POINTER(C.c_int64), _p],
+    "honei_hip_product_smell_dv_staged": [_i, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p, _sz, C.c_int64,
+                                          _p],
 }
 _RESTYPE = {"honei_hip_last_error": C.c_char_p, "honei_hip_version": C.c_char_p}
 
@@ -278,6 +281,23 @@ class DeviceEll:
               BF16: "honei_hip_product_smell_dv_bf16"}[self.vtype]
         call(fn, x.ptr, y.ptr, self.Aj.ptr, self.Ax.ptr, self.Arl.ptr, row_start,
              self.rows if row_end is None else row_end, self.ncpr, self.stride, blocksize, self.threads, stream)
+
+    def block_ranges(self, rows_per_block, stream=None):
+        """(device int32 [nblocks][2] column windows, widest window)."""
+        nb = (self.rows + rows_per_block - 1) // rows_per_block
+        ranges = DeviceArray(max(2 * nb, 2), np.int32)
+        w = C.c_int64()
+        call("honei_hip_ell_block_ranges", self.Aj.ptr, self.Arl.ptr, self.rows, self.ncpr, self.stride,
+             self.threads, rows_per_block, ranges.ptr, C.byref(w), stream)
+        return ranges, w.value
+
+    def product_staged(self, y: DeviceArray, x: DeviceArray, ranges: DeviceArray, window, blocksize=256, chunks=1,
+                       stream=None):
+        """y = A x with each work-group's x window in LDS (ranges made with
+        rows_per_block = chunks * blocksize / threads)."""
+        call("honei_hip_product_smell_dv_staged", self.vtype, x.ptr, y.ptr, self.Aj.ptr, self.Ax.ptr, self.Arl.ptr,
+             self.rows, self.ncpr, self.stride, self.threads, blocksize, chunks, ranges.ptr, ranges.n // 2,
+             int(window), stream)
 
     def defect(self, r: DeviceArray, rhs: DeviceArray, x: DeviceArray, stream=None):
         fn = {F32: "honei_hip_defect_smell_dv_float", F64: "honei_hip_defect_smell_dv_double"}[self.vtype]

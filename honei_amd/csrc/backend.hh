@@ -52,6 +52,19 @@ struct EllArgs {
 // vectors, f64 matrices on f64 vectors.
 hipError_t ell_apply(const EllView &A, const EllArgs &a, const ReduceSlots &slots, hipStream_t s);
 
+// Column window of every group of rows_per_block rows: ranges[2b], [2b+1] =
+// smallest / largest stored column index of rows [b*rpb, (b+1)*rpb) (min >
+// max when the rows hold no entry).  Depends on the pattern only.
+hipError_t ell_block_ranges(const EllView &A, int64_t rows_per_block, int32_t *ranges, hipStream_t s);
+// y = A x with each block's x window staged in LDS (`window` elements per
+// work-group, at most kMaxStageBytes; wider windows are gathered from
+// global memory); a work-group covers `chunks` x block/T rows and `ranges`
+// come from ell_block_ranges with that rows_per_block.  Bit-identical to
+// ell_apply's ELL_PRODUCT over all rows.
+constexpr size_t kMaxStageBytes = 96 * 1024;
+hipError_t ell_product_staged(const EllView &A, void *y, const void *x, const int32_t *ranges, int block, int chunks,
+                              int64_t window, hipStream_t s);
+
 // ---- BLAS-1 / reductions (double accumulation, deterministic order) -------
 // Each reduction writes its result into a device double (out) and, when
 // out_host_pinned is non-null, nowhere else; callers read it back when needed.

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -285,6 +286,48 @@ PRODUCT_ENTRY(honei_hip_product_smell_dv_bf16, VT_BF16)
     }
 DEFECT_ENTRY(honei_hip_defect_smell_dv_float, VT_F32)
 DEFECT_ENTRY(honei_hip_defect_smell_dv_double, VT_F64)
+
+// ---- ELL product with x staged in LDS -------------------------------------------
+int honei_hip_ell_block_ranges(const void *Aj, const void *Arl, size_t rows, size_t ncpr, size_t stride,
+                               size_t threads, size_t rows_per_block, int32_t *ranges, int64_t *max_width,
+                               void *stream) {
+    int st = check_ell(Aj, Aj, Arl, 0, rows, stride, threads, 0);
+    if (st) return st;
+    if (!ranges || rows_per_block < 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "ranges / rows_per_block");
+    EllView A = make_view(Aj, Aj, Arl, rows, ncpr, stride, threads, VT_F32);
+    st = check(ell_block_ranges(A, (int64_t)rows_per_block, ranges, S(stream)), "ell_block_ranges");
+    if (st || !max_width) return st;
+    const size_t nb = (rows + rows_per_block - 1) / rows_per_block;
+    std::vector<int32_t> h(2 * nb);
+    st = check(hipMemcpyAsync(h.data(), ranges, h.size() * 4, hipMemcpyDeviceToHost, S(stream)), "ranges readback");
+    if (!st) st = check(hipStreamSynchronize(S(stream)), "ranges sync");
+    if (st) return st;
+    int64_t w = 0;
+    for (size_t b = 0; b < nb; ++b)
+        if (h[2 * b + 1] >= h[2 * b]) w = std::max<int64_t>(w, (int64_t)h[2 * b + 1] - h[2 * b] + 1);
+    *max_width = w;
+    return HONEI_HIP_SUCCESS;
+}
+
+int honei_hip_product_smell_dv_staged(int vtype, void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                      size_t rows, size_t ncpr, size_t stride, size_t threads, size_t blocksize,
+                                      size_t chunks, const int32_t *ranges, size_t nranges, int64_t window,
+                                      void *stream) {
+    int st = check_ell(Aj, Ax, Arl, 0, rows, stride, threads, blocksize);
+    if (st) return st;
+    if (rows > 0 && (!x || !y || !ranges)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector or ranges");
+    if (chunks < 1 || chunks > 64) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "chunks must be 1 .. 64");
+    const size_t bs = blocksize ? blocksize : 256, rpb = bs / threads * chunks;
+    if (nranges < (rows + rpb - 1) / rpb)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "fewer ranges than blocks: make them with rows_per_block = "
+                                                      "chunks * blocksize / threads");
+    if (vtype < VT_F32 || vtype > VT_BF16) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "vtype");
+    const size_t vs = vtype == VT_F64 ? 8 : 4;
+    if (window < 1 || (size_t)window * vs > kMaxStageBytes)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "window must be 1 .. 96 KiB of vector elements");
+    EllView A = make_view(Aj, Ax, Arl, rows, ncpr, stride, threads, (ValueType)vtype);
+    return check(ell_product_staged(A, y, x, ranges, (int)bs, (int)chunks, window, S(stream)), "product_smell_dv_staged");
+}
 
 // ---- banded Q1 ------------------------------------------------------------------
 static int banded_call(ValueType vt, void *rhs, void *ll, void *ld, void *lu, void *dl, void *dd, void *du, void *ul,

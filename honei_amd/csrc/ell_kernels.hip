@@ -134,7 +134,167 @@ KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc) {
     return nullptr;
 }
 
+// ---- x staged in LDS ----------------------------------------------------------
+// Column window [min, max] of the stored entries k < Arl[r] of the rows of
+// block b (rows_per_block rows), column 0 left out: padding slots inside a
+// row's last T-wide chunk hold column 0, and the staged kernel keeps x[0] in
+// a register.  A block without other columns gets min > max.
+__global__ __launch_bounds__(256) void k_block_ranges(const int32_t *Aj, const uint32_t *Arl, int64_t rows,
+                                                      int64_t stride, int T, int64_t rpb, int32_t *ranges) {
+    const int64_t b = blockIdx.x;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    const int64_t r0 = b * rpb, slots = rpb * T;
+    for (int64_t i = threadIdx.x; i < slots; i += blockDim.x) {
+        const int64_t row = r0 + i / T;
+        if (row >= rows) break;
+        const int64_t lane = i % T;
+        const uint32_t len = Arl[row];
+        for (uint32_t k = 0; k < len; ++k) {
+            const int32_t j = Aj[row * T + lane + (int64_t)k * stride];
+            if (j == 0) continue;
+            lo = j < lo ? j : lo;
+            hi = j > hi ? j : hi;
+        }
+    }
+    __shared__ int32_t rl[256 / kWave], rh[256 / kWave];
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int32_t ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+    }
+    if (threadIdx.x % kWave == 0) {
+        rl[threadIdx.x / kWave] = lo;
+        rh[threadIdx.x / kWave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) {
+            lo = rl[w] < lo ? rl[w] : lo;
+            hi = rh[w] > hi ? rh[w] : hi;
+        }
+        ranges[2 * b] = lo;
+        ranges[2 * b + 1] = hi;
+    }
+}
+
+// y = A x, one block of `chunks` x blockDim/T rows per work-group: the
+// block's x window is copied into LDS (coalesced) once and the row sums of
+// its chunks gather from it; a window wider than `cap` elements is gathered
+// from global memory.  The row sum is ell_lane_sum's, term for term, so y
+// is bit-identical to ELL_PRODUCT.
+template <typename VT, int T>
+__global__ __launch_bounds__(1024) void k_product_staged(const int32_t *__restrict__ Aj, const VT *__restrict__ Ax,
+                                                         const uint32_t *__restrict__ Arl, int64_t stride, int64_t rows,
+                                                         const typename VecOf<VT>::type *__restrict__ x,
+                                                         typename VecOf<VT>::type *__restrict__ y,
+                                                         const int32_t *__restrict__ ranges, int64_t cap, int chunks) {
+    typedef typename VecOf<VT>::type V;
+    extern __shared__ unsigned char stage_raw[];
+    V *win = reinterpret_cast<V *>(stage_raw);
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // neighbouring windows on one XCD
+    const int32_t lo = ranges[2 * b], hi = ranges[2 * b + 1];
+    const int64_t width = (int64_t)hi - (int64_t)lo + 1;
+    const bool staged = hi >= lo && width <= cap;
+    if (staged)
+        for (int64_t i = threadIdx.x; i < width; i += blockDim.x) win[i] = x[lo + i];
+    __syncthreads();
+    const V x0 = x[0];
+    const int lane = (int)(threadIdx.x & (T - 1));
+    const int64_t rows_per_chunk = blockDim.x / T;
+    for (int c = 0; c < chunks; ++c) {
+        const int64_t row = (b * chunks + c) * rows_per_chunk + threadIdx.x / T;
+        V s = V(0);
+        if (row < rows) {
+            const uint32_t len = Arl[row];
+            int64_t slot = row * T + lane;
+            bool outside = !staged;
+            if (staged) {
+                // column 0 (padding) reads x0; any other column outside the
+                // window (ranges not made for this block) sends the lane to
+                // the global path.  U entries at a time (8 for 4-byte, 4 for
+                // 8-byte vectors): their index and value loads are all issued
+                // before the first use.
+                constexpr int U = sizeof(V) == 8 ? 4 : 8;
+                auto term = [&](int32_t j, V av) {
+                    const uint32_t o = (uint32_t)(j - lo);
+                    const bool in = o < (uint32_t)width;
+                    outside |= !in && j != 0;
+                    s += av * (in ? win[o] : x0);
+                };
+                uint32_t k = 0;
+                for (; k + U <= len; k += U) {
+                    int32_t j[U];
+                    V av[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) j[u] = __builtin_nontemporal_load(Aj + slot + u * stride);
+#pragma unroll
+                    for (int u = 0; u < U; ++u) av[u] = load_value_nt(Ax + slot + u * stride);
+#pragma unroll
+                    for (int u = 0; u < U; ++u) term(j[u], av[u]);
+                    slot += U * stride;
+                }
+                for (; k < len; ++k) {
+                    term(__builtin_nontemporal_load(Aj + slot), load_value_nt(Ax + slot));
+                    slot += stride;
+                }
+            }
+            if (outside) s = ell_lane_sum<VT, T>(Aj, Ax, row, lane, len, stride, x);
+        }
+        s = lane_group_sum<T>(s);
+        if (row < rows && lane == 0) y[row] = s;
+    }
+}
+
+typedef void (*StagedFn)(const int32_t *, const void *, const uint32_t *, int64_t, int64_t, const void *, void *,
+                         const int32_t *, int64_t, int);
+
+template <typename VT>
+StagedFn pick_staged(int T) {
+    switch (T) {
+        case 1: return (StagedFn)k_product_staged<VT, 1>;
+        case 2: return (StagedFn)k_product_staged<VT, 2>;
+        case 4: return (StagedFn)k_product_staged<VT, 4>;
+        case 8: return (StagedFn)k_product_staged<VT, 8>;
+        case 16: return (StagedFn)k_product_staged<VT, 16>;
+        case 32: return (StagedFn)k_product_staged<VT, 32>;
+        case 64: return (StagedFn)k_product_staged<VT, 64>;
+    }
+    return nullptr;
+}
+
 }  // namespace
+
+hipError_t ell_block_ranges(const EllView &A, int64_t rows_per_block, int32_t *ranges, hipStream_t s) {
+    const int T = A.threads;
+    if (rows_per_block < 1 || T < 1 || T > 64 || (T & (T - 1)) || !ranges) return hipErrorInvalidValue;
+    const int64_t nb = (A.rows + rows_per_block - 1) / rows_per_block;
+    if (nb <= 0) return hipSuccess;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_block_ranges, dim3((unsigned)nb), dim3(256), 0, s, A.Aj, A.Arl, A.rows, A.stride, T,
+                       rows_per_block, ranges);
+    return hipGetLastError();
+}
+
+hipError_t ell_product_staged(const EllView &A, void *y, const void *x, const int32_t *ranges, int block, int chunks,
+                              int64_t window, hipStream_t s) {
+    const int T = A.threads;
+    StagedFn fn = nullptr;
+    switch (A.vtype) {
+        case VT_F32: fn = pick_staged<float>(T); break;
+        case VT_F64: fn = pick_staged<double>(T); break;
+        case VT_BF16: fn = pick_staged<bf16>(T); break;
+    }
+    if (!fn || block % kWave || block > 1024 || block < T || !ranges || chunks < 1) return hipErrorInvalidValue;
+    const int64_t rpb = (int64_t)(block / T) * chunks;
+    const int64_t nb = (A.rows + rpb - 1) / rpb;
+    if (nb <= 0) return hipSuccess;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    const size_t vs = A.vtype == VT_F64 ? 8 : 4;
+    if (window < 1 || (size_t)window * vs > kMaxStageBytes) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(block), (size_t)window * vs, s, A.Aj, A.Ax, A.Arl, A.stride, A.rows,
+                       x, y, ranges, window, chunks);
+    return hipGetLastError();
+}
 
 hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slots, hipStream_t s) {
     const int T = A.threads;

@@ -19,7 +19,11 @@ template <> struct Product<tags::GPU::HIP> {
         void *y = hip::wr(result);
         const unsigned long bs = hip::blocksize("hip::product_smell_dv");
         int st;
-        if (m.vtype == HONEI_HIP_F64)
+        if (m.window > 0)
+            st = honei_hip_product_smell_dv_staged(m.vtype, x, y, m.Aj, m.Ax, m.Arl, a.rows(), a.num_cols_per_row(),
+                                                   a.stride(), a.threads(), kStageBlock, 1, (const int32_t *)m.ranges,
+                                                   m.nranges, m.window, nullptr);
+        else if (m.vtype == HONEI_HIP_F64)
             st = honei_hip_product_smell_dv_double(x, y, m.Aj, m.Ax, m.Arl, 0, a.rows(), a.num_cols_per_row(),
                                                    a.stride(), bs, a.threads(), nullptr);
         else if (m.vtype == HONEI_HIP_BF16)

@@ -29,12 +29,22 @@ template <> struct HipType<double> { static const int value = HONEI_HIP_F64; };
 struct DeviceEllMirror {
     void *Aj = nullptr, *Ax = nullptr, *Arl = nullptr;
     int vtype = HONEI_HIP_F32;
+    // x staged in LDS (honei_hip_product_smell_dv_staged): column windows of
+    // every stage_rows rows, their count and the LDS elements to reserve
+    // (0: staging not used)
+    void *ranges = nullptr;
+    unsigned long nranges = 0;
+    long long window = 0;
     ~DeviceEllMirror() {
         if (Aj) honei_hip_free(Aj);
         if (Ax) honei_hip_free(Ax);
         if (Arl) honei_hip_free(Arl);
+        if (ranges) honei_hip_free(ranges);
     }
 };
+
+// Work-group shape of the staged product: 1024 threads, one chunk.
+constexpr unsigned long kStageBlock = 1024;
 
 template <typename DT>
 class SparseMatrixELL {
@@ -43,7 +53,7 @@ class SparseMatrixELL {
         DenseVector<unsigned long> Aj, Arl;
         DenseVector<DT> Ax;
         std::shared_ptr<DeviceEllMirror> dev;
-        bool bf16 = false;
+        bool bf16 = false, stage_x = false;
         Imp(unsigned long r, unsigned long c, unsigned long s, unsigned long n, unsigned long t,
             const DenseVector<unsigned long> &aj, const DenseVector<DT> &ax, const DenseVector<unsigned long> &arl)
             : rows(r), columns(c), stride(s), num_cols_per_row(n), threads(t), Aj(aj), Arl(arl), Ax(ax) {}
@@ -152,6 +162,15 @@ class SparseMatrixELL {
         _imp->bf16 = on;
         _imp->dev.reset();
     }
+    // Products gather x from an LDS copy of each work-group's column window
+    // (pays off for scattered columns inside a band; same results).  The
+    // windows are computed with the device copy; a matrix whose widest
+    // window does not fit keeps the plain product.
+    void set_stage_x(bool on) {
+        _imp->stage_x = on;
+        _imp->dev.reset();
+    }
+    bool stage_x() const { return _imp->stage_x; }
     // Drop the device copy (after host-side modification of Aj/Ax/Arl).
     void invalidate_device() const { _imp->dev.reset(); }
 
@@ -176,6 +195,17 @@ class SparseMatrixELL {
                 m->vtype = HipType<DT>::value;
                 hip_check(honei_hip_malloc(&m->Ax, sizeof(DT) * (size ? size : 1)), "ELL Ax allocation");
                 hip_check(honei_hip_upload(m->Ax, _imp->Ax.elements(), sizeof(DT) * size, nullptr), "ELL Ax upload");
+            }
+            if (_imp->stage_x && rows() > 0) {
+                const unsigned long rpb = kStageBlock / threads();
+                m->nranges = (rows() + rpb - 1) / rpb;
+                hip_check(honei_hip_malloc(&m->ranges, 8 * m->nranges), "ELL column window allocation");
+                int64_t w = 0;
+                hip_check(honei_hip_ell_block_ranges(m->Aj, m->Arl, rows(), num_cols_per_row(), stride(), threads(), rpb,
+                                                     (int32_t *)m->ranges, &w, nullptr),
+                          "ELL column windows");
+                const long long cap = (96 * 1024) / (m->vtype == HONEI_HIP_F64 ? 8 : 4);
+                m->window = (w > 0 && w <= cap) ? w : 0;
             }
             _imp->dev = m;
         }

@@ -96,6 +96,31 @@ int honei_hip_product_smell_dv_bf16(void *x, void *y, const void *Aj, const void
                                     size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
                                     size_t blocksize, size_t threads, void *stream);
 
+/* ---- ELL product with x staged in LDS (new) --------------------------------
+ * Product<tags::GPU::HIP> on SparseMatrixELL for matrices whose rows keep
+ * their columns near the diagonal (banded / FEM orderings, la/product.hh:
+ * 148-206 computes the same sums): each work-group copies the x window of
+ * its rows into LDS once and gathers from there instead of from L2.
+ * honei_hip_ell_block_ranges: per group of rows_per_block rows the smallest
+ * and largest stored column other than 0 (padding slots hold column 0; x[0]
+ * is read separately) as int32 pairs, device, [ceil(rows/rpb)][2], and,
+ * when max_width is non-null, the widest window (host; synchronises).  It
+ * depends on the pattern only: compute once per matrix.
+ * honei_hip_product_smell_dv_staged: y = A x over all rows; a work-group
+ * covers chunks (1..64) x blocksize/threads rows and the ranges are made
+ * with that rows_per_block (blocksize 0 = 256; nranges = pairs in `ranges`,
+ * at least the number of work-groups, else an error); window = LDS
+ * elements per work-group (1 .. 96 KiB worth; use max_width when it fits).
+ * A block whose window is wider, or a column outside its range, is gathered
+ * from global memory.  Bit-identical to honei_hip_product_smell_dv_*.     */
+int honei_hip_ell_block_ranges(const void *Aj, const void *Arl, size_t rows, size_t num_cols_per_row, size_t stride,
+                               size_t threads, size_t rows_per_block, int32_t *ranges, int64_t *max_width,
+                               void *stream);
+int honei_hip_product_smell_dv_staged(int vtype, void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                      size_t rows, size_t num_cols_per_row, size_t stride, size_t threads,
+                                      size_t blocksize, size_t chunks, const int32_t *ranges, size_t nranges,
+                                      int64_t window, void *stream);
+
 /* ---- banded Q1 operators ---------------------------------------------------
  * BandedMatrixQx<Q1Type, DT>: nine device bands of `size` = m*m values
  * (LL, LD, LU, DL, DD, DU, UL, UD, UU: coefficients towards columns i-m-1,

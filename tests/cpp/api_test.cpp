@@ -164,6 +164,13 @@ void product_ell() {
                 for (auto &kv : sms.row(i)) scale += std::fabs((double)kv.second * x[kv.first]);
                 TEST_CHECK_EQUAL_WITHIN_EPS(prod[i], ref[i], eps * scale + 1e-30);
             }
+            // x staged in LDS: the same bits
+            SparseMatrixELL<DT> sm1(sms);
+            sm1.set_stage_x(true);
+            DenseVector<DT> prod_s(size, DT(4711));
+            Product<tags::GPU::HIP>::value(prod_s, sm1, dv1);
+            TEST_CHECK(sm1.device().window > 0);
+            for (unsigned long i = 0; i < size; ++i) TEST_CHECK(prod_s[i] == prod[i]);
         }
     }
     Configuration::instance()->set_value("ell::threads", 1);

@@ -349,7 +349,72 @@ def test_full_size_spmv_configs(dtype):
     assert np.array_equal(_gpu_product(A, x), oracle.product(A, x))
     R = _device_random_ell(1 << 22, vt)
     xr = uniform_pm1(2, R.rows).astype(dtype)
-    assert np.array_equal(_gpu_product(R, xr), oracle.product(R, xr))
+    yr = oracle.product(R, xr)
+    assert np.array_equal(_gpu_product(R, xr), yr)
+    ys, _ = _staged(H.DeviceEll.from_oracle(R), xr, 1024, chunks=2)
+    assert np.array_equal(ys, yr)
+
+
+# ---- ELL product with x staged in LDS ---------------------------------------------------
+
+def _staged(D, x, block, window=None, rpb=None, chunks=1):
+    ranges, w = D.block_ranges(rpb or block // D.threads * chunks)
+    dx = H.DeviceArray.from_host(x, D.vec_dtype)
+    dy = H.DeviceArray(D.rows, D.vec_dtype)
+    vs = 8 if D.vtype == H.F64 else 4
+    D.product_staged(dy, dx, ranges, window or min(max(w, 1), 98304 // vs), blocksize=block, chunks=chunks)
+    return dy.get(), w
+
+
+@pytest.mark.parametrize("vtype", [H.F32, H.F64, H.BF16])
+@pytest.mark.parametrize("threads", [1, 2, 4, 16])
+@pytest.mark.parametrize("block,chunks", [(256, 1), (1024, 1), (512, 3)])
+def test_staged_product_bitexact(vtype, threads, block, chunks):
+    """Staged gathers give the bits of the plain kernel (and, for T = 1, of
+    Product<tags::CPU>)."""
+    n = 30000
+    dtype = np.float64 if vtype == H.F64 else np.float32
+    print(f"staged product n={n} T={threads} block={block} chunks={chunks} vt={vtype}", flush=True)
+    rp, col, val = random_band_csr(n, seed=23, max_len=27, band=4096, dtype=dtype)
+    A = oracle.ell_from_sm(n, n, rp, col, val, threads)
+    D = H.DeviceEll.from_oracle(A, vtype)
+    x = uniform_pm1(3, n).astype(dtype)
+    ys, w = _staged(D, x, block, chunks=chunks)
+    assert w <= block // threads * chunks + 2 * 4096
+    yp = _gpu_product(A, x, vtype, blocksize=block)
+    assert np.array_equal(ys.view(np.uint8), yp.view(np.uint8))
+    if threads == 1 and vtype != H.BF16:
+        assert np.array_equal(ys.view(np.uint8), oracle.product(A, x).view(np.uint8))
+
+
+def test_staged_block_ranges_and_fallbacks():
+    """Windows equal the per-block min/max columns; windows wider than the
+    LDS allotment and ranges made for another block size fall back to global
+    gathers with the same bits."""
+    n = 5000
+    rp, col, val = random_band_csr(n, seed=5, max_len=27, band=2000, dtype=np.float64)
+    A = oracle.ell_from_sm(n, n, rp, col, val, 1)
+    D = H.DeviceEll.from_oracle(A)
+    ranges, w = D.block_ranges(256)
+    got = ranges.get().reshape(-1, 2)
+    for b in range(got.shape[0]):
+        cols = col[rp[b * 256]:rp[min(n, (b + 1) * 256)]]
+        cols = cols[cols != 0]   # column 0 stays out of the windows
+        assert got[b, 0] == cols.min() and got[b, 1] == cols.max()
+    assert w == int((got[:, 1] - got[:, 0] + 1).max())
+    x = uniform_pm1(4, n)
+    ref = oracle.product(A, x)
+    for window, rpb in ((64, None), (w, 64)):
+        y, _ = _staged(D, x, 256, window=window, rpb=rpb)
+        assert np.array_equal(y.view(np.uint8), ref.view(np.uint8)), (window, rpb)
+    dx, dy = H.DeviceArray.from_host(x), H.DeviceArray(n, np.float64)
+    with pytest.raises(H.HipError):   # more LDS than allowed
+        D.product_staged(dy, dx, ranges, 98304 // 8 + 1)
+    coarse, _ = D.block_ranges(1024)
+    with pytest.raises(H.HipError):   # ranges made for larger blocks: too few
+        D.product_staged(dy, dx, coarse, w, blocksize=256)
+    with pytest.raises(H.HipError):
+        D.product_staged(dy, dx, coarse, w, blocksize=256, chunks=0)
 
 
 # ---- banded Q1 product / defect --------------------------------------------------------
