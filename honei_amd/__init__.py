@@ -407,7 +407,8 @@ class BatchConfig(C.Structure):
     """honei_hip_batch_config (include/honei_hip.h)."""
     _fields_ = [("inner_vtype", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", _d),
                 ("tolerance", _d), ("max_outer", C.c_int32), ("averaging", C.c_int32),
-                ("iters_per_launch", C.c_int32), ("autotune", C.c_int32), ("outer_method", C.c_int32)]
+                ("iters_per_launch", C.c_int32), ("autotune", C.c_int32), ("outer_method", C.c_int32),
+                ("fused_levels", C.c_int32)]
 
 
 class SlotInfo(C.Structure):

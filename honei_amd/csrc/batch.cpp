@@ -341,7 +341,9 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
     auto mat = [&](const BLevel &lv) {
         return BMat{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, (lv.n * lv.T + lv.block - 1) / lv.block};
     };
-    for (int64_t l = max_level_; l >= 2; --l) {
+    // levels 1..lf as one launch (one work-group per slot), the rest level by level
+    const int64_t lf = cfg_.fused_levels >= 2 ? std::min<int64_t>(cfg_.fused_levels, max_level_) : 1;
+    for (int64_t l = max_level_; l > lf; --l) {
         BLevel &lv = L_[(size_t)l];
         const BMat m = mat(lv);
         const BWork w = level_work(l);
@@ -357,8 +359,41 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_DEFECT, m, w, lv.cap, lv.d, lv.x, lv.rhs, nullptr, s));
         HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
     }
-    HX(b_coarse_cg(vt, L_[1], level_work(1), s));
-    for (int64_t l = 2; l <= max_level_; ++l) {
+    if (lf >= 2) {
+        FusedCycle fc{};
+        fc.top = (int)lf;
+        fc.n_pre = cfg_.n_pre;
+        fc.n_post = cfg_.n_post;
+        for (int64_t l = 1; l <= lf; ++l) {
+            const BLevel &lv = L_[(size_t)l];
+            FusedLevel &f = fc.lv[l];
+            f.Aj = lv.Aj;
+            f.Arl = lv.Arl;
+            f.Ax = lv.Ax;
+            f.stride = lv.stride;
+            f.ax_ld = lv.ax_ld;
+            f.n = lv.n;
+            f.vld = lv.vld;
+            f.side = lv.side;
+            f.cap = lv.cap;
+            f.ncpr = lv.ncpr;
+            f.T = lv.T;
+            f.dinv = lv.dinv;
+            f.rhs = lv.rhs;
+            f.x = lv.x;
+            f.d = lv.d;
+            f.t = lv.t;
+        }
+        fc.g = L_[1].g;
+        fc.u = L_[1].u;
+        fc.en = L_[1].en;
+        fc.konv = 1.1920928955078125e-07;   // FP32 epsilon, as the level-1 launch
+        fc.max_iters = (int)(4 * L_[1].n + 64);
+        HX(b_coarse_vcycle(vt, fc, level_work(1), L_[1].cap, s));
+    } else {
+        HX(b_coarse_cg(vt, L_[1], level_work(1), s));
+    }
+    for (int64_t l = lf + 1; l <= max_level_; ++l) {
         BLevel &lv = L_[(size_t)l];
         const BMat m = mat(lv);
         const BWork w = level_work(l);

@@ -72,6 +72,9 @@ struct BatchConfig {
     // MIXED driver), 1 conjugate gradients in FP64 preconditioned by the
     // V-cycle (robust where the rediscretised hierarchy over-corrects).
     int32_t outer = 1;
+    // Levels 1..fused_levels of every V-cycle in one launch (one work-group
+    // per slot); 0 or 1: level by level.
+    int32_t fused_levels = 4;
 };
 
 // Inner level l of the pool: slots [0, cap) present.
@@ -313,6 +316,28 @@ struct SlotIds {
     int32_t n;
     int32_t id[kMaxSlotIds];
 };
+// The coarse end of the V-cycle in one launch: levels 1..top of every
+// listed slot by one work-group per slot (top = the slot's finest level if
+// that is below `top`), the same sweeps, transfers and coarse CG as the
+// level-by-level launches, in the same order and arithmetic.
+constexpr int kMaxFusedLevels = 6;
+struct FusedLevel {
+    const int32_t *Aj;
+    const uint32_t *Arl;
+    const void *Ax;
+    int64_t stride, ax_ld, n, vld, side, cap, ncpr;
+    int T;
+    float *dinv, *rhs, *x, *d, *t;
+};
+struct FusedCycle {
+    int top, n_pre, n_post;
+    FusedLevel lv[kMaxFusedLevels + 1];   // index = level (1..top)
+    float *g, *u, *en;                    // level-1 CG scratch
+    double konv;
+    int max_iters;
+};
+hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, int64_t max_slots, hipStream_t s);
+
 // Fresh scalars in `state` for slots[0..n), one launch per 256 ids.
 hipError_t b_set_states(SlotScalars *sc, const int64_t *slots, int64_t n, int32_t state, hipStream_t s);
 hipError_t b_fill_state(SlotScalars *sc, int64_t nslots, int32_t state, hipStream_t s);

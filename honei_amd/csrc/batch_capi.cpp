@@ -41,6 +41,7 @@ BatchConfig to_config(const honei_hip_batch_config &c) {
     b.iters_per_launch = c.iters_per_launch;
     b.autotune = c.autotune;
     b.outer = c.outer_method;
+    b.fused_levels = c.fused_levels;
     return b;
 }
 
@@ -71,6 +72,7 @@ int honei_hip_batch_config_default(honei_hip_batch_config *cfg) {
     cfg->iters_per_launch = b.iters_per_launch;
     cfg->autotune = b.autotune;
     cfg->outer_method = b.outer;
+    cfg->fused_levels = b.fused_levels;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -95,6 +97,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
     if (cfg->averaging < 0 || cfg->averaging > 2) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "averaging 0..2");
     if (cfg->outer_method != 0 && cfg->outer_method != 1)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "outer_method 0 or 1");
+    if (cfg->fused_levels < 0 || cfg->fused_levels > kMaxFusedLevels)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "fused_levels 0..6");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");

@@ -456,6 +456,118 @@ __global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t
     }
 }
 
+// ---- the coarse end of the V-cycle, one work-group per slot -------------------------
+// One sweep of level L of slot `ls` by the whole work-group (T lanes per row;
+// the loop bound is a multiple of blockDim so every lane of a row joins its
+// lane reduction).  Same row sum and epilogue as k_b_ell.
+template <typename VT, int T, int MODE>
+__device__ inline void fused_sweep(const FusedLevel &L, int64_t ls, float *out, const float *x, const float *rhs,
+                                   const float *dinv) {
+    const VT *Ax = (const VT *)L.Ax + ls * L.ax_ld;
+    const int64_t vo = ls * L.vld;
+    const float *xs = x + vo, *rs = rhs + vo, *ds = dinv + vo;
+    float *os = out + vo;
+    const int64_t total = ((L.n * T + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    for (int64_t gid = threadIdx.x; gid < total; gid += blockDim.x) {
+        const int lane = (int)(gid & (T - 1));
+        const int64_t row = gid / T;
+        const bool active = row < L.n;
+        float s = 0.f;
+        if (active) {
+            if constexpr (MODE == ELL_JACOBI0)
+                s = ell_lane_sum<VT, T, true, true>(L.Aj, Ax, row, lane, L.Arl[row], L.stride, rs, ds);
+            else
+                s = ell_lane_sum<VT, T, false, true>(L.Aj, Ax, row, lane, L.Arl[row], L.stride, xs);
+        }
+        s = lane_group_sum<T>(s);
+        if (active && lane == 0) {
+            if constexpr (MODE == ELL_DEFECT) {
+                os[row] = rs[row] - s;
+            } else {
+                const float xin = (MODE == ELL_JACOBI0) ? ds[row] * rs[row] : xs[row];
+                const float t = rs[row] - s;
+                os[row] = xin + t * ds[row];
+            }
+        }
+    }
+}
+
+template <typename VT, int MODE>
+__device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, float *out, const float *x, const float *rhs,
+                                     const float *dinv) {
+    switch (L.T) {
+        case 1: fused_sweep<VT, 1, MODE>(L, ls, out, x, rhs, dinv); break;
+        case 2: fused_sweep<VT, 2, MODE>(L, ls, out, x, rhs, dinv); break;
+        case 4: fused_sweep<VT, 4, MODE>(L, ls, out, x, rhs, dinv); break;
+        default: fused_sweep<VT, 8, MODE>(L, ls, out, x, rhs, dinv); break;
+    }
+    __syncthreads();
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w) {
+    const int64_t a0 = active_first(w), na = active_count(w);
+    const Mask8 mk{0xffu};   // all Dirichlet, as the level-by-level transfers
+    for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
+        const int64_t g = w.list[a0 + a];
+        int top = fc.top;
+        while (top > 1 && g >= fc.lv[top].cap) --top;   // slots whose finest level is lower
+        // down: n_pre sweeps from zero (first two fused), defect, restriction
+        for (int l = top; l >= 2; --l) {
+            const FusedLevel &L = fc.lv[l];
+            const int rest = fc.n_pre - 2;
+            float *src = (rest % 2 == 0) ? L.x : L.t;
+            fused_sweep_t<VT, ELL_JACOBI0>(L, g, src, nullptr, L.rhs, L.dinv);
+            for (int k = 0; k < rest; ++k) {
+                float *dst = (src == L.x) ? L.t : L.x;
+                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, L.rhs, L.dinv);
+                src = dst;
+            }
+            fused_sweep_t<VT, ELL_DEFECT>(L, g, L.d, L.x, L.rhs, nullptr);
+            const FusedLevel &C = fc.lv[l - 1];
+            const int64_t m = C.side - 1;
+            for (int64_t i = threadIdx.x; i < C.n; i += blockDim.x)
+                restrict_node<float>(C.rhs + g * C.vld, L.d + g * L.vld, m, mk, i);
+            __syncthreads();
+        }
+        {   // coarse CG on level 1
+            const FusedLevel &L = fc.lv[1];
+            EllView A{};
+            A.rows = L.n;
+            A.cols = L.n;
+            A.stride = L.stride;
+            A.ncpr = L.ncpr;
+            A.threads = L.T;
+            A.Aj = L.Aj;
+            A.Ax = (const VT *)L.Ax + g * L.ax_ld;
+            A.Arl = L.Arl;
+            const int64_t o = g * L.vld;
+            (void)coarse_cg_block<VT>(A, L.rhs + o, L.x + o, fc.g + o, fc.u + o, fc.en + o, fc.konv, fc.max_iters);
+            __syncthreads();
+        }
+        // up: x += P x_coarse, n_post sweeps (result in x)
+        for (int l = 2; l <= top; ++l) {
+            const FusedLevel &L = fc.lv[l];
+            const FusedLevel &C = fc.lv[l - 1];
+            const int64_t m = C.side - 1;
+            for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x)
+                prolong_node<float, true>(nullptr, C.x + g * C.vld, m, mk, L.x + g * L.vld, 1.0f, i);
+            __syncthreads();
+            float *src = L.x, *dst = L.t;
+            for (int k = 0; k < fc.n_post; ++k) {
+                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, L.rhs, L.dinv);
+                float *tmp = src;
+                src = dst;
+                dst = tmp;
+            }
+            if (src != L.x) {
+                for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x) L.x[g * L.vld + i] = src[g * L.vld + i];
+                __syncthreads();
+            }
+        }
+    }
+}
+
 // Fresh scalars in `state` for the slots of one id list (passed by value).
 __global__ void k_b_set_states(SlotScalars *sc, SlotIds ids, int32_t state) {
     const int k = (int)threadIdx.x;
@@ -682,6 +794,25 @@ hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream
     else if (vt == VT_BF16)
         hipLaunchKernelGGL(k_b_coarse_cg<bf16>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
                            (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, int64_t max_slots, hipStream_t s) {
+    if (max_slots <= 0) return hipSuccess;
+    if (fc.top < 1 || fc.top > kMaxFusedLevels || fc.n_pre < 2 || fc.n_post < 0) return hipErrorInvalidValue;
+    for (int l = 1; l <= fc.top; ++l) {
+        const int T = fc.lv[l].T;
+        if (T != 1 && T != 2 && T != 4 && T != 8) return hipErrorInvalidValue;
+        if (l > 1 && (fc.lv[l].side != 2 * (fc.lv[l - 1].side - 1) + 1 || fc.lv[l].cap > fc.lv[l - 1].cap))
+            return hipErrorInvalidValue;
+    }
+    const int grid = (int)(max_slots < 2048 ? max_slots : 2048);
+    if (vt == VT_F32)
+        hipLaunchKernelGGL(k_b_coarse_vcycle<float>, dim3(grid), dim3(256), 0, s, fc, w);
+    else if (vt == VT_BF16)
+        hipLaunchKernelGGL(k_b_coarse_vcycle<bf16>, dim3(grid), dim3(256), 0, s, fc, w);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

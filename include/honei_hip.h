@@ -356,6 +356,8 @@ typedef struct {
     int32_t outer_method;      /* 0 defect correction x += V(b - A x) (the reference's MIXED
                                   driver); 1 FP64 conjugate gradients preconditioned by the V-cycle,
                                   converged residual confirmed by a recomputed b - A x (default) */
+    int32_t fused_levels;      /* levels 1..fused_levels (0..6, default 4) of every V-cycle run
+                                  as one launch, one work-group per slot (same arithmetic) */
 } honei_hip_batch_config;
 
 typedef struct {

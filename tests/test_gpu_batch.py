@@ -137,6 +137,20 @@ def test_batch_alone_equals_in_the_mix():
     assert np.array_equal(a_x, m_x)
 
 
+@pytest.mark.parametrize("outer", [0, 1])
+def test_fused_coarse_levels_equal_level_by_level(outer):
+    """The coarse end of the V-cycle in one launch (levels 1..k, one
+    work-group per slot) gives the bits of the level-by-level launches, for
+    slots whose finest level is above, at and below k."""
+    systems = {(c, j): _system(lv, 300 + 10 * c + j) for c, lv in enumerate([6, 4, 3, 2]) for j in range(2)}
+    base, _, _ = _run_batch([6, 4, 3, 2], [2, 2, 2, 2], systems, outer_method=outer, fused_levels=0)
+    for k in (2, 4, 6):
+        fused, _, _ = _run_batch([6, 4, 3, 2], [2, 2, 2, 2], systems, outer_method=outer, fused_levels=k)
+        for key in systems:
+            assert base[key][0] == fused[key][0], (k, key)
+            assert np.array_equal(base[key][1], fused[key][1]), (k, key)
+
+
 def test_batch_slot_reuse_and_late_admission():
     """A slot reloaded after its first system retired, while another slot is
     still running, solves the new system exactly as a fresh batch does."""
