@@ -585,21 +585,50 @@ def main():
     H.call("honei_hip_product_smell_dv_double", dxt.ptr, db.ptr, A64.Aj.ptr, A64.Ax.ptr, A64.Arl.ptr, 0, n, 9,
            A64.stride, 0, 1, None)
     del A64, dxt
-    mg = H.Multigrid(H.F32, 1, LEVEL)
-    mg.configure(H.mg_config(n_max_iter=1, tolerance=1e-8, zero_inner_start=True, initial_zero=True))
-    mg.build_q1(0.7)
-    dx = H.DeviceArray(n, np.float64)
-    dx.zero()
-    mg.solve_mixed(db, dx)
-    t1 = []
-    for _ in range(5):
-        dx.zero()
-        H.synchronize()
-        t0 = time.perf_counter()
-        outer, inner, rd = mg.solve_mixed(db, dx)
-        t1.append(time.perf_counter() - t0)
-    err = float(np.abs(dx.get() - xt).max())
-    del mg, dx
+    def single_solve(inner_vt, tol):
+        """Median time, cycles, final defect (relative for MIXED, absolute
+        for the FP64 FIXED driver) and max error vs x_true on the resident
+        2049^2 hierarchy: MIXED (one cycle per FP64 outer step) for an FP32
+        inner hierarchy, pure FP64 multigrid (FIXED, cycles until the
+        defect falls below tol times the initial one) for an FP64 one."""
+        mg = H.Multigrid(inner_vt, 1, LEVEL)
+        if inner_vt == H.F64:
+            mg.configure(H.mg_config(n_max_iter=100, tolerance=tol))
+        else:
+            mg.configure(H.mg_config(n_max_iter=1, tolerance=tol, zero_inner_start=True, initial_zero=True))
+        mg.build_q1(0.7)
+        dx = H.DeviceArray(n, np.float64)
+
+        def run():
+            dx.zero()
+            if inner_vt == H.F64:
+                it, fd = mg.solve_fixed(db, dx)
+                return it, fd
+            outer, _inner, rd = mg.solve_mixed(db, dx)
+            return outer, rd
+
+        run()
+        ts = []
+        for _ in range(5):
+            H.synchronize()
+            t0 = time.perf_counter()
+            cyc, rd = run()
+            ts.append(time.perf_counter() - t0)
+        e = float(np.abs(dx.get() - xt).max())
+        del mg, dx
+        return float(np.median(ts)), cyc, rd, e
+
+    t1m, outer, rd, err = single_solve(H.F32, 1e-8)
+    c3 = {}
+    for key, vt in (("mixed_fp32_inner", H.F32), ("fp64_mg", H.F64)):
+        tm, oc, r, e = single_solve(vt, 1e-10)
+        c3[key] = {"solves_per_s": round(1.0 / tm, 3), "ms": round(tm * 1e3, 3), "cycles": oc,
+                   ("final_defect" if vt == H.F64 else "rel_residual"): r, "max_err_vs_x_true": e}
+        log(f"[2049] {key} to 1e-10: {tm * 1e3:.2f} ms, {oc} cycles, defect {r:.2e}, err {e:.2e}")
+    c3["speedup_mixed_vs_fp64"] = round(c3["mixed_fp32_inner"]["solves_per_s"] / c3["fp64_mg"]["solves_per_s"], 2)
+    c3["what"] = ("Q1 2049^2 (11 grids, 3x3 coarsest), b = A x_true (x_true ~ U[-1,1), seed 3), V(2,2) Jacobi 0.7, "
+                  "CG on 3x3, relative defect 1e-10: MIXED (FP32 V-cycle, one per FP64 outer step) vs pure FP64 "
+                  "multigrid (FIXED driver, FP64 V-cycles)")
     # the same system as a one-slot graph-launched pool (kappa = 1: the same operator)
     P1 = H.Batch([LEVEL], [1], H.batch_config(omega=0.7, iters_per_launch=8))
     ones = H.DeviceArray((SIDE - 1) ** 2, np.float64)
@@ -650,7 +679,8 @@ def main():
         "fp64_single_gpu": fp64,
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
-        "single_2049": {"solves_per_s": round(1.0 / float(np.median(t1)), 3), "outer_iterations": outer,
+        "config3_2049": c3,
+        "single_2049": {"solves_per_s": round(1.0 / t1m, 3), "outer_iterations": outer,
                         "rel_residual": rd, "max_err_vs_x_true": err,
                         "what": "one Q1 2049^2 constant-coefficient system, b = A x_true, mixed MG to 1e-8",
                         "as_one_slot_pool": {"solves_per_s": round(1.0 / float(np.median(tp[1:])), 3),

@@ -26,7 +26,7 @@ inline int stream_grid(int64_t n) {
 inline int reduce_grid(int64_t n) {
     int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
     if (g < 1) g = 1;
-    if (g > 1024) g = 1024;
+    if (g > 2048) g = 2048;
     return (int)g;
 }
 
@@ -64,27 +64,39 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
     GRID_STRIDE(i, (int64_t)n) p[i] = 0u;
 }
 
-// MODE 0: sum x*y ; 1: sum x*x ; 2: x += a*y then sum x*x
+// MODE 0: sum x*y ; 1: sum x*x ; 2: x += a*y then sum x*x.  Each thread
+// walks its grid-stride elements four at a time into four accumulators
+// (the four loads in flight together), summed in a fixed order at the end;
+// the grid depends on n only, so the result is reproducible.
 template <typename V, int MODE>
 __global__ __launch_bounds__(kBlock) void k_reduce(V *x, const V *y, V a, const double *a_dev, int64_t n,
                                                    double *out, ReduceSlots slots) {
-    double acc = 0.0;
     V aa = a;
     if constexpr (MODE == 2) {
         if (a_dev) aa = (V)*a_dev;
     }
-    GRID_STRIDE(i, n) {
+    auto term = [&](int64_t i) -> double {
         if constexpr (MODE == 0) {
-            acc += (double)x[i] * (double)y[i];
+            return (double)x[i] * (double)y[i];
         } else if constexpr (MODE == 1) {
-            acc += (double)x[i] * (double)x[i];
+            return (double)x[i] * (double)x[i];
         } else {
             const V v = x[i] + y[i] * aa;
             x[i] = v;
-            acc += (double)v * (double)v;
+            return (double)v * (double)v;
         }
+    };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (; i + 3 * stride < n; i += 4 * stride) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += term(i + u * stride);
     }
-    const double b = block_sum(acc);
+#pragma unroll
+    for (int u = 0; u < 3; ++u)   // at most three left; compile-time accumulator index
+        if (i + u * stride < n) acc[u] += term(i + u * stride);
+    const double b = block_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (threadIdx.x == 0) slots.partials[blockIdx.x] = b;
 }
 
