@@ -205,33 +205,72 @@ def time_spmv_staged(D, reps, x, y, label, block=1024, chunks=1):
             "what": "honei_hip_product_smell_dv_staged, column windows per block computed once per matrix"}
 
 
+def config1_cg(with_cpu):
+    """CGSolver<..., NONE> in FP64 on the Q1 129^2 operator (16,641 rows,
+    ELL width 9), b = A x_true (x_true ~ U[-1,1), seed 1), x0 = 0, to
+    relative residual 1e-8: the GPU solve (device ELL from the host
+    conversion) beside the oracle's C restatement on one host core."""
+    import oracle
+    side = 129
+    A = oracle.q1_matrix(side, np.float64)
+    xt = uniform_pm1(1, A.rows)
+    b = oracle.product(A, xt)
+    D = H.DeviceEll.from_oracle(A)
+    db = H.DeviceArray.from_host(b)
+    dx = H.DeviceArray(A.rows, np.float64)
+    ts, its = [], 0
+    for k in range(6):
+        dx.zero()
+        H.synchronize()
+        t0 = time.perf_counter()
+        its, fd = H.cg_solve(D, db, dx, max_iters=10000, eps=1e-8)
+        if k:
+            ts.append(time.perf_counter() - t0)
+    err = float(np.abs(dx.get() - xt).max())
+    out = {"gpu_ms": round(1e3 * float(np.median(ts)), 3), "iterations": int(its), "max_err_vs_x_true": err,
+           "what": "Q1 129^2 FP64 CG to 1e-8 (config 1); GPU: honei_hip_cg_solve"}
+    if with_cpu:
+        cs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _x, cits = oracle.cg_solver(A, b, eps=1e-8)
+            cs.append(time.perf_counter() - t0)
+        out["cpu_port_ms"] = round(1e3 * float(np.median(cs)), 3)
+        out["cpu_iterations"] = int(cits)
+    log(f"[cg 129^2] GPU {out['gpu_ms']} ms ({its} iterations), CPU port {out.get('cpu_port_ms')} ms")
+    return out
+
+
 def time_banded_q1(side, dtype, reps=50):
     """Banded Q1 product (BandedMatrixQx, no index stream): average launch
-    time and algorithmic bytes 9n*s_v + n*(s_x + s_y)."""
+    time and algorithmic bytes 9n*s_v + n*(s_x + s_y).  The FP32 working set
+    at 2049^2 (185 MB) would fit the 256 MB MALL, so launches alternate
+    between two copies of the operands and every launch streams from HBM."""
     n = side * side
     sv = np.dtype(dtype).itemsize
     sfx = "float" if dtype == np.float32 else "double"
-    bands = [H.DeviceArray(n, dtype) for _ in range(9)]
-    for k, b in enumerate(bands):
-        b.zero()   # values do not change the traffic
-    x = H.DeviceArray.from_host(uniform_pm1(2, n).astype(dtype))
-    y = H.DeviceArray(n, dtype)
-    args = [b.ptr for b in bands] + [x.ptr, y.ptr, n, 0, side, None]
+    sets = []
+    for c in range(2):
+        bands = [H.DeviceArray.from_host(uniform_pm1(10 + 9 * c + k, n).astype(dtype)) for k in range(9)]
+        x = H.DeviceArray.from_host(uniform_pm1(2 + c, n).astype(dtype))
+        y = H.DeviceArray(n, dtype)
+        sets.append((bands, x, y, [b.ptr for b in bands] + [x.ptr, y.ptr, n, 0, side, None]))
     fn = f"honei_hip_product_bmdv_q1_{sfx}"
-    for _ in range(3):
-        H.call(fn, *args)
+    for k in range(4):
+        H.call(fn, *sets[k % 2][3])
     H.synchronize()
     e0, e1 = H.Event(), H.Event()
     e0.record()
-    for _ in range(reps):
-        H.call(fn, *args)
+    for k in range(reps):
+        H.call(fn, *sets[k % 2][3])
     e1.record()
     ms = e0.elapsed_ms(e1) / reps
     alg = 9 * n * sv + 2 * n * sv
     gbs = alg / (ms * 1e-3) / 1e9
     log(f"[spmv] banded q1 {side}^2 {sfx}: {ms * 1e3:.1f} us/launch, {alg / 1e6:.1f} MB, {gbs:.0f} GB/s")
+    del sets
     return {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
-            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
+            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": 2}
 
 
 def read_traffic():
@@ -482,6 +521,7 @@ def main():
         spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
         del R, xr, yr
     spmv["hbm_ceiling"] = hbm_ceiling()
+    cg129 = config1_cg(not args.no_cpu)
     if not args.no_cpu:
         spmv["cpu_q1_2049_f32"] = cpu_spmv_q1(SIDE)
 
@@ -679,6 +719,7 @@ def main():
         "fp64_single_gpu": fp64,
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
+        "config1_cg_129": cg129,
         "config3_2049": c3,
         "single_2049": {"solves_per_s": round(1.0 / t1m, 3), "outer_iterations": outer,
                         "rel_residual": rd, "max_err_vs_x_true": err,

@@ -32,16 +32,23 @@ __global__ __launch_bounds__(256) void k_q1_banded(Bands9 bands, const V *__rest
     const int64_t nb = gridDim.x;
     const int64_t i = xcd_block(blockIdx.x, nb) * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    auto band = [&](int k) { return __builtin_nontemporal_load((const V *)bands.b[k] + i); };
-    V r = band(4) * x[i];                                           // DD
-    if (i - m - 1 >= 0) r += band(0) * x[i - m - 1];                // LL
-    if (i - m >= 0) r += band(1) * x[i - m];                        // LD
-    if (i - m + 1 >= 0) r += band(2) * x[i - m + 1];                // LU
-    if (i - 1 >= 0) r += band(3) * x[i - 1];                        // DL
-    if (i + 1 < n) r += band(5) * x[i + 1];                         // DU
-    if (i + m - 1 < n) r += band(6) * x[i + m - 1];                 // UL
-    if (i + m < n) r += band(7) * x[i + m];                         // UD
-    if (i + m + 1 < n) r += band(8) * x[i + m + 1];                 // UU
+    // all nine band values and gathered x first (column indices clamped into
+    // [0, n) so every load is in bounds), then the guarded sums in the CPU
+    // order; a select keeps a term out exactly where the CPU loop skips it
+    const int64_t off[9] = {-m - 1, -m, -m + 1, -1, 0, 1, m - 1, m, m + 1};
+    V a[9], xv[9];
+    bool ok[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int64_t j = i + off[k];
+        ok[k] = j >= 0 && j < n;
+        a[k] = __builtin_nontemporal_load((const V *)bands.b[k] + i);
+        xv[k] = x[ok[k] ? j : i];
+    }
+    V r = a[4] * xv[4];                                             // DD
+#pragma unroll
+    for (int k = 0; k < 9; ++k)                                     // LL LD LU DL DU UL UD UU
+        if (k != 4) r = ok[k] ? r + a[k] * xv[k] : r;
     if constexpr (DEFECT)
         y[i] = __builtin_nontemporal_load(rhs + i) - r;
     else

@@ -114,6 +114,73 @@ __device__ inline void ell_lane_sum_group_w(const int32_t *__restrict__ Aj, cons
     }
 }
 
+// ---- buffer loads -----------------------------------------------------------
+// A buffer resource over [p, p + bytes) (raw, 32-bit offsets; reads past the
+// end return 0).  Loads through it take a per-lane 32-bit byte offset plus a
+// uniform scalar one, so the NC slots of a row (slot0 + k*stride) share one
+// offset register and differ only in the scalar k*stride*size.
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kBufNT = 2;   // cache policy bit: non-temporal
+
+template <typename T> struct BufLoad;
+template <> struct BufLoad<float> {
+    template <int AUX> static __device__ inline float load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, AUX));
+    }
+};
+template <> struct BufLoad<double> {
+    template <int AUX> static __device__ inline double load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, AUX));
+    }
+};
+template <> struct BufLoad<int32_t> {
+    template <int AUX> static __device__ inline int32_t load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, AUX);
+    }
+};
+template <> struct BufLoad<bf16> {   // the bf16 value widened to float
+    template <int AUX> static __device__ inline float load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        const uint32_t h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, AUX);
+        return __uint_as_float(h << 16);
+    }
+};
+
+// ell_lane_sum for a matrix of stored width NC whose index and value
+// streams are read through buffer resources (rAj over Aj, rAx over Ax): the
+// NC indices first, then the NC values and the gathered x (plain loads: the
+// length of x is not known here), then the guarded sums -- each sum is the
+// same sequence of roundings as ell_lane_sum (slots k >= len are read,
+// padding holds Aj = 0 and Ax = 0, but never added).  NC*stride slots must
+// fit 31-bit byte offsets (the caller checks).
+template <typename VT, int T, int NC, bool SCALED = false>
+__device__ inline typename VecOf<VT>::type ell_lane_sum_buf(__amdgpu_buffer_rsrc_t rAj, __amdgpu_buffer_rsrc_t rAx,
+                                                            int64_t row, int lane, uint32_t len, int64_t stride,
+                                                            const typename VecOf<VT>::type *__restrict__ x,
+                                                            const typename VecOf<VT>::type *__restrict__ dinv) {
+    typedef typename VecOf<VT>::type V;
+    const uint32_t slot0 = (uint32_t)(row * T + lane);
+    int32_t j[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        j[k] = BufLoad<int32_t>::template load<kBufNT>(rAj, slot0 * 4u, (uint32_t)(k * stride) * 4u);
+    V av[NC], xv[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        av[k] = BufLoad<VT>::template load<kBufNT>(rAx, slot0 * (uint32_t)sizeof(VT),
+                                                   (uint32_t)(k * stride) * (uint32_t)sizeof(VT));
+        if constexpr (SCALED)
+            xv[k] = dinv[j[k]] * x[j[k]];
+        else
+            xv[k] = x[j[k]];
+    }
+    V s = V(0);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) s = ((uint32_t)k < len) ? s + av[k] * xv[k] : s;
+    return s;
+}
+
 // Stored width of the Q1 operator (9 entries per row) at T lanes per row.
 constexpr int q1_width(int T) { return (9 + T - 1) / T; }
 

@@ -13,6 +13,8 @@
 // roundings (no FMA contraction) so that for T = 1 the result is
 // bit-identical to Product<tags::CPU> (la/product.hh:148-206).
 
+#include <algorithm>
+
 #include "backend.hh"
 #include "ell_device.hh"
 
@@ -36,12 +38,15 @@ struct KArgs {
     int64_t row_start, row_end;
     int64_t nlogical;
     ReduceSlots slots;
+    uint32_t aj_bytes, ax_bytes;   // buffer extents of the fixed-width form
 };
 
 // One block serves `per_block` consecutive logical blocks of blockDim/T rows
 // (per_block = 1 unless the launch was capped, as it is for launches that
-// carry a reduction: few blocks keep the arrival counter cold).
-template <typename VT, int T, int MODE, bool NORM, bool ACC>
+// carry a reduction: few blocks keep the arrival counter cold).  W: the
+// stored width is q1_width(T) (the Q1 operators among others) and the row
+// sum takes the fixed-width form through buffer loads.
+template <typename VT, int T, int MODE, bool NORM, bool ACC, bool W>
 __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     typedef typename VecOf<VT>::type V;
     const VT *__restrict__ Ax = (const VT *)a.Ax;
@@ -64,10 +69,19 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
 
         V s = V(0);
         if (active) {
-            if constexpr (MODE == ELL_JACOBI0)
-                s = ell_lane_sum<VT, T, true>(Aj, Ax, row, lane, a.Arl[row], a.stride, rhs, dinv);
-            else
-                s = ell_lane_sum<VT, T>(Aj, Ax, row, lane, a.Arl[row], a.stride, x);
+            if constexpr (W) {
+                const auto rAj = buffer_rsrc(Aj, a.aj_bytes), rAx = buffer_rsrc(Ax, a.ax_bytes);
+                if constexpr (MODE == ELL_JACOBI0)
+                    s = ell_lane_sum_buf<VT, T, q1_width(T), true>(rAj, rAx, row, lane, a.Arl[row], a.stride, rhs,
+                                                                    dinv);
+                else
+                    s = ell_lane_sum_buf<VT, T, q1_width(T)>(rAj, rAx, row, lane, a.Arl[row], a.stride, x, nullptr);
+            } else {
+                if constexpr (MODE == ELL_JACOBI0)
+                    s = ell_lane_sum<VT, T, true>(Aj, Ax, row, lane, a.Arl[row], a.stride, rhs, dinv);
+                else
+                    s = ell_lane_sum<VT, T>(Aj, Ax, row, lane, a.Arl[row], a.stride, x);
+            }
         }
         s = lane_group_sum<T>(s);
 
@@ -103,33 +117,40 @@ __global__ void k_finish_norm(const double *partials, int64_t nparts, double *ou
 
 typedef void (*KernelFn)(KArgs);
 
-template <typename VT, int T>
-KernelFn pick_mode(EllMode mode, bool norm, bool acc) {
+template <typename VT, int T, bool W>
+KernelFn pick_mode_w(EllMode mode, bool norm, bool acc) {
     switch (mode) {
-        case ELL_PRODUCT: return ell_row_kernel<VT, T, ELL_PRODUCT, false, false>;
+        case ELL_PRODUCT: return ell_row_kernel<VT, T, ELL_PRODUCT, false, false, W>;
         case ELL_DEFECT:
-            return norm ? ell_row_kernel<VT, T, ELL_DEFECT, true, false>
-                        : ell_row_kernel<VT, T, ELL_DEFECT, false, false>;
+            return norm ? ell_row_kernel<VT, T, ELL_DEFECT, true, false, W>
+                        : ell_row_kernel<VT, T, ELL_DEFECT, false, false, W>;
         case ELL_JACOBI:
-            return acc ? ell_row_kernel<VT, T, ELL_JACOBI, false, true>
-                       : ell_row_kernel<VT, T, ELL_JACOBI, false, false>;
+            return acc ? ell_row_kernel<VT, T, ELL_JACOBI, false, true, W>
+                       : ell_row_kernel<VT, T, ELL_JACOBI, false, false, W>;
         case ELL_JACOBI0:
-            return acc ? ell_row_kernel<VT, T, ELL_JACOBI0, false, true>
-                       : ell_row_kernel<VT, T, ELL_JACOBI0, false, false>;
+            return acc ? ell_row_kernel<VT, T, ELL_JACOBI0, false, true, W>
+                       : ell_row_kernel<VT, T, ELL_JACOBI0, false, false, W>;
     }
     return nullptr;
 }
 
+template <typename VT, int T>
+KernelFn pick_mode(EllMode mode, bool norm, bool acc, bool wide) {
+    if constexpr (T <= 8)
+        if (wide) return pick_mode_w<VT, T, true>(mode, norm, acc);
+    return pick_mode_w<VT, T, false>(mode, norm, acc);
+}
+
 template <typename VT>
-KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc) {
+KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc, bool wide) {
     switch (T) {
-        case 1: return pick_mode<VT, 1>(mode, norm, acc);
-        case 2: return pick_mode<VT, 2>(mode, norm, acc);
-        case 4: return pick_mode<VT, 4>(mode, norm, acc);
-        case 8: return pick_mode<VT, 8>(mode, norm, acc);
-        case 16: return pick_mode<VT, 16>(mode, norm, acc);
-        case 32: return pick_mode<VT, 32>(mode, norm, acc);
-        case 64: return pick_mode<VT, 64>(mode, norm, acc);
+        case 1: return pick_mode<VT, 1>(mode, norm, acc, wide);
+        case 2: return pick_mode<VT, 2>(mode, norm, acc, wide);
+        case 4: return pick_mode<VT, 4>(mode, norm, acc, wide);
+        case 8: return pick_mode<VT, 8>(mode, norm, acc, wide);
+        case 16: return pick_mode<VT, 16>(mode, norm, acc, wide);
+        case 32: return pick_mode<VT, 32>(mode, norm, acc, wide);
+        case 64: return pick_mode<VT, 64>(mode, norm, acc, wide);
     }
     return nullptr;
 }
@@ -300,11 +321,15 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     const int T = A.threads;
     const bool norm = (e.mode == ELL_DEFECT) && e.norm2 != nullptr;
     const bool acc = (e.mode == ELL_JACOBI || e.mode == ELL_JACOBI0) && e.acc != nullptr;
+    // fixed-width form: stored width q1_width(T) and every byte offset within 31 bits
+    const int64_t vsz = A.vtype == VT_F64 ? 8 : (A.vtype == VT_BF16 ? 2 : 4);
+    const int64_t slots_total = A.ncpr * A.stride;
+    const bool wide = T <= 8 && A.ncpr == q1_width(T) && slots_total * 8 < (int64_t(1) << 31);
     KernelFn fn = nullptr;
     switch (A.vtype) {
-        case VT_F32: fn = pick_threads<float>(T, e.mode, norm, acc); break;
-        case VT_F64: fn = pick_threads<double>(T, e.mode, norm, acc); break;
-        case VT_BF16: fn = pick_threads<bf16>(T, e.mode, norm, acc); break;
+        case VT_F32: fn = pick_threads<float>(T, e.mode, norm, acc, wide); break;
+        case VT_F64: fn = pick_threads<double>(T, e.mode, norm, acc, wide); break;
+        case VT_BF16: fn = pick_threads<bf16>(T, e.mode, norm, acc, wide); break;
     }
     if (!fn) return hipErrorInvalidValue;
     const int64_t nrows = e.row_end - e.row_start;
@@ -333,6 +358,8 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     k.row_end = e.row_end;
     k.nlogical = nlogical;
     k.slots = slots;
+    k.aj_bytes = (uint32_t)(slots_total * 4);
+    k.ax_bytes = (uint32_t)(slots_total * vsz);
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
     if (norm) hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, (const double *)slots.partials, grid, e.norm2);
     return hipGetLastError();
