@@ -45,46 +45,14 @@ __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restric
 }
 
 // ell_lane_sum for G value arrays that share one index array (G systems of
-// one sparsity pattern): per array exactly ell_lane_sum's products and sums,
-// in its order, with each index loaded once for all G.  All G arrays are
+// one sparsity pattern) whose stored width ncpr is the compile-time NC: the
+// lane's NC column indices are loaded once for all G, then all values and
+// gathered x, so a row costs two dependent memory round trips instead of one
+// pair per unrolled step.  Slots k >= len are read too (padding holds Aj =
+// 0, Ax = 0, inside the ncpr*stride arrays) but never added: per array each
+// sum is the same sequence of roundings as ell_lane_sum.  Every array is
 // read (a caller with fewer live systems points the spare entries at a live
-// one and ignores their sums), so the loop body has no branches and the
-// unrolled iterations' loads are all in flight together.
-template <typename VT, int T, int G, bool SCALED = false>
-__device__ inline void ell_lane_sum_group(const int32_t *__restrict__ Aj, const VT *const (&Ax)[G], int64_t row,
-                                          int lane, uint32_t len, int64_t stride,
-                                          const typename VecOf<VT>::type *const (&x)[G],
-                                          const typename VecOf<VT>::type *const (&dinv)[G],
-                                          typename VecOf<VT>::type (&s)[G]) {
-    typedef typename VecOf<VT>::type V;
-#pragma unroll
-    for (int g = 0; g < G; ++g) s[g] = V(0);
-    int64_t slot = row * T + lane;
-#pragma unroll 4
-    for (uint32_t k = 0; k < len; ++k) {
-        const int32_t j = Aj[slot];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const V av = load_value_nt(Ax[g] + slot);
-            V xv;
-            if constexpr (SCALED)
-                xv = dinv[g][j] * x[g][j];
-            else
-                xv = x[g][j];
-            s[g] += av * xv;
-        }
-        slot += stride;
-    }
-}
-
-// ell_lane_sum_group for a matrix whose stored width ncpr is the
-// compile-time NC: the lane's NC column indices are loaded first, then all
-// values and gathered x of the G arrays, so a row costs two dependent memory
-// round trips instead of one pair per unrolled step.  Slots k >= len are
-// read too (padding holds Aj = 0, Ax = 0, inside the ncpr*stride arrays) but
-// never added: each sum is the same sequence of roundings as ell_lane_sum.
-// As in ell_lane_sum_group every array is read (spare ones point at a live
-// one).
+// one and ignores their sums).
 template <typename VT, int T, int G, int NC, bool SCALED = false>
 __device__ inline void ell_lane_sum_group_w(const int32_t *__restrict__ Aj, const VT *const (&Ax)[G], int64_t row,
                                             int lane, uint32_t len, int64_t stride,
