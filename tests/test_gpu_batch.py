@@ -191,6 +191,14 @@ def test_batch_not_converged_and_budget():
     with pytest.raises(H.HipError) as e:
         H.Batch([1], [1], H.batch_config())
     assert e.value.code == 1
+    for bad in (dict(fused_levels=7), dict(fused_levels=-1), dict(outer_method=2), dict(n_pre=1),
+                dict(iters_per_launch=0), dict(averaging=3)):
+        with pytest.raises(H.HipError) as e:
+            H.Batch([5], [1], H.batch_config(**bad))
+        assert e.value.code == 1, bad
+    with pytest.raises(H.HipError) as e:   # finest level above the pool's limit
+        H.Batch([14], [1], H.batch_config())
+    assert e.value.code == 1
 
 
 def test_batch_bf16_inner_converges():
