@@ -79,6 +79,8 @@ _SIGS = {
     "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
                            C.POINTER(_d), _p],
     "honei_hip_cg_konv": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _d, _sz, C.POINTER(_sz), _p],
+    "honei_hip_pcg_jacobi": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _p, _sz, _d, C.POINTER(_sz),
+                             C.POINTER(_d), _p],
     "honei_hip_mg_create": [C.POINTER(_p), _i, _sz, _sz],
     "honei_hip_mg_destroy": [_p],
     "honei_hip_mg_configure": [_p, _p],
@@ -572,4 +574,19 @@ def cg_solve(D: DeviceEll, b: DeviceArray, x: DeviceArray, max_iters=10000, eps=
                                   b.ptr, x.ptr, max_iters, eps, C.byref(used), C.byref(fd), stream)
     if rc != SUCCESS and (check or rc != 5):
         raise HipError(rc, "honei_hip_cg_solve", last_error())
+    return used.value, fd.value
+
+
+def pcg_jacobi(D: DeviceEll, b: DeviceArray, x: DeviceArray, dd_inverted: DeviceArray, max_iters=10000, eps=1e-8,
+               stream=None, check=True):
+    """ConjugateGradients<tags::GPU::HIP, JAC>::value(A, b, x, dd_inverted,
+    max_iters, used_iters, eps_relative) (math/conjugate_gradients.hh:
+    1110-1154).  Returns (used_iters, final ||r||); with check=False a run
+    that reaches max_iters returns normally, as the reference does."""
+    used, fd = _sz(), _d()
+    rc = lib().honei_hip_pcg_jacobi(D.vtype, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, D.rows, D.ncpr, D.stride, D.threads,
+                                    b.ptr, x.ptr, dd_inverted.ptr, max_iters, eps, C.byref(used), C.byref(fd),
+                                    stream)
+    if rc != SUCCESS and (check or rc != 5):
+        raise HipError(rc, "honei_hip_pcg_jacobi", last_error())
     return used.value, fd.value

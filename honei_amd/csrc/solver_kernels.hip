@@ -128,6 +128,118 @@ __global__ void k_cg_init(CGScalars *sc, double eps_rel, int max_iters) {
     }
 }
 
+// ---- Jacobi-preconditioned CG ------------------------------------------------
+// ConjugateGradients<Tag, JAC>::value on ELL (math/conjugate_gradients.hh:
+// 1110-1154) with its cg_kernel (:884-913).  CGScalars carries alpha = r.z,
+// lambda = the step (the reference's alpha), talpha = the direction factor
+// (its beta), initial / current = ||r||.  z = dinv .* r is never stored: the
+// update and direction kernels form it where they need it (same rounding).
+// t0 = A d and d.t0 come from k_cg_spmv_dot (p = d, v = t0).
+
+// d = dinv .* r and partial r.z (one block per slot of the partials)
+template <typename V>
+__global__ __launch_bounds__(256) void k_pcg_start(V *d, const V *r, const V *dinv, int64_t n, ReduceSlots sl) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const V ri = r[i];
+        const V zi = dinv[i] * ri;
+        d[i] = zi;
+        acc += (double)ri * (double)zi;
+    }
+    const double b = block_sum(acc);
+    if (threadIdx.x == 0) sl.partials[blockIdx.x] = b;
+}
+
+// r.z from the partials of k_pcg_start; ||r||^2 was left in sc->current by
+// the defect kernel (one block)
+template <typename V>
+__global__ void k_pcg_init(const double *partials, int64_t nparts, CGScalars *sc, double eps_rel, int max_iters) {
+    const double rz = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) {
+        sc->alpha = (double)(V)rz;
+        const V initial = sqrt((V)sc->current);
+        sc->initial = (double)initial;
+        sc->current = sc->initial;
+        sc->eps_rel = eps_rel;
+        sc->iters = 0;
+        sc->max_iters = max_iters;
+        sc->done = max_iters <= 0 ? 1 : 0;
+        sc->converged = 0;
+    }
+}
+
+// step = (r.z) / (d.t0), the denominator clamped to eps when |d.t0| < eps
+// (one block)
+template <typename V>
+__global__ void k_pcg_step(const double *partials, int64_t nparts, CGScalars *sc) {
+    if (sc->done) return;
+    const double tot = sum_partials(partials, nparts, 1, 0);
+    if (threadIdx.x == 0) {
+        const V dt = (V)tot;
+        const V e = eps_of<V>();
+        const V rz = (V)sc->alpha;
+        sc->lambda = (double)(((dt < 0 ? -dt : dt) >= e) ? rz / dt : rz / e);
+    }
+}
+
+// x += step d ; r += -step t0 ; partials of z.r and r.r (z = dinv .* r)
+template <typename V>
+__global__ __launch_bounds__(256) void k_pcg_update(V *x, const V *d, V *r, const V *t0, const V *dinv, int64_t n,
+                                                    CGScalars *sc, ReduceSlots sl) {
+    if (sc->done) return;
+    const V step = (V)sc->lambda;
+    const V mstep = -step;
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += d[i] * step;
+        const V ri = r[i] + t0[i] * mstep;
+        r[i] = ri;
+        const V zi = dinv[i] * ri;
+        acc[0] += (double)zi * (double)ri;
+        acc[1] += (double)ri * (double)ri;
+    }
+    block_sum_first<2>(acc);
+    if (threadIdx.x == 0) {
+        sl.partials[2 * blockIdx.x] = acc[0];
+        sl.partials[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+// direction factor = (z.r)_new / (r.z)_old (clamped like the step), ||r||,
+// iteration count and the test ||r|| < initial * eps_rel (one block)
+template <typename V>
+__global__ void k_pcg_beta(const double *partials, int64_t nparts, CGScalars *sc) {
+    if (sc->done) return;
+    const double zr_t = sum_partials(partials, nparts, 2, 0);
+    const double rr_t = sum_partials(partials, nparts, 2, 1);
+    if (threadIdx.x == 0) {
+        const V e = eps_of<V>();
+        const V rz_old = (V)sc->alpha;
+        const V zr = (V)zr_t;
+        sc->talpha = (double)(((rz_old < 0 ? -rz_old : rz_old) >= e) ? zr / rz_old : zr / e);
+        sc->alpha = (double)zr;
+        sc->iters += 1;
+        const V current = sqrt((V)rr_t);
+        sc->current = (double)current;
+        const bool conv = current < (V)sc->initial * (V)sc->eps_rel;
+        sc->converged = conv ? 1 : 0;
+        if (conv || sc->iters >= sc->max_iters) sc->done = 1;
+    }
+}
+
+// d = z + d * factor with z = dinv .* r (ScaledSum(d, z, t0, beta) after
+// copy(d, t0))
+template <typename V>
+__global__ __launch_bounds__(256) void k_pcg_direction(V *d, const V *r, const V *dinv, int64_t n,
+                                                       const CGScalars *sc) {
+    if (sc->done) return;
+    const V t = (V)sc->talpha;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const V zi = dinv[i] * r[i];
+        d[i] = zi + d[i] * t;
+    }
+}
+
 template <typename VT>
 __global__ __launch_bounds__(256) void k_coarse_cg(EllView A, const typename VecOf<VT>::type *rhs,
                                                    typename VecOf<VT>::type *x, typename VecOf<VT>::type *g,
@@ -187,6 +299,43 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
     return hipGetLastError();
 }
 
+template <typename VT>
+hipError_t pcg_iter_t(const EllView &A, void *x, void *d, void *r, void *t0, const void *dinv, CGScalars *sc,
+                      const ReduceSlots &sl, hipStream_t s) {
+    typedef typename VecOf<VT>::type V;
+    const int T = A.threads;
+    const int64_t nlogical = (A.rows * T + 255) / 256;
+    const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
+#define PCGK(TT)                                                                                             \
+    case TT:                                                                                                 \
+        hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)d,  \
+                           (V *)t0, sc, sl, nlogical);                                                       \
+        break;
+    switch (T) {
+        PCGK(1) PCGK(2) PCGK(4) PCGK(8) PCGK(16) PCGK(32) PCGK(64)
+        default: return hipErrorInvalidValue;
+    }
+#undef PCGK
+    hipLaunchKernelGGL((k_pcg_step<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
+    const int g2 = grid_for(A.rows, 2048);
+    hipLaunchKernelGGL((k_pcg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)d, (V *)r, (const V *)t0,
+                       (const V *)dinv, A.rows, sc, sl);
+    hipLaunchKernelGGL((k_pcg_beta<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g2, sc);
+    hipLaunchKernelGGL((k_pcg_direction<V>), dim3(grid_for(A.rows)), dim3(256), 0, s, (V *)d, (const V *)r,
+                       (const V *)dinv, A.rows, (const CGScalars *)sc);
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t pcg_start_t(void *d, const void *r, const void *dinv, int64_t n, CGScalars *sc, double eps_rel,
+                       int max_iters, const ReduceSlots &sl, hipStream_t s) {
+    const int g = grid_for(n, 2048);
+    hipLaunchKernelGGL((k_pcg_start<V>), dim3(g), dim3(256), 0, s, (V *)d, (const V *)r, (const V *)dinv, n, sl);
+    hipLaunchKernelGGL((k_pcg_init<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, sc,
+                       eps_rel, max_iters);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
@@ -197,6 +346,22 @@ hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CG
         case VT_BF16: return cg_iter_t<bf16>(A, x, p, r, v, sc, sl, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t pcg_iteration(const EllView &A, void *x, void *d, void *r, void *t0, const void *dinv, CGScalars *sc,
+                         const ReduceSlots &sl, hipStream_t s) {
+    switch (A.vtype) {
+        case VT_F32: return pcg_iter_t<float>(A, x, d, r, t0, dinv, sc, sl, s);
+        case VT_F64: return pcg_iter_t<double>(A, x, d, r, t0, dinv, sc, sl, s);
+        case VT_BF16: return pcg_iter_t<bf16>(A, x, d, r, t0, dinv, sc, sl, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t pcg_start(ValueType vt, void *d, const void *r, const void *dinv, int64_t n, CGScalars *sc,
+                     double eps_rel, int max_iters, const ReduceSlots &sl, hipStream_t s) {
+    if (vt == VT_F64) return pcg_start_t<double>(d, r, dinv, n, sc, eps_rel, max_iters, sl, s);
+    return pcg_start_t<float>(d, r, dinv, n, sc, eps_rel, max_iters, sl, s);
 }
 
 hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s) {

@@ -124,6 +124,71 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     return HONEI_HIP_SUCCESS;
 }
 
+int honei_hip_pcg_jacobi(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                         size_t num_cols_per_row, size_t stride, size_t threads, const void *b, void *x,
+                         const void *dd_inverted, size_t max_iters, double eps_relative, size_t *used_iters,
+                         double *final_defect, void *stream) {
+    if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64 && vtype != HONEI_HIP_BF16)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
+    if (!Aj || !Ax || !Arl || !b || !x || !dd_inverted) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
+    if (!pow2_threads(threads) || rows * threads > stride)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad ELL layout");
+    if (max_iters > 0x7fffffff) max_iters = 0x7fffffff;
+    Workspace *w = workspace_for(S(stream));
+    if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    const ValueType vt = (ValueType)vtype;
+    const size_t es = vec_size(vt);
+    const size_t vbytes = ((rows * es + 255) / 256) * 256;
+    char *base = (char *)w->ensure_scratch(3 * vbytes + 256);
+    if (!base) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    void *d = base, *r = base + vbytes, *t0 = base + 2 * vbytes;
+    CGScalars *sc = (CGScalars *)(base + 3 * vbytes);
+    EllView A;
+    A.rows = (int64_t)rows;
+    A.cols = (int64_t)rows;
+    A.stride = (int64_t)stride;
+    A.ncpr = (int64_t)num_cols_per_row;
+    A.threads = (int)threads;
+    A.Aj = (const int32_t *)Aj;
+    A.Ax = Ax;
+    A.Arl = (const uint32_t *)Arl;
+    A.vtype = vt;
+    hipStream_t s = S(stream);
+    // r = b - A x ; ||r||^2 ; d = z = dinv .* r ; r.z
+    // (math/conjugate_gradients.hh:1125-1134)
+    EllArgs a;
+    a.mode = ELL_DEFECT;
+    a.out = r;
+    a.x = x;
+    a.rhs = b;
+    a.norm2 = &sc->current;
+    a.row_start = 0;
+    a.row_end = (int64_t)rows;
+    int st = check(ell_apply(A, a, w->slots, s), "pcg defect");
+    if (st) return st;
+    if ((st = check(pcg_start(vt, d, r, dd_inverted, (int64_t)rows, sc, eps_relative, (int)max_iters, w->slots, s),
+                    "pcg start")))
+        return st;
+    // enqueue iterations in chunks; one flag read per chunk
+    CGScalars h{};
+    size_t chunk = rows < 100000 ? 16 : 4;
+    size_t enqueued = 0;
+    while (true) {
+        for (size_t k = 0; k < chunk && enqueued < max_iters; ++k, ++enqueued)
+            if ((st = check(pcg_iteration(A, x, d, r, t0, dd_inverted, sc, w->slots, s), "pcg iteration")))
+                return st;
+        if ((st = check(hipMemcpyAsync(&h, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s), "pcg readback")))
+            return st;
+        if ((st = check(hipStreamSynchronize(s), "pcg sync"))) return st;
+        if (h.done || enqueued >= max_iters) break;
+        if (chunk < 64) chunk *= 2;
+    }
+    if (used_iters) *used_iters = (size_t)h.iters;
+    if (final_defect) *final_defect = h.current;
+    if (max_iters > 0 && !h.converged) return fail(HONEI_HIP_ERROR_NOT_CONVERGED, "PCG-Jacobi reached max_iters");
+    return HONEI_HIP_SUCCESS;
+}
+
 int honei_hip_cg_konv(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
                       size_t num_cols_per_row, size_t stride, size_t threads, const void *b, void *x,
                       double konv_rad, size_t max_iters, size_t *iters, void *stream) {

@@ -1,6 +1,8 @@
 // honei/math/cg.hh -- CGSolver<tags::GPU::HIP, methods::NONE>
-// (math/cg.hh:63-133) and ConjugateGradients<tags::GPU::HIP, NONE> with the
-// konv_rad stopping rule (math/conjugate_gradients.hh:587-623).
+// (math/cg.hh:63-133), ConjugateGradients<tags::GPU::HIP, NONE> with the
+// konv_rad stopping rule (math/conjugate_gradients.hh:587-623) and
+// ConjugateGradients<tags::GPU::HIP, JAC> (math/conjugate_gradients.hh:
+// 1110-1154).
 #pragma once
 
 #include "honei/backends/hip/device.hh"
@@ -34,6 +36,31 @@ template <> struct CGSolver<tags::GPU::HIP, methods::NONE> {
 };
 
 template <typename Tag_, typename Method_> struct ConjugateGradients;
+
+// Jacobi-preconditioned CG on ELL (math/conjugate_gradients.hh:1110-1154):
+// the whole solve on the device, the convergence flag read once per chunk of
+// iterations.  A run that hits max_iters returns normally with used_iters ==
+// max_iters, as the reference does (which only prints then).
+template <> struct ConjugateGradients<tags::GPU::HIP, methods::JAC> {
+    template <typename DT>
+    static void value(const SparseMatrixELL<DT> &A, const DenseVector<DT> &b, DenseVector<DT> &x,
+                      const DenseVector<DT> &dd_inverted, unsigned long max_iters, unsigned long &used_iters,
+                      DT eps_relative = DT(1e-8)) {
+        hip::check_size(b.size(), A.rows());
+        hip::check_size(x.size(), A.rows());
+        hip::check_size(dd_inverted.size(), A.rows());
+        const DeviceEllMirror &m = A.device();
+        size_t used = 0;
+        double fd = 0;
+        void *pb = hip::rd(b);
+        void *pd = hip::rd(dd_inverted);
+        const int st = honei_hip_pcg_jacobi(m.vtype, m.Aj, m.Ax, m.Arl, A.rows(), A.num_cols_per_row(), A.stride(),
+                                            A.threads(), pb, hip::rw(x), pd, max_iters, (double)eps_relative, &used,
+                                            &fd, nullptr);
+        if (max_iters > 0) used_iters = used;
+        if (st != HONEI_HIP_ERROR_NOT_CONVERGED) hip_check(st, "ConjugateGradients<tags::GPU::HIP, JAC>");
+    }
+};
 
 template <> struct ConjugateGradients<tags::GPU::HIP, methods::NONE> {
     template <typename DT>

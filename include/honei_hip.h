@@ -251,6 +251,20 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
                        size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
                        void *stream);
 
+/* ConjugateGradients<Tag, methods::JAC>::value(A, b, x, dd_inverted,
+ * max_iters, used_iters, eps_relative) on ELL
+ * (math/conjugate_gradients.hh:1110-1154, kernel :884-913): Jacobi-
+ * preconditioned CG, z = dd_inverted .* r, stopping once
+ * ||r|| < ||r_0|| * eps_relative.  dd_inverted is a device vector of the
+ * vector type (double for F64 matrices, float otherwise).  *used_iters is
+ * the iteration count (0 for max_iters = 0); *final_defect the last ||r||.
+ * Returns HONEI_HIP_ERROR_NOT_CONVERGED when max_iters is reached first (x
+ * and *used_iters are still written; the reference returns normally then). */
+int honei_hip_pcg_jacobi(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                         size_t num_cols_per_row, size_t stride, size_t threads, const void *b_dev, void *x_dev,
+                         const void *dd_inverted_dev, size_t max_iters, double eps_relative, size_t *used_iters,
+                         double *final_defect, void *stream);
+
 /* ConjugateGradients<Tag, methods::NONE>::value(A, b, x, konv_rad)
  * (math/conjugate_gradients.hh:154-201, 587-623), the multigrid coarse
  * solver: x = 0, iterate until ||x||^2 grows by no more than konv_rad.

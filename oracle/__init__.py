@@ -419,6 +419,25 @@ def cg_solver(A: Ell, b, x0=None, max_iters=10000, eps=1e-8, history=False):
     return x, used.value
 
 
+def pcg_jacobi(A: Ell, b, dinv, x0=None, max_iters=10000, eps=1e-8, history=False):
+    """ConjugateGradients<tags::CPU, JAC>::value(A, b, x, dd_inverted,
+    max_iters, used_iters, eps_relative) on ELL
+    (math/conjugate_gradients.hh:1110-1154).  used_iters is 0 when the
+    reference leaves it unset (max_iters = 0)."""
+    dt = A.dtype
+    b = np.ascontiguousarray(b, dt)
+    dinv = np.ascontiguousarray(dinv, dt)
+    x = np.zeros(A.rows, dt) if x0 is None else np.array(x0, dt, copy=True)
+    used = _u64(0)
+    hist = np.zeros(max(max_iters, 1), dt) if history else None
+    getattr(lib(), f"oracle_pcg_jacobi_{_sfx(dt)}")(
+        *_ell_args(A), _ptr(b), _ptr(x), _ptr(dinv), _u64(max_iters), _scalar(dt, eps), C.byref(used),
+        _ptr(hist) if history else None)
+    if history:
+        return x, used.value, hist[:used.value]
+    return x, used.value
+
+
 def cg_konv(A: Ell, b, konv=None):
     """ConjugateGradients<tags::CPU, NONE>::value(A, b, x, konv_rad)."""
     dt = A.dtype

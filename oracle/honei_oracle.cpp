@@ -509,6 +509,41 @@ void cg_solver(const Ell<DT> &A, const DT *b, DT *x, u64 max_iters, u64 *used_it
     }
 }
 
+// ConjugateGradients<Tag, JAC>::value on ELL (math/conjugate_gradients.hh:
+// 1110-1154) with its cg_kernel (math/conjugate_gradients.hh:884-913):
+// Jacobi-preconditioned CG, z = dinv .* r.  Stops once ||r|| < initial *
+// eps_relative; used_iters is written only when the loop converges or
+// reaches max_iters (it stays untouched for max_iters = 0).  history (if
+// given) receives ||r|| after every iteration.
+template <typename DT>
+void pcg_jacobi(const Ell<DT> &A, const DT *b, DT *x, const DT *dinv, u64 max_iters, u64 *used_iters,
+                DT eps_relative, DT *history) {
+    const u64 n = A.rows;
+    const DT eps = std::numeric_limits<DT>::epsilon();
+    std::vector<DT> r(n), z(n), d(n), t0(n);
+    defect_ell(r.data(), b, x, A);
+    const DT initial = std::sqrt(norm2(r.data(), n));
+    vprod(z.data(), dinv, r.data(), n);
+    vcopy(d.data(), z.data(), n);
+    for (u64 i = 0; i < max_iters; ++i) {
+        DT beta = dot(r.data(), z.data(), n);
+        product_ell(t0.data(), d.data(), A, 0, n);
+        DT alpha = dot(d.data(), t0.data(), n);
+        alpha = std::abs(alpha) >= eps ? beta / alpha : beta / eps;
+        scaled_sum2(x, d.data(), alpha, n);
+        scaled_sum2(r.data(), t0.data(), DT(-alpha), n);
+        vprod(z.data(), dinv, r.data(), n);
+        const DT zr = dot(z.data(), r.data(), n);
+        beta = std::abs(beta) >= eps ? zr / beta : zr / eps;
+        vcopy(t0.data(), d.data(), n);
+        scaled_sum3(d.data(), z.data(), t0.data(), beta, n);
+        const DT current = std::sqrt(norm2(r.data(), n));
+        if (history) history[i] = current;
+        if (current < initial * eps_relative) { *used_iters = i + 1; break; }
+        if (i == max_iters - 1) *used_iters = i + 1;
+    }
+}
+
 // math/conjugate_gradients.hh:154-201 (cg_kernel, ELL, gradient form) driven
 // by the konv_rad loop of math/conjugate_gradients.hh:587-623: x starts at 0
 // and iterations stop once ||x||^2 grows by no more than konv_rad.
@@ -852,6 +887,8 @@ u64 oracle_prolmat_csr(u64 fine_dim, u64 *rp, u64 *col, double *val) { return pr
 // ---- solvers -----------------------------------------------------------------------
 void oracle_cg_solver_f(ORACLE_ELL_ARGS(float), const float *b, float *x, u64 max_iters, float eps, u64 *used, float *hist) { ORACLE_MAKE_ELL(float); cg_solver(A, b, x, max_iters, used, eps, hist); }
 void oracle_cg_solver_d(ORACLE_ELL_ARGS(double), const double *b, double *x, u64 max_iters, double eps, u64 *used, double *hist) { ORACLE_MAKE_ELL(double); cg_solver(A, b, x, max_iters, used, eps, hist); }
+void oracle_pcg_jacobi_f(ORACLE_ELL_ARGS(float), const float *b, float *x, const float *dinv, u64 max_iters, float eps, u64 *used, float *hist) { ORACLE_MAKE_ELL(float); pcg_jacobi(A, b, x, dinv, max_iters, used, eps, hist); }
+void oracle_pcg_jacobi_d(ORACLE_ELL_ARGS(double), const double *b, double *x, const double *dinv, u64 max_iters, double eps, u64 *used, double *hist) { ORACLE_MAKE_ELL(double); pcg_jacobi(A, b, x, dinv, max_iters, used, eps, hist); }
 u64 oracle_cg_konv_f(ORACLE_ELL_ARGS(float), const float *b, float *x, double konv) { ORACLE_MAKE_ELL(float); return cg_konv(A, b, x, konv); }
 u64 oracle_cg_konv_d(ORACLE_ELL_ARGS(double), const double *b, double *x, double konv) { ORACLE_MAKE_ELL(double); return cg_konv(A, b, x, konv); }
 

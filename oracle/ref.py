@@ -35,6 +35,8 @@ def lib():
         _lib.ref_ell_from_csr_d.restype = _p
         _lib.ref_prolmat_d.restype = _p
         _lib.ref_cg_solver_d.restype = _u64
+        _lib.ref_pcg_jacobi_f.restype = _u64
+        _lib.ref_pcg_jacobi_d.restype = _u64
         _lib.ref_mg_mixed.restype = _u64
         atexit.register(_free_all)
     return _lib
@@ -125,6 +127,17 @@ def cg_solver(A: RefEll, b, x0, max_iters, eps):
     b = np.ascontiguousarray(b, np.float64)
     x = np.array(x0, np.float64, copy=True)
     used = lib().ref_cg_solver_d(_p(A.h), _ptr(b), _ptr(x), _u64(max_iters), C.c_double(eps))
+    return x, used
+
+
+def pcg_jacobi(A: RefEll, b, dinv, x0, max_iters, eps):
+    """ConjugateGradients<tags::CPU, JAC>::value on the reference's ELL."""
+    dt = A.dtype
+    b = np.ascontiguousarray(b, dt)
+    dinv = np.ascontiguousarray(dinv, dt)
+    x = np.array(x0, dt, copy=True)
+    sc = C.c_float if A.s == "f" else C.c_double
+    used = getattr(lib(), f"ref_pcg_jacobi_{A.s}")(_p(A.h), _ptr(b), _ptr(x), _ptr(dinv), _u64(max_iters), sc(eps))
     return x, used
 
 

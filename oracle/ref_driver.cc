@@ -211,6 +211,29 @@ u64 ref_cg_solver_d(void *h, const double *b, double *x, u64 max_iters, double e
     return used;
 }
 
+// ConjugateGradients<tags::CPU, JAC>::value(A, b, x, dd_inverted, max_iters,
+// used_iters, eps) (math/conjugate_gradients.hh:1110-1154); its prints are
+// swallowed.  Returns used_iters (0 when the reference leaves it unset).
+extern "C++" template <typename DT>
+u64 pcg_jacobi(void *h, const DT *b, DT *x, const DT *dinv, u64 max_iters, DT eps) {
+    SparseMatrixELL<DT> &m = *(SparseMatrixELL<DT> *)h;
+    DenseVector<DT> bv(m.rows()), xv(m.rows()), dv(m.rows());
+    for (u64 i = 0; i < m.rows(); ++i) { bv[i] = b[i]; xv[i] = x[i]; dv[i] = dinv[i]; }
+    unsigned long used_ul = 0;
+    {
+        CoutCapture cap;
+        ConjugateGradients<tags::CPU, methods::JAC>::value(m, bv, xv, dv, (unsigned long)max_iters, used_ul, eps);
+    }
+    copy_out(xv, x);
+    return used_ul;
+}
+u64 ref_pcg_jacobi_f(void *h, const float *b, float *x, const float *dinv, u64 max_iters, float eps) {
+    return pcg_jacobi(h, b, x, dinv, max_iters, eps);
+}
+u64 ref_pcg_jacobi_d(void *h, const double *b, double *x, const double *dinv, u64 max_iters, double eps) {
+    return pcg_jacobi(h, b, x, dinv, max_iters, eps);
+}
+
 // ConjugateGradients<tags::CPU, NONE>::value(A, b, x, konv_rad)
 void ref_cg_konv_f(void *h, const float *b, float *x, double konv) {
     SparseMatrixELL<float> &m = *(SparseMatrixELL<float> *)h;

@@ -333,6 +333,28 @@ void cg_solver() {
     TEST_CHECK(err < 1e-5);
 }
 
+// math/dune_regression_TEST.cc:66-178: PCG-Jacobi on the area51 system from
+// its start vector, checked against the FEATFLOW2 solution with that test's
+// tolerance (3e-5).
+void pcg_jacobi_area51() {
+    const std::string file = g_golden + "/l2/area51_";
+    SparseMatrixELL<double> A(MatrixIO<io_formats::ELL>::read_matrix(file + "full_0.ell", double(0)));
+    DenseVector<double> rhs(VectorIO<io_formats::EXP>::read_vector(file + "rhs_0", double(0)));
+    DenseVector<double> x(VectorIO<io_formats::EXP>::read_vector(file + "init_0", double(0)));
+    DenseVector<double> sol(VectorIO<io_formats::EXP>::read_vector(file + "sol_0", double(0)));
+    const unsigned long n = A.rows();
+    DenseVector<double> dinv(n, 0.0);
+    for (unsigned long r = 0; r < n; ++r)
+        for (unsigned long k = 0; k < A.Arl()[r]; ++k) {
+            const unsigned long slot = r + k * A.stride();
+            if (A.Aj()[slot] == r) dinv[r] = 1.0 / A.Ax()[slot];
+        }
+    unsigned long used = 0;
+    ConjugateGradients<tags::GPU::HIP, methods::JAC>::value(A, rhs, x, dinv, 10000ul, used, 1e-8);
+    TEST_CHECK(used > 0 && used < 10000);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK_EQUAL_WITHIN_EPS(x[i], sol[i], 3e-5);
+}
+
 // math/poisson_mg_mixed_ell_TEST.cc:42-230 set-up, Dirichlet on all edges
 void multigrid_mixed() {
     const unsigned long max_level = 6, side = (1ul << max_level) + 1, n = side * side;
@@ -543,6 +565,7 @@ int main(int argc, char **argv) {
         {"transfers<float>", transfers<float>},
         {"transfers<double>", transfers<double>},
         {"cg_solver<double>", cg_solver},
+        {"pcg_jacobi_area51<double>", pcg_jacobi_area51},
         {"multigrid_mixed", multigrid_mixed},
         {"mg_solver_quad_w_cycle", mg_solver_quad},
         {"batch_solver", batch_solver},

@@ -24,6 +24,8 @@ FILES = (
     + [f"poisson_advanced4/sort_0/A_{i}.ell" for i in range(1, 5)]
     + [f"poisson_advanced4/sort_0/prol_{i}.ell" for i in range(2, 5)]
     + ["poisson_advanced4/sort_0/rhs_4", "poisson_advanced4/sort_0/init_4", "poisson_advanced4/sort_0/sol_4"]
+    # PCG-Jacobi ELL system and FEATFLOW2 solution of math/dune_regression_TEST.cc:66-178
+    + ["l2/area51_full_0.ell", "l2/area51_rhs_0", "l2/area51_init_0", "l2/area51_sol_0"]
 )
 
 for f in FILES:

@@ -19,6 +19,7 @@ import pytest
 import honei_amd as H
 import oracle
 from tests.patterns import uniform_pm1
+from tests.spd import diag_inverse, lognormal_q1_spd
 
 pytestmark = pytest.mark.gpu
 
@@ -154,6 +155,99 @@ def test_cg_not_converged_reports():
     with pytest.raises(H.HipError) as e:
         H.cg_solve(D, db, dx, 3, 1e-12)
     assert e.value.code == 5
+
+
+def _bf16_round(A):
+    u = A.Ax.astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    return oracle.Ell(A.rows, A.cols, A.stride, A.ncpr, A.threads, A.Aj, r, A.Arl)
+
+
+def _pcg_case(side, vtype, threads, seed):
+    dt = np.float64 if vtype == H.F64 else np.float32
+    A = lognormal_q1_spd(side, seed, dt, threads)
+    if vtype == H.BF16:
+        A = _bf16_round(A)
+    xt = uniform_pm1(seed + 1, A.rows).astype(dt)
+    b = oracle.product(A, xt)
+    x0 = uniform_pm1(seed + 2, A.rows).astype(dt)
+    return A, b, x0, diag_inverse(A)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+@pytest.mark.parametrize("vtype", [H.F64, H.F32, H.BF16])
+@pytest.mark.parametrize("side", [33, 129])
+def test_pcg_jacobi_matches_oracle(side, vtype, threads):
+    """ConjugateGradients<JAC> on a lognormal-coefficient SPD Q1 system from a
+    random start: the iteration count of the oracle (reference control flow),
+    the FP64 true residual below the tolerance and the iterate within
+    rounding of the oracle's."""
+    A, b, x0, dinv = _pcg_case(side, vtype, threads, 20 + side)
+    eps = 1e-8 if vtype == H.F64 else 1e-5
+    xo, used_o = oracle.pcg_jacobi(A, b, dinv, x0, 5000, eps)
+    D = H.DeviceEll.from_oracle(A, vtype)
+    db, dx, dd = H.DeviceArray.from_host(b), H.DeviceArray.from_host(x0), H.DeviceArray.from_host(dinv)
+    print(f"pcg side={side} vtype={vtype} T={threads} oracle_iters={used_o}", flush=True)
+    used, fd = H.pcg_jacobi(D, db, dx, dd, 5000, eps)
+    x = dx.get()
+    assert abs(used - used_o) <= (1 if vtype == H.F64 else 3)
+    A64 = A.astype(np.float64)
+    r0 = np.linalg.norm(oracle.defect(A64, b.astype(np.float64), x0.astype(np.float64)))
+    r = np.linalg.norm(oracle.defect(A64, b.astype(np.float64), x.astype(np.float64)))
+    assert r < (eps * (1.5 if vtype == H.F64 else 3.0)) * r0
+    tol = 1e-7 if vtype == H.F64 else 1e-3
+    assert np.abs(x - xo).max() <= tol * np.abs(xo).max()
+
+
+def test_pcg_jacobi_area51_featflow():
+    """math/dune_regression_TEST.cc:66-178 on the device: the area51 system
+    from its start vector, FEATFLOW2's solution within 3e-5."""
+    d = os.path.join(os.path.dirname(__file__), "golden", "honei_testdata", "l2")
+    A = oracle.read_ell(os.path.join(d, "area51_full_0.ell"))
+    b = oracle.read_exp(os.path.join(d, "area51_rhs_0"))
+    x0 = oracle.read_exp(os.path.join(d, "area51_init_0"))
+    sol = oracle.read_exp(os.path.join(d, "area51_sol_0"))
+    dinv = diag_inverse(A)
+    _xo, used_o = oracle.pcg_jacobi(A, b, dinv, x0, 10000, 1e-8)
+    D = H.DeviceEll.from_oracle(A)
+    db, dx, dd = H.DeviceArray.from_host(b), H.DeviceArray.from_host(x0), H.DeviceArray.from_host(dinv)
+    used, _fd = H.pcg_jacobi(D, db, dx, dd, 10000, 1e-8)
+    assert abs(used - used_o) <= 1
+    assert np.abs(dx.get() - sol).max() < 3e-5
+
+
+def test_pcg_jacobi_limits():
+    """max_iters = 0: x untouched, 0 iterations; a capped run reports
+    HONEI_HIP_ERROR_NOT_CONVERGED (5) with used == max_iters, or returns
+    normally with check=False, as the reference does."""
+    A, b, x0, dinv = _pcg_case(65, H.F64, 1, 7)
+    D = H.DeviceEll.from_oracle(A)
+    db, dd = H.DeviceArray.from_host(b), H.DeviceArray.from_host(dinv)
+    dx = H.DeviceArray.from_host(x0)
+    used, _ = H.pcg_jacobi(D, db, dx, dd, 0, 1e-8)
+    assert used == 0 and np.array_equal(dx.get(), x0)
+    with pytest.raises(H.HipError) as e:
+        H.pcg_jacobi(D, db, dx, dd, 3, 1e-12)
+    assert e.value.code == 5
+    dx = H.DeviceArray.from_host(x0)
+    used, _ = H.pcg_jacobi(D, db, dx, dd, 3, 1e-12, check=False)
+    xo, used_o = oracle.pcg_jacobi(A, b, dinv, x0, 3, 1e-12)
+    assert used == used_o == 3
+    assert np.abs(dx.get() - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_pcg_jacobi_large_fp64_residual():
+    """A 1025^2 lognormal SPD system (about 1M unknowns) to 1e-8: the FP64 true
+    residual of the device result meets the tolerance (size-independent
+    check; the oracle's run at this size is not needed for it)."""
+    A, b, x0, dinv = _pcg_case(1025, H.F64, 1, 3)
+    D = H.DeviceEll.from_oracle(A)
+    db, dx, dd = H.DeviceArray.from_host(b), H.DeviceArray.from_host(x0), H.DeviceArray.from_host(dinv)
+    used, fd = H.pcg_jacobi(D, db, dx, dd, 20000, 1e-8)
+    r0 = np.linalg.norm(oracle.defect(A, b, x0))
+    r = np.linalg.norm(oracle.defect(A, b, dx.get()))
+    print(f"pcg 1025^2 iters={used} rel={r / r0:.3e}", flush=True)
+    assert 0 < used < 20000 and r < 1.5e-8 * r0
 
 
 @pytest.mark.parametrize("level", [2, 4, 6, 8])

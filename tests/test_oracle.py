@@ -15,6 +15,7 @@ import pytest
 import oracle
 from oracle import ref
 from tests.patterns import honei_pattern_csr, random_band_csr, uniform_pm1
+from tests.spd import diag_inverse, lognormal_q1_spd
 
 needs_ref = pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
 
@@ -286,6 +287,98 @@ def test_cg_solver_bitexact_vs_reference():
     xr, used_r = ref.cg_solver(R, b, x0, 3000, 1e-8)
     assert used == used_r
     assert np.array_equal(x, xr)
+
+
+def test_pcg_jacobi_area51_featflow(golden):
+    """math/dune_regression_TEST.cc:66-178: PCG-Jacobi to 1e-8 from the stored
+    start vector reproduces FEATFLOW2's solution within that test's eps
+    (3 * (m*8 + b) = 3e-5 for double)."""
+    d = os.path.join(golden, "l2")
+    A = oracle.read_ell(os.path.join(d, "area51_full_0.ell"))
+    b = oracle.read_exp(os.path.join(d, "area51_rhs_0"))
+    x0 = oracle.read_exp(os.path.join(d, "area51_init_0"))
+    sol = oracle.read_exp(os.path.join(d, "area51_sol_0"))
+    x, used = oracle.pcg_jacobi(A, b, diag_inverse(A), x0, 10000, 1e-8)
+    assert 0 < used < 10000
+    assert np.abs(x - sol).max() < 3e-5
+
+
+def test_pcg_jacobi_diagonal_system_one_iteration():
+    """With A diagonal and dinv = 1/diag(A), z_0 is the exact error: one
+    iteration solves the system (step = 1, r_1 = 0).  Without the
+    preconditioner CG needs one iteration per distinct diagonal value, so a
+    dropped or misplaced dinv fails here."""
+    n = 40
+    diag = np.repeat(np.array([1.0, 2.0, 5.0, 11.0]), n // 4)
+    rp = np.arange(n + 1, dtype=np.uint64)
+    A = oracle.ell_from_sm(n, n, rp, np.arange(n, dtype=np.uint64), diag)
+    b = uniform_pm1(3, n)
+    x, used = oracle.pcg_jacobi(A, b, 1.0 / diag, None, 100, 1e-12)
+    assert used == 1
+    np.testing.assert_allclose(x, b / diag, rtol=4e-16, atol=0)
+    _x, used_plain = oracle.pcg_jacobi(A, b, np.ones(n), None, 100, 1e-12)
+    assert used_plain == 4
+
+
+def test_pcg_jacobi_unit_preconditioner_is_plain_cg():
+    """dinv = 1 makes z = r exactly, and every PCG-Jacobi quantity then equals
+    CGSolver's (r.z = ||r||^2, the same step, the same direction update with
+    operands commuted): iterates and defect histories agree bit-for-bit with
+    the independently restated CGSolver."""
+    A = lognormal_q1_spd(33, 4, np.float64)
+    b = uniform_pm1(5, A.rows)
+    xp, used_p, hp = oracle.pcg_jacobi(A, b, np.ones(A.rows), None, 3000, 1e-8, history=True)
+    xc, used_c, hc = oracle.cg_solver(A, b, None, 3000, 1e-8, history=True)
+    assert used_p == used_c > 10
+    assert np.array_equal(xp, xc) and np.array_equal(hp, hc)
+
+
+def test_pcg_jacobi_limits():
+    """max_iters = 0 leaves x and used_iters alone; a capped run reports
+    max_iters and is a prefix of the full run; the preconditioned solve of a
+    lognormal-coefficient system needs fewer iterations than plain CG."""
+    A = lognormal_q1_spd(33, 6, np.float64, sigma=1.5)
+    b = uniform_pm1(7, A.rows)
+    dinv = diag_inverse(A)
+    x0 = uniform_pm1(8, A.rows)
+    x, used = oracle.pcg_jacobi(A, b, dinv, x0, 0, 1e-8)
+    assert used == 0 and np.array_equal(x, x0)
+    xf, used_f, hf = oracle.pcg_jacobi(A, b, dinv, x0, 5000, 1e-8, history=True)
+    x3, used_3, h3 = oracle.pcg_jacobi(A, b, dinv, x0, 3, 1e-8, history=True)
+    assert used_3 == 3 and np.array_equal(h3, hf[:3])
+    assert np.linalg.norm(oracle.defect(A, b, xf)) < 1e-8 * np.linalg.norm(oracle.defect(A, b, x0)) * 1.01
+    _xc, used_c = oracle.cg_solver(A, b, x0, 5000, 1e-8)
+    assert used_f < used_c
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype,threads", [(np.float32, 1), (np.float64, 1), (np.float64, 4)])
+def test_pcg_jacobi_bitexact_vs_reference(dtype, threads):
+    A = lognormal_q1_spd(33, 9, dtype, threads)
+    rp, col, val = A.to_csr()
+    R = ref.ell_from_sm(A.rows, A.cols, rp, col, val, threads)
+    b = uniform_pm1(10, A.rows).astype(dtype)
+    x0 = uniform_pm1(11, A.rows).astype(dtype)
+    dinv = diag_inverse(A)
+    x, used = oracle.pcg_jacobi(A, b, dinv, x0, 2000, 1e-6 if dtype == np.float32 else 1e-8)
+    xr, used_r = ref.pcg_jacobi(R, b, dinv, x0, 2000, 1e-6 if dtype == np.float32 else 1e-8)
+    assert used == used_r > 0
+    assert np.array_equal(x, xr)
+
+
+@needs_ref
+def test_pcg_jacobi_area51_bitexact_vs_reference(golden):
+    d = os.path.join(golden, "l2")
+    A = oracle.read_ell(os.path.join(d, "area51_full_0.ell"))
+    rp, col, val = A.to_csr()
+    R = ref.ell_from_sm(A.rows, A.cols, rp, col, val, 1)
+    b = oracle.read_exp(os.path.join(d, "area51_rhs_0"))
+    x0 = oracle.read_exp(os.path.join(d, "area51_init_0"))
+    dinv = diag_inverse(A)
+    for its in (0, 2, 10000):
+        x, used = oracle.pcg_jacobi(A, b, dinv, x0, its, 1e-8)
+        xr, used_r = ref.pcg_jacobi(R, b, dinv, x0, its, 1e-8)
+        assert used == used_r and np.array_equal(x, xr)
 
 
 @needs_ref
