@@ -55,9 +55,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def device_q1(side, vt):
+def device_q1(side, vt, with_dinv=False):
     """Q1 operator assembled on the device (bit-identical to the reference's
-    FillMatrix -> SparseMatrix -> SparseMatrixELL chain)."""
+    FillMatrix -> SparseMatrix -> SparseMatrixELL chain); with_dinv also
+    fills D.dinv = 1 / diag(A)."""
     n = side * side
     stride = 32 * ((n + 31) // 32)
 
@@ -69,7 +70,9 @@ def device_q1(side, vt):
     D.Aj = H.DeviceArray(9 * stride, np.int32)
     D.Ax = H.DeviceArray(9 * stride, np.float64 if vt == H.F64 else np.float32)
     D.Arl = H.DeviceArray(n, np.uint32)
-    H.call("honei_hip_q1_assemble", vt, side, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, stride, None, None, 0.0, 1, None)
+    D.dinv = H.DeviceArray(n, np.float64 if vt == H.F64 else np.float32) if with_dinv else None
+    H.call("honei_hip_q1_assemble", vt, side, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, stride, None,
+           D.dinv.ptr if with_dinv else None, 1.0, 1, None)
     D.nnz = 9 * (side - 2) ** 2 + 4 * (side - 1)
     return D
 
@@ -206,38 +209,85 @@ def time_spmv_staged(D, reps, x, y, label, block=1024, chunks=1):
 
 
 def config1_cg(with_cpu):
-    """CGSolver<..., NONE> in FP64 on the Q1 129^2 operator (16,641 rows,
-    ELL width 9), b = A x_true (x_true ~ U[-1,1), seed 1), x0 = 0, to
-    relative residual 1e-8: the GPU solve (device ELL from the host
-    conversion) beside the oracle's C restatement on one host core."""
-    import oracle
+    """CGSolver<..., NONE> and ConjugateGradients<..., JAC> (PCG-Jacobi) in
+    FP64 on the Q1 129^2 operator (16,641 rows, ELL width 9), b = A x_true
+    (x_true ~ U[-1,1), seed 1), x0 = 0, to relative residual 1e-8: operator,
+    dinv and b built on the device; with_cpu times the oracle's C
+    restatement of both solvers on one host core beside them."""
     side = 129
-    A = oracle.q1_matrix(side, np.float64)
-    xt = uniform_pm1(1, A.rows)
-    b = oracle.product(A, xt)
-    D = H.DeviceEll.from_oracle(A)
-    db = H.DeviceArray.from_host(b)
-    dx = H.DeviceArray(A.rows, np.float64)
-    ts, its = [], 0
-    for k in range(6):
+    D = device_q1(side, H.F64, with_dinv=True)
+    n = D.rows
+    xt = uniform_pm1(1, n)
+    dxt = H.DeviceArray.from_host(xt)
+    db = H.DeviceArray(n, np.float64)
+    H.call("honei_hip_product_smell_dv_double", dxt.ptr, db.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, n, D.ncpr,
+           D.stride, 0, 1, None)
+    dx = H.DeviceArray(n, np.float64)
+    out = {}
+    for name, solve in (("cg", lambda: H.cg_solve(D, db, dx, max_iters=10000, eps=1e-8)),
+                        ("pcg_jacobi", lambda: H.pcg_jacobi(D, db, dx, D.dinv, max_iters=10000, eps=1e-8))):
+        ts, its = [], 0
+        for k in range(6):
+            dx.zero()
+            H.synchronize()
+            t0 = time.perf_counter()
+            its, _fd = solve()
+            if k:
+                ts.append(time.perf_counter() - t0)
+        err = float(np.abs(dx.get() - xt).max())
+        out[name] = {"gpu_ms": round(1e3 * float(np.median(ts)), 3), "iterations": int(its),
+                     "max_err_vs_x_true": err}
+    out["what"] = ("Q1 129^2 FP64 to 1e-8 (config 1); GPU: honei_hip_cg_solve (CGSolver) and "
+                   "honei_hip_pcg_jacobi (ConjugateGradients<JAC>, dinv = 1/diag)")
+    if with_cpu:
+        import oracle
+        A = oracle.q1_matrix(side, np.float64)
+        b = oracle.product(A, xt)
+        dinv = 1.0 / oracle.q1_bands(side, np.float64)[4]
+        for name, solve in (("cg", lambda: oracle.cg_solver(A, b, eps=1e-8)),
+                            ("pcg_jacobi", lambda: oracle.pcg_jacobi(A, b, dinv, eps=1e-8))):
+            cs = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                _x, cits = solve()
+                cs.append(time.perf_counter() - t0)
+            out[name]["cpu_port_ms"] = round(1e3 * float(np.median(cs)), 3)
+            out[name]["cpu_iterations"] = int(cits)
+    for name in ("cg", "pcg_jacobi"):
+        log(f"[{name} 129^2] GPU {out[name]['gpu_ms']} ms ({out[name]['iterations']} iterations), "
+            f"CPU port {out[name].get('cpu_port_ms')} ms")
+    return out
+
+
+def pcg_iteration_rate(side=2049, iters=200):
+    """PCG-Jacobi iterations in FP64 on the Q1 side^2 operator (device
+    assembly, b = A 1): `iters` iterations with an unreachable tolerance,
+    time per iteration and its algorithmic bytes 12 nnz + 16 n (t0 = A d, d
+    gathered once) + 56 n (x, r update with z.r, r.r) + 32 n (direction)."""
+    D = device_q1(side, H.F64, with_dinv=True)
+    n = D.rows
+    ones = H.DeviceArray.from_host(np.ones(n))
+    db = H.DeviceArray(n, np.float64)
+    H.call("honei_hip_product_smell_dv_double", ones.ptr, db.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, n, D.ncpr,
+           D.stride, 0, 1, None)
+    dx = H.DeviceArray(n, np.float64)
+    ts = []
+    for k in range(4):
         dx.zero()
         H.synchronize()
         t0 = time.perf_counter()
-        its, fd = H.cg_solve(D, db, dx, max_iters=10000, eps=1e-8)
+        used, _ = H.pcg_jacobi(D, db, dx, D.dinv, max_iters=iters, eps=1e-30, check=False)
         if k:
             ts.append(time.perf_counter() - t0)
-    err = float(np.abs(dx.get() - xt).max())
-    out = {"gpu_ms": round(1e3 * float(np.median(ts)), 3), "iterations": int(its), "max_err_vs_x_true": err,
-           "what": "Q1 129^2 FP64 CG to 1e-8 (config 1); GPU: honei_hip_cg_solve"}
-    if with_cpu:
-        cs = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            _x, cits = oracle.cg_solver(A, b, eps=1e-8)
-            cs.append(time.perf_counter() - t0)
-        out["cpu_port_ms"] = round(1e3 * float(np.median(cs)), 3)
-        out["cpu_iterations"] = int(cits)
-    log(f"[cg 129^2] GPU {out['gpu_ms']} ms ({its} iterations), CPU port {out.get('cpu_port_ms')} ms")
+    per_it = float(np.median(ts)) / used
+    nbytes = 12 * D.nnz + 104 * n
+    out = {"side": side, "iterations": int(used), "us_per_iteration": round(1e6 * per_it, 2),
+           "algorithmic_bytes_per_iteration": int(nbytes),
+           "achieved_GBps": round(nbytes / per_it / 1e9, 1),
+           "frac_of_8TBps": round(nbytes / per_it / 8e12, 3),
+           "what": "PCG-Jacobi FP64 iteration (5 kernels: A d + d.t0, step, update + 2 dots, factor, "
+                   "direction); whole solve call incl. defect, start and flag reads, divided by iterations"}
+    log(f"[pcg {side}^2] {out['us_per_iteration']} us/iteration, {out['achieved_GBps']} GB/s")
     return out
 
 
@@ -588,6 +638,7 @@ def main():
         del R, xr, yr
     spmv["hbm_ceiling"] = hbm_ceiling()
     cg129 = config1_cg(not args.no_cpu)
+    pcg2049 = pcg_iteration_rate()
     if not args.no_cpu:
         spmv["cpu_q1_2049_f32"] = cpu_spmv_q1(SIDE)
 
@@ -786,6 +837,7 @@ def main():
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
         "config1_cg_129": cg129,
+        "pcg_jacobi_2049_fp64": pcg2049,
         "config3_2049": c3,
         "single_2049": {"solves_per_s": round(1.0 / t1m, 3), "outer_iterations": outer,
                         "rel_residual": rd, "max_err_vs_x_true": err,

@@ -63,6 +63,25 @@ template <> struct ConjugateGradients<tags::GPU::HIP, methods::JAC> {
 };
 
 template <> struct ConjugateGradients<tags::GPU::HIP, methods::NONE> {
+    // value(A, b, x, max_iters) (math/conjugate_gradients.hh:570-585, kernel
+    // :204-249): CG from x with the fixed relative tolerance 1e-8 (and the
+    // absolute stop ||r|| < 1e-8); the same iteration as CGSolver.
+    template <typename DT>
+    static void value(const SparseMatrixELL<DT> &A, const DenseVector<DT> &b, DenseVector<DT> &x,
+                      unsigned long max_iters) {
+        hip::check_size(b.size(), A.rows());
+        hip::check_size(x.size(), A.rows());
+        const DeviceEllMirror &m = A.device();
+        size_t used = 0;
+        double fd = 0;
+        void *pb = hip::rd(b);
+        const int st = honei_hip_cg_solve(m.vtype, m.Aj, m.Ax, m.Arl, A.rows(), A.num_cols_per_row(), A.stride(),
+                                          A.threads(), pb, hip::rw(x), max_iters, 1e-8, &used, &fd, nullptr);
+        if (st != HONEI_HIP_ERROR_NOT_CONVERGED) hip_check(st, "ConjugateGradients<tags::GPU::HIP, NONE>");
+    }
+
+    // value(A, b, x, konv_rad) (math/conjugate_gradients.hh:587-623), the
+    // multigrid coarse solver: x = 0, iterate until ||x|| stagnates.
     template <typename DT>
     static void value(const SparseMatrixELL<DT> &A, const DenseVector<DT> &b, DenseVector<DT> &x, double konv_rad) {
         const DeviceEllMirror &m = A.device();

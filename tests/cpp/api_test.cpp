@@ -331,6 +331,11 @@ void cg_solver() {
     double err = 0;
     for (unsigned long i = 0; i < n; ++i) err = std::max(err, std::fabs(x[i] - xt[i]));
     TEST_CHECK(err < 1e-5);
+    // ConjugateGradients<NONE>::value(A, b, x, max_iters): the same iteration
+    // with the fixed 1e-8 tolerance, so the same result
+    DenseVector<double> x2(n, 0.0);
+    ConjugateGradients<tags::GPU::HIP, methods::NONE>::value(A, b, x2, 10000ul);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK(x2[i] == x[i]);
 }
 
 // math/dune_regression_TEST.cc:66-178: PCG-Jacobi on the area51 system from
