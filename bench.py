@@ -555,7 +555,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the single-system FP64 comparison")
     ap.add_argument("--batch-only", action="store_true", help="only the batched step (for profiling)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="with --batch-only: enqueue the step's kernels directly instead of graph launches "
+                         "(same kernels and results; for kernel-trace profiling)")
+    ap.add_argument("--no-pair", action="store_true",
+                    help="with --batch-only: one launch per smoothing pass (pair_sweeps = 0), for comparison")
     ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section")
+    ap.add_argument("--pcg-only", action="store_true",
+                    help="only the PCG-Jacobi iteration rate on Q1 2049^2 FP64 (for profiling)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream (for profiling)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 5 launches of the roofline SpMV and 5 of a 2^26-float scale kernel "
@@ -568,7 +575,7 @@ def main():
 
     # the all-cores CPU leg forks worker processes: run it before the GPU is touched
     cpu_all = None
-    if not (args.no_cpu or args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe):
+    if not (args.no_cpu or args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe or args.pcg_only):
         os.environ["OMP_NUM_THREADS"] = "1"
         cpu_all = cpu_baseline_all_cores([int(v) for v in batch_levels(NSYS, BATCH_SEED)],
                                          max(1, min(16, os.cpu_count() or 1)))
@@ -589,9 +596,13 @@ def main():
         print(json.dumps({"pmc_probe": "done"}))
         return
 
+    if args.pcg_only:
+        print(json.dumps({"pcg_only": pcg_iteration_rate()}))
+        return
+
     if args.batch_only:
         levels, kap, rhs = batched_workload()
-        B, slots = batched_pool(levels)
+        B, slots = batched_pool(levels, use_graphs=0 if args.no_graphs else 1, pair_sweeps=0 if args.no_pair else 1)
         for _ in range(args.warmup):
             batched_step(B, slots, kap, rhs)
         t0 = time.perf_counter()

@@ -228,6 +228,8 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     if (cfg.iters_per_launch < 1 || cfg.iters_per_launch > 64 || cfg.max_outer < 1) return hipErrorInvalidValue;
     if (cfg.averaging < 0 || cfg.averaging > 2) return hipErrorInvalidValue;
     if (cfg.outer != 0 && cfg.outer != 1) return hipErrorInvalidValue;
+    if (cfg.use_graphs != 0 && cfg.use_graphs != 1) return hipErrorInvalidValue;
+    if (cfg.pair_sweeps != 0 && cfg.pair_sweeps != 1) return hipErrorInvalidValue;
     std::vector<int> order((size_t)nclass);
     for (int c = 0; c < nclass; ++c) {
         if (levels[c] < 2 || levels[c] > 13 || counts[c] < 1 || counts[c] > 65535) return hipErrorInvalidValue;
@@ -274,6 +276,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
         lv.block = 256;
         lv.group = 1;
         if (cfg_.autotune && l > 1) HX(tune_level(lv));
+        if (paired(lv)) lv.T = 1;   // the paired launches read the T = 1 layout
         const Level0 g = level_geometry(l, lv.T, lv.block);
         lv.stride = g.stride;
         lv.ncpr = g.ncpr;
@@ -347,6 +350,12 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         BLevel &lv = L_[(size_t)l];
         const BMat m = mat(lv);
         const BWork w = level_work(l);
+        if (paired(lv) && cfg_.n_pre == 2) {
+            // x = Jacobi0, d = rhs - A x: one launch
+            HX(b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s));
+            HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
+            continue;
+        }
         // x = n_pre Jacobi sweeps from zero (first two fused), ending in x
         const int rest = cfg_.n_pre - 2;
         float *src = (rest % 2 == 0) ? lv.x : lv.t;
@@ -397,7 +406,13 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         BLevel &lv = L_[(size_t)l];
         const BMat m = mat(lv);
         const BWork w = level_work(l);
-        HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, s));
+        if (paired(lv) && cfg_.n_post == 2) {
+            // t = x + P e, then x = J(J(t)): one launch for both sweeps
+            HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.t, s));
+            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s));
+            continue;
+        }
+        HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.x, s));
         float *src = lv.x, *dst = lv.t;
         for (int k = 0; k < cfg_.n_post; ++k) {
             HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
@@ -493,6 +508,7 @@ hipError_t BatchMG::capture_iterations(int iters, hipGraph_t *graph, hipGraphExe
 // so finished work does not sit behind up to iters_per_launch - 1 empty
 // iterations.
 hipError_t BatchMG::capture() {
+    if (!cfg_.use_graphs) return hipSuccess;
     HX(capture_iterations(cfg_.iters_per_launch, &graph_, &exec_));
     if (cfg_.iters_per_launch > 1) HX(capture_iterations(1, &graph1_, &exec1_));
     return hipSuccess;
@@ -628,8 +644,12 @@ hipError_t BatchMG::activate_ready(bool wait) {
 hipError_t BatchMG::step() {
     HX(activate_ready(false));
     // a tail: few slots still running and none waiting to join
-    const bool tail = exec1_ && loading_.empty() && running_ <= std::max<int64_t>(1, nslots_ / 32);
-    HX(hipGraphLaunch(tail ? exec1_ : exec_, s_main_));
+    const bool tail = cfg_.iters_per_launch > 1 && loading_.empty() && running_ <= std::max<int64_t>(1, nslots_ / 32);
+    if (cfg_.use_graphs) {
+        HX(hipGraphLaunch(tail ? exec1_ : exec_, s_main_));
+    } else {
+        for (int i = 0; i < (tail ? 1 : cfg_.iters_per_launch); ++i) HX(enqueue_iteration(s_main_));
+    }
     ++launches_;
     return hipSuccess;
 }

@@ -59,6 +59,15 @@ struct SlotScalars {
 
 constexpr int kMaxClasses = 16;
 
+// Paired row passes (b_pair): tiles of at most kPairWidth columns (62 + a
+// halo column each side fill a wavefront) and kPairRows rows (kPairTileRows
+// with the halo, one wavefront per row: 1024-thread work-groups), on levels
+// from kPairMinSide^2 nodes up.
+enum PairKind { PAIR_PRE = 0, PAIR_POST = 1 };
+constexpr int64_t kPairWidth = 62;
+constexpr int kPairTileRows = 16, kPairRows = kPairTileRows - 2;
+constexpr int64_t kPairMinSide = 65;
+
 struct BatchConfig {
     int32_t inner_vtype = VT_F32;    // VT_F32 or VT_BF16 matrix values (vectors FP32)
     int32_t n_pre = 2, n_post = 2;   // Jacobi sweeps (n_pre >= 2)
@@ -75,6 +84,13 @@ struct BatchConfig {
     // Levels 1..fused_levels of every V-cycle in one launch (one work-group
     // per slot); 0 or 1: level by level.
     int32_t fused_levels = 4;
+    // 1: each step is one graph launch; 0: the same kernels enqueued on the
+    // stream at every step (for tools that cannot follow graph launches).
+    int32_t use_graphs = 1;
+    // 1: on levels of side >= kPairMinSide (T = 1 there), the Jacobi0 +
+    // defect and the two post-smoothing sweeps each run as one launch that
+    // reads the matrix once (n_pre == 2 / n_post == 2); 0: one launch per pass.
+    int32_t pair_sweeps = 1;
 };
 
 // Inner level l of the pool: slots [0, cap) present.
@@ -274,6 +290,10 @@ class BatchMG {
     int64_t build_kappa_cap_ = 0;
     hipEvent_t build_ev_ = nullptr;
     hipError_t tune_level(BLevel &lv);
+    // level runs the paired launches (b_pair) of its smoothing passes
+    bool paired(const BLevel &lv) const {
+        return cfg_.pair_sweeps && lv.side >= kPairMinSide && (cfg_.n_pre == 2 || cfg_.n_post == 2);
+    }
     hipError_t capture();
     hipError_t capture_iterations(int iters, hipGraph_t *graph, hipGraphExec_t *exec);
     hipError_t enqueue_iteration(hipStream_t s);
@@ -307,7 +327,14 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
 hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
                     double tol, int32_t max_outer, hipStream_t s);
 hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
-hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
+// out = fine.x + P coarse.x (out: fine.x or fine.t)
+hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, float *out, hipStream_t s);
+// Two dependent row passes of a T = 1 Q1 level in one launch (k_b_pair):
+// PAIR_PRE writes x = Jacobi0 to out0 and d = rhs - A x to out1; PAIR_POST
+// writes two Jacobi sweeps of xin to out0 (xin != out0).  Bit-identical to
+// the separate b_ell launches.
+hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
+                  const float *xin, hipStream_t s);
 hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);
 hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s);
 // Slot ids of one state-setting launch (kernel argument).

@@ -42,6 +42,8 @@ BatchConfig to_config(const honei_hip_batch_config &c) {
     b.autotune = c.autotune;
     b.outer = c.outer_method;
     b.fused_levels = c.fused_levels;
+    b.use_graphs = c.use_graphs;
+    b.pair_sweeps = c.pair_sweeps;
     return b;
 }
 
@@ -73,6 +75,8 @@ int honei_hip_batch_config_default(honei_hip_batch_config *cfg) {
     cfg->autotune = b.autotune;
     cfg->outer_method = b.outer;
     cfg->fused_levels = b.fused_levels;
+    cfg->use_graphs = b.use_graphs;
+    cfg->pair_sweeps = b.pair_sweeps;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -99,6 +103,10 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "outer_method 0 or 1");
     if (cfg->fused_levels < 0 || cfg->fused_levels > kMaxFusedLevels)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "fused_levels 0..6");
+    if (cfg->use_graphs != 0 && cfg->use_graphs != 1)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "use_graphs 0 or 1");
+    if (cfg->pair_sweeps != 0 && cfg->pair_sweeps != 1)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "pair_sweeps 0 or 1");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");

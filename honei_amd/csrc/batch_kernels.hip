@@ -151,6 +151,107 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
     }
 }
 
+// ---- two dependent row passes of a Q1 level in one launch --------------------
+// The levels of the batch are lexicographic side x side Q1 grids: an
+// interior row holds its 9-point stencil in column order (neighbour k at
+// grid offset (k/3 - 1, k%3 - 1)), a boundary row its diagonal alone
+// (q1_row), T = 1 layout.  k_b_pair runs two passes that depend on each
+// other in one launch.  A work-group of kPairTileRows wavefronts owns a tile
+// of one slot's grid: kPairRows rows by `width` (<= 62) columns.  Wavefront
+// i forms the first pass for grid row r0 - 1 + i on the tile's columns plus
+// one halo column each side (a lane per column) into LDS; after one
+// barrier, wavefronts 1 .. kPairRows form the second pass for their row from
+// LDS, with the matrix row still in registers.  So the matrix is read from
+// HBM once for both passes: halo rows and columns are read again by the
+// neighbouring tiles, through L2 (cached loads, not streaming ones).  Every
+// node's sum is the T = 1 sum of k_b_ell (same terms, same order, same
+// epilogue), so the result is bit-identical to two separate launches.
+//   PAIR_PRE : x = Jacobi0 (two sweeps from zero) -> out0, then the defect
+//              d = rhs - A x -> out1;
+//   PAIR_POST: out0 = J(J(xin)) (xin != out0).
+struct PairGeom {
+    int32_t side, width, strips, segs;
+};
+
+template <typename VT> __device__ inline float load_value_cached(const VT *p) { return *p; }
+template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { return bf16_to_float(p->bits); }
+
+template <typename VT, int PAIR>
+__global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork w, PairGeom pg, float *out0,
+                                                                  float *out1, const float *xin, const float *rhs,
+                                                                  const float *dinv) {
+    __shared__ float tile[kPairTileRows][kWave];
+    const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;   // tile row, lane
+    const int64_t a0 = active_first(w), na = active_count(w);
+    const int64_t ups = (int64_t)pg.strips * pg.segs;
+    const int64_t total = na * ups;
+    const int S = pg.side;
+    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+        int64_t e, rem, strip, seg;
+        div32(lb, ups, e, rem);
+        div32(rem, pg.segs, strip, seg);
+        const int64_t ls = w.list[a0 + e] - w.base;
+        const int64_t vo = ls * m.vld;
+        const VT *ax = (const VT *)m.Ax + ls * m.ax_ld;
+        const float *xs = xin + vo, *rs = rhs + vo, *ds = dinv + vo;
+        const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
+        const int c0 = (int)strip * pg.width, c1 = min(S, c0 + pg.width);
+        const int col = c0 - 1 + lane;
+        const bool in1 = col >= 0 && col < S && col <= c1;        // first-pass lanes
+        const bool in2 = lane >= 1 && col < c1;                    // second-pass lanes
+        const int colc = in1 ? col : (col < 0 ? 0 : S - 1);       // clamped, for addresses
+        const int r = r0 - 1 + i;
+        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);       // clamped row: computed, unused
+        const int64_t node = (int64_t)rc * S + colc;
+        const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
+        // ---- first pass, grid row r ----
+        float av[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
+        const float rh = rs[node], di = ds[node];
+        {
+            float xv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int64_t j = bnd ? node : node + (int64_t)(k / 3 - 1) * S + (k % 3 - 1);
+                if constexpr (PAIR == PAIR_PRE)
+                    xv[k] = ds[j] * rs[j];
+                else
+                    xv[k] = xs[j];
+            }
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
+            const float x0 = (PAIR == PAIR_PRE) ? di * rh : xs[node];
+            const float t = rh - s;
+            const float v1 = x0 + t * di;
+            tile[i][lane] = v1;
+            if (PAIR == PAIR_PRE && in2 && i >= 1 && r < r1) out0[vo + node] = v1;
+        }
+        __syncthreads();
+        // ---- second pass, grid row r for tile rows 1 .. kPairRows ----
+        if (i >= 1 && i <= kPairRows && r < r1 && in2) {
+            float xv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int ik = bnd ? i : i + (k / 3 - 1);
+                const int lk = bnd ? lane : lane + (k % 3 - 1);
+                xv[k] = tile[ik][lk];
+            }
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
+            if constexpr (PAIR == PAIR_PRE) {
+                out1[vo + node] = rh - s;
+            } else {
+                const float t = rh - s;
+                out0[vo + node] = tile[i][lane] + t * di;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- FP64 outer work, all classes in one launch ------------------------------
 // Both outer kernels order their logical blocks class by class (per-class
 // prefix in LDS, indexed by the class) and walk each slot's rows in order.
@@ -431,10 +532,12 @@ __global__ __launch_bounds__(256) void k_b_restrict(BWork w, float *rhs_c, int64
 }
 
 // x_f[g] += P x_c[g]
+// x_f + P x_c -> out_f (out_f may be x_f)
 __global__ __launch_bounds__(256) void k_b_prolong_axpy(BWork w, float *x_f, int64_t vld_f, const float *x_c, int64_t vld_c, int64_t m,
-                                 Mask8 mk) {
+                                 Mask8 mk, float *out_f) {
     const int64_t nf = (2 * m + 1) * (2 * m + 1);
-    FOR_SLOT_ENTRIES(w, nf, (prolong_node<float, true>(nullptr, x_c + g * vld_c, m, mk, x_f + g * vld_f, 1.0f, i)));
+    FOR_SLOT_ENTRIES(w, nf, (prolong_node<float, true>(nullptr, x_c + g * vld_c, m, mk, x_f + g * vld_f, 1.0f, i,
+                                                       out_f + g * vld_f)));
 }
 
 __global__ __launch_bounds__(256) void k_b_copy(BWork w, float *dst, const float *src, int64_t vld, int64_t n) {
@@ -760,11 +863,46 @@ hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, 
     return hipGetLastError();
 }
 
-hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
+hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, float *out, hipStream_t s) {
     const int64_t m = coarse.side - 1;
     if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || !entry_fits(fine.cap, fine.n)) return hipErrorInvalidValue;
+    if (out != fine.x && out != fine.t) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_b_prolong_axpy, dim3(entry_grid(fine.cap, fine.n)), dim3(256), 0, s, w, fine.x, fine.vld,
-                       (const float *)coarse.x, coarse.vld, m, pack_mask(nullptr));
+                       (const float *)coarse.x, coarse.vld, m, pack_mask(nullptr), out);
+    return hipGetLastError();
+}
+
+hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
+                  const float *xin, hipStream_t s) {
+    if (lv.T != 1 || lv.side < 3 || lv.ncpr != 9 || lv.side > 65535) return hipErrorInvalidValue;
+    if (pair != PAIR_PRE && pair != PAIR_POST) return hipErrorInvalidValue;
+    if (!out0 || (pair == PAIR_PRE && !out1) || (pair == PAIR_POST && (!xin || xin == out0)))
+        return hipErrorInvalidValue;
+    PairGeom pg;
+    pg.side = (int32_t)lv.side;
+    pg.strips = (int32_t)((lv.side + kPairWidth - 1) / kPairWidth);
+    pg.width = (int32_t)((lv.side + pg.strips - 1) / pg.strips);
+    pg.segs = (int32_t)((lv.side + kPairRows - 1) / kPairRows);
+    if (lv.cap <= 0) return hipSuccess;
+    const int64_t units = lv.cap * (int64_t)pg.strips * pg.segs;
+    if (units > kMaxLogicalBlocks) return hipErrorInvalidValue;
+    const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0};
+    const int threads = kPairTileRows * kWave;
+    const dim3 grid(batch_grid(units, threads)), block(threads);
+    const float *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
+    if (vt == VT_F32) {
+        if (pair == PAIR_PRE)
+            hipLaunchKernelGGL((k_b_pair<float, PAIR_PRE>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
+        else
+            hipLaunchKernelGGL((k_b_pair<float, PAIR_POST>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
+    } else if (vt == VT_BF16) {
+        if (pair == PAIR_PRE)
+            hipLaunchKernelGGL((k_b_pair<bf16, PAIR_PRE>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
+        else
+            hipLaunchKernelGGL((k_b_pair<bf16, PAIR_POST>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -120,10 +120,11 @@ __device__ inline void restrict_node(V *__restrict__ coarse, const V *__restrict
 }
 
 // fine[R,C] from coarse, in the reference's accumulation order; optionally
-// x[R,C] += alpha * fine[R,C] (the MG correction ScaledSum, fused).
+// x[R,C] + alpha * fine[R,C] (the MG correction ScaledSum, fused) stored to
+// xo[R,C] (xo null: back to x).
 template <typename V, bool AXPY>
 __device__ inline void prolong_node(V *__restrict__ fine, const V *__restrict__ coarse, int64_t m, const Mask8 &mk,
-                                    V *__restrict__ x, V alpha, int64_t id) {
+                                    V *x, V alpha, int64_t id, V *xo = nullptr) {
     const int64_t nc = m + 1, nf = 2 * m + 1;
     const int64_t R = (uint32_t)id / (uint32_t)nf, C = (uint32_t)id % (uint32_t)nf;
     auto Cn = [&](int64_t cr, int64_t cc) { return coarse[cr * nc + cc]; };
@@ -145,7 +146,7 @@ __device__ inline void prolong_node(V *__restrict__ fine, const V *__restrict__ 
         f = f + Cn((R + 1) / 2, (C + 1) / 2) * V(0.25);
     }
     if (fine) fine[id] = f;
-    if constexpr (AXPY) x[id] = x[id] + f * alpha;
+    if constexpr (AXPY) (xo ? xo : x)[id] = x[id] + f * alpha;
 }
 
 }  // namespace hipx

@@ -372,6 +372,12 @@ typedef struct {
                                   converged residual confirmed by a recomputed b - A x (default) */
     int32_t fused_levels;      /* levels 1..fused_levels (0..6, default 4) of every V-cycle run
                                   as one launch, one work-group per slot (same arithmetic) */
+    int32_t use_graphs;        /* 1 (default): a step is one hipGraph launch; 0: the same kernels
+                                  enqueued directly at every step (same results; for profilers
+                                  that cannot follow graph launches) */
+    int32_t pair_sweeps;       /* 1 (default): on levels of 65^2 nodes and up, Jacobi0 + defect and
+                                  the two post-smoothing sweeps each run as one launch that reads
+                                  the matrix once (n_pre == 2, n_post == 2; same results) */
 } honei_hip_batch_config;
 
 typedef struct {

@@ -151,6 +151,38 @@ def test_fused_coarse_levels_equal_level_by_level(outer):
             assert np.array_equal(base[key][1], fused[key][1]), (k, key)
 
 
+@pytest.mark.parametrize("outer", [0, 1])
+def test_direct_enqueue_equals_graph_launch(outer):
+    """use_graphs = 0 enqueues the captured iteration's kernels at every step
+    instead of launching the graph: the same bits, iteration counts and
+    number of steps."""
+    systems = {(c, j): _system(lv, 500 + 10 * c + j) for c, lv in enumerate([6, 4, 2]) for j in range(2)}
+    g, lg, _ = _run_batch([6, 4, 2], [2, 2, 2], systems, outer_method=outer, iters_per_launch=3)
+    d, ld, _ = _run_batch([6, 4, 2], [2, 2, 2], systems, outer_method=outer, iters_per_launch=3, use_graphs=0)
+    assert lg == ld
+    for key in systems:
+        assert g[key][0] == d[key][0], key
+        assert np.array_equal(g[key][1], d[key][1]), key
+
+
+@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+@pytest.mark.parametrize("outer", [0, 1])
+def test_paired_passes_equal_separate_launches(outer, inner):
+    """Levels of 65^2 nodes and up run Jacobi0 + defect and the two
+    post-smoothing sweeps as one launch each (pair_sweeps = 1, the default):
+    every slot's iterations, defect and x equal those of one launch per pass
+    (pair_sweeps = 0), with T = 1 on every level (autotune off)."""
+    systems = {(c, j): _system(lv, 700 + 10 * c + j) for c, lv in enumerate([8, 7, 6, 3]) for j in range(2)}
+    kw = dict(outer_method=outer, inner_vtype=inner, autotune=0, iters_per_launch=2)
+    sep, _, _ = _run_batch([8, 7, 6, 3], [2, 2, 2, 2], systems, pair_sweeps=0, **kw)
+    par, _, _ = _run_batch([8, 7, 6, 3], [2, 2, 2, 2], systems, pair_sweeps=1, **kw)
+    for key in systems:
+        assert sep[key][0] == par[key][0], key
+        assert np.array_equal(sep[key][1], par[key][1]), key
+        if outer == 1:
+            assert sep[key][0][2] == 0, key   # converged
+
+
 def test_batch_slot_reuse_and_late_admission():
     """A slot reloaded after its first system retired, while another slot is
     still running, solves the new system exactly as a fresh batch does."""
@@ -192,7 +224,8 @@ def test_batch_not_converged_and_budget():
         H.Batch([1], [1], H.batch_config())
     assert e.value.code == 1
     for bad in (dict(fused_levels=7), dict(fused_levels=-1), dict(outer_method=2), dict(n_pre=1),
-                dict(iters_per_launch=0), dict(averaging=3)):
+                dict(iters_per_launch=0), dict(averaging=3), dict(use_graphs=2),
+                dict(pair_sweeps=2)):
         with pytest.raises(H.HipError) as e:
             H.Batch([5], [1], H.batch_config(**bad))
         assert e.value.code == 1, bad
