@@ -23,12 +23,15 @@ namespace honei {
 namespace hipx {
 namespace {
 
-// v = A p and partial v.p; the last block forms lambda = alpha / v.p.  At
-// most 2048 blocks, each walking a contiguous range of 256/T-row chunks.
-template <typename VT, int T>
+// v = A p and partial v.p (one partial per block, summed by the next
+// kernel).  Each block walks a contiguous range of 256/T-row chunks.  W:
+// stored width q1_width(T), index and value streams read through buffer
+// resources of aj_bytes / ax_bytes (the fixed-width row sum of the ELL row
+// kernel; same sums).
+template <typename VT, int T, bool W>
 __global__ __launch_bounds__(256) void k_cg_spmv_dot(EllView A, const typename VecOf<VT>::type *p,
                                                      typename VecOf<VT>::type *v, CGScalars *sc, ReduceSlots sl,
-                                                     int64_t nlogical) {
+                                                     int64_t nlogical, uint32_t aj_bytes, uint32_t ax_bytes) {
     typedef typename VecOf<VT>::type V;
     if (sc->done) return;
     const int64_t pb = xcd_block(blockIdx.x, gridDim.x);
@@ -42,7 +45,13 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot(EllView A, const typename V
         const int64_t row = gid / T;
         const bool active = row < A.rows;
         V s = V(0);
-        if (active) s = ell_lane_sum<VT, T>(A.Aj, (const VT *)A.Ax, row, lane, A.Arl[row], A.stride, p);
+        if (active) {
+            if constexpr (W)
+                s = ell_lane_sum_buf<VT, T, q1_width(T)>(buffer_rsrc(A.Aj, aj_bytes), buffer_rsrc(A.Ax, ax_bytes), row,
+                                                         lane, A.Arl[row], A.stride, p, nullptr);
+            else
+                s = ell_lane_sum<VT, T>(A.Aj, (const VT *)A.Ax, row, lane, A.Arl[row], A.stride, p);
+        }
         s = lane_group_sum<T>(s);
         if (active && lane == 0) {
             v[row] = s;
@@ -272,6 +281,41 @@ inline int grid_for(int64_t n, int per = 1024) {
     return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
 }
 
+#define HX_RET(expr)                          \
+    do {                                      \
+        const hipError_t e_ = (expr);         \
+        if (e_ != hipSuccess) return e_;      \
+    } while (0)
+
+// t = A p with the partials of t.p; the fixed-width form where the stored
+// width is that of the Q1 operator and every byte offset fits 31 bits
+template <typename VT>
+hipError_t launch_spmv_dot(const EllView &A, const void *p, void *t, CGScalars *sc, const ReduceSlots &sl,
+                           int64_t nlogical, int64_t grid, hipStream_t s) {
+    typedef typename VecOf<VT>::type V;
+    const int T = A.threads;
+    const int64_t slots_total = A.ncpr * A.stride;
+    const bool wide = T <= 8 && A.ncpr == q1_width(T) && slots_total * 8 < (int64_t(1) << 31);
+    const uint32_t ajb = (uint32_t)(wide ? slots_total * 4 : 0);
+    const uint32_t axb = (uint32_t)(wide ? slots_total * (int64_t)sizeof(VT) : 0);
+    const dim3 g((unsigned)grid), b(256);
+#define SDK(TT)                                                                                                  \
+    case TT:                                                                                                     \
+        if (wide && TT <= 8)                                                                                     \
+            hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT, (TT <= 8)>), g, b, 0, s, A, (const V *)p, (V *)t, sc, sl,  \
+                               nlogical, ajb, axb);                                                              \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT, false>), g, b, 0, s, A, (const V *)p, (V *)t, sc, sl,      \
+                               nlogical, ajb, axb);                                                              \
+        break;
+    switch (T) {
+        SDK(1) SDK(2) SDK(4) SDK(8) SDK(16) SDK(32) SDK(64)
+        default: return hipErrorInvalidValue;
+    }
+#undef SDK
+    return hipGetLastError();
+}
+
 template <typename VT>
 hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc, const ReduceSlots &sl,
                      hipStream_t s) {
@@ -279,16 +323,7 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
     const int T = A.threads;
     const int64_t nlogical = (A.rows * T + 255) / 256;
     const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
-#define CGK(TT)                                                                                              \
-    case TT:                                                                                                 \
-        hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)p,  \
-                           (V *)v, sc, sl, nlogical);                                                        \
-        break;
-    switch (T) {
-        CGK(1) CGK(2) CGK(4) CGK(8) CGK(16) CGK(32) CGK(64)
-        default: return hipErrorInvalidValue;
-    }
-#undef CGK
+    HX_RET(launch_spmv_dot<VT>(A, p, v, sc, sl, nlogical, grid, s));
     hipLaunchKernelGGL((k_cg_lambda<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     const int g2 = grid_for(A.rows, 2048);
     hipLaunchKernelGGL((k_cg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)p, (V *)r, (const V *)v,
@@ -306,16 +341,7 @@ hipError_t pcg_iter_t(const EllView &A, void *x, void *d, void *r, void *t0, con
     const int T = A.threads;
     const int64_t nlogical = (A.rows * T + 255) / 256;
     const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
-#define PCGK(TT)                                                                                             \
-    case TT:                                                                                                 \
-        hipLaunchKernelGGL((k_cg_spmv_dot<VT, TT>), dim3((unsigned)grid), dim3(256), 0, s, A, (const V *)d,  \
-                           (V *)t0, sc, sl, nlogical);                                                       \
-        break;
-    switch (T) {
-        PCGK(1) PCGK(2) PCGK(4) PCGK(8) PCGK(16) PCGK(32) PCGK(64)
-        default: return hipErrorInvalidValue;
-    }
-#undef PCGK
+    HX_RET(launch_spmv_dot<VT>(A, d, t0, sc, sl, nlogical, grid, s));
     hipLaunchKernelGGL((k_pcg_step<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     const int g2 = grid_for(A.rows, 2048);
     hipLaunchKernelGGL((k_pcg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)d, (V *)r, (const V *)t0,
