@@ -432,6 +432,7 @@ BClassTab BatchMG::class_table() const {
         BClassEntry &e = t.c[k];
         e.first = c.first;
         e.n = c.n;
+        e.side = c.side;
         e.vld = c.vld;
         e.stride = c.stride;
         e.ax_ld = c.ax_ld;

@@ -142,7 +142,7 @@ struct BWork {
 // Per-class data of the FP64 outer launches, which cover the active slots of
 // every class in one launch (logical block = 256 rows of one slot).
 struct BClassEntry {
-    int64_t first, n, vld, stride, ax_ld, bps, vld_top;
+    int64_t first, n, side, vld, stride, ax_ld, bps, vld_top;
     int64_t pst;   // partial stride per slot: 4 bps
     int64_t nps;   // partials per slot of the CG reductions: one per 64-row chunk
     const int32_t *Aj;

@@ -265,6 +265,34 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork 
 //     wavefront sum of that chunk (nps = ceil(n / 64) partials per slot), so
 //     no launch waits at a work-group barrier.
 
+// Row sum of G slots' FP64 Q1 operators (as q1_row builds them, T = 1):
+// the neighbour columns come from the grid position (interior row: the
+// 9-point stencil in column order; boundary row: the diagonal alone), so no
+// index array is read; the values, the guarded sum and its order are those of
+// ell_lane_sum_group_w over the stored matrix, hence the same bits.
+template <int G>
+__device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const double *const (&Ax)[G], int64_t stride,
+                                            const double *const (&x)[G], double (&s)[G]) {
+    const uint32_t r = (uint32_t)row / (uint32_t)side, c = (uint32_t)row - r * (uint32_t)side;
+    const bool bnd = r == 0 || r == side - 1 || c == 0 || c == side - 1;
+    int64_t j[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) j[k] = bnd ? row : row + (int64_t)(k / 3 - 1) * side + (k % 3 - 1);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        double av[9], xv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            av[k] = load_value_nt(Ax[g] + row + (int64_t)k * stride);
+            xv[k] = x[g][j[k]];
+        }
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) t = (k < (bnd ? 1 : 9)) ? t + av[k] * xv[k] : t;
+        s[g] = t;
+    }
+}
+
 // Row sums (defect, A p): logical block = 256 rows of a group of G
 // consecutive listed slots of one class (a short last group repeats its
 // first slot in the spare places and stores only its own slots); each
@@ -307,9 +335,9 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
                 unused[k] = nullptr;
             }
             double sum[G];
-            // the FP64 outer operator: Q1, one lane per row, stored width 9
-            ell_lane_sum_group_w<double, 1, G, q1_width(1), false>(e.Aj, ax, row, 0, e.Arl[row], e.stride, xv, unused,
-                                                                   sum);
+            // the FP64 outer operator: Q1 (q1_row), one lane per row
+            (void)unused;
+            q1_stencil_sum_group<G>(e.side, row, ax, e.stride, xv, sum);
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const int64_t vi = ls[k] * e.vld + row;
