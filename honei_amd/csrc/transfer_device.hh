@@ -127,24 +127,23 @@ __device__ inline void prolong_node(V *__restrict__ fine, const V *__restrict__ 
                                     V *x, V alpha, int64_t id, V *xo = nullptr) {
     const int64_t nc = m + 1, nf = 2 * m + 1;
     const int64_t R = (uint32_t)id / (uint32_t)nf, C = (uint32_t)id % (uint32_t)nf;
-    auto Cn = [&](int64_t cr, int64_t cc) { return coarse[cr * nc + cc]; };
-    V f;
-    if (dirichlet_zero((uint32_t)R, (uint32_t)C, (uint32_t)nf, mk)) {
-        f = V(0);
-    } else if ((R & 1) == 0 && (C & 1) == 0) {
-        f = Cn(R / 2, C / 2);
-    } else if ((R & 1) == 0) {
-        f = V(0) + Cn(R / 2, (C - 1) / 2) * V(0.5);
-        f = f + Cn(R / 2, (C + 1) / 2) * V(0.5);
-    } else if ((C & 1) == 0) {
-        f = V(0) + Cn((R - 1) / 2, C / 2) * V(0.5);
-        f = f + Cn((R + 1) / 2, C / 2) * V(0.5);
-    } else {
-        f = V(0) + Cn((R - 1) / 2, (C - 1) / 2) * V(0.25);
-        f = f + Cn((R - 1) / 2, (C + 1) / 2) * V(0.25);
-        f = f + Cn((R + 1) / 2, (C - 1) / 2) * V(0.25);
-        f = f + Cn((R + 1) / 2, (C + 1) / 2) * V(0.25);
-    }
+    // The up to four coarse neighbours a (rlo, clo), b (rlo, chi), c (rhi,
+    // clo), d (rhi, chi) -- one coarse row / column when R / C is even -- are
+    // all loaded, every case is formed and one is selected, so neighbouring
+    // lanes (alternating parities) take no separate paths.  Each case keeps
+    // the reference's terms and order: even-even a; even R a/2 + b/2; even C
+    // a/2 + c/2; odd-odd a/4 + b/4 + c/4 + d/4 (each from a zero start).
+    const bool oddR = (R & 1) != 0, oddC = (C & 1) != 0;
+    const int64_t rlo = (R - (oddR ? 1 : 0)) / 2, rhi = (R + (oddR ? 1 : 0)) / 2;
+    const int64_t clo = (C - (oddC ? 1 : 0)) / 2, chi = (C + (oddC ? 1 : 0)) / 2;
+    const V a = coarse[rlo * nc + clo], b = coarse[rlo * nc + chi];
+    const V c = coarse[rhi * nc + clo], d = coarse[rhi * nc + chi];
+    const V h = V(0.5), q = V(0.25);
+    const V f_rc = ((V(0) + a * q + b * q) + c * q) + d * q;
+    const V f_c = (V(0) + a * h) + b * h;
+    const V f_r = (V(0) + a * h) + c * h;
+    V f = oddR ? (oddC ? f_rc : f_r) : (oddC ? f_c : a);
+    if (dirichlet_zero((uint32_t)R, (uint32_t)C, (uint32_t)nf, mk)) f = V(0);
     if (fine) fine[id] = f;
     if constexpr (AXPY) (xo ? xo : x)[id] = x[id] + f * alpha;
 }
