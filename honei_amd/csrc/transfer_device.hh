@@ -70,37 +70,42 @@ __device__ inline void restrict_node(V *__restrict__ coarse, const V *__restrict
                                      int64_t id) {
     const int64_t nc = m + 1, nf = 2 * m + 1;
     const int64_t r = (uint32_t)id / (uint32_t)nc, c = (uint32_t)id % (uint32_t)nc;
-    if (dirichlet_zero((uint32_t)r, (uint32_t)c, (uint32_t)nc, mk)) {
-        coarse[id] = V(0);
-        return;
-    }
+    const bool dirichlet = dirichlet_zero((uint32_t)r, (uint32_t)c, (uint32_t)nc, mk);
     auto F = [&](int64_t fr, int64_t fc) { return fine[fr * nf + fc]; };
     const int32_t NONE = INT32_MAX;
     const int32_t ri = (int32_t)r, ci = (int32_t)c;
+    // all nine fine values are loaded (rows / columns outside the grid
+    // clamped to a valid neighbour) and the absent terms selected away, so the
+    // loads are issued together
+    const bool up = r >= 1, dn = r + 1 <= m, lf = c >= 1, rt = c + 1 <= m;
+    const int64_t rm = up ? 2 * r - 1 : 2 * r, rp = dn ? 2 * r + 1 : 2 * r;
+    const int64_t cm = lf ? 2 * c - 1 : 2 * c, cp = rt ? 2 * c + 1 : 2 * c;
+    const V fA = F(rm, 2 * c), fB = F(rp, 2 * c), fC = F(2 * r, cp), fD = F(2 * r, cm);
+    const V fE = F(rp, cp), fF = F(rp, cm), fG = F(rm, cp), fH = F(rm, cm), f0 = F(2 * r, 2 * c);
     // tag = sweep index i (1..m) * 8 + sub-step (A..H = 0..7)
     int32_t k[8];
     V v[8];
     const V h = V(0.5), q = V(0.25);
     // A_i (i=r): row i += .5 F(2i-1, 2c)
-    k[0] = r >= 1 ? ri * 8 + 0 : NONE;           v[0] = r >= 1 ? h * F(2 * r - 1, 2 * c) : V(0);
+    k[0] = up ? ri * 8 + 0 : NONE;             v[0] = up ? h * fA : V(0);
     // B_i (i=r+1): row i-1 += .5 F(2i-1, 2c)
-    k[1] = r + 1 <= m ? (ri + 1) * 8 + 1 : NONE; v[1] = r + 1 <= m ? h * F(2 * r + 1, 2 * c) : V(0);
+    k[1] = dn ? (ri + 1) * 8 + 1 : NONE;       v[1] = dn ? h * fB : V(0);
     // C_i (i=c+1): col i-1 += .5 F(2r, 2i-1)
-    k[2] = c + 1 <= m ? (ci + 1) * 8 + 2 : NONE; v[2] = c + 1 <= m ? h * F(2 * r, 2 * c + 1) : V(0);
+    k[2] = rt ? (ci + 1) * 8 + 2 : NONE;       v[2] = rt ? h * fC : V(0);
     // D_i (i=c): col i += .5 F(2r, 2i-1)
-    k[3] = c >= 1 ? ci * 8 + 3 : NONE;           v[3] = c >= 1 ? h * F(2 * r, 2 * c - 1) : V(0);
+    k[3] = lf ? ci * 8 + 3 : NONE;             v[3] = lf ? h * fD : V(0);
     // E_i (i=r+1, j=c<m): (i-1, j) += .25 F(2i-1, 2j+1)
-    k[4] = (r + 1 <= m && c < m) ? (ri + 1) * 8 + 4 : NONE;
-    v[4] = (r + 1 <= m && c < m) ? q * F(2 * r + 1, 2 * c + 1) : V(0);
+    k[4] = (dn && c < m) ? (ri + 1) * 8 + 4 : NONE;
+    v[4] = (dn && c < m) ? q * fE : V(0);
     // F_i (i=r+1, j=c-1): (i-1, j+1) += .25 F(2i-1, 2j+1)
-    k[5] = (r + 1 <= m && c >= 1) ? (ri + 1) * 8 + 5 : NONE;
-    v[5] = (r + 1 <= m && c >= 1) ? q * F(2 * r + 1, 2 * c - 1) : V(0);
+    k[5] = (dn && lf) ? (ri + 1) * 8 + 5 : NONE;
+    v[5] = (dn && lf) ? q * fF : V(0);
     // G_i (i=r, j=c<m): (i, j) += .25 F(2i-1, 2j+1)
-    k[6] = (r >= 1 && c < m) ? ri * 8 + 6 : NONE;
-    v[6] = (r >= 1 && c < m) ? q * F(2 * r - 1, 2 * c + 1) : V(0);
+    k[6] = (up && c < m) ? ri * 8 + 6 : NONE;
+    v[6] = (up && c < m) ? q * fG : V(0);
     // H_i (i=r, j=c-1): (i, j+1) += .25 F(2i-1, 2j+1)
-    k[7] = (r >= 1 && c >= 1) ? ri * 8 + 7 : NONE;
-    v[7] = (r >= 1 && c >= 1) ? q * F(2 * r - 1, 2 * c - 1) : V(0);
+    k[7] = (up && lf) ? ri * 8 + 7 : NONE;
+    v[7] = (up && lf) ? q * fH : V(0);
     // Batcher odd-even merge sort network for 8 keys (19 comparators)
     cswap(k[0], v[0], k[1], v[1]); cswap(k[2], v[2], k[3], v[3]);
     cswap(k[4], v[4], k[5], v[5]); cswap(k[6], v[6], k[7], v[7]);
@@ -112,11 +117,10 @@ __device__ inline void restrict_node(V *__restrict__ coarse, const V *__restrict
     cswap(k[2], v[2], k[4], v[4]); cswap(k[3], v[3], k[5], v[5]);
     cswap(k[1], v[1], k[2], v[2]); cswap(k[3], v[3], k[4], v[4]);
     cswap(k[5], v[5], k[6], v[6]);
-    V s = F(2 * r, 2 * c);
+    V s = f0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-        if (k[t] != NONE) s = s + v[t];
-    coarse[id] = s;
+    for (int t = 0; t < 8; ++t) s = (k[t] != NONE) ? s + v[t] : s;
+    coarse[id] = dirichlet ? V(0) : s;
 }
 
 // fine[R,C] from coarse, in the reference's accumulation order; optionally
