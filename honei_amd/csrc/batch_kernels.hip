@@ -180,7 +180,14 @@ template <typename VT, int PAIR>
 __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork w, PairGeom pg, float *out0,
                                                                   float *out1, const float *xin, const float *rhs,
                                                                   const float *dinv) {
+    // first-pass results of the tile and its halo
     __shared__ float tile[kPairTileRows][kWave];
+    // the first pass's gathered operand (PAIR_PRE: dinv .* rhs; PAIR_POST:
+    // xin) on grid rows r0 - 2 .. r0 + kPairTileRows - 1 and columns
+    // c0 - 2 .. c0 + kWave - 1 (entry [a][b] = grid (r0 - 2 + a, c0 - 2 + b),
+    // positions outside the grid clamped to it: loaded, never used)
+    __shared__ float src[kPairTileRows + 2][kWave + 2];
+    constexpr int kExtra = 2 * kWave + 2 * (kPairTileRows + 2);   // src entries beyond one per thread
     const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;   // tile row, lane
     const int64_t a0 = active_first(w), na = active_count(w);
     const int64_t ups = (int64_t)pg.strips * pg.segs;
@@ -204,20 +211,45 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork 
         const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);       // clamped row: computed, unused
         const int64_t node = (int64_t)rc * S + colc;
         const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
+        auto operand = [&](int gr, int gc) -> float {
+            gr = gr < 0 ? 0 : (gr > S - 1 ? S - 1 : gr);
+            gc = gc < 0 ? 0 : (gc > S - 1 ? S - 1 : gc);
+            const int64_t j = (int64_t)gr * S + gc;
+            if constexpr (PAIR == PAIR_PRE)
+                return ds[j] * rs[j];
+            else
+                return xs[j];
+        };
+        // ---- stage the operand ----
+        src[i + 1][lane + 1] = operand(r, col);
+        if (threadIdx.x < kExtra) {
+            const int t = threadIdx.x;
+            int a, b;
+            if (t < kWave) {                    // grid row r0 - 2
+                a = 0;
+                b = t + 1;
+            } else if (t < 2 * kWave) {         // grid row r0 + kPairTileRows - 1
+                a = kPairTileRows + 1;
+                b = t - kWave + 1;
+            } else {                            // grid columns c0 - 2 and c0 + kWave - 1
+                a = (t - 2 * kWave) / 2;
+                b = ((t - 2 * kWave) % 2) ? kWave + 1 : 0;
+            }
+            src[a][b] = operand(r0 - 2 + a, c0 - 2 + b);
+        }
         // ---- first pass, grid row r ----
         float av[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
         const float rh = rs[node], di = ds[node];
+        __syncthreads();
         {
             float xv[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                const int64_t j = bnd ? node : node + (int64_t)(k / 3 - 1) * S + (k % 3 - 1);
-                if constexpr (PAIR == PAIR_PRE)
-                    xv[k] = ds[j] * rs[j];
-                else
-                    xv[k] = xs[j];
+                const int a = bnd ? i + 1 : i + 1 + (k / 3 - 1);
+                const int b = bnd ? lane + 1 : lane + 1 + (k % 3 - 1);
+                xv[k] = src[a][b];
             }
             float s = 0.f;
 #pragma unroll
