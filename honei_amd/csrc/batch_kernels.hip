@@ -327,8 +327,9 @@ __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const dou
 
 // Row sums (defect, A p): logical block = 256 rows of a group of G
 // consecutive listed slots of one class (a short last group repeats its
-// first slot in the spare places and stores only its own slots); each
-// column index is loaded once for the group.
+// first slot in the spare places and stores only its own slots).  The
+// neighbour columns come from the grid position, so slots share no loads;
+// the launches use G = kOuterRowGroup.
 template <int OP, int G, bool DC>
 __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, const SlotScalars *sc) {
     static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_SPMV_DOT, "row-sum op");
@@ -869,7 +870,7 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
 
 // Slots per logical block of the row-sum launches, 256-row blocks per
 // logical block of the element-wise launches.
-constexpr int kOuterRowGroup = 2, kOuterElemBlocks = 4;
+constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4;
 
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s) {
