@@ -438,10 +438,24 @@ def batch_kernel_roofline(B, level):
     # per system: FP32 values + x, rhs, dinv, out per row; once: the shared indices and row lengths
     bytes_ = ns * (nnz * 4 + n * 4 * 4) + nnz * 4 + n * 4
     gbs = bytes_ / (us * 1e-6) / 1e9
-    return {"kernel": f"k_b_ell<float,{T},JACOBI,{group}> level {level} ({side}^2), block {block}", "slots": ns,
-            "us": round(us, 2), "algorithmic_bytes": int(bytes_), "achieved_gbs": round(gbs, 1),
-            "frac": round(gbs / PEAK_GBS, 4),
-            "bytes_rule": "slots*(nnz*4 + n*16) + nnz*4 + n*4: the column indices are shared by all slots"}
+    out = {"row_sweep": {"kernel": f"k_b_ell<float,{T},JACOBI,{group}> level {level} ({side}^2), block {block}",
+                         "slots": ns, "us": round(us, 2), "algorithmic_bytes": int(bytes_),
+                         "achieved_gbs": round(gbs, 1), "frac": round(gbs / PEAK_GBS, 4),
+                         "bytes_rule": "slots*(nnz*4 + n*16) + nnz*4 + n*4: the column indices are shared by all "
+                                       "slots",
+                         "note": "one pass per launch; levels from 65^2 up run the paired launches below"}}
+    pre, post = B.pair_timing(level)
+    pbytes = ns * (nnz * 4 + n * 16)
+    for name, t, what in (("paired_pre", pre, "k_b_pair<float,PRE>: Jacobi0 then the defect"),
+                          ("paired_post", post, "k_b_pair<float,POST>: two Jacobi sweeps")):
+        if t > 0:
+            g = pbytes / (t * 1e-6) / 1e9
+            out[name] = {"kernel": f"{what}, level {level} ({side}^2)", "slots": ns, "us": round(t, 2),
+                         "algorithmic_bytes": int(pbytes), "achieved_gbs": round(g, 1),
+                         "frac": round(g / PEAK_GBS, 4),
+                         "bytes_rule": "slots*(nnz*4 + n*16): values once for both passes, rhs, dinv, the "
+                                       "input or first output, the output; no index stream"}
+    return out
 
 
 def cpu_baseline_batched(levels, budget_s=15.0):
@@ -611,7 +625,8 @@ def main():
         total = time.perf_counter() - t0
         print(json.dumps({"batch_only": {"solves_per_s": round(len(levels) * args.steps / total, 2),
                                          "steps": args.steps, "launches_last_step": nl,
-                                         "shapes": {L: B.tuning(L) for L in range(2, max(levels) + 1)}}}))
+                                         "shapes": {L: B.tuning(L) for L in range(2, max(levels) + 1)},
+                                         "pair_us": {L: B.pair_timing(L) for L in range(2, max(levels) + 1)}}}))
         return
 
     if args.serve_only:

@@ -103,6 +103,7 @@ _SIGS = {
     "honei_hip_batch_destroy": [_p],
     "honei_hip_batch_info": [_p, C.POINTER(C.c_int64), C.POINTER(_u64), C.POINTER(C.c_int64)],
     "honei_hip_batch_slot": [_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64)],
+    "honei_hip_batch_pair_timing": [_p, C.c_int32, C.POINTER(_d), C.POINTER(_d)],
     "honei_hip_batch_load": [_p, C.c_int64, _p, _p],
     "honei_hip_batch_load_many": [_p, C.c_int64, _p, _p, _p],
     "honei_hip_batch_load_seeded": [_p, C.c_int64, _p, _p, _p, _d],
@@ -550,6 +551,13 @@ class Batch:
         t, b, g, us, ns = C.c_int32(), C.c_int32(), C.c_int32(), _d(), C.c_int64()
         call("honei_hip_batch_shape", self.h, level, C.byref(t), C.byref(b), C.byref(g), C.byref(us), C.byref(ns))
         return t.value, b.value, g.value, us.value, ns.value
+
+    def pair_timing(self, level):
+        """(paired Jacobi0 + defect us, paired two-sweep us) from the autotuner
+        over the tuning batch of `tuning(level)`; zeros when not paired."""
+        a, b = _d(), _d()
+        call("honei_hip_batch_pair_timing", self.h, level, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def launches(self):
         k = C.c_int64()

@@ -174,6 +174,41 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
                 }
             }
         }
+        if (T == 1 && paired(lv) && err == hipSuccess) {
+            // the paired launches this level will run, timed the same way
+            BLevel tl;
+            tl.level = lv.level;
+            tl.side = g.side;
+            tl.n = n;
+            tl.cap = ns;
+            tl.vld = g.vld;
+            tl.T = 1;
+            tl.stride = g.stride;
+            tl.ncpr = g.ncpr;
+            tl.ax_ld = g.ax_ld;
+            tl.Aj = (int32_t *)Aj;
+            tl.Arl = (uint32_t *)Arl;
+            tl.Ax = Ax;
+            tl.dinv = (float *)vec;
+            tl.x = tl.dinv + ns * g.vld;
+            tl.rhs = tl.x + ns * g.vld;
+            float *out = tl.rhs + ns * g.vld;
+            for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
+                auto launch = [&]() {
+                    return pass == 0 ? b_pair(vt, PAIR_PRE, tl, w, tl.x, out, nullptr, s_main_)
+                                     : b_pair(vt, PAIR_POST, tl, w, out, nullptr, tl.x, s_main_);
+                };
+                const int reps = n >= 65536 ? 10 : 30;
+                for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch();
+                if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
+                for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+                if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                (pass == 0 ? best.pre_us : best.post_us) = 1000.0 * ms / reps;
+            }
+        }
         (void)hipStreamSynchronize(s_main_);
         for (void *p : {Aj, Arl, Ax, vec, scd, lst})
             if (p) (void)hipFree(p);

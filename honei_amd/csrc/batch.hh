@@ -121,6 +121,7 @@ struct LaunchShape {
     int T = 1, block = 256, group = 1;
     double us = 0.0;       // measured time of one batched Jacobi sweep
     int64_t slots = 0;     // slots of the tuning batch
+    double pre_us = 0.0, post_us = 0.0;   // paired launches (b_pair) on the same batch, T = 1
 };
 
 // Batched kernels (batch_kernels.hip).
@@ -258,6 +259,10 @@ class BatchMG {
     const std::vector<BLevel> &levels() const { return L_; }
     const BatchConfig &config() const { return cfg_; }
 
+    // level runs the paired launches (b_pair) of its smoothing passes
+    bool paired(const BLevel &lv) const {
+        return cfg_.pair_sweeps && lv.side >= kPairMinSide && (cfg_.n_pre == 2 || cfg_.n_post == 2);
+    }
   private:
     BatchConfig cfg_;
     std::vector<BLevel> L_;          // indexed by level (0 unused)
@@ -290,10 +295,6 @@ class BatchMG {
     int64_t build_kappa_cap_ = 0;
     hipEvent_t build_ev_ = nullptr;
     hipError_t tune_level(BLevel &lv);
-    // level runs the paired launches (b_pair) of its smoothing passes
-    bool paired(const BLevel &lv) const {
-        return cfg_.pair_sweeps && lv.side >= kPairMinSide && (cfg_.n_pre == 2 || cfg_.n_post == 2);
-    }
     hipError_t capture();
     hipError_t capture_iterations(int iters, hipGraph_t *graph, hipGraphExec_t *exec);
     hipError_t enqueue_iteration(hipStream_t s);

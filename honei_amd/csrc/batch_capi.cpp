@@ -242,6 +242,21 @@ int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t 
     return HONEI_HIP_SUCCESS;
 }
 
+int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, double *post_us) {
+    if (!batch) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "batch is null");
+    BatchMG *b = (BatchMG *)batch;
+    const auto &L = b->levels();
+    if (level < 1 || level >= (int32_t)L.size()) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "level out of range");
+    LaunchShape sh;
+    int dev = 0;
+    const bool tuned = hipGetDevice(&dev) == hipSuccess &&
+                       batch_tuned_shape(dev, level, b->config().inner_vtype, &sh) && b->config().autotune &&
+                       b->paired(L[(size_t)level]);
+    if (pre_us) *pre_us = tuned ? sh.pre_us : 0.0;
+    if (post_us) *post_us = tuned ? sh.post_us : 0.0;
+    return HONEI_HIP_SUCCESS;
+}
+
 int honei_hip_serve_run(const honei_hip_serve_config *cfg, const honei_hip_request *reqs, int64_t nreq,
                         honei_hip_request_result *results, honei_hip_serve_stats *stats) {
     if (!cfg || (nreq > 0 && (!reqs || !results)) || nreq < 0)

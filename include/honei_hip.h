@@ -424,6 +424,11 @@ int honei_hip_batch_run(void *batch, int64_t *launches);
  * batch (microseconds, else 0) and that batch's slot count.                */
 int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, int32_t *group,
                           double *sweep_us, int64_t *tune_slots);
+/* Levels that run the paired launches (pair_sweeps, levels of 65^2 nodes and
+ * up): the autotuner's measured time of one paired Jacobi0 + defect launch
+ * and of one paired two-sweep launch over the same tuning batch
+ * (microseconds; 0 when the level is not paired or not tuned).            */
+int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, double *post_us);
 
 /* ---- serving loop -------------------------------------------------------------------
  * No reference counterpart.  Continuous batching over a batch pool: requests
