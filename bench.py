@@ -443,7 +443,7 @@ def batch_kernel_roofline(B, level):
                          "achieved_gbs": round(gbs, 1), "frac": round(gbs / PEAK_GBS, 4),
                          "bytes_rule": "slots*(nnz*4 + n*16) + nnz*4 + n*4: the column indices are shared by all "
                                        "slots",
-                         "note": "one pass per launch; levels from 65^2 up run the paired launches below"}}
+                         "note": "one pass per launch; levels from 33^2 up run the paired launches below"}}
     pre, post = B.pair_timing(level)
     pbytes = ns * (nnz * 4 + n * 16)
     for name, t, what in (("paired_pre", pre, "k_b_pair<float,PRE>: Jacobi0 then the defect"),

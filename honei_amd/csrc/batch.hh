@@ -66,7 +66,7 @@ constexpr int kMaxClasses = 16;
 enum PairKind { PAIR_PRE = 0, PAIR_POST = 1 };
 constexpr int64_t kPairWidth = 62;
 constexpr int kPairTileRows = 16, kPairRows = kPairTileRows - 2;
-constexpr int64_t kPairMinSide = 65;
+constexpr int64_t kPairMinSide = 33;
 
 struct BatchConfig {
     int32_t inner_vtype = VT_F32;    // VT_F32 or VT_BF16 matrix values (vectors FP32)

@@ -375,7 +375,7 @@ typedef struct {
     int32_t use_graphs;        /* 1 (default): a step is one hipGraph launch; 0: the same kernels
                                   enqueued directly at every step (same results; for profilers
                                   that cannot follow graph launches) */
-    int32_t pair_sweeps;       /* 1 (default): on levels of 65^2 nodes and up, Jacobi0 + defect and
+    int32_t pair_sweeps;       /* 1 (default): on levels of 33^2 nodes and up, Jacobi0 + defect and
                                   the two post-smoothing sweeps each run as one launch that reads
                                   the matrix once (n_pre == 2, n_post == 2; same results) */
 } honei_hip_batch_config;
@@ -424,7 +424,7 @@ int honei_hip_batch_run(void *batch, int64_t *launches);
  * batch (microseconds, else 0) and that batch's slot count.                */
 int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t *block, int32_t *group,
                           double *sweep_us, int64_t *tune_slots);
-/* Levels that run the paired launches (pair_sweeps, levels of 65^2 nodes and
+/* Levels that run the paired launches (pair_sweeps, levels of 33^2 nodes and
  * up): the autotuner's measured time of one paired Jacobi0 + defect launch
  * and of one paired two-sweep launch over the same tuning batch
  * (microseconds; 0 when the level is not paired or not tuned).            */

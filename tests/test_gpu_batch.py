@@ -168,7 +168,7 @@ def test_direct_enqueue_equals_graph_launch(outer):
 @pytest.mark.parametrize("inner", [H.F32, H.BF16])
 @pytest.mark.parametrize("outer", [0, 1])
 def test_paired_passes_equal_separate_launches(outer, inner):
-    """Levels of 65^2 nodes and up run Jacobi0 + defect and the two
+    """Levels of 33^2 nodes and up run Jacobi0 + defect and the two
     post-smoothing sweeps as one launch each (pair_sweeps = 1, the default):
     every slot's iterations, defect and x equal those of one launch per pass
     (pair_sweeps = 0), with T = 1 on every level (autotune off)."""
