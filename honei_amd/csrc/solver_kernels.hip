@@ -128,6 +128,7 @@ __global__ void k_cg_init(CGScalars *sc, double eps_rel, int max_iters) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const V alpha = (V)sc->alpha;
         sc->alpha = (double)alpha;
+        sc->carry[0] = (double)alpha;
         sc->initial = (double)sqrt(alpha);
         sc->current = sc->initial;
         sc->eps_rel = eps_rel;
@@ -166,6 +167,7 @@ __global__ void k_pcg_init(const double *partials, int64_t nparts, CGScalars *sc
     const double rz = sum_partials(partials, nparts, 1, 0);
     if (threadIdx.x == 0) {
         sc->alpha = (double)(V)rz;
+        sc->carry[0] = (double)(V)rz;
         const V initial = sqrt((V)sc->current);
         sc->initial = (double)initial;
         sc->current = sc->initial;
@@ -249,6 +251,122 @@ __global__ __launch_bounds__(256) void k_pcg_direction(V *d, const V *r, const V
     }
 }
 
+// ---- three-kernel iteration for small systems --------------------------------
+// When the SpMV and update launches have few blocks (<= kFewParts), the
+// one-block scalar kernels are folded into their consumers: every block of
+// the update sums the SpMV's partials itself (one block_sum, the same value
+// in every block), and every block of the direction kernel sums the
+// update's partials; block 0 of the direction kernel publishes the scalars.
+// The carried scalar of iteration `it` is read from carry[it & 1] while the
+// next one goes to carry[(it + 1) & 1].  Partials of the update live at
+// kFusedOff so the SpMV of the next iteration cannot overwrite them early.
+constexpr int kFewParts = 256;
+constexpr int64_t kFusedOff = 8192;
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_cg_update_f(V *x, const V *p, V *r, const V *v, int64_t n, CGScalars *sc,
+                                                     ReduceSlots sl, int64_t nparts, int it) {
+    if (sc->done) return;
+    const double tot = sum_partials(sl.partials, nparts, 1, 0);
+    const V temp = (V)tot;
+    const V e = eps_of<V>();
+    const V alpha = (V)sc->carry[it & 1];
+    const V lambda = alpha / ((temp < 0 ? -temp : temp) > e ? temp : e);
+    const V mlambda = -lambda;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += p[i] * lambda;
+        const V ri = r[i] + v[i] * mlambda;
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    }
+    const double b = block_sum(acc);
+    if (threadIdx.x == 0) sl.partials[kFusedOff + blockIdx.x] = b;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_cg_direction_f(V *p, const V *r, int64_t n, CGScalars *sc, ReduceSlots sl,
+                                                        int64_t nparts, int it) {
+    if (sc->done) return;
+    const double tot = sum_partials(sl.partials + kFusedOff, nparts, 1, 0);
+    const V e = eps_of<V>();
+    const V alpha_old = (V)sc->carry[it & 1];
+    const V alpha = (V)tot;
+    const V talpha = alpha / ((alpha_old < 0 ? -alpha_old : alpha_old) > e ? alpha_old : e);
+    const V current = sqrt(alpha);
+    const bool stop = current < (V)sc->eps_rel * (V)sc->initial || current < (V)sc->eps_rel || it + 1 >= sc->max_iters;
+    if (!stop) {
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            const V q = p[i] * talpha;
+            p[i] = q + r[i];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->carry[(it + 1) & 1] = (double)alpha;
+        sc->alpha = (double)alpha;
+        sc->talpha = (double)talpha;
+        sc->iters = it + 1;
+        sc->current = (double)current;
+        if (stop) sc->done = 1;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_pcg_update_f(V *x, const V *d, V *r, const V *t0, const V *dinv, int64_t n,
+                                                      CGScalars *sc, ReduceSlots sl, int64_t nparts, int it) {
+    if (sc->done) return;
+    const double tot = sum_partials(sl.partials, nparts, 1, 0);
+    const V dt = (V)tot;
+    const V e = eps_of<V>();
+    const V rz = (V)sc->carry[it & 1];
+    const V step = ((dt < 0 ? -dt : dt) >= e) ? rz / dt : rz / e;
+    const V mstep = -step;
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += d[i] * step;
+        const V ri = r[i] + t0[i] * mstep;
+        r[i] = ri;
+        const V zi = dinv[i] * ri;
+        acc[0] += (double)zi * (double)ri;
+        acc[1] += (double)ri * (double)ri;
+    }
+    block_sum_first<2>(acc);
+    if (threadIdx.x == 0) {
+        sl.partials[kFusedOff + 2 * blockIdx.x] = acc[0];
+        sl.partials[kFusedOff + 2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_pcg_direction_f(V *d, const V *r, const V *dinv, int64_t n, CGScalars *sc,
+                                                         ReduceSlots sl, int64_t nparts, int it) {
+    if (sc->done) return;
+    const double zr_t = sum_partials(sl.partials + kFusedOff, nparts, 2, 0);
+    const double rr_t = sum_partials(sl.partials + kFusedOff, nparts, 2, 1);
+    const V e = eps_of<V>();
+    const V rz_old = (V)sc->carry[it & 1];
+    const V zr = (V)zr_t;
+    const V factor = ((rz_old < 0 ? -rz_old : rz_old) >= e) ? zr / rz_old : zr / e;
+    const V current = sqrt((V)rr_t);
+    const bool conv = current < (V)sc->initial * (V)sc->eps_rel;
+    const bool stop = conv || it + 1 >= sc->max_iters;
+    if (!stop) {
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            const V zi = dinv[i] * r[i];
+            d[i] = zi + d[i] * factor;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->carry[(it + 1) & 1] = (double)zr;
+        sc->alpha = (double)zr;
+        sc->talpha = (double)factor;
+        sc->iters = it + 1;
+        sc->current = (double)current;
+        sc->converged = conv ? 1 : 0;
+        if (stop) sc->done = 1;
+    }
+}
+
 template <typename VT>
 __global__ __launch_bounds__(256) void k_coarse_cg(EllView A, const typename VecOf<VT>::type *rhs,
                                                    typename VecOf<VT>::type *x, typename VecOf<VT>::type *g,
@@ -318,14 +436,21 @@ hipError_t launch_spmv_dot(const EllView &A, const void *p, void *t, CGScalars *
 
 template <typename VT>
 hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc, const ReduceSlots &sl,
-                     hipStream_t s) {
+                     int it, hipStream_t s) {
     typedef typename VecOf<VT>::type V;
     const int T = A.threads;
     const int64_t nlogical = (A.rows * T + 255) / 256;
     const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
     HX_RET(launch_spmv_dot<VT>(A, p, v, sc, sl, nlogical, grid, s));
-    hipLaunchKernelGGL((k_cg_lambda<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     const int g2 = grid_for(A.rows, 2048);
+    if (grid <= kFewParts && g2 <= kFewParts) {
+        hipLaunchKernelGGL((k_cg_update_f<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)p, (V *)r, (const V *)v,
+                           A.rows, sc, sl, grid, it);
+        hipLaunchKernelGGL((k_cg_direction_f<V>), dim3(grid_for(A.rows)), dim3(256), 0, s, (V *)p, (const V *)r,
+                           A.rows, sc, sl, (int64_t)g2, it);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_cg_lambda<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     hipLaunchKernelGGL((k_cg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)p, (V *)r, (const V *)v,
                        A.rows, sc, sl);
     hipLaunchKernelGGL((k_cg_alpha<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g2, sc);
@@ -336,14 +461,21 @@ hipError_t cg_iter_t(const EllView &A, void *x, void *p, void *r, void *v, CGSca
 
 template <typename VT>
 hipError_t pcg_iter_t(const EllView &A, void *x, void *d, void *r, void *t0, const void *dinv, CGScalars *sc,
-                      const ReduceSlots &sl, hipStream_t s) {
+                      const ReduceSlots &sl, int it, hipStream_t s) {
     typedef typename VecOf<VT>::type V;
     const int T = A.threads;
     const int64_t nlogical = (A.rows * T + 255) / 256;
     const int64_t grid = nlogical < sl.capacity ? nlogical : sl.capacity;
     HX_RET(launch_spmv_dot<VT>(A, d, t0, sc, sl, nlogical, grid, s));
-    hipLaunchKernelGGL((k_pcg_step<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     const int g2 = grid_for(A.rows, 2048);
+    if (grid <= kFewParts && g2 <= kFewParts) {
+        hipLaunchKernelGGL((k_pcg_update_f<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)d, (V *)r,
+                           (const V *)t0, (const V *)dinv, A.rows, sc, sl, grid, it);
+        hipLaunchKernelGGL((k_pcg_direction_f<V>), dim3(grid_for(A.rows)), dim3(256), 0, s, (V *)d, (const V *)r,
+                           (const V *)dinv, A.rows, sc, sl, (int64_t)g2, it);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_pcg_step<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, grid, sc);
     hipLaunchKernelGGL((k_pcg_update<V>), dim3(g2), dim3(256), 0, s, (V *)x, (const V *)d, (V *)r, (const V *)t0,
                        (const V *)dinv, A.rows, sc, sl);
     hipLaunchKernelGGL((k_pcg_beta<V>), dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g2, sc);
@@ -365,21 +497,21 @@ hipError_t pcg_start_t(void *d, const void *r, const void *dinv, int64_t n, CGSc
 }  // namespace
 
 hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
-                        const ReduceSlots &sl, hipStream_t s) {
+                        const ReduceSlots &sl, int it, hipStream_t s) {
     switch (A.vtype) {
-        case VT_F32: return cg_iter_t<float>(A, x, p, r, v, sc, sl, s);
-        case VT_F64: return cg_iter_t<double>(A, x, p, r, v, sc, sl, s);
-        case VT_BF16: return cg_iter_t<bf16>(A, x, p, r, v, sc, sl, s);
+        case VT_F32: return cg_iter_t<float>(A, x, p, r, v, sc, sl, it, s);
+        case VT_F64: return cg_iter_t<double>(A, x, p, r, v, sc, sl, it, s);
+        case VT_BF16: return cg_iter_t<bf16>(A, x, p, r, v, sc, sl, it, s);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t pcg_iteration(const EllView &A, void *x, void *d, void *r, void *t0, const void *dinv, CGScalars *sc,
-                         const ReduceSlots &sl, hipStream_t s) {
+                         const ReduceSlots &sl, int it, hipStream_t s) {
     switch (A.vtype) {
-        case VT_F32: return pcg_iter_t<float>(A, x, d, r, t0, dinv, sc, sl, s);
-        case VT_F64: return pcg_iter_t<double>(A, x, d, r, t0, dinv, sc, sl, s);
-        case VT_BF16: return pcg_iter_t<bf16>(A, x, d, r, t0, dinv, sc, sl, s);
+        case VT_F32: return pcg_iter_t<float>(A, x, d, r, t0, dinv, sc, sl, it, s);
+        case VT_F64: return pcg_iter_t<double>(A, x, d, r, t0, dinv, sc, sl, it, s);
+        case VT_BF16: return pcg_iter_t<bf16>(A, x, d, r, t0, dinv, sc, sl, it, s);
     }
     return hipErrorInvalidValue;
 }

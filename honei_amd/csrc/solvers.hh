@@ -22,12 +22,17 @@ namespace hipx {
 // Device-resident CG state (math/cg.hh:63-133 scalars).
 struct CGScalars {
     double alpha, lambda, talpha, initial, current, eps_rel;
+    // the scalar an iteration carries to the next (CG: |r|^2, PCG: r.z) of
+    // iteration k in carry[k & 1] (read by the three-kernel iteration while
+    // block 0 writes the next one)
+    double carry[2];
     int done, iters, max_iters, converged;   // converged: set by the PCG-Jacobi test only
 };
 
 hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s);
+// Iteration `it` (0, 1, ... in order) of the loop.
 hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
-                        const ReduceSlots &sl, hipStream_t s);
+                        const ReduceSlots &sl, int it, hipStream_t s);
 // Jacobi-preconditioned CG (math/conjugate_gradients.hh:884-913,
 // 1110-1154).  pcg_start expects r = b - A x in r and ||r||^2 in
 // sc->current; it forms d = dinv .* r and the scalars.  Vectors (and dinv)
@@ -35,7 +40,7 @@ hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CG
 hipError_t pcg_start(ValueType vt, void *d, const void *r, const void *dinv, int64_t n, CGScalars *sc,
                      double eps_rel, int max_iters, const ReduceSlots &sl, hipStream_t s);
 hipError_t pcg_iteration(const EllView &A, void *x, void *d, void *r, void *t0, const void *dinv, CGScalars *sc,
-                         const ReduceSlots &sl, hipStream_t s);
+                         const ReduceSlots &sl, int it, hipStream_t s);
 hipError_t coarse_cg(const EllView &A, const void *rhs, void *x, void *g, void *u, void *en, double konv,
                      int max_iters, int *iters_dev, hipStream_t s);
 hipError_t boundary_guess(double *guess, const double *b, int64_t n, int64_t side, hipStream_t s);

@@ -109,7 +109,7 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     size_t enqueued = 0;
     while (true) {
         for (size_t k = 0; k < chunk && enqueued < max_iters; ++k, ++enqueued)
-            if ((st = check(cg_iteration(A, x, p, r, v, sc, w->slots, s), "cg iteration"))) return st;
+            if ((st = check(cg_iteration(A, x, p, r, v, sc, w->slots, (int)enqueued, s), "cg iteration"))) return st;
         if ((st = check(hipMemcpyAsync(&h, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s), "cg readback")))
             return st;
         if ((st = check(hipStreamSynchronize(s), "cg sync"))) return st;
@@ -175,7 +175,7 @@ int honei_hip_pcg_jacobi(int vtype, const void *Aj, const void *Ax, const void *
     size_t enqueued = 0;
     while (true) {
         for (size_t k = 0; k < chunk && enqueued < max_iters; ++k, ++enqueued)
-            if ((st = check(pcg_iteration(A, x, d, r, t0, dd_inverted, sc, w->slots, s), "pcg iteration")))
+            if ((st = check(pcg_iteration(A, x, d, r, t0, dd_inverted, sc, w->slots, (int)enqueued, s), "pcg iteration")))
                 return st;
         if ((st = check(hipMemcpyAsync(&h, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s), "pcg readback")))
             return st;
