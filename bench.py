@@ -208,7 +208,7 @@ def time_spmv_staged(D, reps, x, y, label, block=1024, chunks=1):
             "what": "honei_hip_product_smell_dv_staged, column windows per block computed once per matrix"}
 
 
-def config1_cg(with_cpu):
+def cg_q1_129(with_cpu):
     """CGSolver<..., NONE> and ConjugateGradients<..., JAC> (PCG-Jacobi) in
     FP64 on the Q1 129^2 operator (16,641 rows, ELL width 9), b = A x_true
     (x_true ~ U[-1,1), seed 1), x0 = 0, to relative residual 1e-8: operator,
@@ -237,7 +237,7 @@ def config1_cg(with_cpu):
         err = float(np.abs(dx.get() - xt).max())
         out[name] = {"gpu_ms": round(1e3 * float(np.median(ts)), 3), "iterations": int(its),
                      "max_err_vs_x_true": err}
-    out["what"] = ("Q1 129^2 FP64 to 1e-8 (config 1); GPU: honei_hip_cg_solve (CGSolver) and "
+    out["what"] = ("Q1 129^2 FP64 to 1e-8; GPU: honei_hip_cg_solve (CGSolver) and "
                    "honei_hip_pcg_jacobi (ConjugateGradients<JAC>, dinv = 1/diag)")
     if with_cpu:
         import oracle
@@ -663,7 +663,7 @@ def main():
         spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
         del R, xr, yr
     spmv["hbm_ceiling"] = hbm_ceiling()
-    cg129 = config1_cg(not args.no_cpu)
+    cg129 = cg_q1_129(not args.no_cpu)
     pcg2049 = pcg_iteration_rate()
     if not args.no_cpu:
         spmv["cpu_q1_2049_f32"] = cpu_spmv_q1(SIDE)
@@ -862,9 +862,9 @@ def main():
         "fp64_single_gpu": fp64,
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
-        "config1_cg_129": cg129,
+        "cg_q1_129_fp64": cg129,
         "pcg_jacobi_2049_fp64": pcg2049,
-        "config3_2049": c3,
+        "mg_q1_2049_to_1e-10": c3,
         "single_2049": {"solves_per_s": round(1.0 / t1m, 3), "outer_iterations": outer,
                         "rel_residual": rd, "max_err_vs_x_true": err,
                         "what": "one Q1 2049^2 constant-coefficient system, b = A x_true, mixed MG to 1e-8",

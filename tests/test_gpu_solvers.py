@@ -128,8 +128,8 @@ def test_cg_matches_oracle(dtype, side):
     assert np.abs(x - xo).max() <= tol * np.abs(xo).max()
 
 
-def test_cg_config1_fp64():
-    """Config-1 problem: Q1 129x129, b = A x_true, x0 with Dirichlet values,
+def test_cg_q1_129_fp64_boundary_start():
+    """Q1 129x129 problem: b = A x_true, x0 with Dirichlet values,
     FP64 CG to 1e-8 relative."""
     A, xt, b = _problem(129, 1)
     x0 = np.zeros_like(b)
@@ -319,7 +319,7 @@ def test_mg_mixed_prolmat_hierarchy_from_host():
 
 @pytest.mark.parametrize("level", [9, 11])
 def test_mg_mixed_large_zero_start(level):
-    """Config-3 shape (level 11 = 2049^2, 4.2M unknowns): FP32 V(2,2) inner,
+    """Q1 2049^2 shape (level 11, 4.2M unknowns): FP32 V(2,2) inner,
     FP64 outer to 1e-10; error vs x_true and vs an FP64 MG solve."""
     side = 2 ** level + 1
     n = side * side

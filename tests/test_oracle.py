@@ -265,7 +265,7 @@ def _q1_problem(side, seed):
 
 
 def test_cg_solver_closed_form():
-    """CGSolver to 1e-8 relative on b = A x_true (config-1 shape, smaller)."""
+    """CGSolver to 1e-8 relative on b = A x_true (the Q1 CG workload, smaller)."""
     A, xt, b = _q1_problem(33, 1)
     x0 = np.zeros_like(b)
     s = 33
