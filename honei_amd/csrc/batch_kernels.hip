@@ -325,14 +325,14 @@ __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const dou
     }
 }
 
-// Row sums (defect, A p): logical block = 256 rows of a group of G
+// Defects b - A x: logical block = 256 rows of a group of G
 // consecutive listed slots of one class (a short last group repeats its
 // first slot in the spare places and stores only its own slots).  The
 // neighbour columns come from the grid position, so slots share no loads;
 // the launches use G = kOuterRowGroup.
 template <int OP, int G, bool DC>
 __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, const SlotScalars *sc) {
-    static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE || OP == OP_SPMV_DOT, "row-sum op");
+    static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE, "defect op (q = A p: k_b_outer_spmv_dot)");
     __shared__ int64_t pre[kMaxClasses + 1];
     if (threadIdx.x == 0) {
         int64_t acc = 0;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 ax[k] = e.Ax + ls[k] * e.ax_ld;
-                xv[k] = (OP == OP_SPMV_DOT ? e.p : e.x) + ls[k] * e.vld;
+                xv[k] = e.x + ls[k] * e.vld;
                 unused[k] = nullptr;
             }
             double sum[G];
@@ -374,15 +374,9 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const int64_t vi = ls[k] * e.vld + row;
-                if constexpr (OP == OP_SPMV_DOT) {
-                    const double pv = e.p[vi];
-                    if (k < cnt) e.q[vi] = sum[k];
-                    part[k] = pv * sum[k];
-                } else {
-                    const double r = e.b[vi] - sum[k];
-                    if (k < cnt) e.r[vi] = r;
-                    part[k] = r * r;
-                }
+                const double r = e.b[vi] - sum[k];
+                if (k < cnt) e.r[vi] = r;
+                part[k] = r * r;
             }
         }
         if constexpr (DC) {
@@ -400,6 +394,68 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
                 const double v = wave_sum(part[k]);
                 if (threadIdx.x % kWave == 0 && k < cnt && chunk < e.nps) e.partials[ls[k] * e.pst + chunk] = v;
             }
+        }
+    }
+}
+
+// q = A p with the CG partials of p.q (one wavefront sum per 64-row chunk,
+// the layout of k_b_outer_rows): logical block = R consecutive 256-row blocks
+// of one listed slot, thread t takes row t of each with every load of the R
+// rows issued before any sum (rows past the end load the last row and store
+// nothing).  Same sums as k_b_outer_rows<OP_SPMV_DOT>.
+template <int R>
+__global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w) {
+    __shared__ int64_t pre[kMaxClasses + 1];
+    if (threadIdx.x == 0) {
+        int64_t acc = 0;
+        pre[0] = 0;
+        for (int c = 0; c < tab.nclass; ++c) {
+            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * ((tab.c[c].bps + R - 1) / R);
+            pre[c + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int64_t total = pre[tab.nclass];
+    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+        int c = 0;
+        while (lb >= pre[c + 1]) ++c;
+        const BClassEntry &e = tab.c[c];
+        int64_t gi, hb;   // listed slot of the class, group of R row blocks
+        div32(lb - pre[c], (e.bps + R - 1) / R, gi, hb);
+        const int64_t ls = w.list[w.bounds[c] + gi] - e.first;
+        const double *ax = e.Ax + ls * e.ax_ld;
+        const double *pv = e.p + ls * e.vld;
+        const int64_t S = e.side;
+        double av[R][9], xv[R][9], pr[R];
+        bool bnd[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t row0 = (hb * R + r) * 256 + threadIdx.x;
+            const int64_t row = row0 < e.n ? row0 : e.n - 1;
+            const uint32_t gr = (uint32_t)row / (uint32_t)S, gc = (uint32_t)row - gr * (uint32_t)S;
+            bnd[r] = gr == 0 || gr == S - 1 || gc == 0 || gc == S - 1;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int64_t j = bnd[r] ? row : row + (int64_t)(k / 3 - 1) * S + (k % 3 - 1);
+                av[r][k] = load_value_nt(ax + row + (int64_t)k * e.stride);
+                xv[r][k] = pv[j];
+            }
+            pr[r] = pv[row];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t row = (hb * R + r) * 256 + threadIdx.x;
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) t = (k < (bnd[r] ? 1 : 9)) ? t + av[r][k] * xv[r][k] : t;
+            double part = 0.0;
+            if (row < e.n) {
+                e.q[ls * e.vld + row] = t;
+                part = pr[r] * t;
+            }
+            const int64_t chunk = (hb * R + r) * (256 / kWave) + threadIdx.x / kWave;
+            const double v = wave_sum(part);
+            if (threadIdx.x % kWave == 0 && chunk < e.nps) e.partials[ls * e.pst + chunk] = v;
         }
     }
 }
@@ -870,7 +926,7 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
 
 // Slots per logical block of the row-sum launches, 256-row blocks per
 // logical block of the element-wise launches.
-constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4;
+constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4, kOuterSpmvBlocks = 2;
 
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s) {
@@ -878,13 +934,14 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
     if (max_blocks > kMaxLogicalBlocks) return hipErrorInvalidValue;
     constexpr int G = kOuterRowGroup, R = kOuterElemBlocks;
     const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)), ge(batch_grid((max_blocks + R - 1) / R, 256));
+    const dim3 gs(batch_grid((max_blocks + kOuterSpmvBlocks - 1) / kOuterSpmvBlocks, 256));
     switch (op) {
         case OP_DEFECT_ALL: hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_ALL, G, true>), gr, block, 0, s, tab, w, sc); break;
         case OP_DEFECT_SETTLE:
             hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false>), gr, block, 0, s, tab, w, sc);
             break;
         case OP_SPMV_DOT:
-            hipLaunchKernelGGL((k_b_outer_rows<OP_SPMV_DOT, G, false>), gr, block, 0, s, tab, w, sc);
+            hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks>), gs, block, 0, s, tab, w);
             break;
         case OP_SCALE: hipLaunchKernelGGL((k_b_outer_elem<OP_SCALE, R>), ge, block, 0, s, tab, w, sc); break;
         case OP_ACC: hipLaunchKernelGGL((k_b_outer_elem<OP_ACC, R>), ge, block, 0, s, tab, w, sc); break;
