@@ -157,11 +157,13 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
 // grid offset (k/3 - 1, k%3 - 1)), a boundary row its diagonal alone
 // (q1_row), T = 1 layout.  k_b_pair runs two passes that depend on each
 // other in one launch.  A work-group of kPairTileRows wavefronts owns a tile
-// of one slot's grid: kPairRows rows by `width` (<= 62) columns.  Wavefront
-// i forms the first pass for grid row r0 - 1 + i on the tile's columns plus
-// one halo column each side (a lane per column) into LDS; after one
-// barrier, wavefronts 1 .. kPairRows form the second pass for their row from
-// LDS, with the matrix row still in registers.  So the matrix is read from
+// of one slot's grid: kPairRows rows by `width` (<= 62) columns.  The
+// work-group first stages the first pass's gathered operand for the tile and
+// a two-node halo in LDS; after a barrier, wavefront i forms the first pass
+// for grid row r0 - 1 + i on the tile's columns plus one halo column each
+// side (a lane per column) from it into LDS; after a second barrier,
+// wavefronts 1 .. kPairRows form the second pass for their row from LDS,
+// with the matrix row still in registers.  So the matrix is read from
 // HBM once for both passes: halo rows and columns are read again by the
 // neighbouring tiles, through L2 (cached loads, not streaming ones).  Every
 // node's sum is the T = 1 sum of k_b_ell (same terms, same order, same
