@@ -198,7 +198,11 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork 
     for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
         int64_t e, rem, strip, seg;
         div32(lb, ups, e, rem);
-        div32(rem, pg.segs, strip, seg);
+        // strips minor: the tiles side by side in a band of rows are
+        // consecutive logical blocks, so they run together on one XCD and the
+        // cache lines their rows share (a row of a tile starts at an arbitrary
+        // column) are fetched once into that XCD's L2
+        div32(rem, pg.strips, seg, strip);
         const int64_t ls = w.list[a0 + e] - w.base;
         const int64_t vo = ls * m.vld;
         const VT *ax = (const VT *)m.Ax + ls * m.ax_ld;
