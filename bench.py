@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import honei_amd as H  # noqa: E402
+from tests import patterns  # noqa: E402
 from tests.patterns import batch_levels, poisson_system_seeds, uniform01, uniform_pm1  # noqa: E402
 
 METRIC = "Poisson solves/s + p99 latency at 1e-8 rel. residual; ELL SpMV % of HBM peak"
@@ -412,18 +413,9 @@ def fp64_single_solves(levels, kap, rhs, limit_s=60.0):
 
 
 def serving_requests(seconds, rate=SERVE_RATE, seed=SERVE_SEED):
-    """Poisson arrivals (exponential gaps -ln(1-U)/rate), side uniform over
-    2^k+1, k in SERVE_LEVELS; per-request seeds poisson_system_seeds(seed, i)."""
-    n_max = int(rate * seconds * 1.5) + 100
-    gaps = -np.log1p(-uniform01(seed, n_max)) / rate
-    t = np.cumsum(gaps)
-    n = int(np.searchsorted(t, seconds))
-    pick = np.minimum((uniform01((seed << 8) + 1, n) * len(SERVE_LEVELS)).astype(int), len(SERVE_LEVELS) - 1)
-    reqs = []
-    for i in range(n):
-        cs, bs = poisson_system_seeds(seed, i)
-        reqs.append((float(t[i]), SERVE_LEVELS[pick[i]], cs, bs))
-    return reqs
+    """The serving stream (tests/patterns.py: Poisson arrivals, side uniform
+    over 2^k+1 for k in SERVE_LEVELS, per-request seeds)."""
+    return patterns.serving_requests(seconds, rate, seed, SERVE_LEVELS)
 
 
 def batch_kernel_roofline(B, level):

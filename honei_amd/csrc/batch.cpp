@@ -43,15 +43,19 @@ Level0 level_geometry(int64_t level, int T, int block) {
 
 BatchMG::~BatchMG() { release(); }
 
+// step() and load_many() return before their work has run, so everything
+// the launched graphs and builds read (graph executables with their kernel
+// arguments, pool memory, staging buffers) is released only after both
+// streams have drained.
 void BatchMG::release() {
+    if (s_main_) (void)hipStreamSynchronize(s_main_);
+    if (s_load_) (void)hipStreamSynchronize(s_load_);
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (exec1_) (void)hipGraphExecDestroy(exec1_);
     if (graph1_) (void)hipGraphDestroy(graph1_);
     exec_ = exec1_ = nullptr;
     graph_ = graph1_ = nullptr;
-    if (s_main_) (void)hipStreamSynchronize(s_main_);
-    if (s_load_) (void)hipStreamSynchronize(s_load_);
     for (hipEvent_t e : load_ev_)
         if (e) (void)hipEventDestroy(e);
     load_ev_.clear();

@@ -109,6 +109,19 @@ template <int G, typename T> __device__ inline void block_sum_first(T (&v)[G]) {
     __syncthreads();
 }
 
+// A flag read once per work-group (thread 0, broadcast through LDS): every
+// wavefront of the work-group takes the same branch on it even while another
+// work-group of the same launch is writing the flag, so no wavefront leaves
+// early while the others go on to a barrier or a block-wide sum.
+__device__ inline int block_uniform_flag(const int *flag) {
+    __shared__ int f;
+    if (threadIdx.x == 0) f = *(const volatile int *)flag;
+    __syncthreads();
+    const int v = f;
+    __syncthreads();
+    return v;
+}
+
 // Cross-block reduction slots.  A reducing kernel stores one partial per
 // block with a plain store; a one-block finisher launched behind it sums the
 // partials in a fixed order.  (A last-block-arrives scheme would need an

@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void k_cg_update_f(V *x, const V *p, V *r, con
 template <typename V>
 __global__ __launch_bounds__(256) void k_cg_direction_f(V *p, const V *r, int64_t n, CGScalars *sc, ReduceSlots sl,
                                                         int64_t nparts, int it) {
-    if (sc->done) return;
+    // block 0 of this launch publishes sc->done: read it once per block
+    if (block_uniform_flag(&sc->done)) return;
     const double tot = sum_partials(sl.partials + kFusedOff, nparts, 1, 0);
     const V e = eps_of<V>();
     const V alpha_old = (V)sc->carry[it & 1];
@@ -340,7 +341,8 @@ __global__ __launch_bounds__(256) void k_pcg_update_f(V *x, const V *d, V *r, co
 template <typename V>
 __global__ __launch_bounds__(256) void k_pcg_direction_f(V *d, const V *r, const V *dinv, int64_t n, CGScalars *sc,
                                                          ReduceSlots sl, int64_t nparts, int it) {
-    if (sc->done) return;
+    // block 0 of this launch publishes sc->done: read it once per block
+    if (block_uniform_flag(&sc->done)) return;
     const double zr_t = sum_partials(sl.partials + kFusedOff, nparts, 2, 0);
     const double rr_t = sum_partials(sl.partials + kFusedOff, nparts, 2, 1);
     const V e = eps_of<V>();

@@ -438,6 +438,57 @@ def pcg_jacobi(A: Ell, b, dinv, x0=None, max_iters=10000, eps=1e-8, history=Fals
     return x, used.value
 
 
+def csr_arrays(A: Ell):
+    """Row pointers, columns and values of A's stored non-zeros, rows in
+    ascending column order: the entries of Ell.to_csr
+    (la/sparse_matrix.hh:157-185), vectorised for large matrices.  Slot
+    s = r*T + t + k*stride belongs to row (s mod stride) / T and counts while
+    k < Arl[r]; explicit zeros are dropped as SparseMatrix drops them."""
+    T, S = A.threads, A.stride
+    nch = len(A.Ax) // S if S else 0
+    lanes = (np.arange(A.rows, dtype=np.int64)[:, None] * T + np.arange(T, dtype=np.int64)[None, :]).reshape(-1)
+    rows = np.repeat(np.arange(A.rows, dtype=np.int64), T)
+    arl = A.Arl.astype(np.int64)[rows]
+    k = np.arange(nch, dtype=np.int64)[:, None]
+    slots = (lanes[None, :] + k * S)[k < arl[None, :]]
+    rr = np.broadcast_to(rows[None, :], (nch, rows.size))[k < arl[None, :]]
+    vals = A.Ax[slots]
+    keep = vals != 0
+    slots, rr, vals = slots[keep], rr[keep], vals[keep]
+    cols = A.Aj[slots].astype(np.int64)
+    order = np.lexsort((cols, rr))
+    rp = np.zeros(A.rows + 1, np.int64)
+    np.add.at(rp, rr + 1, 1)
+    return np.cumsum(rp), cols[order], vals[order]
+
+
+def direct_solve(A: Ell, b, tol=1e-14, max_refine=4):
+    """FP64 reference solution of A x = b (no reference counterpart): a
+    sparse LU factorisation (SuperLU 5 through scipy.sparse.linalg.splu, COLAMD
+    column ordering), then iterative refinement x += LU^-1 (b - A x) with the
+    residual formed by the oracle's Defect<tags::CPU> (math/defect.hh:449-504)
+    until ||b - A x|| <= tol ||b|| or max_refine steps.  Returns (x, relative
+    residual, refinement steps).  Pinned by closed forms (b = A x_true gives
+    back x_true), by agreement with CGSolver run to a tight tolerance and by
+    its own residual (tests/test_oracle.py)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    A64 = A if A.dtype == np.float64 else A.astype(np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    rp, cols, vals = csr_arrays(A64)
+    M = sp.csc_matrix(sp.csr_matrix((vals, cols, rp), shape=(A64.rows, A64.cols)))
+    lu = spl.splu(M, permc_spec="COLAMD")
+    x = lu.solve(b)
+    nb = float(np.linalg.norm(b)) or 1.0
+    rel, steps = float(np.linalg.norm(defect(A64, b, x))) / nb, 0
+    while rel > tol and steps < max_refine:
+        x = x + lu.solve(defect(A64, b, x))
+        rel = float(np.linalg.norm(defect(A64, b, x))) / nb
+        steps += 1
+    return x, rel, steps
+
+
 def cg_konv(A: Ell, b, konv=None):
     """ConjugateGradients<tags::CPU, NONE>::value(A, b, x, konv_rad)."""
     dt = A.dtype

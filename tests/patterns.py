@@ -108,3 +108,23 @@ def batch_levels(nsys: int, seed: int, kmin: int = 5, kmax: int = 9) -> np.ndarr
     log-uniform over the valid geometric-multigrid sides)."""
     u = uniform01(seed, nsys)
     return kmin + np.minimum((u * (kmax - kmin + 1)).astype(np.int64), kmax - kmin)
+
+
+SERVE_LEVELS = (6, 7, 8, 9, 10)   # sides 65 .. 1025
+
+
+def serving_requests(seconds: float, rate: float = 2000.0, seed: int = 5, levels=SERVE_LEVELS):
+    """Serving stream: Poisson arrivals (exponential gaps -ln(1-U)/rate) over
+    `seconds`, side 2^k+1 with k uniform over `levels`; request i carries
+    the seeds poisson_system_seeds(seed, i).  Tuples (arrival s, level,
+    coefficient seed, right-hand-side seed)."""
+    n_max = int(rate * seconds * 1.5) + 100
+    gaps = -np.log1p(-uniform01(seed, n_max)) / rate
+    t = np.cumsum(gaps)
+    n = int(np.searchsorted(t, seconds))
+    pick = np.minimum((uniform01((seed << 8) + 1, n) * len(levels)).astype(int), len(levels) - 1)
+    reqs = []
+    for i in range(n):
+        cs, bs = poisson_system_seeds(seed, i)
+        reqs.append((float(t[i]), int(levels[pick[i]]), cs, bs))
+    return reqs

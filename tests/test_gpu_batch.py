@@ -183,6 +183,22 @@ def test_paired_passes_equal_separate_launches(outer, inner):
             assert sep[key][0][2] == 0, key   # converged
 
 
+@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+def test_paired_passes_equal_separate_launches_513_and_1025(inner):
+    """The tile shapes of the headline batch's finest level (513^2: 9 column
+    strips of 57, 37 row bands) and of the serving stream's largest size
+    (1025^2: 17 strips of 61, 74 bands), one slot each: paired launches and
+    one launch per pass give the same iterations, defect and x."""
+    systems = {(0, 0): _system(10, 900), (1, 0): _system(9, 901)}
+    kw = dict(outer_method=1, inner_vtype=inner, autotune=0, iters_per_launch=4)
+    sep, _, _ = _run_batch([10, 9], [1, 1], systems, pair_sweeps=0, **kw)
+    par, _, _ = _run_batch([10, 9], [1, 1], systems, pair_sweeps=1, **kw)
+    for key in systems:
+        assert sep[key][0] == par[key][0], key
+        assert sep[key][0][2] == 0, key
+        assert np.array_equal(sep[key][1], par[key][1]), key
+
+
 def test_batch_slot_reuse_and_late_admission():
     """A slot reloaded after its first system retired, while another slot is
     still running, solves the new system exactly as a fresh batch does."""

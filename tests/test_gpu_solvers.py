@@ -107,9 +107,15 @@ def _problem(side, seed):
     return A, xt, oracle.product(A, xt)
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("side", [33, 129])
+# Q1 256^2 = 65,536 rows is the largest system whose CG iteration runs as
+# three kernels (at most 256 SpMV blocks of 256 rows); 257^2 and 513^2 take
+# the five-kernel iteration (one-block scalar kernels between the passes).
+@pytest.mark.parametrize("dtype,side", [(np.float64, 33), (np.float64, 129), (np.float32, 33), (np.float32, 129),
+                                        (np.float64, 256), (np.float64, 257), (np.float64, 513)])
 def test_cg_matches_oracle(dtype, side):
+    """CGSolver (math/cg.hh:63-133) on Q1 from x0 = 0: the oracle's iteration
+    count (+-1 FP64, +-3 FP32), the FP64 true residual below the tolerance
+    and the iterate within rounding of the oracle's."""
     A, xt, b = _problem(side, 1)
     A = A.astype(dtype)
     b = b.astype(dtype)

@@ -559,3 +559,52 @@ def test_banded_q1_vs_reference_and_ell(side, dtype):
         tol = 4 * np.finfo(dtype).eps * (np.abs(bands).sum(0) * np.abs(x).max() + 1e-30)
         assert np.all(np.abs(y - ye) <= tol)
         assert np.array_equal(r, (rhs - y).astype(dtype))
+
+
+# ---------------------------------------------------------------------------
+# FP64 direct solve (the error reference of the solver results)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("threads", [1, 4])
+def test_csr_arrays_equal_sparse_matrix_rows(golden, threads):
+    """The vectorised CSR extraction gives the rows SparseMatrix(ELL) gives
+    (la/sparse_matrix.hh:157-185): on the reference's 5pt_10x10 file and on
+    a threads-per-row layout with explicit zeros stored."""
+    F = oracle.read_ell(os.path.join(golden, "5pt_10x10.ell"))
+    kappa = np.exp(uniform_pm1(3, 64))
+    B = oracle.q1_coeff_matrix(9, kappa, threads=threads)
+    B.Ax[B.Aj == 40] = 0.0   # stored zeros are not entries
+    for A in (F, B):
+        rp, c, v = A.to_csr()
+        rp2, c2, v2 = oracle.csr_arrays(A)
+        assert np.array_equal(rp.astype(np.int64), rp2)
+        assert np.array_equal(c.astype(np.int64), c2)
+        assert np.array_equal(v, v2)
+
+
+@pytest.mark.parametrize("side", [17, 65])
+def test_direct_solve_recovers_known_solution(side):
+    """b = A x_true on the constant and the lognormal-coefficient Q1
+    operators: the direct solve gives x_true back to rounding, and its own
+    residual (the oracle's Defect) is at machine precision."""
+    from tests.patterns import lognormal_cells
+    for A in (oracle.q1_matrix(side), oracle.q1_coeff_matrix(side, lognormal_cells(side, (side - 1) ** 2))):
+        xt = uniform_pm1(side + 1, side * side)
+        b = oracle.product(A, xt)
+        x, rel, steps = oracle.direct_solve(A, b)
+        assert rel <= 1e-14 and steps <= 4
+        assert np.abs(x - xt).max() <= 1e-11
+
+
+def test_direct_solve_agrees_with_cg_solver():
+    """CGSolver (math/cg.hh:63-133) run to 1e-13 and the direct solve agree
+    to the accuracy the CG tolerance implies; a dropped or transposed term in
+    either would separate them far beyond it."""
+    A = oracle.q1_matrix(33)
+    b = uniform_pm1(44, A.rows)
+    xc, used = oracle.cg_solver(A, b, np.zeros_like(b), 10000, 1e-13)
+    xd, rel, _ = oracle.direct_solve(A, b)
+    assert rel <= 1e-14
+    assert np.abs(xc - xd).max() <= 1e-8 * np.abs(xd).max()   # measured 1.0e-9
+    # the transpose solves a different system (the Dirichlet rows make A non-symmetric)
+    xt_, _, _ = oracle.direct_solve(oracle.transpose(A), b)
+    assert np.abs(xt_ - xd).max() > 1e-3 * np.abs(xd).max()

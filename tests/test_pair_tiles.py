@@ -179,3 +179,64 @@ def test_pair_tiles_equal_two_passes(side, pair):
     assert np.array_equal(a0.view(np.uint32), b0.view(np.uint32))
     if a1 is not None:
         assert np.array_equal(a1.view(np.uint32), b1.view(np.uint32))
+
+
+# ---- logical-block enumeration of the launch (b_pair / k_b_pair) ----------
+KXCDS = 8                      # kXcds (hip_common.hh)
+KRESIDENT_THREADS = 256 * 2048  # kResidentThreads (batch_kernels.hip)
+
+
+def xcd_block(b, nblocks):
+    """hip_common.hh xcd_block: work-group b -> logical block."""
+    q, r = divmod(nblocks, KXCDS)
+    x, i = b % KXCDS, b // KXCDS
+    return x * q + min(x, r) + i
+
+
+def batch_grid(items, block):
+    cap = KRESIDENT_THREADS // block
+    return 1 if items < 1 else min(items, cap)
+
+
+def pair_geometry(S, W=KPAIR_WIDTH, R=KPAIR_TILE_ROWS - 2):
+    strips = (S + W - 1) // W
+    width = (S + strips - 1) // strips
+    segs = (S + R - 1) // R
+    return strips, width, segs
+
+
+@pytest.mark.parametrize("side,slots", [(33, 61), (65, 3), (257, 53), (513, 1), (513, 51), (1025, 1), (1025, 7)])
+def test_pair_launch_visits_every_tile_once(side, slots):
+    """The launch's grid of 1024-thread work-groups, each walking logical
+    blocks lb = xcd_block(b) + k * grid, with lb -> (listed slot e, seg,
+    strip) by the kernel's two divisions (strips minor): every tile of every
+    listed slot is visited exactly once, consecutive logical blocks are the
+    side-by-side tiles of one band of rows, and the tiles of a slot cover its
+    grid exactly once with widths <= 62."""
+    strips, width, segs = pair_geometry(side)
+    assert width <= KPAIR_WIDTH and (strips - 1) * width < side
+    R = KPAIR_TILE_ROWS - 2
+    ups = strips * segs
+    total = slots * ups
+    grid = batch_grid(total, KPAIR_TILE_ROWS * 64)
+    seen = np.zeros((slots, segs, strips), np.int32)
+    first_lb = {}
+    for b in range(grid):
+        lb = xcd_block(b, grid)
+        while lb < total:
+            e, rem = divmod(lb, ups)
+            seg, strip = divmod(rem, strips)
+            seen[e, seg, strip] += 1
+            first_lb[(e, seg, strip)] = lb
+            lb += grid
+    assert (seen == 1).all()
+    # strips minor: tile (seg, strip + 1) is the next logical block after (seg, strip)
+    for (e, seg, strip), lb in list(first_lb.items())[:2000]:
+        if strip + 1 < strips:
+            assert first_lb[(e, seg, strip + 1)] == lb + 1
+    cover = np.zeros((side, side), np.int32)
+    for seg in range(segs):
+        for strip in range(strips):
+            r0, c0 = seg * R, strip * width
+            cover[r0:min(side, r0 + R), c0:min(side, c0 + width)] += 1
+    assert (cover == 1).all()
