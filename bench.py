@@ -50,6 +50,7 @@ SERVE_RATE, SERVE_SEED = 2000.0, 5  # serving workload
 SERVE_FP64_SAMPLE = 250  # requests of the stream timed with the plain FP64 solve
 SERVE_LEVELS = [6, 7, 8, 9, 10]     # sides 65 .. 1025
 SERVE_BUDGET = 64 << 30
+ERR_SAMPLE = 2                      # systems per size whose error vs the FP64 direct solve is stated
 
 
 def log(*a):
@@ -98,43 +99,67 @@ def device_random_ell(n, vt, seed=2, max_len=27, band=4096, threads=1):
     return D
 
 
-def time_spmv(D, reps, x, y, label):
+def time_spmv(sets, reps, label, idx64=False):
     """Average duration of one ELL SpMV launch (HIP events on the launch
-    stream) and algorithmic bytes per launch: nnz*(s_v+4) + n*(s_x+s_y)."""
-    fn = {H.F32: "honei_hip_product_smell_dv_float", H.F64: "honei_hip_product_smell_dv_double"}[D.vtype]
-    args = (x.ptr, y.ptr, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, 0, D.rows, D.ncpr, D.stride, 0, D.threads, None)
-    for _ in range(3):
-        H.call(fn, *args)
+    stream) and algorithmic bytes per launch nnz*(s_v+s_i) + n*(s_x+s_y),
+    s_i = 4 (8 with idx64: the reference's unsigned long Aj read by the
+    *_idx64 entry).  `sets` holds (matrix, x, y) operand copies; the
+    launches alternate between them, so no launch finds its operands in the
+    256 MB MALL from the launch before."""
+    D0 = sets[0][0]
+    sfx = "float" if D0.vtype == H.F32 else "double"
+    fn = f"honei_hip_product_smell_dv_{sfx}" + ("_idx64" if idx64 else "")
+
+    def args(k):
+        D, x, y = sets[k % len(sets)]
+        Aj, Arl = (D.Aj64, D.Arl64) if idx64 else (D.Aj, D.Arl)
+        return (x.ptr, y.ptr, Aj.ptr, D.Ax.ptr, Arl.ptr, 0, D.rows, D.ncpr, D.stride, 0, D.threads, None)
+    for k in range(2 * len(sets)):
+        H.call(fn, *args(k))
     H.synchronize()
     e0, e1 = H.Event(), H.Event()
     e0.record()
-    for _ in range(reps):
-        H.call(fn, *args)
+    for k in range(reps):
+        H.call(fn, *args(k))
     e1.record()
     ms = e0.elapsed_ms(e1) / reps
-    sv = 8 if D.vtype == H.F64 else 4
-    alg = D.nnz * (sv + 4) + D.rows * 2 * sv
+    sv = 8 if D0.vtype == H.F64 else 4
+    nnz = sum(st[0].nnz for st in sets) / len(sets)   # the copies may differ in pattern
+    alg = nnz * (sv + (8 if idx64 else 4)) + D0.rows * 2 * sv
     gbs = alg / (ms * 1e-3) / 1e9
     log(f"[spmv] {label}: {ms * 1e3:.1f} us/launch, {alg / 1e6:.1f} MB, {gbs:.0f} GB/s = {100 * gbs / PEAK_GBS:.1f}%")
     return {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
-            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
+            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": len(sets)}
+
+
+def with_idx64(D):
+    """Attach the reference's 64-bit index arrays (Aj, Arl as unsigned long)
+    of a device matrix, widened on the host from its 32-bit copies."""
+    D.Aj64 = H.DeviceArray.from_host(D.Aj.get().view(np.uint32).astype(np.uint64))
+    D.Arl64 = H.DeviceArray.from_host(D.Arl.get().astype(np.uint64))
+    return D
 
 
 def hbm_ceiling(reps=20):
-    """Measured ceilings over 2^28 floats (1 GiB), HIP events on the launch
-    stream: a device-to-device copy and the in-place scale kernel (bytes =
-    read + write), and the squared-norm reduction (read only)."""
-    nf = 1 << 28
-    a = H.DeviceArray(nf, np.float32)
-    b = H.DeviceArray(nf, np.float32)
-    r = H.DeviceArray(1, np.float64)
+    """Measured streaming ceilings over 1 GiB arrays, HIP events on the
+    launch stream: the probe kernels (16-byte non-temporal loads, eight in
+    flight per thread, honei_hip_stream_probe) reading (bytes = read) and
+    copying (bytes = read + write), at two grid sizes, beside hipMemcpy
+    device-to-device."""
+    nbytes = 1 << 30
+    a = H.DeviceArray(nbytes // 4, np.float32)
+    b = H.DeviceArray(nbytes // 4, np.float32)
+    part = H.DeviceArray(65536, np.float32)
     a.zero()
     b.zero()
     out = {}
-    for label, fn, nbytes in (
-            ("d2d_copy", lambda: H.call("honei_hip_copy", b.ptr, a.ptr, nf * 4, None), 2 * nf * 4),
-            ("scale_kernel", lambda: H.call("honei_hip_scale_one", H.F32, a.ptr, 1.0, nf, None), 2 * nf * 4),
-            ("read_norm_kernel", lambda: H.call("honei_hip_norm_l2_dev", H.F32, r.ptr, a.ptr, nf, None), nf * 4)):
+    cases = [("d2d_copy_hipmemcpy", lambda: H.call("honei_hip_copy", b.ptr, a.ptr, nbytes, None), 2 * nbytes)]
+    for grid in (2048, 8192):
+        cases.append((f"probe_read_grid{grid}",
+                      lambda g=grid: H.call("honei_hip_stream_probe", 0, part.ptr, a.ptr, nbytes, g, None), nbytes))
+        cases.append((f"probe_copy_grid{grid}",
+                      lambda g=grid: H.call("honei_hip_stream_probe", 1, b.ptr, a.ptr, nbytes, g, None), 2 * nbytes))
+    for label, fn, nb in cases:
         for _ in range(3):
             fn()
         H.synchronize()
@@ -144,10 +169,12 @@ def hbm_ceiling(reps=20):
             fn()
         e1.record()
         ms = e0.elapsed_ms(e1) / reps
-        gbs = nbytes / (ms * 1e-3) / 1e9
+        gbs = nb / (ms * 1e-3) / 1e9
         out[label] = {"us": round(ms * 1e3, 1), "gbs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2)}
         log(f"[hbm] {label}: {gbs:.0f} GB/s")
-    del a, b, r
+    out["read_ceiling_gbs"] = max(v["gbs"] for k, v in out.items() if k.startswith("probe_read"))
+    out["copy_ceiling_gbs"] = max(v["gbs"] for k, v in out.items() if k.startswith("probe_copy"))
+    del a, b, part
     return out
 
 
@@ -418,6 +445,281 @@ def serving_requests(seconds, rate=SERVE_RATE, seed=SERVE_SEED):
     return patterns.serving_requests(seconds, rate, seed, SERVE_LEVELS)
 
 
+def q1_nnz(side):
+    """Stored entries of the Q1 operator: 9 per interior row, 1 per Dirichlet row."""
+    return 9 * (side - 2) ** 2 + 4 * (side - 1)
+
+
+def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5):
+    """Algorithmic bytes of every launch one slot of finest level L takes
+    part in during a batched step with `iters` outer (CG) iterations, each
+    launch counted by its own compulsory traffic (SURVEY-style: values and
+    vectors once per launch, index streams only where a launch reads one):
+
+      build    : coefficient read from its source and written, every coarser
+                 coefficient from four finer cells, every level's values
+                 (vs bytes each) and dinv, the FP64 operator, b copied, x0;
+      settle   : r = b - A x in FP64 at the start and at the verification
+                 (nnz*8 + 24n each; neighbours from the grid, no indices);
+      per iteration, on the finest level n = n_L:
+                 scale r/|r| -> FP32 (12n), r.z (12n), p = z + beta p (20n),
+                 q = A p with p.q (nnz*8 + 16n), x, r update with |r|^2 (48n);
+      per iteration, per level l (n_l, nnz_l):
+                 l >= pair_from: paired Jacobi0 + defect and paired two
+                 sweeps (nnz_l*vs + 16 n_l each), restriction
+                 (4 n_l + 4 n_{l-1}), prolongation + correction
+                 (8 n_l + 4 n_{l-1});
+                 2 <= l <= fused: the fused coarse launch's passes, each
+                 with its index stream: Jacobi0, one Jacobi sweep, the
+                 defect, two post sweeps (nnz_l*(vs+4) + 16 n_l, the defect
+                 12 n_l), restriction and prolongation as above;
+                 level 1 (3x3, coarse CG): not counted."""
+    def n_(l):
+        return (2 ** l + 1) ** 2
+
+    def nnz_(l):
+        return q1_nnz(2 ** l + 1)
+
+    def cells(l):
+        return 4 ** l
+
+    n, nnz = n_(L), nnz_(L)
+    build = 16 * cells(L) + 24 * n + nnz * 8
+    for l in range(1, L + 1):
+        build += 8 * cells(l) + nnz_(l) * vs + 4 * n_(l)
+        if l < L:
+            build += 8 * cells(l + 1) + 8 * cells(l)
+    settle = 2 * (nnz * 8 + 24 * n)
+    it = 12 * n + 12 * n + 20 * n + (nnz * 8 + 16 * n) + 48 * n
+    for l in range(2, L + 1):
+        nl, zl, nc = n_(l), nnz_(l), n_(l - 1)
+        transfers = (4 * nl + 4 * nc) + (8 * nl + 4 * nc)
+        if l >= pair_from:
+            it += 2 * (zl * vs + 16 * nl) + transfers
+        elif l <= fused:
+            sweep = zl * (vs + 4) + 16 * nl
+            it += 4 * sweep + (zl * (vs + 4) + 12 * nl) + transfers
+    return build + settle + iters * it
+
+
+def step_roofline(levels, iters, seconds, vs=4):
+    """Step-level roofline: the algorithmic bytes of every launch of one
+    batched step (slot_solve_bytes summed over the systems with their own
+    iteration counts) over the step's wall time, against 8 TB/s."""
+    total = sum(slot_solve_bytes(L, int(k), vs) for L, k in zip(levels, iters))
+    gbs = total / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_GBS, 4), "bytes_per_step": int(total),
+            "seconds_per_step": round(seconds, 6),
+            "what": ("algorithmic bytes of every launch of the step (hierarchy build, FP64 outer CG ops, every "
+                     "V-cycle level's passes and transfers; per system from its outer iteration count) over "
+                     "the step's wall time, against 8 TB/s")}
+
+
+def device_fields(L, cs, bs):
+    """Host copies of the coefficient and right-hand side the device draws
+    from a system's seeds (the fields the batch and the serving loop solve)."""
+    side = 2 ** L + 1
+    k = H.DeviceArray((side - 1) ** 2, np.float64)
+    b = H.DeviceArray(side * side, np.float64)
+    H.fill_lognormal(k, 1.0, cs)
+    H.fill_uniform(b, -1.0, 1.0, bs)
+    H.synchronize()
+    return k.get(), b.get()
+
+
+def sample_batch_results(B, levels, slots, info, per_level=ERR_SAMPLE):
+    """(level, coefficient, b, x, device relative residual) of the first
+    `per_level` systems of each size in workload order, for the FP64
+    direct-solve error of the CPU leg."""
+    out, seen = [], {}
+    for i, (L, g) in enumerate(zip(levels, slots)):
+        if seen.get(L, 0) >= per_level:
+            continue
+        seen[L] = seen.get(L, 0) + 1
+        x = H.DeviceArray((2 ** L + 1) ** 2, np.float64)
+        B.result(g, x)
+        kappa, b = device_fields(L, *poisson_system_seeds(BATCH_SEED, i))
+        out.append((L, kappa, b, x.get(), float(info[g].rel)))
+    return out
+
+
+def level_launch_rates(B, levels, level, reps=20):
+    """The step's own launch shape at full activity: honei_hip_batch_time_level
+    over every slot present on `level` (all systems of that size and larger),
+    and the FP64 q = A p over every slot; bytes by the rules of
+    slot_solve_bytes.  Overwrites the pool's vectors (after the results are
+    taken)."""
+    pre, rst, pro, post, spmv = B.time_level(level, reps)
+    side = 2 ** level + 1
+    n, nnz, nc = side * side, q1_nnz(side), (2 ** (level - 1) + 1) ** 2
+    cap = sum(1 for L in levels if L >= level)
+    rows = {"paired_pre": (pre, cap * (nnz * 4 + 16 * n), "k_b_pair<float,PRE>: Jacobi0 then the defect"),
+            "restriction": (rst, cap * (4 * n + 4 * nc), "k_b_restrict"),
+            "prolongation": (pro, cap * (8 * n + 4 * nc), "k_b_prolong_axpy: t = x + P e"),
+            "paired_post": (post, cap * (nnz * 4 + 16 * n), "k_b_pair<float,POST>: two Jacobi sweeps")}
+    out = {"level": level, "side": side, "slots": cap}
+    for key, (us, nb, what) in rows.items():
+        if us > 0:
+            g = nb / (us * 1e-6) / 1e9
+            out[key] = {"kernel": what, "us": round(us, 2), "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
+                        "frac": round(g / PEAK_GBS, 4)}
+    if spmv > 0:
+        nb = sum(q1_nnz(2 ** L + 1) * 8 + 16 * (2 ** L + 1) ** 2 for L in levels)
+        g = nb / (spmv * 1e-6) / 1e9
+        out["outer_spmv_dot"] = {"kernel": "k_b_outer_spmv_dot: FP64 q = A p, p.q, every slot", "us": round(spmv, 2),
+                                 "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
+                                 "frac": round(g / PEAK_GBS, 4)}
+    out["what"] = ("launches of the batched step's shape over every slot of the level (full activity), timed "
+                   "back to back with HIP events on the pool's stream after the timed steps")
+    log("[batch] level %d launches over %d slots: " % (level, cap)
+        + ", ".join(f"{k} {v['us']} us ({v['frac']:.2f})" for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
+def serving_capacity(seconds, rates=(2000.0, 3000.0, 4000.0, 6000.0, 9000.0)):
+    """Open-loop serving streams of increasing arrival rate (same size mix,
+    seeds and 64 GB budget as the serving run, `seconds` long each):
+    throughput and p50/p99 per rate; the knee is the first rate whose
+    throughput falls below 95 % of the offered rate, the capacity the
+    largest throughput measured."""
+    rows = []
+    for rate in rates:
+        reqs = serving_requests(seconds, rate=rate, seed=SERVE_SEED)
+        _res, st = H.serve(SERVE_LEVELS, reqs, budget_bytes=SERVE_BUDGET)
+        rows.append({"arrival_rate": rate, "requests": len(reqs), "solves_per_s": round(st.solves_per_s, 1),
+                     "p50_ms": round(st.p50_ms, 2), "p99_ms": round(st.p99_ms, 2), "max_queue": st.max_queue,
+                     "not_converged": st.not_converged})
+        log(f"[capacity] {rate:.0f} req/s: {st.solves_per_s:.0f} solves/s, p50 {st.p50_ms:.1f} ms, "
+            f"p99 {st.p99_ms:.1f} ms, max queue {st.max_queue}")
+        if st.solves_per_s < 0.8 * rate:
+            break
+    knee = next((r["arrival_rate"] for r in rows if r["solves_per_s"] < 0.95 * r["arrival_rate"]), None)
+    return {"streams": rows, "knee_arrival_rate": knee,
+            "capacity_solves_per_s": max(r["solves_per_s"] for r in rows), "seconds_per_stream": seconds,
+            "what": "open-loop Poisson streams at increasing rates; wall time includes draining the queue"}
+
+
+# ---- CPU leg: the reference's own CPU path and the FP64 oracle --------------------------
+def cpu_workers():
+    """Host processes of the all-cores CPU leg: the CPUs this process may run
+    on, at most 16 (the host share of a one-GPU job on the MI355X boxes)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, aff))
+
+
+def _reference_cpu_job(q):
+    try:
+        q.put(reference_cpu_path())
+    except Exception as e:   # reported in the JSON line, never fatal to the bench
+        q.put({"error": repr(e)})
+
+
+def start_reference_cpu_path():
+    """reference_cpu_path in a forked host process (started before the GPU is
+    touched; it runs on one core beside the GPU work)."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    p = ctx.Process(target=_reference_cpu_job, args=(q,))
+    p.start()
+    return p, q
+
+
+def host_cpu_info():
+    """Host CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff}
+
+
+def reference_cpu_path():
+    """The reference's own CPU solvers (oracle/_ref: HONEI's tags::CPU code
+    built from its sources), one host core: CGSolver<CPU, NONE> on the Q1
+    129^2 FP64 system (b = A x_true, seed 1, x0 = 0, to 1e-8) and
+    Multigrid<CPU, CPU, NONE, JAC, V, MIXED> on Q1 2049^2 (b = A x_true,
+    seed 3, 11 grids, V(2,2), one V-cycle per FP64 outer step, to 1e-10),
+    median wall time of the solve call.  None where oracle/_ref is absent."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    import oracle
+    out = {"kind": "reference", "cores": 1}
+    A = ref.q1_ell(129)
+    xt = uniform_pm1(1, 129 * 129)
+    b = oracle.product(oracle.q1_matrix(129), xt)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        x, used = ref.cg_solver(A, b, np.zeros_like(b), 10000, 1e-8)
+        ts.append(time.perf_counter() - t0)
+    out["cg_q1_129_fp64"] = {"ms": round(1e3 * float(np.median(ts)), 3), "iterations": int(used),
+                             "max_err_vs_x_true": float(np.abs(x - xt).max()),
+                             "what": "CGSolver<tags::CPU, NONE> of the reference build"}
+    log(f"[cpu ref] CG 129^2: {out['cg_q1_129_fp64']['ms']} ms, {used} iterations")
+    xt = uniform_pm1(3, SIDE * SIDE)
+    b = oracle.product(oracle.q1_matrix(SIDE), xt)
+    ts, cyc = [], 0
+    for _ in range(2):
+        x, cyc, _inner, secs = ref.mg_mixed(LEVEL, b, np.zeros_like(b), n_max_iter=1, n_pre=2, tol=1e-10, timed=True)
+        ts.append(secs)
+    out["mixed_mg_q1_2049"] = {"ms": round(1e3 * float(np.median(ts)), 1), "outer_iterations": int(cyc),
+                               "max_err_vs_x_true": float(np.abs(x - xt).max()),
+                               "what": ("Multigrid<tags::CPU, tags::CPU, NONE, JAC, V, MIXED> of the reference build, "
+                                        "one V(2,2) cycle per outer step, to 1e-10; set-up not timed")}
+    log(f"[cpu ref] MIXED MG 2049^2: {out['mixed_mg_q1_2049']['ms']} ms, {cyc} outer iterations")
+    return out
+
+
+def fp64_oracle_errors(samples, workers=16):
+    """Final error of GPU results against the oracle's FP64 direct solve
+    (sparse LU plus refinement to 1e-14, oracle.direct_solve) of the same
+    systems, in fresh host processes (no GPU state): per group, the largest
+    max|x - x64| / max|x64| and the oracle's own FP64 relative residual of
+    the GPU solution."""
+    import multiprocessing as mp
+    jobs = [(g, L, k, b, x, rel) for g, rows in samples.items() for (L, k, b, x, rel) in rows]
+    if not jobs:
+        return {}
+    with mp.get_context("spawn").Pool(max(1, min(workers, len(jobs)))) as pool:
+        res = pool.map(_direct_error_job, jobs)
+    out = {}
+    for (g, L, _k, _b, _x, rel), (err, orel) in zip(jobs, res):
+        o = out.setdefault(g, {"max_err_vs_fp64_oracle": 0.0, "max_oracle_rel_residual": 0.0, "per_side": {}})
+        o["max_err_vs_fp64_oracle"] = max(o["max_err_vs_fp64_oracle"], err)
+        o["max_oracle_rel_residual"] = max(o["max_oracle_rel_residual"], orel)
+        side = str(2 ** L + 1)
+        o["per_side"][side] = max(o["per_side"].get(side, 0.0), err)
+    for g, o in out.items():
+        o["systems"] = sum(1 for j in jobs if j[0] == g)
+        log(f"[cpu fp64] {g}: max error vs FP64 direct solve {o['max_err_vs_fp64_oracle']:.2e} over "
+            f"{o['systems']} systems, oracle residual <= {o['max_oracle_rel_residual']:.2e}")
+    return out
+
+
+def _direct_error_job(job):
+    import oracle
+    _g, L, kappa, b, x, _rel = job
+    side = 2 ** L + 1
+    A = oracle.q1_coeff_matrix(side, kappa)
+    x64, _r, _s = oracle.direct_solve(A, b)
+    guess = np.zeros_like(b)
+    bd = np.zeros(side * side, bool)
+    bd[:side] = bd[-side:] = True
+    bd[::side] = True
+    guess[bd] = b[bd]
+    r0 = np.linalg.norm(oracle.defect(A, b, guess))
+    orel = float(np.linalg.norm(oracle.defect(A, b, x)) / r0)
+    return float(np.abs(x - x64).max() / np.abs(x64).max()), orel
+
+
 def batch_kernel_roofline(B, level):
     """Bandwidth of the batched Jacobi sweep on `level` from the autotuner's
     own timing (tuning batch of `slots` systems sharing one index array)."""
@@ -558,6 +860,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson arrival stream of the serving run (0: skip)")
+    ap.add_argument("--capacity-seconds", type=float, default=6.0,
+                    help="length of each stream of the serving capacity sweep (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the single-system FP64 comparison")
     ap.add_argument("--batch-only", action="store_true", help="only the batched step (for profiling)")
@@ -579,12 +883,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    # the all-cores CPU leg forks worker processes: run it before the GPU is touched
-    cpu_all = None
-    if not (args.no_cpu or args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe or args.pcg_only):
+    # the CPU legs fork host processes: start them before the GPU is touched
+    cpu_all, ref_proc = None, None
+    full = not (args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe or args.pcg_only)
+    if full and not args.no_cpu:
         os.environ["OMP_NUM_THREADS"] = "1"
-        cpu_all = cpu_baseline_all_cores([int(v) for v in batch_levels(NSYS, BATCH_SEED)],
-                                         max(1, min(16, os.cpu_count() or 1)))
+        cpu_all = cpu_baseline_all_cores([int(v) for v in batch_levels(NSYS, BATCH_SEED)], cpu_workers())
+        cpu_all["host"] = host_cpu_info()
+        ref_proc = start_reference_cpu_path()   # one core, runs beside the GPU work
     H.init(0)
     n = SIDE * SIDE
     if args.pmc_probe:
@@ -631,29 +937,29 @@ def main():
 
     # ---- standalone SpMV (roofline kernel: Q1 2049^2, FP32, T=1) ----------
     spmv = {}
-    D32 = device_q1(SIDE, H.F32)
-    x32 = H.DeviceArray.from_host(uniform_pm1(2, n).astype(np.float32))
-    y32 = H.DeviceArray(n, np.float32)
-    spmv["q1_2049_f32"] = time_spmv(D32, 50, x32, y32, "q1 2049^2 f32")
-    if args.spmv_only:
-        print(json.dumps({"spmv": spmv}))
-        return
-    del D32, x32, y32
-    D64 = device_q1(SIDE, H.F64)
-    x64 = H.DeviceArray.from_host(uniform_pm1(2, n))
-    y64 = H.DeviceArray(n, np.float64)
-    spmv["q1_2049_f64"] = time_spmv(D64, 50, x64, y64, "q1 2049^2 f64")
-    del D64, x64, y64
+    for vt, dt, key in ((H.F32, np.float32, "q1_2049_f32"), (H.F64, np.float64, "q1_2049_f64")):
+        sets = [(with_idx64(device_q1(SIDE, vt)), H.DeviceArray.from_host(uniform_pm1(2 + c, n).astype(dt)),
+                 H.DeviceArray(n, dt)) for c in range(2)]
+        spmv[key] = time_spmv(sets, 50, f"q1 2049^2 {key[-3:]}")
+        spmv[key + "_idx64"] = time_spmv(sets, 50, f"q1 2049^2 {key[-3:]}, 64-bit indices", idx64=True)
+        spmv[key + "_idx64"]["what"] = ("honei_hip_product_smell_dv_*_idx64 on the reference's unsigned long Aj/Arl; "
+                                        "bytes with 8-byte indices")
+        del sets
+        if args.spmv_only and vt == H.F32:
+            print(json.dumps({"spmv": spmv}))
+            return
     spmv["q1_2049_banded_f32"] = time_banded_q1(SIDE, np.float32)
     spmv["q1_2049_banded_f64"] = time_banded_q1(SIDE, np.float64)
     for vt, dt, key in ((H.F32, np.float32, "random_4m_f32"), (H.F64, np.float64, "random_4m_f64")):
-        R = device_random_ell(1 << 22, vt)
-        xr = H.DeviceArray.from_host(uniform_pm1(2, R.rows).astype(dt))
-        yr = H.DeviceArray(R.rows, dt)
-        spmv[key] = time_spmv(R, 30, xr, yr, key)
-        spmv[key]["nnz"] = R.nnz
+        sets = []
+        for c in range(2):
+            R = device_random_ell(1 << 22, vt, seed=2 + c)
+            sets.append((R, H.DeviceArray.from_host(uniform_pm1(2 + c, R.rows).astype(dt)), H.DeviceArray(R.rows, dt)))
+        spmv[key] = time_spmv(sets, 30, key)
+        spmv[key]["nnz"] = sets[0][0].nnz
+        R, xr, yr = sets[0]
         spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
-        del R, xr, yr
+        del sets, R, xr, yr
     spmv["hbm_ceiling"] = hbm_ceiling()
     cg129 = cg_q1_129(not args.no_cpu)
     pcg2049 = pcg_iteration_rate()
@@ -685,6 +991,10 @@ def main():
         f"{launches / args.steps:.1f} graph launches per step")
     value = len(levels) * args.steps / total
     lat_ms = np.array(lat) * 1e3
+    step_rl = step_roofline(levels, iters, total / args.steps)
+    log(f"[batch] step roofline: {step_rl['bytes_per_step'] / 1e9:.1f} GB of launches per step, "
+        f"{step_rl['achieved']:.0f} GB/s = {step_rl['frac']:.3f} of peak")
+    err_samples = {"batch": sample_batch_results(B, levels, slots, info)}
 
     # ---- plain single-system FP64 GPU solve of the same systems -------------------
     fp64 = None
@@ -697,6 +1007,7 @@ def main():
                          "V(2,2) cycle, one system per call, host-driven, same tolerance")}
         log(f"[fp64] {n64} single-system FP64 solves in {t64:.3f} s ({n64 / t64:.1f} solves/s)")
     batch_kernel = batch_kernel_roofline(B, max(levels))
+    batch_kernel["step_shape"] = level_launch_rates(B, levels, max(levels))
     del B
     # the same step with BF16-stored V-cycle matrices (FP32 arithmetic)
     Bb, slots_b = batched_pool(levels, inner_vtype=H.BF16)
@@ -706,6 +1017,7 @@ def main():
         sec, _done, _nl, info_b = batched_step(Bb, slots_b, kap, rhs)
         tb.append(sec)
     it_b = np.array([info_b[g].iters for g in slots_b])
+    err_samples["bf16_inner"] = sample_batch_results(Bb, levels, slots_b, info_b)
     bf16 = {"solves_per_s": round(len(levels) / float(np.median(tb)), 2),
             "outer_iterations_median": int(np.median(it_b)), "outer_iterations_max": int(it_b.max()),
             "max_rel_residual": float(max(info_b[g].rel for g in slots_b)),
@@ -719,7 +1031,18 @@ def main():
         reqs = serving_requests(args.serve_seconds)
         log(f"[serve] {len(reqs)} requests over {args.serve_seconds:.0f} s, levels {SERVE_LEVELS}, "
             f"budget {SERVE_BUDGET >> 30} GiB")
-        res, st = H.serve(SERVE_LEVELS, reqs, budget_bytes=SERVE_BUDGET)
+        # the first ERR_SAMPLE requests of each size return their solution
+        picked, seen = [], {}
+        for i, q in enumerate(reqs):
+            if seen.get(q[1], 0) < ERR_SAMPLE:
+                seen[q[1]] = seen.get(q[1], 0) + 1
+                picked.append(i)
+        xs = {i: H.DeviceArray((2 ** reqs[i][1] + 1) ** 2, np.float64) for i in picked}
+        res, st = H.serve(SERVE_LEVELS, [q + (xs[i],) if i in xs else q for i, q in enumerate(reqs)],
+                          budget_bytes=SERVE_BUDGET)
+        err_samples["serving"] = [(reqs[i][1], *device_fields(reqs[i][1], reqs[i][2], reqs[i][3]), xs[i].get(),
+                                   float(res[i].rel)) for i in picked]
+        del xs
         serving = {"requests": len(reqs), "completed": st.completed, "not_converged": st.not_converged,
                    "rejected": st.rejected, "wall_s": round(st.wall_s, 3),
                    "solves_per_s": round(st.solves_per_s, 2),
@@ -751,6 +1074,8 @@ def main():
                 "what": "the first requests of the stream solved one at a time by honei_hip_mg_solve_pcg in FP64"}
             serving["speedup_vs_fp64_single_gpu"] = round(st.solves_per_s / (nd / tot), 2)
             log(f"[serve] FP64 single-system GPU solve on {nd} of the requests: {nd / tot:.1f}/s")
+        if args.capacity_seconds > 0:
+            serving["capacity"] = serving_capacity(args.capacity_seconds)
 
     # ---- single large system (2049^2, constant coefficient) -----------------------------
     xt = uniform_pm1(3, n)
@@ -868,6 +1193,7 @@ def main():
                                                      "assembly from the coefficient)"}},
         "spmv": spmv,
         "batch_kernel": batch_kernel,
+        "step_roofline": step_rl,
     }
     r = spmv["q1_2049_f32"]
     out["roofline"] = {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
@@ -880,6 +1206,20 @@ def main():
         one = cpu_baseline_batched(levels)
         out["cpu_baseline"] = cpu_all
         out["cpu_baseline_one_core"] = one
+        # final error of the GPU results against the oracle's FP64 direct solve
+        errs = fp64_oracle_errors(err_samples, cpu_workers())
+        for key, obj in (("batch", out["solve"]), ("bf16_inner", bf16), ("serving", serving)):
+            if obj is not None and key in errs:
+                obj["max_err_vs_fp64_oracle"] = errs[key]["max_err_vs_fp64_oracle"]
+                obj["fp64_oracle_check"] = errs[key]
+        out["fp64_oracle_check_what"] = (
+            f"first {ERR_SAMPLE} systems of each size: GPU solution vs the oracle's FP64 direct solve of the same "
+            "system (sparse LU + refinement to 1e-14 relative residual), max |x - x64| / max |x64|, and the "
+            "oracle's FP64 relative residual of the GPU solution")
+        if ref_proc is not None:
+            p, q = ref_proc
+            out["cpu_reference_path"] = q.get()
+            p.join()
     print(json.dumps(out))
 
 

@@ -54,6 +54,12 @@ _SIGS = {
     "honei_hip_product_smell_dv_bf16": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
     "honei_hip_defect_smell_dv_float": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
     "honei_hip_defect_smell_dv_double": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_product_smell_dv_float_idx64": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_product_smell_dv_double_idx64": [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_defect_smell_dv_float_idx64": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_defect_smell_dv_double_idx64": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_narrow_indices": [_p, _p, _sz, _u64, _p],
+    "honei_hip_stream_probe": [_i, _p, _p, _sz, _sz, _p],
     "honei_hip_defect_norm_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_jacobi_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_dot_product_two_float": [C.POINTER(C.c_float), _p, _p, _sz, _p],
@@ -104,6 +110,7 @@ _SIGS = {
     "honei_hip_batch_info": [_p, C.POINTER(C.c_int64), C.POINTER(_u64), C.POINTER(C.c_int64)],
     "honei_hip_batch_slot": [_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64)],
     "honei_hip_batch_pair_timing": [_p, C.c_int32, C.POINTER(_d), C.POINTER(_d)],
+    "honei_hip_batch_time_level": [_p, C.c_int32, C.c_int32, C.POINTER(_d)],
     "honei_hip_batch_load": [_p, C.c_int64, _p, _p],
     "honei_hip_batch_load_many": [_p, C.c_int64, _p, _p, _p],
     "honei_hip_batch_load_seeded": [_p, C.c_int64, _p, _p, _p, _d],
@@ -558,6 +565,14 @@ class Batch:
         a, b = _d(), _d()
         call("honei_hip_batch_pair_timing", self.h, level, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def time_level(self, level, reps=20):
+        """(paired pre, restriction, prolongation, paired post, FP64 q = A p)
+        microseconds per launch over every slot present on `level`
+        (honei_hip_batch_time_level; overwrites the pool's vectors)."""
+        us = (_d * 5)()
+        call("honei_hip_batch_time_level", self.h, level, reps, us)
+        return tuple(us)
 
     def launches(self):
         k = C.c_int64()

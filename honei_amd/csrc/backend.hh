@@ -26,6 +26,10 @@ struct EllView {
     const void *Ax;
     const uint32_t *Arl;
     ValueType vtype;
+    // 32-bit words per stored index of Aj and Arl: 1 (int32 / uint32 arrays,
+    // every kernel) or 2 (the reference's 64-bit arrays; ell_apply's product
+    // and defect only)
+    int index_words = 1;
 };
 
 // What the ELL row kernel does with the row sum s = sum_j A_rj x_j.
@@ -91,6 +95,14 @@ hipError_t vcopy(void *dst, const void *src, int64_t n, ValueType vt, hipStream_
 hipError_t convert_d2f(float *dst, const double *src, int64_t n, hipStream_t s);
 hipError_t convert_f2d(double *dst, const float *src, int64_t n, hipStream_t s);
 hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
+// Bandwidth probe over n4 16-byte elements of src on `grid` 256-thread
+// work-groups: kind 0 reads (one float per work-group into dst), kind 1
+// copies into dst.
+hipError_t stream_probe(int kind, void *dst, const void *src, int64_t n4, int grid, hipStream_t s);
+// d = low 32 bits of the 64-bit indices src; *bad += entries >= bound
+// (*bad is not cleared here).
+hipError_t narrow_indices(uint32_t *d, const uint64_t *src, int64_t n, uint64_t bound, unsigned int *bad,
+                          hipStream_t s);
 
 // Banded Q1 product y = A x (rhs null) or defect y = rhs - A x; bands[9] =
 // LL, LD, LU, DL, DD, DU, UL, UD, UU device arrays of n = m*m values.

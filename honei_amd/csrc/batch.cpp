@@ -719,6 +719,57 @@ hipError_t BatchMG::result(int64_t g, double *dst) {
     return hipErrorInvalidValue;
 }
 
+hipError_t BatchMG::time_level(int64_t level, int reps, double us[5]) {
+    if (level < 2 || level > max_level_ || reps < 1 || !us) return hipErrorInvalidValue;
+    HX(hipStreamSynchronize(s_load_));
+    HX(hipStreamSynchronize(s_main_));
+    for (int k = 0; k < 5; ++k) us[k] = 0.0;
+    // every slot listed; bounds[k] = slots below class boundary k
+    std::vector<int32_t> hl((size_t)nslots_), hb((size_t)kMaxClasses + 1, 0);
+    for (int64_t g = 0; g < nslots_; ++g) hl[(size_t)g] = (int32_t)g;
+    for (size_t k = 0; k < boundaries_.size(); ++k) hb[k] = (int32_t)boundaries_[k];
+    int32_t *dl = nullptr, *db = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipMalloc((void **)&dl, hl.size() * sizeof(int32_t));
+    if (err == hipSuccess) err = hipMalloc((void **)&db, hb.size() * sizeof(int32_t));
+    if (err == hipSuccess) err = hipMemcpy(dl, hl.data(), hl.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(db, hb.data(), hb.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    BLevel &lv = L_[(size_t)level], &lc = L_[(size_t)level - 1];
+    int32_t hi = 0;
+    while (hi < (int32_t)C_.size() && C_[(size_t)hi].level >= level) ++hi;
+    const BWork w{dl, db, 0, hi, 0}, wa{dl, db, 0, (int32_t)C_.size(), 0};
+    const ValueType vt = (ValueType)cfg_.inner_vtype;
+    auto timeit = [&](auto launch, double *out) {
+        if (err != hipSuccess) return;
+        err = launch();
+        if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
+        for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+        if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess) *out = 1000.0 * ms / reps;
+    };
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_); }, &us[0]);
+    timeit([&]() { return b_restrict(lv, lc, w, s_main_); }, &us[1]);
+    timeit([&]() { return b_prolong_axpy(lv, lc, w, lv.t, s_main_); }, &us[2]);
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_); }, &us[3]);
+    if (cfg_.outer == 1) {
+        const BClassTab tab = class_table();
+        int64_t blocks = 0;
+        for (const BClass &c : C_) blocks += c.count * c.bps;
+        timeit([&]() { return b_outer(OP_SPMV_DOT, tab, wa, blocks, sc_, s_main_); }, &us[4]);
+    }
+    (void)hipStreamSynchronize(s_main_);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dl) (void)hipFree(dl);
+    if (db) (void)hipFree(db);
+    return err;
+}
+
 hipError_t BatchMG::run_to_completion(int64_t *launches) {
     HX(activate_ready(true));
     int64_t k = 0;

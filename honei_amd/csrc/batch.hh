@@ -259,6 +259,16 @@ class BatchMG {
     const std::vector<BLevel> &levels() const { return L_; }
     const BatchConfig &config() const { return cfg_; }
 
+    // Time one level's V-cycle launches over every slot present on the level
+    // (the step's launch shape at full activity), HIP events on the pool's
+    // stream, reps launches after one warm-up: us[0] paired pre-smoothing +
+    // defect, us[1] restriction to level - 1, us[2] prolongation + correction
+    // from level - 1, us[3] paired post-smoothing (0 where the level does not
+    // pair), us[4] the FP64 q = A p, p.q over every slot of every class.
+    // Overwrites the pool's vectors: call between steps only (a load
+    // rebuilds them, and results must be copied out first).
+    hipError_t time_level(int64_t level, int reps, double us[5]);
+
     // level runs the paired launches (b_pair) of its smoothing passes
     bool paired(const BLevel &lv) const {
         return cfg_.pair_sweeps && lv.side >= kPairMinSide && (cfg_.n_pre == 2 || cfg_.n_post == 2);

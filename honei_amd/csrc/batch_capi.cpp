@@ -257,6 +257,16 @@ int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, doub
     return HONEI_HIP_SUCCESS;
 }
 
+int honei_hip_batch_time_level(void *batch, int32_t level, int32_t reps, double *us) {
+    if (!batch || !us) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    BatchMG *b = (BatchMG *)batch;
+    if (level < 2 || level >= (int32_t)b->levels().size()) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "level out of range");
+    if (reps < 1 || reps > 1000) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "reps outside 1..1000");
+    if (b->loading_count() > 0 || b->running() > 0)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "the pool has slots loading or running");
+    return check(b->time_level(level, reps, us), "batch_time_level");
+}
+
 int honei_hip_serve_run(const honei_hip_serve_config *cfg, const honei_hip_request *reqs, int64_t nreq,
                         honei_hip_request_result *results, honei_hip_serve_stats *stats) {
     if (!cfg || (nreq > 0 && (!reqs || !results)) || nreq < 0)

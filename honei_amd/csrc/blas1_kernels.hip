@@ -57,6 +57,49 @@ template <typename V> __global__ void k_fill(V *x, V v, int64_t n) { GRID_STRIDE
 template <typename V> __global__ void k_copy(V *d, const V *s, int64_t n) { GRID_STRIDE(i, n) d[i] = s[i]; }
 __global__ void k_d2f(float *d, const double *s, int64_t n) { GRID_STRIDE(i, n) d[i] = (float)s[i]; }
 __global__ void k_f2d(double *d, const float *s, int64_t n) { GRID_STRIDE(i, n) d[i] = (double)s[i]; }
+// d[i] = low word of s[i]; entries >= bound counted into *bad (one vector
+// atomic per thread that met any)
+__global__ void k_narrow_u64(uint32_t *d, const uint64_t *s, int64_t n, uint64_t bound, unsigned int *bad) {
+    unsigned int nb = 0;
+    GRID_STRIDE(i, n) {
+        const uint64_t v = s[i];
+        nb += v >= bound ? 1u : 0u;
+        d[i] = (uint32_t)v;
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+// Bandwidth probes: 16-byte non-temporal loads (and stores), eight in flight
+// per thread, grid-stride over the array.  The read probe keeps a sum per
+// work-group (written to `partials`) so the loads are not removed.
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int kProbeUnroll = 8;
+__global__ __launch_bounds__(256) void k_probe_read(const f4v *__restrict__ src, int64_t n4, float *partials) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (; i + (kProbeUnroll - 1) * stride < n4; i += kProbeUnroll * stride) {
+        f4v v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) acc += v[k];
+    }
+    for (; i < n4; i += stride) acc += __builtin_nontemporal_load(src + i);
+    const float t = block_sum(acc.x + acc.y + acc.z + acc.w);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void k_probe_copy(f4v *__restrict__ dst, const f4v *__restrict__ src, int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * stride < n4; i += kProbeUnroll * stride) {
+        f4v v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) __builtin_nontemporal_store(v[k], dst + i + k * stride);
+    }
+    for (; i < n4; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
 __global__ void k_zero_bytes(unsigned char *p, size_t n) {
     GRID_STRIDE(i, (int64_t)n) p[i] = 0;
 }
@@ -200,6 +243,18 @@ hipError_t convert_d2f(float *d, const double *src, int64_t n, hipStream_t s) {
 }
 hipError_t convert_f2d(double *d, const float *src, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_f2d, dim3(stream_grid(n)), dim3(kBlock), 0, s, d, src, n);
+    return hipGetLastError();
+}
+hipError_t narrow_indices(uint32_t *d, const uint64_t *src, int64_t n, uint64_t bound, unsigned int *bad,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_narrow_u64, dim3(stream_grid(n)), dim3(kBlock), 0, s, d, src, n, bound, bad);
+    return hipGetLastError();
+}
+hipError_t stream_probe(int kind, void *dst, const void *src, int64_t n4, int grid, hipStream_t s) {
+    if (kind == 0)
+        hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(kBlock), 0, s, (const f4v *)src, n4, (float *)dst);
+    else
+        hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(kBlock), 0, s, (f4v *)dst, (const f4v *)src, n4);
     return hipGetLastError();
 }
 hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s) {

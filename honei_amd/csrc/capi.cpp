@@ -16,6 +16,7 @@
 
 #include "backend.hh"
 #include "runtime.hh"
+#include "solvers.hh"
 
 using namespace honei::hipx;
 
@@ -53,7 +54,7 @@ int check_ell(const void *Aj, const void *Ax, const void *Arl, size_t row_start,
 }
 
 EllView make_view(const void *Aj, const void *Ax, const void *Arl, size_t rows, size_t ncpr, size_t stride,
-                  size_t threads, ValueType vt) {
+                  size_t threads, ValueType vt, int index_words = 1) {
     EllView A;
     A.rows = (int64_t)rows;
     A.cols = 0;
@@ -64,17 +65,18 @@ EllView make_view(const void *Aj, const void *Ax, const void *Arl, size_t rows, 
     A.Ax = Ax;
     A.Arl = (const uint32_t *)Arl;
     A.vtype = vt;
+    A.index_words = index_words;
     return A;
 }
 
 int ell_call(ValueType vt, EllArgs a, const void *Aj, const void *Ax, const void *Arl, size_t ncpr, size_t stride,
-             size_t threads, size_t blocksize, void *stream, const char *what) {
+             size_t threads, size_t blocksize, void *stream, const char *what, int index_words = 1) {
     int st = check_ell(Aj, Ax, Arl, (size_t)a.row_start, (size_t)a.row_end, stride, threads, blocksize);
     if (st) return st;
     Workspace *w = workspace_for(S(stream));
     if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
     a.block = (int)blocksize;
-    EllView A = make_view(Aj, Ax, Arl, (size_t)a.row_end, ncpr, stride, threads, vt);
+    EllView A = make_view(Aj, Ax, Arl, (size_t)a.row_end, ncpr, stride, threads, vt, index_words);
     return check(ell_apply(A, a, w->slots, S(stream)), what);
 }
 
@@ -286,6 +288,74 @@ PRODUCT_ENTRY(honei_hip_product_smell_dv_bf16, VT_BF16)
     }
 DEFECT_ENTRY(honei_hip_defect_smell_dv_float, VT_F32)
 DEFECT_ENTRY(honei_hip_defect_smell_dv_double, VT_F64)
+
+// The same two operations on the reference's own index arrays: the glue
+// (la/product-cuda.cc:178-194, math/defect-cuda.cc) hands the launcher the
+// device copies of SparseMatrixELL::Aj() and Arl(), 64-bit unsigned long
+// arrays, which these entries read as they are (low word of each index).
+#define PRODUCT_ENTRY64(NAME, VT)                                                                        \
+    int NAME(void *x, void *y, const void *Aj, const void *Ax, const void *Arl, size_t row_start,          \
+             size_t row_end, size_t ncpr, size_t stride, size_t blocksize, size_t threads, void *stream) {   \
+        if (row_end > row_start && (!x || !y)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector");\
+        EllArgs a;                                                                                        \
+        a.mode = ELL_PRODUCT;                                                                             \
+        a.out = y;                                                                                        \
+        a.x = x;                                                                                          \
+        a.row_start = (int64_t)row_start;                                                                 \
+        a.row_end = (int64_t)row_end;                                                                     \
+        return ell_call(VT, a, Aj, Ax, Arl, ncpr, stride, threads, blocksize, stream, #NAME, 2);          \
+    }
+PRODUCT_ENTRY64(honei_hip_product_smell_dv_float_idx64, VT_F32)
+PRODUCT_ENTRY64(honei_hip_product_smell_dv_double_idx64, VT_F64)
+
+#define DEFECT_ENTRY64(NAME, VT)                                                                          \
+    int NAME(void *rhs, void *result, const void *Aj, const void *Ax, const void *Arl, void *b,            \
+             size_t row_start, size_t row_end, size_t ncpr, size_t stride, size_t blocksize, size_t threads, \
+             void *stream) {                                                                               \
+        if (row_end > row_start && (!rhs || !result || !b))                                               \
+            return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector");                                  \
+        EllArgs a;                                                                                         \
+        a.mode = ELL_DEFECT;                                                                               \
+        a.out = result;                                                                                    \
+        a.x = b;                                                                                           \
+        a.rhs = rhs;                                                                                       \
+        a.row_start = (int64_t)row_start;                                                                  \
+        a.row_end = (int64_t)row_end;                                                                      \
+        return ell_call(VT, a, Aj, Ax, Arl, ncpr, stride, threads, blocksize, stream, #NAME, 2);           \
+    }
+DEFECT_ENTRY64(honei_hip_defect_smell_dv_float_idx64, VT_F32)
+DEFECT_ENTRY64(honei_hip_defect_smell_dv_double_idx64, VT_F64)
+
+int honei_hip_stream_probe(int kind, void *dst, const void *src, size_t bytes, size_t grid, void *stream) {
+    if (kind != 0 && kind != 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "kind must be 0 (read) or 1 (copy)");
+    if (!dst || !src) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bytes and pointers must be multiples of 16");
+    if (grid < 1 || grid > 65536) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "grid outside 1..65536");
+    return check(stream_probe(kind, dst, src, (int64_t)(bytes / 16), (int)grid, S(stream)), "stream_probe");
+}
+
+int honei_hip_narrow_indices(void *dst, const void *src, size_t n, uint64_t bound, void *stream) {
+    if (n == 0) return HONEI_HIP_SUCCESS;
+    if (!dst || !src) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
+    if (n > (size_t)INT64_MAX / 8) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "n too large");
+    Workspace *w = workspace_for(S(stream));
+    if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    const uint64_t lim = bound < (uint64_t(1) << 32) ? bound : (uint64_t(1) << 32);
+    unsigned int *bad = (unsigned int *)(w->scalars + Workspace::kScalars - 1);
+    int st = check(fill_bytes_zero(bad, sizeof(unsigned int), S(stream)), "narrow_indices clear");
+    if (!st)
+        st = check(narrow_indices((uint32_t *)dst, (const uint64_t *)src, (int64_t)n, lim, bad, S(stream)),
+                   "narrow_indices");
+    unsigned int hbad = 0;
+    if (!st) st = check(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, S(stream)), "narrow readback");
+    if (!st) st = check(hipStreamSynchronize(S(stream)), "narrow sync");
+    if (st) return st;
+    if (hbad)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT,
+                    std::to_string(hbad) + " indices at or above " + std::to_string(lim));
+    return HONEI_HIP_SUCCESS;
+}
 
 // ---- ELL product with x staged in LDS -------------------------------------------
 int honei_hip_ell_block_ranges(const void *Aj, const void *Arl, size_t rows, size_t ncpr, size_t stride,

@@ -17,7 +17,11 @@ namespace hipx {
 // separate roundings (callers compile with fp contract off).  Ax is streamed
 // with non-temporal loads; Aj too unless SHARED_AJ (one index array serving
 // many value arrays, as in a batch, where it should stay in L2 / MALL).
-template <typename VT, int T, bool SCALED = false, bool SHARED_AJ = false>
+// IW: 32-bit words per stored index -- 1 for this backend's int32 arrays, 2
+// for the reference's 64-bit unsigned long arrays (la/sparse_matrix_ell.hh:
+// 110-116), of which the low word is read (little-endian; column indices
+// below 2^31).
+template <typename VT, int T, bool SCALED = false, bool SHARED_AJ = false, int IW = 1>
 __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restrict__ Aj, const VT *__restrict__ Ax,
                                                         int64_t row, int lane, uint32_t len, int64_t stride,
                                                         const typename VecOf<VT>::type *__restrict__ x,
@@ -29,9 +33,9 @@ __device__ inline typename VecOf<VT>::type ell_lane_sum(const int32_t *__restric
     for (uint32_t k = 0; k < len; ++k) {
         int32_t j;
         if constexpr (SHARED_AJ)
-            j = Aj[slot];
+            j = Aj[slot * IW];
         else
-            j = __builtin_nontemporal_load(Aj + slot);
+            j = __builtin_nontemporal_load(Aj + slot * IW);
         const V av = load_value_nt(Ax + slot);
         V xv;
         if constexpr (SCALED)
@@ -121,8 +125,8 @@ template <> struct BufLoad<bf16> {   // the bf16 value widened to float
 // length of x is not known here), then the guarded sums -- each sum is the
 // same sequence of roundings as ell_lane_sum (slots k >= len are read,
 // padding holds Aj = 0 and Ax = 0, but never added).  NC*stride slots must
-// fit 31-bit byte offsets (the caller checks).
-template <typename VT, int T, int NC, bool SCALED = false>
+// fit 31-bit byte offsets (the caller checks).  IW as for ell_lane_sum.
+template <typename VT, int T, int NC, bool SCALED = false, int IW = 1>
 __device__ inline typename VecOf<VT>::type ell_lane_sum_buf(__amdgpu_buffer_rsrc_t rAj, __amdgpu_buffer_rsrc_t rAx,
                                                             int64_t row, int lane, uint32_t len, int64_t stride,
                                                             const typename VecOf<VT>::type *__restrict__ x,
@@ -132,7 +136,7 @@ __device__ inline typename VecOf<VT>::type ell_lane_sum_buf(__amdgpu_buffer_rsrc
     int32_t j[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k)
-        j[k] = BufLoad<int32_t>::template load<kBufNT>(rAj, slot0 * 4u, (uint32_t)(k * stride) * 4u);
+        j[k] = BufLoad<int32_t>::template load<kBufNT>(rAj, slot0 * (4u * IW), (uint32_t)(k * stride) * (4u * IW));
     V av[NC], xv[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {

@@ -14,6 +14,7 @@
 // bit-identical to Product<tags::CPU> (la/product.hh:148-206).
 
 #include <algorithm>
+#include <type_traits>
 
 #include "backend.hh"
 #include "ell_device.hh"
@@ -45,8 +46,10 @@ struct KArgs {
 // (per_block = 1 unless the launch was capped, as it is for launches that
 // carry a reduction: few blocks keep the arrival counter cold).  W: the
 // stored width is q1_width(T) (the Q1 operators among others) and the row
-// sum takes the fixed-width form through buffer loads.
-template <typename VT, int T, int MODE, bool NORM, bool ACC, bool W>
+// sum takes the fixed-width form through buffer loads.  IW: 32-bit words per
+// stored index in Aj and Arl (1: int32 arrays; 2: the reference's 64-bit
+// arrays, low word read).
+template <typename VT, int T, int MODE, bool NORM, bool ACC, bool W, int IW = 1>
 __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     typedef typename VecOf<VT>::type V;
     const VT *__restrict__ Ax = (const VT *)a.Ax;
@@ -69,18 +72,19 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
 
         V s = V(0);
         if (active) {
+            const uint32_t len = a.Arl[row * IW];
             if constexpr (W) {
                 const auto rAj = buffer_rsrc(Aj, a.aj_bytes), rAx = buffer_rsrc(Ax, a.ax_bytes);
                 if constexpr (MODE == ELL_JACOBI0)
-                    s = ell_lane_sum_buf<VT, T, q1_width(T), true>(rAj, rAx, row, lane, a.Arl[row], a.stride, rhs,
-                                                                    dinv);
+                    s = ell_lane_sum_buf<VT, T, q1_width(T), true, IW>(rAj, rAx, row, lane, len, a.stride, rhs, dinv);
                 else
-                    s = ell_lane_sum_buf<VT, T, q1_width(T)>(rAj, rAx, row, lane, a.Arl[row], a.stride, x, nullptr);
+                    s = ell_lane_sum_buf<VT, T, q1_width(T), false, IW>(rAj, rAx, row, lane, len, a.stride, x,
+                                                                         nullptr);
             } else {
                 if constexpr (MODE == ELL_JACOBI0)
-                    s = ell_lane_sum<VT, T, true>(Aj, Ax, row, lane, a.Arl[row], a.stride, rhs, dinv);
+                    s = ell_lane_sum<VT, T, true, false, IW>(Aj, Ax, row, lane, len, a.stride, rhs, dinv);
                 else
-                    s = ell_lane_sum<VT, T>(Aj, Ax, row, lane, a.Arl[row], a.stride, x);
+                    s = ell_lane_sum<VT, T, false, false, IW>(Aj, Ax, row, lane, len, a.stride, x);
             }
         }
         s = lane_group_sum<T>(s);
@@ -118,7 +122,15 @@ __global__ void k_finish_norm(const double *partials, int64_t nparts, double *ou
 typedef void (*KernelFn)(KArgs);
 
 template <typename VT, int T, bool W>
-KernelFn pick_mode_w(EllMode mode, bool norm, bool acc) {
+KernelFn pick_mode_w(EllMode mode, bool norm, bool acc, int iw) {
+    if (iw == 2) {   // the reference's 64-bit index arrays: Product and Defect
+        if constexpr (!std::is_same<VT, bf16>::value) {
+            if (mode == ELL_PRODUCT) return ell_row_kernel<VT, T, ELL_PRODUCT, false, false, W, 2>;
+            if (mode == ELL_DEFECT && !norm) return ell_row_kernel<VT, T, ELL_DEFECT, false, false, W, 2>;
+        }
+        return nullptr;
+    }
+    if (iw != 1) return nullptr;
     switch (mode) {
         case ELL_PRODUCT: return ell_row_kernel<VT, T, ELL_PRODUCT, false, false, W>;
         case ELL_DEFECT:
@@ -135,22 +147,22 @@ KernelFn pick_mode_w(EllMode mode, bool norm, bool acc) {
 }
 
 template <typename VT, int T>
-KernelFn pick_mode(EllMode mode, bool norm, bool acc, bool wide) {
+KernelFn pick_mode(EllMode mode, bool norm, bool acc, bool wide, int iw) {
     if constexpr (T <= 8)
-        if (wide) return pick_mode_w<VT, T, true>(mode, norm, acc);
-    return pick_mode_w<VT, T, false>(mode, norm, acc);
+        if (wide) return pick_mode_w<VT, T, true>(mode, norm, acc, iw);
+    return pick_mode_w<VT, T, false>(mode, norm, acc, iw);
 }
 
 template <typename VT>
-KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc, bool wide) {
+KernelFn pick_threads(int T, EllMode mode, bool norm, bool acc, bool wide, int iw) {
     switch (T) {
-        case 1: return pick_mode<VT, 1>(mode, norm, acc, wide);
-        case 2: return pick_mode<VT, 2>(mode, norm, acc, wide);
-        case 4: return pick_mode<VT, 4>(mode, norm, acc, wide);
-        case 8: return pick_mode<VT, 8>(mode, norm, acc, wide);
-        case 16: return pick_mode<VT, 16>(mode, norm, acc, wide);
-        case 32: return pick_mode<VT, 32>(mode, norm, acc, wide);
-        case 64: return pick_mode<VT, 64>(mode, norm, acc, wide);
+        case 1: return pick_mode<VT, 1>(mode, norm, acc, wide, iw);
+        case 2: return pick_mode<VT, 2>(mode, norm, acc, wide, iw);
+        case 4: return pick_mode<VT, 4>(mode, norm, acc, wide, iw);
+        case 8: return pick_mode<VT, 8>(mode, norm, acc, wide, iw);
+        case 16: return pick_mode<VT, 16>(mode, norm, acc, wide, iw);
+        case 32: return pick_mode<VT, 32>(mode, norm, acc, wide, iw);
+        case 64: return pick_mode<VT, 64>(mode, norm, acc, wide, iw);
     }
     return nullptr;
 }
@@ -287,6 +299,7 @@ StagedFn pick_staged(int T) {
 
 hipError_t ell_block_ranges(const EllView &A, int64_t rows_per_block, int32_t *ranges, hipStream_t s) {
     const int T = A.threads;
+    if (A.index_words != 1) return hipErrorInvalidValue;
     if (rows_per_block < 1 || T < 1 || T > 64 || (T & (T - 1)) || !ranges) return hipErrorInvalidValue;
     const int64_t nb = (A.rows + rows_per_block - 1) / rows_per_block;
     if (nb <= 0) return hipSuccess;
@@ -305,7 +318,8 @@ hipError_t ell_product_staged(const EllView &A, void *y, const void *x, const in
         case VT_F64: fn = pick_staged<double>(T); break;
         case VT_BF16: fn = pick_staged<bf16>(T); break;
     }
-    if (!fn || block % kWave || block > 1024 || block < T || !ranges || chunks < 1) return hipErrorInvalidValue;
+    if (!fn || block % kWave || block > 1024 || block < T || !ranges || chunks < 1 || A.index_words != 1)
+        return hipErrorInvalidValue;
     const int64_t rpb = (int64_t)(block / T) * chunks;
     const int64_t nb = (A.rows + rpb - 1) / rpb;
     if (nb <= 0) return hipSuccess;
@@ -322,14 +336,15 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     const bool norm = (e.mode == ELL_DEFECT) && e.norm2 != nullptr;
     const bool acc = (e.mode == ELL_JACOBI || e.mode == ELL_JACOBI0) && e.acc != nullptr;
     // fixed-width form: stored width q1_width(T) and every byte offset within 31 bits
+    const int iw = A.index_words;
     const int64_t vsz = A.vtype == VT_F64 ? 8 : (A.vtype == VT_BF16 ? 2 : 4);
     const int64_t slots_total = A.ncpr * A.stride;
     const bool wide = T <= 8 && A.ncpr == q1_width(T) && slots_total * 8 < (int64_t(1) << 31);
     KernelFn fn = nullptr;
     switch (A.vtype) {
-        case VT_F32: fn = pick_threads<float>(T, e.mode, norm, acc, wide); break;
-        case VT_F64: fn = pick_threads<double>(T, e.mode, norm, acc, wide); break;
-        case VT_BF16: fn = pick_threads<bf16>(T, e.mode, norm, acc, wide); break;
+        case VT_F32: fn = pick_threads<float>(T, e.mode, norm, acc, wide, iw); break;
+        case VT_F64: fn = pick_threads<double>(T, e.mode, norm, acc, wide, iw); break;
+        case VT_BF16: fn = pick_threads<bf16>(T, e.mode, norm, acc, wide, iw); break;
     }
     if (!fn) return hipErrorInvalidValue;
     const int64_t nrows = e.row_end - e.row_start;
@@ -358,7 +373,7 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     k.row_end = e.row_end;
     k.nlogical = nlogical;
     k.slots = slots;
-    k.aj_bytes = (uint32_t)(slots_total * 4);
+    k.aj_bytes = (uint32_t)(slots_total * 4 * iw);
     k.ax_bytes = (uint32_t)(slots_total * vsz);
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
     if (norm) hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, (const double *)slots.partials, grid, e.norm2);

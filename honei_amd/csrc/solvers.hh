@@ -30,6 +30,19 @@ struct CGScalars {
 };
 
 hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s);
+
+// Host side of CGSolver (solvers_capi.cpp), shared by honei_hip_cg_solve and
+// its allocation-free variant: argument checks (HONEI status code), the
+// bytes of one CG vector (256-byte multiple), the matrix view, and the solve
+// on caller-provided vectors p, r, v, scalars and reduction slots.
+int cg_check_args(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows, size_t stride,
+                  size_t threads, const void *b, const void *x);
+size_t cg_vector_bytes(ValueType vt, size_t rows);
+EllView cg_view(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows, size_t ncpr,
+                size_t stride, size_t threads);
+int cg_solve_on(const EllView &A, const void *b, void *x, size_t max_iters, double eps_relative, size_t *used_iters,
+                double *final_defect, hipStream_t s, const ReduceSlots &slots, void *p, void *r, void *v,
+                CGScalars *sc);
 // Iteration `it` (0, 1, ... in order) of the loop.
 hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
                         const ReduceSlots &sl, int it, hipStream_t s);

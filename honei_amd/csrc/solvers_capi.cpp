@@ -56,40 +56,45 @@ int upload_ell(DevEll &M, ValueType vt, size_t rows, size_t cols, size_t stride,
 
 }  // namespace
 
-extern "C" {
+namespace honei {
+namespace hipx {
 
-// ---- CG -------------------------------------------------------------------
-int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
-                       size_t num_cols_per_row, size_t stride, size_t threads, const void *b, void *x,
-                       size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
-                       void *stream) {
+int cg_check_args(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows, size_t stride,
+                  size_t threads, const void *b, const void *x) {
     if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64 && vtype != HONEI_HIP_BF16)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad vtype");
     if (!Aj || !Ax || !Arl || !b || !x) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
     if (!pow2_threads(threads) || rows * threads > stride)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bad ELL layout");
-    if (max_iters > 0x7fffffff) max_iters = 0x7fffffff;
-    Workspace *w = workspace_for(S(stream));
-    if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
-    const ValueType vt = (ValueType)vtype;
-    const size_t es = vec_size(vt);
-    const size_t vbytes = ((rows * es + 255) / 256) * 256;
-    char *base = (char *)w->ensure_scratch(3 * vbytes + 256);
-    if (!base) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
-    void *p = base, *r = base + vbytes, *v = base + 2 * vbytes;
-    CGScalars *sc = (CGScalars *)(base + 3 * vbytes);
+    if (rows > (size_t)INT32_MAX) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "rows exceed 32-bit index range");
+    return HONEI_HIP_SUCCESS;
+}
+
+size_t cg_vector_bytes(ValueType vt, size_t rows) { return ((rows * vec_size(vt) + 255) / 256) * 256; }
+
+EllView cg_view(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows, size_t ncpr,
+                size_t stride, size_t threads) {
     EllView A;
     A.rows = (int64_t)rows;
     A.cols = (int64_t)rows;
     A.stride = (int64_t)stride;
-    A.ncpr = (int64_t)num_cols_per_row;
+    A.ncpr = (int64_t)ncpr;
     A.threads = (int)threads;
     A.Aj = (const int32_t *)Aj;
     A.Ax = Ax;
     A.Arl = (const uint32_t *)Arl;
-    A.vtype = vt;
-    const ValueType vv = vt == VT_F64 ? VT_F64 : VT_F32;
-    hipStream_t s = S(stream);
+    A.vtype = (ValueType)vtype;
+    return A;
+}
+
+// CGSolver<Tag, NONE> (math/cg.hh:63-133) with p, r, v and the scalars in
+// caller-provided device memory and the reductions in `slots`.
+int cg_solve_on(const EllView &A, const void *b, void *x, size_t max_iters, double eps_relative, size_t *used_iters,
+                double *final_defect, hipStream_t s, const ReduceSlots &slots, void *p, void *r, void *v,
+                CGScalars *sc) {
+    if (max_iters > 0x7fffffff) max_iters = 0x7fffffff;
+    const size_t rows = (size_t)A.rows;
+    const ValueType vv = A.vtype == VT_F64 ? VT_F64 : VT_F32;
     // r = b - A x ; p = r ; alpha = ||r||^2 (math/cg.hh:90-95)
     EllArgs a;
     a.mode = ELL_DEFECT;
@@ -99,7 +104,7 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     a.norm2 = &sc->alpha;
     a.row_start = 0;
     a.row_end = (int64_t)rows;
-    int st = check(ell_apply(A, a, w->slots, s), "cg defect");
+    int st = check(ell_apply(A, a, slots, s), "cg defect");
     if (st) return st;
     if ((st = check(vcopy(p, r, (int64_t)rows, vv, s), "cg copy"))) return st;
     if ((st = check(cg_init(vv, sc, eps_relative, (int)max_iters, s), "cg init"))) return st;
@@ -109,7 +114,7 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     size_t enqueued = 0;
     while (true) {
         for (size_t k = 0; k < chunk && enqueued < max_iters; ++k, ++enqueued)
-            if ((st = check(cg_iteration(A, x, p, r, v, sc, w->slots, (int)enqueued, s), "cg iteration"))) return st;
+            if ((st = check(cg_iteration(A, x, p, r, v, sc, slots, (int)enqueued, s), "cg iteration"))) return st;
         if ((st = check(hipMemcpyAsync(&h, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s), "cg readback")))
             return st;
         if ((st = check(hipStreamSynchronize(s), "cg sync"))) return st;
@@ -122,6 +127,28 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
     if (h.iters >= h.max_iters && !(h.current < h.eps_rel * h.initial || h.current < h.eps_rel))
         return fail(HONEI_HIP_ERROR_NOT_CONVERGED, "CG reached max_iters");
     return HONEI_HIP_SUCCESS;
+}
+
+}  // namespace hipx
+}  // namespace honei
+
+extern "C" {
+
+// ---- CG -------------------------------------------------------------------
+int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                       size_t num_cols_per_row, size_t stride, size_t threads, const void *b, void *x,
+                       size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
+                       void *stream) {
+    int st = cg_check_args(vtype, Aj, Ax, Arl, rows, stride, threads, b, x);
+    if (st) return st;
+    Workspace *w = workspace_for(S(stream));
+    if (!w) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    const size_t vbytes = cg_vector_bytes((ValueType)vtype, rows);
+    char *base = (char *)w->ensure_scratch(3 * vbytes + 256);
+    if (!base) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    const EllView A = cg_view(vtype, Aj, Ax, Arl, rows, num_cols_per_row, stride, threads);
+    return cg_solve_on(A, b, x, max_iters, eps_relative, used_iters, final_defect, S(stream), w->slots, base,
+                       base + vbytes, base + 2 * vbytes, (CGScalars *)(base + 3 * vbytes));
 }
 
 int honei_hip_pcg_jacobi(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,

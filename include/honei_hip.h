@@ -153,6 +153,47 @@ int honei_hip_defect_smell_dv_double(void *rhs, void *result, const void *Aj, co
                                      void *b, size_t row_start, size_t row_end, size_t num_cols_per_row,
                                      size_t stride, size_t blocksize, size_t threads, void *stream);
 
+/* ---- the reference's 64-bit index arrays ----------------------------------
+ * SparseMatrixELL keeps Aj and Arl as unsigned long (la/sparse_matrix_ell.hh:
+ * 110-116) and its CUDA glue passes their device copies straight to the
+ * launcher (la/product-cuda.cc:178-194 for Product, math/defect-cuda.cc:
+ * 524-660 for Defect).  These entries take those arrays as they are: Aj
+ * uint64 [ncpr*stride], Arl uint64 [rows], column indices below 2^31 (the
+ * low word of each index is read).  Otherwise identical to the int32 forms
+ * above (same arguments, sub-range convention and bits); they stream 8 bytes
+ * per index slot instead of 4.
+ * Replace cuda_product_smell_dv_{float,double} and
+ * cuda_defect_smell_dv_{float,double} (backends/cuda/operations.hh:41-47,
+ * 87-93) for a caller that keeps the reference's index type.             */
+int honei_hip_product_smell_dv_float_idx64(void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                           size_t row_start, size_t row_end, size_t num_cols_per_row, size_t stride,
+                                           size_t blocksize, size_t threads, void *stream);
+int honei_hip_product_smell_dv_double_idx64(void *x, void *y, const void *Aj, const void *Ax, const void *Arl,
+                                            size_t row_start, size_t row_end, size_t num_cols_per_row,
+                                            size_t stride, size_t blocksize, size_t threads, void *stream);
+int honei_hip_defect_smell_dv_float_idx64(void *rhs, void *result, const void *Aj, const void *Ax, const void *Arl,
+                                          void *b, size_t row_start, size_t row_end, size_t num_cols_per_row,
+                                          size_t stride, size_t blocksize, size_t threads, void *stream);
+int honei_hip_defect_smell_dv_double_idx64(void *rhs, void *result, const void *Aj, const void *Ax,
+                                           const void *Arl, void *b, size_t row_start, size_t row_end,
+                                           size_t num_cols_per_row, size_t stride, size_t blocksize, size_t threads,
+                                           void *stream);
+/* Device-side narrowing of a 64-bit index array (the device copy of
+ * SparseMatrixELL::Aj() or Arl()) into this backend's 32-bit form:
+ * dst_dev_u32[i] = src_dev_u64[i], every value checked < bound (columns for
+ * Aj, num_cols_per_row + 1 for Arl; values at or above 2^32 always fail).
+ * For glue that narrows once per matrix and then calls the int32 entries
+ * (4 instead of 8 bytes per index slot).  Synchronous (reads back the
+ * count of out-of-range values); no reference counterpart.               */
+int honei_hip_narrow_indices(void *dst_dev_u32, const void *src_dev_u64, size_t n, uint64_t bound, void *stream);
+
+/* Bandwidth probe (measurement only, no reference counterpart): `bytes`
+ * (multiple of 16, 16-byte aligned pointers) streamed with 16-byte
+ * non-temporal loads, eight in flight per thread, on `grid` (1..65536)
+ * 256-thread work-groups.  kind 0: read src, one float sum per work-group
+ * into dst (dst holds `grid` floats); kind 1: copy src to dst.           */
+int honei_hip_stream_probe(int kind, void *dst, const void *src, size_t bytes, size_t grid, void *stream);
+
 /* Fused defect + norm (no reference counterpart; the reference runs
  * Defect then Norm, math/multigrid.hh:2495-2498): result = rhs - A x over
  * all rows and *norm2_dev = ||result||^2 (double accumulation), in one pass. */
@@ -429,6 +470,16 @@ int honei_hip_batch_shape(void *batch, int32_t level, int32_t *threads, int32_t 
  * and of one paired two-sweep launch over the same tuning batch
  * (microseconds; 0 when the level is not paired or not tuned).            */
 int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, double *post_us);
+/* Measurement (no reference counterpart): one level's launches timed over
+ * every slot present on that level -- the launch shape of a batched step at
+ * full activity -- with HIP events on the pool's stream, `reps` launches
+ * after one warm-up: us[0] paired Jacobi0 + defect, us[1] restriction to
+ * level - 1, us[2] prolongation + correction from level - 1, us[3] paired
+ * two-sweep launch (0 where the level does not pair), us[4] the FP64
+ * q = A p, p.q over every slot of every class (CG outer).  Overwrites the
+ * pool's vectors and solutions: only between steps, with no slot loading or
+ * running, after the results are copied out.  Synchronous.               */
+int honei_hip_batch_time_level(void *batch, int32_t level, int32_t reps, double *us);
 
 /* ---- serving loop -------------------------------------------------------------------
  * No reference counterpart.  Continuous batching over a batch pool: requests

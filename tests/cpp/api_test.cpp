@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "honei/backends/hip/memory_backend.hh"
 #include "honei/la/algorithm.hh"
 #include "honei/la/difference.hh"
 #include "honei/la/dot_product.hh"
@@ -171,6 +172,125 @@ void product_ell() {
             Product<tags::GPU::HIP>::value(prod_s, sm1, dv1);
             TEST_CHECK(sm1.device().window > 0);
             for (unsigned long i = 0; i < size; ++i) TEST_CHECK(prod_s[i] == prod[i]);
+        }
+    }
+    Configuration::instance()->set_value("ell::threads", 1);
+}
+
+// MemoryBackend<tags::GPU::HIP> against the contract of
+// util/memory_backend_base.hh:32-58 as the arbiter uses it
+// (util/memory_arbiter-impl.hh:380-500): upload of a known block returns its
+// copy without transferring, download writes back to the block's host
+// address, free forgets it; copy, fill and both conversions act on the
+// device copies.
+void memory_backend_api() {
+    MemoryBackend<tags::GPU::HIP> *mb = MemoryBackend<tags::GPU::HIP>::instance();
+    const unsigned long n = 1000;
+    std::vector<double> a(n), b(n, 0.0), c(n, -1.0);
+    std::vector<float> f(n);
+    for (unsigned long i = 0; i < n; ++i) a[i] = urand();
+    const unsigned long blocks0 = mb->blocks();
+    TEST_CHECK(!mb->knows(a.data(), a.data()));
+    void *da = mb->upload(a.data(), a.data(), 8 * n);
+    TEST_CHECK(mb->knows(a.data(), a.data()) && mb->blocks() == blocks0 + 1);
+    const double keep = a[7];
+    a[7] = 12345.0;   // host change without a new lock: the known block is not re-sent
+    TEST_CHECK(mb->upload(a.data(), a.data(), 8 * n) == da);
+    mb->download(a.data(), a.data(), 8 * n);
+    TEST_CHECK(a[7] == keep);
+    void *db = mb->alloc(b.data(), b.data(), 8 * n);
+    TEST_CHECK(mb->alloc(b.data(), b.data(), 8 * n) == db);
+    mb->copy(a.data(), a.data(), b.data(), b.data(), 8 * n);
+    mb->download(b.data(), b.data(), 8 * n);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK(b[i] == a[i]);
+    mb->alloc(f.data(), f.data(), 4 * n);
+    mb->convert_double_float(a.data(), a.data(), f.data(), f.data(), 8 * n);
+    mb->download(f.data(), f.data(), 4 * n);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK(f[i] == (float)a[i]);
+    mb->alloc(c.data(), c.data(), 8 * n);
+    mb->convert_float_double(f.data(), f.data(), c.data(), c.data(), 4 * n);
+    mb->download(c.data(), c.data(), 8 * n);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK(c[i] == (double)f[i]);
+    mb->fill(c.data(), c.data(), 8 * n, 2.5);
+    mb->fill(f.data(), f.data(), 4 * n, -0.75f);
+    mb->download(c.data(), c.data(), 8 * n);
+    mb->download(f.data(), f.data(), 4 * n);
+    for (unsigned long i = 0; i < n; ++i) TEST_CHECK(c[i] == 2.5 && f[i] == -0.75f);
+    for (void *id : {(void *)a.data(), (void *)b.data(), (void *)c.data(), (void *)f.data()}) mb->free(id);
+    TEST_CHECK(!mb->knows(a.data(), a.data()) && mb->blocks() == blocks0);
+    bool threw = false;
+    try {
+        mb->download(a.data(), a.data(), 8 * n);
+    } catch (InternalError &) {
+        threw = true;
+    }
+    TEST_CHECK(threw);
+}
+
+// The reference's glue as written (la/product-cuda.cc:178-194,
+// math/defect-cuda.cc:524-660): the matrix's Aj, Ax, Arl and the vectors are
+// locked through the memory backend -- Aj and Arl as the 64-bit unsigned
+// long blocks SparseMatrixELL holds -- and handed to the launcher.  The
+// *_idx64 launchers give the bits of the int32 path (Product / Defect
+// <tags::GPU::HIP> on the same matrix), and the device-side narrowing of
+// those blocks gives the int32 arrays.
+template <typename DT>
+void reference_glue() {
+    MemoryBackend<tags::GPU::HIP> *mb = MemoryBackend<tags::GPU::HIP>::instance();
+    auto product64 = sizeof(DT) == 4 ? honei_hip_product_smell_dv_float_idx64 : honei_hip_product_smell_dv_double_idx64;
+    auto defect64 = sizeof(DT) == 4 ? honei_hip_defect_smell_dv_float_idx64 : honei_hip_defect_smell_dv_double_idx64;
+    for (unsigned long threads = 1; threads <= 16; threads *= 2) {
+        Configuration::instance()->set_value("ell::threads", (int)threads);
+        for (int which = 0; which < 2; ++which) {
+            SparseMatrix<DT> sms = which == 0 ? pattern<DT>(300, 303, true) : q1<DT>(33);
+            SparseMatrixELL<DT> a(sms);
+            const unsigned long rows = a.rows(), cols = a.columns(), slots = a.Ax().size();
+            DenseVector<DT> x = vec<DT>(rand_vec(cols)), rhs = vec<DT>(rand_vec(rows));
+            DenseVector<DT> y32(rows, DT(0)), r32(rows, DT(0));
+            Product<tags::GPU::HIP>::value(y32, a, x);
+            Defect<tags::GPU::HIP>::value(r32, rhs, a, x);
+            unsigned long *aj = a.Aj().elements(), *arl = a.Arl().elements();
+            DT *ax = a.Ax().elements(), *xe = x.elements(), *re = rhs.elements();
+            std::vector<DT> y(rows, DT(4711)), r(rows, DT(4711));
+            void *Aj_gpu = mb->upload(aj, aj, 8 * slots);
+            void *Ax_gpu = mb->upload(ax, ax, sizeof(DT) * slots);
+            void *Arl_gpu = mb->upload(arl, arl, 8 * rows);
+            void *x_gpu = mb->upload(xe, xe, sizeof(DT) * cols);
+            void *rhs_gpu = mb->upload(re, re, sizeof(DT) * rows);
+            void *y_gpu = mb->alloc(y.data(), y.data(), sizeof(DT) * rows);
+            void *r_gpu = mb->alloc(r.data(), r.data(), sizeof(DT) * rows);
+            hip_check(product64(x_gpu, y_gpu, Aj_gpu, Ax_gpu, Arl_gpu, 0, rows, a.num_cols_per_row(), a.stride(), 0,
+                                a.threads(), nullptr),
+                      "product idx64");
+            hip_check(defect64(rhs_gpu, r_gpu, Aj_gpu, Ax_gpu, Arl_gpu, x_gpu, 0, rows, a.num_cols_per_row(),
+                               a.stride(), 0, a.threads(), nullptr),
+                      "defect idx64");
+            mb->download(y.data(), y.data(), sizeof(DT) * rows);
+            mb->download(r.data(), r.data(), sizeof(DT) * rows);
+            for (unsigned long i = 0; i < rows; ++i) TEST_CHECK(y[i] == y32[i] && r[i] == r32[i]);
+            // a sub-range writes y[i - row_start], as the int32 entry does
+            const unsigned long r0 = rows / 3, r1 = rows - 5;
+            hip_check(product64(x_gpu, y_gpu, Aj_gpu, Ax_gpu, Arl_gpu, r0, r1, a.num_cols_per_row(), a.stride(), 0,
+                                a.threads(), nullptr),
+                      "product idx64 range");
+            mb->download(y.data(), y.data(), sizeof(DT) * rows);
+            for (unsigned long i = r0; i < r1; ++i) TEST_CHECK(y[i - r0] == y32[i]);
+            // device-side narrowing of the 64-bit blocks
+            std::vector<uint32_t> aj32(slots), arl32(rows);
+            void *d32 = nullptr;
+            hip_check(honei_hip_malloc(&d32, 4 * slots + 4 * rows + 8), "malloc");
+            hip_check(honei_hip_narrow_indices(d32, Aj_gpu, slots, cols, nullptr), "narrow Aj");
+            hip_check(honei_hip_download(aj32.data(), d32, 4 * slots, nullptr), "download");
+            hip_check(honei_hip_narrow_indices(d32, Arl_gpu, rows, a.num_cols_per_row() + 1, nullptr), "narrow Arl");
+            hip_check(honei_hip_download(arl32.data(), d32, 4 * rows, nullptr), "download");
+            for (unsigned long i = 0; i < slots; ++i) TEST_CHECK(aj32[i] == aj[i]);
+            for (unsigned long i = 0; i < rows; ++i) TEST_CHECK(arl32[i] == arl[i]);
+            // an index at or above the bound is reported, not narrowed silently
+            TEST_CHECK(honei_hip_narrow_indices(d32, Aj_gpu, slots, 1, nullptr) == HONEI_HIP_ERROR_INVALID_ARGUMENT);
+            hip_check(honei_hip_free(d32), "free");
+            for (void *id : {(void *)aj, (void *)ax, (void *)arl, (void *)xe, (void *)re, (void *)y.data(),
+                             (void *)r.data()})
+                mb->free(id);
         }
     }
     Configuration::instance()->set_value("ell::threads", 1);
@@ -563,6 +683,9 @@ int main(int argc, char **argv) {
         {"sparse_matrix_ell_quick_test<double>", ell_quick<double>},
         {"sparse_matrix_ell_dense_vector_product_test<float>", product_ell<float>},
         {"sparse_matrix_ell_dense_vector_product_test<double>", product_ell<double>},
+        {"memory_backend_api", memory_backend_api},
+        {"reference_glue_idx64<float>", reference_glue<float>},
+        {"reference_glue_idx64<double>", reference_glue<double>},
         {"defect_and_blas1<float>", defect_and_blas1<float>},
         {"defect_and_blas1<double>", defect_and_blas1<double>},
         {"jacobi<float>", jacobi<float>},

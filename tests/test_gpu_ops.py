@@ -82,6 +82,68 @@ def test_product_reference_pattern(size, threads):
         assert (err <= TOL[dtype] * _abs_row_sums(A, x) + 1e-30).all()
 
 
+def _idx64_arrays(A):
+    """Device copies of the matrix exactly as the reference's glue hands them
+    to its launcher: Aj and Arl as the 64-bit unsigned long arrays of
+    SparseMatrixELL (la/product-cuda.cc:178-194), Ax in its value type."""
+    return (H.DeviceArray.from_host(A.Aj, np.uint64), H.DeviceArray.from_host(A.Ax, A.Ax.dtype),
+            H.DeviceArray.from_host(A.Arl, np.uint64))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("threads", [1, 2, 4, 8, 16, 32, 64])
+def test_product_and_defect_on_reference_index_arrays(dtype, threads):
+    """honei_hip_{product,defect}_smell_dv_*_idx64 read the 64-bit Aj / Arl
+    as the reference stores them: the same bits as the int32 entries (and
+    the oracle's at T = 1), sub-ranges included; the narrowed arrays are the
+    int32 ones."""
+    n = 20000
+    rp, col, val = random_band_csr(n, seed=31, max_len=27, band=4096, dtype=dtype)
+    A = oracle.ell_from_sm(n, n, rp, col, val, threads)
+    x = uniform_pm1(4, n).astype(dtype)
+    rhs = uniform_pm1(5, n).astype(dtype)
+    sfx = "float" if dtype == np.float32 else "double"
+    Aj, Ax, Arl = _idx64_arrays(A)
+    D = H.DeviceEll.from_oracle(A)
+    dx, drhs = H.DeviceArray.from_host(x), H.DeviceArray.from_host(rhs)
+    y32, y64 = H.DeviceArray(n, dtype), H.DeviceArray(n, dtype)
+    D.product(y32, dx)
+    H.call(f"honei_hip_product_smell_dv_{sfx}_idx64", dx.ptr, y64.ptr, Aj.ptr, Ax.ptr, Arl.ptr, 0, n, A.ncpr,
+           A.stride, 0, threads, None)
+    assert np.array_equal(y64.get().view(np.uint8), y32.get().view(np.uint8))
+    if threads == 1:
+        assert np.array_equal(y64.get().view(np.uint8), oracle.product(A, x).view(np.uint8))
+    ys = H.DeviceArray(n, dtype)
+    H.call(f"honei_hip_product_smell_dv_{sfx}_idx64", dx.ptr, ys.ptr, Aj.ptr, Ax.ptr, Arl.ptr, 777, 15000, A.ncpr,
+           A.stride, 256, threads, None)
+    assert np.array_equal(ys.get()[:15000 - 777], y32.get()[777:15000])
+    r32, r64 = H.DeviceArray(n, dtype), H.DeviceArray(n, dtype)
+    D.defect(r32, drhs, dx)
+    H.call(f"honei_hip_defect_smell_dv_{sfx}_idx64", drhs.ptr, r64.ptr, Aj.ptr, Ax.ptr, Arl.ptr, dx.ptr, 0, n,
+           A.ncpr, A.stride, 0, threads, None)
+    assert np.array_equal(r64.get().view(np.uint8), r32.get().view(np.uint8))
+    nj = H.DeviceArray(Aj.n, np.uint32)
+    H.call("honei_hip_narrow_indices", nj.ptr, Aj.ptr, Aj.n, n, None)
+    assert np.array_equal(nj.get(), D.Aj.get().view(np.uint32))
+    with pytest.raises(H.HipError) as e:
+        H.call("honei_hip_narrow_indices", nj.ptr, Aj.ptr, Aj.n, n // 2, None)
+    assert e.value.code == 1
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_product_on_reference_index_arrays_q1_2049(dtype):
+    """Full size (Q1 2049^2, the fixed-width buffer-load row sum): the 64-bit
+    index entry is bit-exact against the CPU path."""
+    A = oracle.q1_matrix(2049, dtype)
+    x = uniform_pm1(2, A.rows).astype(dtype)
+    sfx = "float" if dtype == np.float32 else "double"
+    Aj, Ax, Arl = _idx64_arrays(A)
+    dx, dy = H.DeviceArray.from_host(x), H.DeviceArray(A.rows, dtype)
+    H.call(f"honei_hip_product_smell_dv_{sfx}_idx64", dx.ptr, dy.ptr, Aj.ptr, Ax.ptr, Arl.ptr, 0, A.rows, A.ncpr,
+           A.stride, 0, 1, None)
+    assert np.array_equal(dy.get().view(np.uint8), oracle.product(A, x).view(np.uint8))
+
+
 def test_product_row_range_offsets_output():
     """Sub-range convention of cuda_product_smell_dv_*: y[r - row_start]."""
     A = oracle.q1_matrix(65)
