@@ -132,6 +132,40 @@ def time_spmv(sets, reps, label, idx64=False):
             "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": len(sets)}
 
 
+def time_sell(sets, reps, label):
+    """time_spmv for the sliced copy of each operand matrix
+    (honei_hip_product_sell; the copies are built outside the timed
+    launches); the same algorithmic bytes as the ELL product, and the slots
+    actually stored beside them."""
+    sells = [(H.Sell(D), x, y) for D, x, y in sets]
+    for k in range(2 * len(sells)):
+        S, x, y = sells[k % len(sells)]
+        S.product(y, x)
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record()
+    for k in range(reps):
+        S, x, y = sells[k % len(sells)]
+        S.product(y, x)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    D0 = sets[0][0]
+    sv = 8 if D0.vtype == H.F64 else 4
+    nnz = sum(st[0].nnz for st in sets) / len(sets)
+    alg = nnz * (sv + 4) + D0.rows * 2 * sv
+    pad = sum(S.padded_slots for S, _, _ in sells) / len(sells)
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"[spmv] {label} sliced: {ms * 1e3:.1f} us/launch, {gbs:.0f} GB/s = {100 * gbs / PEAK_GBS:.1f}% "
+        f"({pad / nnz:.3f} stored slots per non-zero)")
+    out = {"us_per_launch": round(ms * 1e3, 2), "algorithmic_bytes": int(alg), "achieved_gbs": round(gbs, 1),
+           "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": len(sets),
+           "stored_slots_per_nonzero": round(pad / nnz, 4),
+           "what": ("honei_hip_product_sell: rows sorted by length within 2048-row windows, 64-row slices "
+                    "padded to their longest row, terms in stored order (bit-identical to the ELL product)")}
+    del sells
+    return out
+
+
 def with_idx64(D):
     """Attach the reference's 64-bit index arrays (Aj, Arl as unsigned long)
     of a device matrix, widened on the host from its 32-bit copies."""
@@ -577,6 +611,46 @@ def level_launch_rates(B, levels, level, reps=20):
     return out
 
 
+def serving_payload(seconds, sources=8):
+    """The serving stream with a host payload: each request's coefficient and
+    right-hand side are copied from pinned host memory when it is admitted
+    (`sources` distinct systems per size, request i of a size taking source
+    i mod `sources`), and its solution is copied back to pinned host memory
+    (one buffer per size) before the request counts as done; latency covers
+    both copies."""
+    reqs = serving_requests(seconds)
+    src, xbuf, seen = {}, {}, {}
+    for L in SERVE_LEVELS:
+        src[L] = []
+        for k in range(sources):
+            cs, bs = poisson_system_seeds(SERVE_SEED + 100, 16 * L + k)
+            kappa, b = device_fields(L, cs, bs)
+            pk, pb = H.PinnedArray(kappa.size), H.PinnedArray(b.size)
+            pk.a[:] = kappa
+            pb.a[:] = b
+            src[L].append((pk, pb))
+        xbuf[L] = H.PinnedArray((2 ** L + 1) ** 2)
+    mix = []
+    for q in reqs:
+        k = seen.get(q[1], 0)
+        seen[q[1]] = k + 1
+        pk, pb = src[q[1]][k % sources]
+        mix.append(q + (None, pk, pb, xbuf[q[1]]))
+    _res, st = H.serve(SERVE_LEVELS, mix, budget_bytes=SERVE_BUDGET)
+    mb = sum(24 * (2 ** q[1] + 1) ** 2 for q in reqs) / 1e6
+    out = {"requests": len(reqs), "completed": st.completed, "not_converged": st.not_converged,
+           "solves_per_s": round(st.solves_per_s, 2), "wall_s": round(st.wall_s, 3),
+           "latency_ms": {"p50": round(st.p50_ms, 2), "p99": round(st.p99_ms, 2), "max": round(st.max_ms, 2)},
+           "host_link_MB": round(mb, 1), "host_link_GBps": round(mb / 1e3 / max(st.wall_s, 1e-9), 2),
+           "what": (f"the serving stream for {seconds:.0f} s with coefficient and right-hand side copied from pinned "
+                    "host memory at admission and the solution copied back to pinned host memory; latency from "
+                    "arrival to the end of the return copy")}
+    log(f"[serve payload] {st.completed} solves in {st.wall_s:.2f} s ({st.solves_per_s:.1f}/s), p50 {st.p50_ms:.1f} ms "
+        f"p99 {st.p99_ms:.1f} ms, {out['host_link_GBps']} GB/s over the host link")
+    del src, xbuf
+    return out
+
+
 def serving_capacity(seconds, rates=(2000.0, 3000.0, 4000.0, 6000.0, 9000.0)):
     """Open-loop serving streams of increasing arrival rate (same size mix,
     seeds and 64 GB budget as the serving run, `seconds` long each):
@@ -862,6 +936,8 @@ def main():
                     help="length of the Poisson arrival stream of the serving run (0: skip)")
     ap.add_argument("--capacity-seconds", type=float, default=6.0,
                     help="length of each stream of the serving capacity sweep (0: skip)")
+    ap.add_argument("--payload-seconds", type=float, default=10.0,
+                    help="length of the serving stream with host payloads (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the single-system FP64 comparison")
     ap.add_argument("--batch-only", action="store_true", help="only the batched step (for profiling)")
@@ -870,6 +946,10 @@ def main():
                          "(same kernels and results; for kernel-trace profiling)")
     ap.add_argument("--no-pair", action="store_true",
                     help="with --batch-only: one launch per smoothing pass (pair_sweeps = 0), for comparison")
+    ap.add_argument("--no-mf", action="store_true",
+                    help="with --batch-only: read the stored matrix values (matrix_free = 0), for comparison")
+    ap.add_argument("--iters-per-launch", type=int, default=0,
+                    help="with --batch-only: outer iterations per graph launch (0: the pool's default)")
     ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section")
     ap.add_argument("--pcg-only", action="store_true",
                     help="only the PCG-Jacobi iteration rate on Q1 2049^2 FP64 (for profiling)")
@@ -914,7 +994,11 @@ def main():
 
     if args.batch_only:
         levels, kap, rhs = batched_workload()
-        B, slots = batched_pool(levels, use_graphs=0 if args.no_graphs else 1, pair_sweeps=0 if args.no_pair else 1)
+        kw = dict(use_graphs=0 if args.no_graphs else 1, pair_sweeps=0 if args.no_pair else 1,
+                  matrix_free=0 if args.no_mf else 1)
+        if args.iters_per_launch > 0:
+            kw["iters_per_launch"] = args.iters_per_launch
+        B, slots = batched_pool(levels, **kw)
         for _ in range(args.warmup):
             batched_step(B, slots, kap, rhs)
         t0 = time.perf_counter()
@@ -957,6 +1041,7 @@ def main():
             sets.append((R, H.DeviceArray.from_host(uniform_pm1(2 + c, R.rows).astype(dt)), H.DeviceArray(R.rows, dt)))
         spmv[key] = time_spmv(sets, 30, key)
         spmv[key]["nnz"] = sets[0][0].nnz
+        spmv[key + "_sliced"] = time_sell(sets, 30, key)
         R, xr, yr = sets[0]
         spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
         del sets, R, xr, yr
@@ -1076,6 +1161,8 @@ def main():
             log(f"[serve] FP64 single-system GPU solve on {nd} of the requests: {nd / tot:.1f}/s")
         if args.capacity_seconds > 0:
             serving["capacity"] = serving_capacity(args.capacity_seconds)
+        if args.payload_seconds > 0:
+            serving["host_payload"] = serving_payload(args.payload_seconds)
 
     # ---- single large system (2049^2, constant coefficient) -----------------------------
     xt = uniform_pm1(3, n)

@@ -32,6 +32,8 @@ _SIGS = {
     "honei_hip_init": [_i],
     "honei_hip_shutdown": [],
     "honei_hip_malloc": [C.POINTER(_p), _sz],
+    "honei_hip_host_alloc": [C.POINTER(_p), _sz],
+    "honei_hip_host_free": [_p],
     "honei_hip_free": [_p],
     "honei_hip_upload": [_p, _p, _sz, _p],
     "honei_hip_download": [_p, _p, _sz, _p],
@@ -60,6 +62,10 @@ _SIGS = {
     "honei_hip_defect_smell_dv_double_idx64": [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _sz, _p],
     "honei_hip_narrow_indices": [_p, _p, _sz, _u64, _p],
     "honei_hip_stream_probe": [_i, _p, _p, _sz, _sz, _p],
+    "honei_hip_sell_create": [C.POINTER(_p), _i, _p, _p, _p, _sz, _sz, _sz, _sz, _sz, _p],
+    "honei_hip_sell_info": [_p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)],
+    "honei_hip_product_sell": [_p, _p, _p, _p],
+    "honei_hip_sell_destroy": [_p],
     "honei_hip_defect_norm_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_jacobi_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_dot_product_two_float": [C.POINTER(C.c_float), _p, _p, _sz, _p],
@@ -324,6 +330,36 @@ class DeviceEll:
              None if acc is None else acc.ptr, stream)
 
 
+class Sell:
+    """Sliced copy of a threads = 1 DeviceEll (honei_hip_sell_*): rows sorted
+    by length within windows of 2048, slices of 64 padded to their longest
+    row; products bit-identical to the ELL product."""
+
+    def __init__(self, D: "DeviceEll", stream=None):
+        p = _p()
+        call("honei_hip_sell_create", C.byref(p), D.vtype, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, D.rows, D.cols, D.ncpr,
+             D.stride, D.threads, stream)
+        self.h = p.value
+        self.rows = D.rows
+        sl, pad, nb = _sz(), _sz(), _sz()
+        call("honei_hip_sell_info", self.h, C.byref(sl), C.byref(pad), C.byref(nb))
+        self.slices, self.padded_slots, self.device_bytes = sl.value, pad.value, nb.value
+
+    def product(self, y: "DeviceArray", x: "DeviceArray", stream=None):
+        call("honei_hip_product_sell", self.h, x.ptr, y.ptr, stream)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().honei_hip_sell_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class MGConfig(C.Structure):
     """honei_hip_mg_config (include/honei_hip.h)."""
     _fields_ = [("n_max_iter", _sz), ("n_pre_smooth", _sz), ("n_post_smooth", _sz),
@@ -418,7 +454,8 @@ class BatchConfig(C.Structure):
     _fields_ = [("inner_vtype", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", _d),
                 ("tolerance", _d), ("max_outer", C.c_int32), ("averaging", C.c_int32),
                 ("iters_per_launch", C.c_int32), ("autotune", C.c_int32), ("outer_method", C.c_int32),
-                ("fused_levels", C.c_int32), ("use_graphs", C.c_int32), ("pair_sweeps", C.c_int32)]
+                ("fused_levels", C.c_int32), ("use_graphs", C.c_int32), ("pair_sweeps", C.c_int32),
+                ("matrix_free", C.c_int32)]
 
 
 class SlotInfo(C.Structure):
@@ -440,7 +477,7 @@ def batch_config(**kw):
 class Request(C.Structure):
     """honei_hip_request (include/honei_hip.h)."""
     _fields_ = [("arrival_s", _d), ("level", C.c_int32), ("pad", C.c_int32), ("coeff_seed", _u64),
-                ("rhs_seed", _u64), ("x_dev", _p)]
+                ("rhs_seed", _u64), ("x_dev", _p), ("coeff_host", _p), ("b_host", _p), ("x_host", _p)]
 
 
 class RequestResult(C.Structure):
@@ -462,10 +499,36 @@ class ServeConfig(C.Structure):
                 ("levels", C.c_int64 * 16), ("counts", C.c_int64 * 16), ("budget_bytes", _u64), ("sigma", _d)]
 
 
+class PinnedArray:
+    """Pinned host memory (honei_hip_host_alloc) with a numpy view."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        p = _p()
+        call("honei_hip_host_alloc", C.byref(p), max(1, self.n * self.dtype.itemsize))
+        self.ptr = p.value
+        buf = (C.c_char * max(1, self.n * self.dtype.itemsize)).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=self.n)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.a = None
+            lib().honei_hip_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def serve(levels, requests, counts=None, budget_bytes=0, sigma=1.0, max_slots_per_class=256, **batch_kw):
     """Run the continuous-batching loop over `requests` (list of
-    (arrival_s, level, coeff_seed, rhs_seed[, DeviceArray x])).  Returns
-    (list of RequestResult, ServeStats)."""
+    (arrival_s, level, coeff_seed, rhs_seed[, DeviceArray x[, PinnedArray
+    coeff, PinnedArray b, PinnedArray x_host]])).  Returns (list of
+    RequestResult, ServeStats)."""
     cfg = ServeConfig()
     cfg.batch = batch_config(**batch_kw)
     cfg.nclass = len(levels)
@@ -479,6 +542,10 @@ def serve(levels, requests, counts=None, budget_bytes=0, sigma=1.0, max_slots_pe
     for i, r in enumerate(requests):
         rq[i].arrival_s, rq[i].level, rq[i].coeff_seed, rq[i].rhs_seed = r[0], r[1], r[2], r[3]
         rq[i].x_dev = r[4].ptr if len(r) > 4 and r[4] is not None else None
+        if len(r) > 5:
+            rq[i].coeff_host = r[5].ptr if r[5] is not None else None
+            rq[i].b_host = r[6].ptr if r[6] is not None else None
+            rq[i].x_host = r[7].ptr if len(r) > 7 and r[7] is not None else None
     res = (RequestResult * max(1, len(requests)))()
     st = ServeStats()
     call("honei_hip_serve_run", C.byref(cfg), rq, len(requests), res, C.byref(st))

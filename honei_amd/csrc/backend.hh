@@ -50,6 +50,7 @@ struct EllArgs {
     double *norm2 = nullptr;      // DEFECT: ||out||^2 into this device double
     int64_t row_start = 0, row_end = 0;
     int block = 256;
+    const int *gate = nullptr;    // non-null: the launch does nothing unless *gate != 0 (device flag)
 };
 
 // Vector element type follows the matrix: f32 and bf16 matrices act on f32
@@ -76,9 +77,10 @@ hipError_t dot(const void *x, const void *y, int64_t n, ValueType vt, double *ou
                const ReduceSlots &slots, hipStream_t s);
 hipError_t norm2(const void *x, int64_t n, ValueType vt, double *out, const ReduceSlots &slots,
                  hipStream_t s);
-// x += b*y ; b either a host value (b_dev null) or read from device memory.
+// x += b*y ; b either a host value (b_dev null) or read from device memory;
+// skip non-null: nothing happens while *skip != 0 (device flag).
 hipError_t axpy(void *x, const void *y, double b, const double *b_dev, int64_t n, ValueType vt,
-                hipStream_t s);
+                hipStream_t s, const int *skip = nullptr);
 // r = x + b*y
 hipError_t scaled_sum3(void *r, const void *x, const void *y, double b, int64_t n, ValueType vt,
                        hipStream_t s);
@@ -104,13 +106,28 @@ hipError_t stream_probe(int kind, void *dst, const void *src, int64_t n4, int gr
 hipError_t narrow_indices(uint32_t *d, const uint64_t *src, int64_t n, uint64_t bound, unsigned int *bad,
                           hipStream_t s);
 
+// ---- sliced ELL (sell_kernels.hip) ----------------------------------------
+// Rows sorted by length (longest first) within windows of sell_window()
+// rows, in slices of sell_slice_rows(); see sell_kernels.hip.
+int sell_window();
+int sell_slice_rows();
+hipError_t sell_sort(const uint32_t *Arl, int64_t rows, int32_t *perm, uint32_t *len, hipStream_t s);
+hipError_t sell_widths(const uint32_t *len, int64_t rows, int64_t slices, int32_t *width, hipStream_t s);
+hipError_t sell_fill(ValueType vt, const int32_t *Aj, const void *Ax, int64_t stride, int64_t rows,
+                     const int32_t *perm, const uint32_t *len, const int32_t *width, const int64_t *base,
+                     int32_t *sAj, void *sAx, hipStream_t s);
+hipError_t sell_spmv(ValueType vt, const int32_t *sAj, const void *sAx, const int32_t *perm, const uint32_t *len,
+                     const int32_t *width, const int64_t *base, int64_t rows, const void *x, void *y, hipStream_t s);
+
 // Banded Q1 product y = A x (rhs null) or defect y = rhs - A x; bands[9] =
 // LL, LD, LU, DL, DD, DU, UL, UD, UU device arrays of n = m*m values.
 hipError_t q1_banded(ValueType vt, const void *const *bands, const void *x, const void *rhs, void *y, int64_t n,
                      int64_t m, int block, hipStream_t s);
-// x += alpha*y and return ||x||^2 (fused; alpha from device memory when non-null)
+// x += alpha*y and return ||x||^2 (fused; alpha from device memory when
+// non-null); skip as for axpy (out then keeps its value).
 hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_dev, int64_t n,
-                      ValueType vt, double *out, const ReduceSlots &slots, hipStream_t s);
+                      ValueType vt, double *out, const ReduceSlots &slots, hipStream_t s,
+                      const int *skip = nullptr);
 
 // ---- geometric transfers on (2^L+1)^2 lexicographic grids --------------
 // mask: 8 entries (corners then edges) on the host; value 2 = Dirichlet.

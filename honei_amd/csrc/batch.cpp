@@ -74,9 +74,7 @@ void BatchMG::release() {
     build_host_ = nullptr;
     if (build_dev_) (void)hipFree(build_dev_);
     build_dev_ = nullptr;
-    if (build_kappa_) (void)hipFree(build_kappa_);
-    build_kappa_ = nullptr;
-    build_cap_ = build_kappa_cap_ = 0;
+    build_cap_ = 0;
     bytes_ = 0;
 }
 
@@ -197,21 +195,29 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
             tl.x = tl.dinv + ns * g.vld;
             tl.rhs = tl.x + ns * g.vld;
             float *out = tl.rhs + ns * g.vld;
-            for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
-                auto launch = [&]() {
-                    return pass == 0 ? b_pair(vt, PAIR_PRE, tl, w, tl.x, out, nullptr, s_main_)
-                                     : b_pair(vt, PAIR_POST, tl, w, out, nullptr, tl.x, s_main_);
-                };
-                const int reps = n >= 65536 ? 10 : 30;
-                for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch();
-                if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
-                for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
-                if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
-                if (err == hipSuccess) err = hipEventSynchronize(e1);
-                float ms = 0.f;
-                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-                (pass == 0 ? best.pre_us : best.post_us) = 1000.0 * ms / reps;
+            tl.kld = round_up((g.side - 1) * (g.side - 1), 32);
+            err = hipMalloc((void **)&tl.kappa, (size_t)(ns * tl.kld) * 8);
+            if (err == hipSuccess) err = vfill(tl.kappa, 1.0, ns * tl.kld, VT_F64, s_main_);
+            // both forms: matrix values read (mf 0) and formed from the cell coefficients (mf 1)
+            for (int mf = 0; mf < 2; ++mf) {
+                for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
+                    auto launch = [&]() {
+                        return pass == 0 ? b_pair(vt, PAIR_PRE, tl, w, tl.x, out, nullptr, s_main_, mf == 1)
+                                         : b_pair(vt, PAIR_POST, tl, w, out, nullptr, tl.x, s_main_, mf == 1);
+                    };
+                    const int reps = n >= 65536 ? 10 : 30;
+                    for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch();
+                    if (err == hipSuccess) err = hipEventRecord(e0, s_main_);
+                    for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+                    if (err == hipSuccess) err = hipEventRecord(e1, s_main_);
+                    if (err == hipSuccess) err = hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                    (pass == 0 ? best.pre_us[mf] : best.post_us[mf]) = 1000.0 * ms / reps;
+                }
             }
+            (void)hipStreamSynchronize(s_main_);
+            if (tl.kappa) (void)hipFree(tl.kappa);
         }
         (void)hipStreamSynchronize(s_main_);
         for (void *p : {Aj, Arl, Ax, vec, scd, lst})
@@ -243,13 +249,16 @@ uint64_t BatchMG::plan_bytes() const {
     const size_t vs = vt_size((ValueType)cfg_.inner_vtype);
     for (int64_t l = 1; l <= max_level_; ++l) {
         const BLevel &lv = L_[(size_t)l];
-        b += (uint64_t)lv.ax_ld * 4 + (uint64_t)lv.n * 4;                          // Aj, Arl
-        b += (uint64_t)(lv.cap * lv.ax_ld) * vs;                                    // Ax
+        if (stores_values(lv)) {
+            b += (uint64_t)lv.ax_ld * 4 + (uint64_t)lv.n * 4;                      // Aj, Arl
+            b += (uint64_t)(lv.cap * lv.ax_ld) * vs;                                // Ax
+        }
         b += (uint64_t)(lv.cap * lv.vld) * 4 * (l == 1 ? 8 : 5);                    // vectors
+        b += (uint64_t)(lv.cap * lv.kld) * 8;                                       // cell coefficients
     }
     for (const BClass &c : C_) {
-        b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4;
-        b += (uint64_t)(c.count * c.ax_ld) * 8 + (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
+        if (stores_outer_values()) b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4 + (uint64_t)(c.count * c.ax_ld) * 8;
+        b += (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
         b += (uint64_t)(c.count * 4 * c.bps) * 8;   // partials
     }
     b += (uint64_t)nslots_ * sizeof(SlotScalars);
@@ -269,6 +278,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     if (cfg.outer != 0 && cfg.outer != 1) return hipErrorInvalidValue;
     if (cfg.use_graphs != 0 && cfg.use_graphs != 1) return hipErrorInvalidValue;
     if (cfg.pair_sweeps != 0 && cfg.pair_sweeps != 1) return hipErrorInvalidValue;
+    if (cfg.matrix_free != 0 && cfg.matrix_free != 1) return hipErrorInvalidValue;
     std::vector<int> order((size_t)nclass);
     for (int c = 0; c < nclass; ++c) {
         if (levels[c] < 2 || levels[c] > 13 || counts[c] < 1 || counts[c] > 65535) return hipErrorInvalidValue;
@@ -321,6 +331,7 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
         lv.ncpr = g.ncpr;
         lv.ax_ld = g.ax_ld;
         lv.vld = g.vld;
+        lv.kld = round_up((lv.side - 1) * (lv.side - 1), 32);
     }
     if (budget_bytes && plan_bytes() > budget_bytes) {
         over_budget_ = true;
@@ -331,24 +342,27 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     const size_t vs = vt_size(vt);
     for (int64_t l = 1; l <= max_level_; ++l) {
         BLevel &lv = L_[(size_t)l];
+        const size_t vb = (size_t)(lv.cap * lv.vld) * 4;
+        for (float **p : {&lv.dinv, &lv.rhs, &lv.x, &lv.d, &lv.t}) HX(alloc((void **)p, vb));
+        HX(alloc((void **)&lv.kappa, (size_t)(lv.cap * lv.kld) * 8));
+        if (l == 1)
+            for (float **p : {&lv.g, &lv.u, &lv.en}) HX(alloc((void **)p, vb));
+        if (!stores_values(lv)) continue;   // matrix-free paired level: no index arrays or values
         HX(alloc((void **)&lv.Aj, (size_t)lv.ax_ld * 4));
         HX(alloc((void **)&lv.Arl, (size_t)lv.n * 4));
         HX(alloc(&lv.Ax, (size_t)(lv.cap * lv.ax_ld) * vs));
-        const size_t vb = (size_t)(lv.cap * lv.vld) * 4;
-        for (float **p : {&lv.dinv, &lv.rhs, &lv.x, &lv.d, &lv.t}) HX(alloc((void **)p, vb));
-        if (l == 1)
-            for (float **p : {&lv.g, &lv.u, &lv.en}) HX(alloc((void **)p, vb));
         // shared index arrays (values of slot 0 are placeholders until a load)
         HX(q1_assemble(vt, lv.side, lv.Aj, lv.Ax, lv.Arl, lv.stride, nullptr, lv.dinv, cfg_.omega, s_main_, lv.T));
     }
     for (BClass &c : C_) {
-        HX(alloc((void **)&c.Aj, (size_t)c.ax_ld * 4));
-        HX(alloc((void **)&c.Arl, (size_t)c.n * 4));
-        HX(alloc((void **)&c.Ax, (size_t)(c.count * c.ax_ld) * 8));
         for (double **p : {&c.b, &c.x, &c.d}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
         if (cfg_.outer == 1)
             for (double **p : {&c.p, &c.q}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
         HX(alloc((void **)&c.partials, (size_t)(c.count * 4 * c.bps) * 8));   // 64-row chunks
+        if (!stores_outer_values()) continue;
+        HX(alloc((void **)&c.Aj, (size_t)c.ax_ld * 4));
+        HX(alloc((void **)&c.Arl, (size_t)c.n * 4));
+        HX(alloc((void **)&c.Ax, (size_t)(c.count * c.ax_ld) * 8));
         HX(q1_assemble(VT_F64, c.side, c.Aj, c.Ax, c.Arl, c.stride, nullptr, nullptr, 0.0, s_main_, 1));
     }
     HX(alloc((void **)&sc_, (size_t)nslots_ * sizeof(SlotScalars)));
@@ -391,7 +405,7 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         const BWork w = level_work(l);
         if (paired(lv) && cfg_.n_pre == 2) {
             // x = Jacobi0, d = rhs - A x: one launch
-            HX(b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s));
+            HX(b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s, cfg_.matrix_free != 0));
             HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
             continue;
         }
@@ -448,7 +462,7 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         if (paired(lv) && cfg_.n_post == 2) {
             // t = x + P e, then x = J(J(t)): one launch for both sweeps
             HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.t, s));
-            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s));
+            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s, cfg_.matrix_free != 0));
             continue;
         }
         HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.x, s));
@@ -490,6 +504,8 @@ BClassTab BatchMG::class_table() const {
         e.partials = c.partials;
         e.rhs_top = top.rhs;
         e.x_top = top.x;
+        e.kappa = top.kappa;
+        e.kld = top.kld;
     }
     return t;
 }
@@ -508,13 +524,13 @@ hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     const BWork w = all_work();
     int64_t blocks = 0;
     for (const BClass &c : C_) blocks += c.count * c.bps;
-    const bool cg = cfg_.outer == 1;
+    const bool cg = cfg_.outer == 1, mf = cfg_.matrix_free != 0;
     if (cg) {   // CG keeps its recursive residual: only new / verifying slots need b - A x
         HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_UNSETTLED, s));
-        HX(b_outer(OP_DEFECT_SETTLE, tab, w, blocks, sc_, s));
+        HX(b_outer(OP_DEFECT_SETTLE, tab, w, blocks, sc_, s, mf));
     }
     HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_NONIDLE, s));
-    if (!cg) HX(b_outer(OP_DEFECT_ALL, tab, w, blocks, sc_, s));
+    if (!cg) HX(b_outer(OP_DEFECT_ALL, tab, w, blocks, sc_, s, mf));
     HX(b_finish(cg ? FIN_CHECK_CG : FIN_CHECK_DC, tab, w, nslots_, sc_, cfg_.tolerance, cfg_.max_outer, s));
     HX(b_compact(sc_, nslots_, boundaries_.data(), nb, list_, bounds_, LIST_ACTIVE, s));
     HX(b_outer(OP_SCALE, tab, w, blocks, sc_, s));
@@ -523,7 +539,7 @@ hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     HX(b_outer(OP_DOT_RZ, tab, w, blocks, sc_, s));
     HX(b_finish(FIN_BETA, tab, w, nslots_, sc_, 0.0, 0, s));
     HX(b_outer(OP_PUPDATE, tab, w, blocks, sc_, s));
-    HX(b_outer(OP_SPMV_DOT, tab, w, blocks, sc_, s));
+    HX(b_outer(OP_SPMV_DOT, tab, w, blocks, sc_, s, mf));
     HX(b_finish(FIN_ALPHA, tab, w, nslots_, sc_, 0.0, 0, s));
     return b_outer(OP_UPDATE, tab, w, blocks, sc_, s);
 }
@@ -574,7 +590,6 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
     const bool seeded = coeffs == nullptr;
     if (seeded ? (!cseeds || !bseeds) : !bs) return hipErrorInvalidValue;
     std::vector<int64_t> cls((size_t)n);
-    int64_t kappa_need = 0;
     int maxl = 1;
     for (int64_t e = 0; e < n; ++e) {
         const int64_t g = slots[e];
@@ -585,9 +600,7 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
         size_t k = 0;
         while (!(g >= C_[k].first && g < C_[k].first + C_[k].count)) ++k;
         cls[(size_t)e] = (int64_t)k;
-        const int L = (int)C_[k].level;
-        maxl = std::max(maxl, L);
-        for (int l = 1; l <= L; ++l) kappa_need += (1ll << (2 * l));
+        maxl = std::max(maxl, (int)C_[k].level);
     }
     hipStream_t s = s_load_;
     // staging and scratch of the previous build must be free
@@ -602,19 +615,13 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
         HX(hipMalloc((void **)&build_dev_, sizeof(BuildEntry) * (size_t)n));
         build_cap_ = n;
     }
-    if (kappa_need > build_kappa_cap_) {
-        if (build_kappa_) (void)hipFree(build_kappa_);
-        build_kappa_ = nullptr;
-        build_kappa_cap_ = 0;
-        HX(hipMalloc((void **)&build_kappa_, sizeof(double) * (size_t)kappa_need));
-        build_kappa_cap_ = kappa_need;
-    }
     const ValueType vt = (ValueType)cfg_.inner_vtype;
     const size_t vs = vt_size(vt);
     BuildParams P{};
     P.entries = (int32_t)n;
     P.averaging = cfg_.averaging;
     P.vt = vt;
+    P.outer_values = stores_outer_values() ? 1 : 0;
     P.sigma = sigma;
     P.omega = cfg_.omega;
     for (int64_t l = 1; l <= max_level_; ++l) {
@@ -623,7 +630,6 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
         P.T[l] = L_[(size_t)l].T;
     }
     for (const BClass &c : C_) P.stride64[c.level] = c.stride;
-    int64_t koff = 0;
     for (int64_t e = 0; e < n; ++e) {
         const int64_t g = slots[e];
         const BClass &c = C_[(size_t)cls[(size_t)e]];
@@ -638,12 +644,11 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
         b.b_src = seeded ? nullptr : bs[e];
         for (int64_t l = 1; l <= c.level; ++l) {
             const BLevel &lv = L_[(size_t)l];
-            b.kappa[l] = build_kappa_ + koff;
-            koff += (1ll << (2 * l));
-            b.Ax[l] = (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs;
+            b.kappa[l] = lv.kappa + g * lv.kld;
+            b.Ax[l] = lv.Ax ? (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs : nullptr;
             b.dinv[l] = lv.dinv + g * lv.vld;
         }
-        b.Ax64 = c.Ax + ls * c.ax_ld;
+        b.Ax64 = c.Ax ? c.Ax + ls * c.ax_ld : nullptr;
         b.b = c.b + ls * c.vld;
         b.x = c.x + ls * c.vld;
     }
@@ -709,6 +714,15 @@ hipError_t BatchMG::poll(std::vector<SlotScalars> &out, std::vector<int64_t> *re
     return hipSuccess;
 }
 
+const double *BatchMG::solution(int64_t g, int64_t *n) const {
+    for (const BClass &c : C_)
+        if (g >= c.first && g < c.first + c.count) {
+            if (n) *n = c.n;
+            return c.x + (g - c.first) * c.vld;
+        }
+    return nullptr;
+}
+
 hipError_t BatchMG::result(int64_t g, double *dst) {
     if (g < 0 || g >= nslots_ || !dst) return hipErrorInvalidValue;
     for (const BClass &c : C_)
@@ -752,15 +766,16 @@ hipError_t BatchMG::time_level(int64_t level, int reps, double us[5]) {
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
         if (err == hipSuccess) *out = 1000.0 * ms / reps;
     };
-    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_); }, &us[0]);
+    const bool mf = cfg_.matrix_free != 0;
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_, mf); }, &us[0]);
     timeit([&]() { return b_restrict(lv, lc, w, s_main_); }, &us[1]);
     timeit([&]() { return b_prolong_axpy(lv, lc, w, lv.t, s_main_); }, &us[2]);
-    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_); }, &us[3]);
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, mf); }, &us[3]);
     if (cfg_.outer == 1) {
         const BClassTab tab = class_table();
         int64_t blocks = 0;
         for (const BClass &c : C_) blocks += c.count * c.bps;
-        timeit([&]() { return b_outer(OP_SPMV_DOT, tab, wa, blocks, sc_, s_main_); }, &us[4]);
+        timeit([&]() { return b_outer(OP_SPMV_DOT, tab, wa, blocks, sc_, s_main_, mf); }, &us[4]);
     }
     (void)hipStreamSynchronize(s_main_);
     if (e0) (void)hipEventDestroy(e0);

@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -91,6 +92,11 @@ struct BatchConfig {
     // defect and the two post-smoothing sweeps each run as one launch that
     // reads the matrix once (n_pre == 2 / n_post == 2); 0: one launch per pass.
     int32_t pair_sweeps = 1;
+    // 1: the paired launches and the FP64 outer row sums form each row's
+    // nine weights from the slot's cell coefficients (8 bytes per node) with
+    // the assembly's own expressions instead of reading the stored values
+    // (9 per row): the same bits; 0: read the stored values.
+    int32_t matrix_free = 1;
 };
 
 // Inner level l of the pool: slots [0, cap) present.
@@ -103,6 +109,8 @@ struct BLevel {
     void *Ax = nullptr;
     float *dinv = nullptr, *rhs = nullptr, *x = nullptr, *d = nullptr, *t = nullptr;
     float *g = nullptr, *u = nullptr, *en = nullptr;   // coarse CG scratch (level 1 only)
+    double *kappa = nullptr;   // cell coefficients [cap][kld] ((side-1)^2 per slot)
+    int64_t kld = 0;
 };
 
 // Class c: slots [first, first + count), finest level `level`, FP64 outer data.
@@ -121,7 +129,9 @@ struct LaunchShape {
     int T = 1, block = 256, group = 1;
     double us = 0.0;       // measured time of one batched Jacobi sweep
     int64_t slots = 0;     // slots of the tuning batch
-    double pre_us = 0.0, post_us = 0.0;   // paired launches (b_pair) on the same batch, T = 1
+    // paired launches (b_pair) on the same batch, T = 1, with the matrix
+    // values read [0] and formed from cell coefficients [1]
+    double pre_us[2] = {0.0, 0.0}, post_us[2] = {0.0, 0.0};
 };
 
 // Batched kernels (batch_kernels.hip).
@@ -152,6 +162,8 @@ struct BClassEntry {
     double *x, *r, *p, *q, *partials;
     float *rhs_top;     // inner right-hand side of the class's finest level
     const float *x_top; // inner solution of the class's finest level
+    const double *kappa;   // cell coefficients of the finest level, [global slot][kld]
+    int64_t kld;
 };
 struct BClassTab {
     int32_t nclass;
@@ -180,6 +192,8 @@ struct BMat {
     const uint32_t *Arl;
     const void *Ax;
     int64_t stride, ax_ld, n, vld, bps;
+    const double *kappa;   // cell coefficients [slot][kld] (matrix-free launches)
+    int64_t kld;
 };
 
 // ---- bulk hierarchy build (many slots per launch) ----------------------------
@@ -193,13 +207,14 @@ struct BuildEntry {
     int32_t seeded;               // 1: coefficient exp(sigma N(0,1)), b ~ U[-1,1) from the seeds
     uint64_t coeff_seed, rhs_seed;
     const double *coeff_src, *b_src;
-    double *kappa[kMaxLevel + 1]; // coefficient of level l ((2^l)^2 cells), scratch; kappa[level] finest
+    double *kappa[kMaxLevel + 1]; // coefficient of level l ((2^l)^2 cells) of this slot; kappa[level] finest
     void *Ax[kMaxLevel + 1];      // this slot's inner values per level
     float *dinv[kMaxLevel + 1];   // this slot's omega / diag per level
     double *Ax64, *b, *x;         // FP64 outer values, right-hand side, start vector
 };
 struct BuildParams {
-    int32_t entries, averaging, vt, pad;
+    int32_t entries, averaging, vt;
+    int32_t outer_values;         // 0: the pool stores no FP64 outer values (matrix-free outer)
     double sigma, omega;
     int64_t side[kMaxLevel + 1], stride[kMaxLevel + 1], stride64[kMaxLevel + 1];
     int32_t T[kMaxLevel + 1];
@@ -254,6 +269,10 @@ class BatchMG {
     bool busy(int64_t g) const { return host_state_[(size_t)g] != 0; }
     bool loading(int64_t g) const { return host_state_[(size_t)g] == 1; }
     int64_t running() const { return running_; }
+    // stream of the bulk builds (work enqueued on it before load_many runs first)
+    hipStream_t load_stream() const { return s_load_; }
+    // FP64 solution of slot g (device, side^2 values; *n receives side^2), or null
+    const double *solution(int64_t g, int64_t *n) const;
     int64_t loading_count() const { return (int64_t)loading_.size(); }
     int64_t launches() const { return launches_; }
     const std::vector<BLevel> &levels() const { return L_; }
@@ -273,6 +292,17 @@ class BatchMG {
     bool paired(const BLevel &lv) const {
         return cfg_.pair_sweeps && lv.side >= kPairMinSide && (cfg_.n_pre == 2 || cfg_.n_post == 2);
     }
+    // Some kernel reads level l's stored values: every level except those
+    // whose smoothing passes all run as matrix-free paired launches outside
+    // the fused coarse launch.
+    bool stores_values(const BLevel &lv) const {
+        const int64_t lf = cfg_.fused_levels >= 2 ? std::min<int64_t>(cfg_.fused_levels, max_level_) : 1;
+        const bool all_paired = paired(lv) && cfg_.n_pre == 2 && (cfg_.n_post == 2 || cfg_.n_post == 0);
+        return !(cfg_.matrix_free && all_paired && lv.level > lf);
+    }
+    // the FP64 outer values are read only when the outer row sums do not
+    // form them from the cell coefficients
+    bool stores_outer_values() const { return !cfg_.matrix_free; }
   private:
     BatchConfig cfg_;
     std::vector<BLevel> L_;          // indexed by level (0 unused)
@@ -298,11 +328,9 @@ class BatchMG {
     hipEvent_t null_ev_ = nullptr;
 
     hipError_t alloc(void **p, size_t bytes);
-    // bulk build state: device entry table, pinned staging, coefficient scratch
+    // bulk build state: device entry table, pinned staging
     BuildEntry *build_dev_ = nullptr, *build_host_ = nullptr;
     int64_t build_cap_ = 0;
-    double *build_kappa_ = nullptr;
-    int64_t build_kappa_cap_ = 0;
     hipEvent_t build_ev_ = nullptr;
     hipError_t tune_level(BLevel &lv);
     hipError_t capture();
@@ -332,8 +360,10 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
                  int64_t max_slots, void *out, const void *x, const void *rhs, const void *dinv, hipStream_t s);
 // FP64 outer work over the active slots of all classes (w: lo = 0, hi = nclass).
 // max_blocks: 256-row blocks of all slots of all classes.
+// mf: row sums from the cell coefficients (BClassEntry::kappa) instead of
+// the stored FP64 values.
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
-                   hipStream_t s);
+                   hipStream_t s, bool mf = false);
 // One block per listed slot: reduce its partials and update its scalars.
 hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
                     double tol, int32_t max_outer, hipStream_t s);
@@ -344,8 +374,9 @@ hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork 
 // PAIR_PRE writes x = Jacobi0 to out0 and d = rhs - A x to out1; PAIR_POST
 // writes two Jacobi sweeps of xin to out0 (xin != out0).  Bit-identical to
 // the separate b_ell launches.
+// mf: the matrix weights from lv.kappa (bit-identical to the stored ones).
 hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
-                  const float *xin, hipStream_t s);
+                  const float *xin, hipStream_t s, bool mf = false);
 hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);
 hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s);
 // Slot ids of one state-setting launch (kernel argument).

@@ -44,6 +44,7 @@ BatchConfig to_config(const honei_hip_batch_config &c) {
     b.fused_levels = c.fused_levels;
     b.use_graphs = c.use_graphs;
     b.pair_sweeps = c.pair_sweeps;
+    b.matrix_free = c.matrix_free;
     return b;
 }
 
@@ -77,6 +78,7 @@ int honei_hip_batch_config_default(honei_hip_batch_config *cfg) {
     cfg->fused_levels = b.fused_levels;
     cfg->use_graphs = b.use_graphs;
     cfg->pair_sweeps = b.pair_sweeps;
+    cfg->matrix_free = b.matrix_free;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -107,6 +109,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "use_graphs 0 or 1");
     if (cfg->pair_sweeps != 0 && cfg->pair_sweeps != 1)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "pair_sweeps 0 or 1");
+    if (cfg->matrix_free != 0 && cfg->matrix_free != 1)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "matrix_free 0 or 1");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");
@@ -252,8 +256,9 @@ int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, doub
     const bool tuned = hipGetDevice(&dev) == hipSuccess &&
                        batch_tuned_shape(dev, level, b->config().inner_vtype, &sh) && b->config().autotune &&
                        b->paired(L[(size_t)level]);
-    if (pre_us) *pre_us = tuned ? sh.pre_us : 0.0;
-    if (post_us) *post_us = tuned ? sh.post_us : 0.0;
+    const int mf = b->config().matrix_free != 0 ? 1 : 0;
+    if (pre_us) *pre_us = tuned ? sh.pre_us[mf] : 0.0;
+    if (post_us) *post_us = tuned ? sh.post_us[mf] : 0.0;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -290,8 +295,10 @@ int honei_hip_serve_run(const honei_hip_serve_config *cfg, const honei_hip_reque
     for (int64_t r = 0; r < nreq; ++r) {
         if (r > 0 && reqs[r].arrival_s < reqs[r - 1].arrival_s)
             return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "arrival times must be non-decreasing");
+        if ((reqs[r].coeff_host == nullptr) != (reqs[r].b_host == nullptr))
+            return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "coeff_host and b_host: both or neither");
         rq[(size_t)r] = ServeRequest{reqs[r].arrival_s, reqs[r].level, reqs[r].coeff_seed, reqs[r].rhs_seed,
-                                     reqs[r].x_dev};
+                                     reqs[r].x_dev, reqs[r].coeff_host, reqs[r].b_host, reqs[r].x_host};
     }
     std::vector<ServeRecord> rec((size_t)nreq);
     ServeStats st;

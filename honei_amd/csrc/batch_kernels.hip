@@ -178,8 +178,8 @@ struct PairGeom {
 template <typename VT> __device__ inline float load_value_cached(const VT *p) { return *p; }
 template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { return bf16_to_float(p->bits); }
 
-template <typename VT, int PAIR>
-__global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork w, PairGeom pg, float *out0,
+template <typename VT, int PAIR, bool MF>
+__global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWork w, PairGeom pg, float *out0,
                                                                   float *out1, const float *xin, const float *rhs,
                                                                   const float *dinv) {
     // first-pass results of the tile and its halo
@@ -245,8 +245,12 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork 
         }
         // ---- first pass, grid row r ----
         float av[9];
+        if constexpr (MF) {
+            q1_stored_weights<VT, float>(S, rc, colc, bnd, m.kappa + ls * m.kld, av);
+        } else {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
+            for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
+        }
         const float rh = rs[node], di = ds[node];
         __syncthreads();
         {
@@ -308,9 +312,12 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_pair(BMat m, BWork 
 // 9-point stencil in column order; boundary row: the diagonal alone), so no
 // index array is read; the values, the guarded sum and its order are those of
 // ell_lane_sum_group_w over the stored matrix, hence the same bits.
-template <int G>
+// MF: the values from the slots' cell coefficients (kap[g], q1_stored_weights)
+// instead of Ax[g] -- the same doubles.
+template <int G, bool MF>
 __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const double *const (&Ax)[G], int64_t stride,
-                                            const double *const (&x)[G], double (&s)[G]) {
+                                            const double *const (&kap)[G], const double *const (&x)[G],
+                                            double (&s)[G]) {
     const uint32_t r = (uint32_t)row / (uint32_t)side, c = (uint32_t)row - r * (uint32_t)side;
     const bool bnd = r == 0 || r == side - 1 || c == 0 || c == side - 1;
     int64_t j[9];
@@ -319,9 +326,10 @@ __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const dou
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         double av[9], xv[9];
+        if constexpr (MF) q1_stored_weights<double, double>(side, r, c, bnd, kap[g], av);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            av[k] = load_value_nt(Ax[g] + row + (int64_t)k * stride);
+            if constexpr (!MF) av[k] = load_value_nt(Ax[g] + row + (int64_t)k * stride);
             xv[k] = x[g][j[k]];
         }
         double t = 0.0;
@@ -336,7 +344,7 @@ __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const dou
 // first slot in the spare places and stores only its own slots).  The
 // neighbour columns come from the grid position, so slots share no loads;
 // the launches use G = kOuterRowGroup.
-template <int OP, int G, bool DC>
+template <int OP, int G, bool DC, bool MF>
 __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, const SlotScalars *sc) {
     static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE, "defect op (q = A p: k_b_outer_spmv_dot)");
     __shared__ int64_t pre[kMaxClasses + 1];
@@ -366,17 +374,16 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 #pragma unroll
         for (int k = 0; k < G; ++k) part[k] = 0.0;
         if (row < e.n) {
-            const double *ax[G], *xv[G], *unused[G];
+            const double *ax[G], *xv[G], *kp[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 ax[k] = e.Ax + ls[k] * e.ax_ld;
                 xv[k] = e.x + ls[k] * e.vld;
-                unused[k] = nullptr;
+                kp[k] = MF ? e.kappa + (ls[k] + e.first) * e.kld : nullptr;
             }
             double sum[G];
             // the FP64 outer operator: Q1 (q1_row), one lane per row
-            (void)unused;
-            q1_stencil_sum_group<G>(e.side, row, ax, e.stride, xv, sum);
+            q1_stencil_sum_group<G, MF>(e.side, row, ax, e.stride, kp, xv, sum);
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 const int64_t vi = ls[k] * e.vld + row;
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 // of one listed slot, thread t takes row t of each with every load of the R
 // rows issued before any sum (rows past the end load the last row and store
 // nothing).  Same sums as k_b_outer_rows<OP_SPMV_DOT>.
-template <int R>
+template <int R, bool MF>
 __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w) {
     __shared__ int64_t pre[kMaxClasses + 1];
     if (threadIdx.x == 0) {
@@ -430,6 +437,7 @@ __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w
         div32(lb - pre[c], (e.bps + R - 1) / R, gi, hb);
         const int64_t ls = w.list[w.bounds[c] + gi] - e.first;
         const double *ax = e.Ax + ls * e.ax_ld;
+        const double *kap = MF ? e.kappa + (ls + e.first) * e.kld : nullptr;
         const double *pv = e.p + ls * e.vld;
         const int64_t S = e.side;
         double av[R][9], xv[R][9], pr[R];
@@ -440,10 +448,11 @@ __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w
             const int64_t row = row0 < e.n ? row0 : e.n - 1;
             const uint32_t gr = (uint32_t)row / (uint32_t)S, gc = (uint32_t)row - gr * (uint32_t)S;
             bnd[r] = gr == 0 || gr == S - 1 || gc == 0 || gc == S - 1;
+            if constexpr (MF) q1_stored_weights<double, double>(S, gr, gc, bnd[r], kap, av[r]);
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 const int64_t j = bnd[r] ? row : row + (int64_t)(k / 3 - 1) * S + (k % 3 - 1);
-                av[r][k] = load_value_nt(ax + row + (int64_t)k * e.stride);
+                if constexpr (!MF) av[r][k] = load_value_nt(ax + row + (int64_t)k * e.stride);
                 xv[r][k] = pv[j];
             }
             pr[r] = pv[row];
@@ -843,20 +852,27 @@ __global__ void k_bb_coarsen(const BuildEntry *entries, BuildParams P, int l) {
         coarsen_cell(e.kappa[l + 1], 2 * nc, e.kappa[l], nc, P.averaging, id);
 }
 
-// inner values and dinv of level l (shared Aj / Arl untouched)
+// inner values and dinv of level l (shared Aj / Arl untouched); a level
+// whose values are not stored (e.Ax[l] null) gets its dinv only
 template <typename VT>
 __global__ void k_bb_level(const BuildEntry *entries, BuildParams P, int l) {
     const BuildEntry &e = entries[blockIdx.y];
     if (e.level < l) return;
     const int T = P.T[l];
-    const int64_t rows = P.stride[l] / T;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
-        q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], e.dinv[l], P.omega);
+    const int64_t rows = P.stride[l] / T, n = P.side[l] * P.side[l];
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        if (e.Ax[l])
+            q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], e.dinv[l], P.omega);
+        else if (r < n)
+            q1_dinv_row<VT>(P.side[l], r, e.kappa[l], e.dinv[l], P.omega);
+    }
 }
 
-// FP64 outer values of the finest level
+// FP64 outer values of the finest level (entries without stored FP64
+// values are skipped)
 __global__ void k_bb_outer(const BuildEntry *entries, BuildParams P) {
     const BuildEntry &e = entries[blockIdx.y];
+    if (!e.Ax64) return;
     const int L = e.level;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < P.stride64[L];
          r += (int64_t)gridDim.x * blockDim.x)
@@ -935,19 +951,33 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
 constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4, kOuterSpmvBlocks = 2;
 
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
-                   hipStream_t s) {
+                   hipStream_t s, bool mf) {
     if (max_blocks <= 0) return hipSuccess;
     if (max_blocks > kMaxLogicalBlocks) return hipErrorInvalidValue;
+    if (mf)
+        for (int c = 0; c < tab.nclass; ++c)
+            if (!tab.c[c].kappa || tab.c[c].kld < (tab.c[c].side - 1) * (tab.c[c].side - 1)) return hipErrorInvalidValue;
     constexpr int G = kOuterRowGroup, R = kOuterElemBlocks;
     const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)), ge(batch_grid((max_blocks + R - 1) / R, 256));
     const dim3 gs(batch_grid((max_blocks + kOuterSpmvBlocks - 1) / kOuterSpmvBlocks, 256));
     switch (op) {
-        case OP_DEFECT_ALL: hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_ALL, G, true>), gr, block, 0, s, tab, w, sc); break;
+        case OP_DEFECT_ALL:
+            if (mf)
+                hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_ALL, G, true, true>), gr, block, 0, s, tab, w, sc);
+            else
+                hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_ALL, G, true, false>), gr, block, 0, s, tab, w, sc);
+            break;
         case OP_DEFECT_SETTLE:
-            hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false>), gr, block, 0, s, tab, w, sc);
+            if (mf)
+                hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false, true>), gr, block, 0, s, tab, w, sc);
+            else
+                hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false, false>), gr, block, 0, s, tab, w, sc);
             break;
         case OP_SPMV_DOT:
-            hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks>), gs, block, 0, s, tab, w);
+            if (mf)
+                hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks, true>), gs, block, 0, s, tab, w);
+            else
+                hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks, false>), gs, block, 0, s, tab, w);
             break;
         case OP_SCALE: hipLaunchKernelGGL((k_b_outer_elem<OP_SCALE, R>), ge, block, 0, s, tab, w, sc); break;
         case OP_ACC: hipLaunchKernelGGL((k_b_outer_elem<OP_ACC, R>), ge, block, 0, s, tab, w, sc); break;
@@ -997,11 +1027,12 @@ hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork 
 }
 
 hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
-                  const float *xin, hipStream_t s) {
+                  const float *xin, hipStream_t s, bool mf) {
     if (lv.T != 1 || lv.side < 3 || lv.ncpr != 9 || lv.side > 65535) return hipErrorInvalidValue;
     if (pair != PAIR_PRE && pair != PAIR_POST) return hipErrorInvalidValue;
     if (!out0 || (pair == PAIR_PRE && !out1) || (pair == PAIR_POST && (!xin || xin == out0)))
         return hipErrorInvalidValue;
+    if (mf && (!lv.kappa || lv.kld < (lv.side - 1) * (lv.side - 1))) return hipErrorInvalidValue;
     PairGeom pg;
     pg.side = (int32_t)lv.side;
     pg.strips = (int32_t)((lv.side + kPairWidth - 1) / kPairWidth);
@@ -1010,23 +1041,28 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, floa
     if (lv.cap <= 0) return hipSuccess;
     const int64_t units = lv.cap * (int64_t)pg.strips * pg.segs;
     if (units > kMaxLogicalBlocks) return hipErrorInvalidValue;
-    const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0};
+    const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0, lv.kappa, lv.kld};
     const int threads = kPairTileRows * kWave;
     const dim3 grid(batch_grid(units, threads)), block(threads);
     const float *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
-    if (vt == VT_F32) {
-        if (pair == PAIR_PRE)
-            hipLaunchKernelGGL((k_b_pair<float, PAIR_PRE>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
-        else
-            hipLaunchKernelGGL((k_b_pair<float, PAIR_POST>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
-    } else if (vt == VT_BF16) {
-        if (pair == PAIR_PRE)
-            hipLaunchKernelGGL((k_b_pair<bf16, PAIR_PRE>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
-        else
-            hipLaunchKernelGGL((k_b_pair<bf16, PAIR_POST>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv);
-    } else {
+#define PAIR_LAUNCH(VT, P, MFV) \
+    hipLaunchKernelGGL((k_b_pair<VT, P, MFV>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv)
+#define PAIR_VT(VT)                                                           \
+    do {                                                                      \
+        if (pair == PAIR_PRE) {                                               \
+            if (mf) PAIR_LAUNCH(VT, PAIR_PRE, true); else PAIR_LAUNCH(VT, PAIR_PRE, false);    \
+        } else {                                                              \
+            if (mf) PAIR_LAUNCH(VT, PAIR_POST, true); else PAIR_LAUNCH(VT, PAIR_POST, false);  \
+        }                                                                     \
+    } while (0)
+    if (vt == VT_F32)
+        PAIR_VT(float);
+    else if (vt == VT_BF16)
+        PAIR_VT(bf16);
+    else
         return hipErrorInvalidValue;
-    }
+#undef PAIR_VT
+#undef PAIR_LAUNCH
     return hipGetLastError();
 }
 
@@ -1102,7 +1138,7 @@ hipError_t b_build(const BuildEntry *entries, const BuildParams &P, int max_leve
         else
             return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_bb_outer, grid(P.stride64[max_level]), dim3(256), 0, s, entries, P);
+    if (P.outer_values) hipLaunchKernelGGL(k_bb_outer, grid(P.stride64[max_level]), dim3(256), 0, s, entries, P);
     return hipGetLastError();
 }
 

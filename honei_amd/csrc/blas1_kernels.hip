@@ -33,7 +33,8 @@ inline int reduce_grid(int64_t n) {
 #define GRID_STRIDE(i, n) \
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
-template <typename V> __global__ void k_axpy(V *x, const V *y, V b, const double *b_dev, int64_t n) {
+template <typename V> __global__ void k_axpy(V *x, const V *y, V b, const double *b_dev, int64_t n, const int *skip) {
+    if (skip && block_uniform_flag(skip)) return;
     const V bb = b_dev ? (V)*b_dev : b;
     GRID_STRIDE(i, n) x[i] += y[i] * bb;
 }
@@ -113,7 +114,8 @@ __global__ void k_zero_words(uint32_t *p, size_t n) {
 // the grid depends on n only, so the result is reproducible.
 template <typename V, int MODE>
 __global__ __launch_bounds__(kBlock) void k_reduce(V *x, const V *y, V a, const double *a_dev, int64_t n,
-                                                   double *out, ReduceSlots slots) {
+                                                   double *out, ReduceSlots slots, const int *skip) {
+    if (skip && block_uniform_flag(skip)) return;
     V aa = a;
     if constexpr (MODE == 2) {
         if (a_dev) aa = (V)*a_dev;
@@ -143,7 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(V *x, const V *y, V a, const 
     if (threadIdx.x == 0) slots.partials[blockIdx.x] = b;
 }
 
-__global__ void k_finish_sum(const double *partials, int64_t nparts, double *out) {
+__global__ void k_finish_sum(const double *partials, int64_t nparts, double *out, const int *skip) {
+    if (skip && block_uniform_flag(skip)) return;
     const double t = sum_partials(partials, nparts, 1, 0);
     if (threadIdx.x == 0) *out = t;
 }
@@ -163,8 +166,9 @@ hipError_t dot(const void *x, const void *y, int64_t n, ValueType vt, double *ou
         typedef decltype(z) V;
         const int g = reduce_grid(n);
         hipLaunchKernelGGL((k_reduce<V, 0>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)y,
-                           V(0), (const double *)nullptr, n, out, sl);
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
+                           V(0), (const double *)nullptr, n, out, sl, (const int *)nullptr);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out,
+                           (const int *)nullptr);
         return hipGetLastError();
     });
 }
@@ -173,19 +177,21 @@ hipError_t norm2(const void *x, int64_t n, ValueType vt, double *out, const Redu
         typedef decltype(z) V;
         const int g = reduce_grid(n);
         hipLaunchKernelGGL((k_reduce<V, 1>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)x,
-                           V(0), (const double *)nullptr, n, out, sl);
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
+                           V(0), (const double *)nullptr, n, out, sl, (const int *)nullptr);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out,
+                           (const int *)nullptr);
         return hipGetLastError();
     });
 }
 hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_dev, int64_t n, ValueType vt,
-                      double *out, const ReduceSlots &sl, hipStream_t s) {
+                      double *out, const ReduceSlots &sl, hipStream_t s, const int *skip) {
     return by_type(vt, [&](auto z) {
         typedef decltype(z) V;
         const int g = reduce_grid(n);
         hipLaunchKernelGGL((k_reduce<V, 2>), dim3(g), dim3(kBlock), 0, s, (V *)x, (const V *)y,
-                           (V)alpha, alpha_dev, n, out, sl);
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out);
+                           (V)alpha, alpha_dev, n, out, sl, skip);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, (const double *)sl.partials, (int64_t)g, out,
+                           skip);
         return hipGetLastError();
     });
 }
@@ -198,9 +204,10 @@ hipError_t axpy_norm2(void *x, const void *y, double alpha, const double *alpha_
         return hipGetLastError();                                        \
     })
 
-hipError_t axpy(void *x, const void *y, double b, const double *b_dev, int64_t n, ValueType vt, hipStream_t s) {
+hipError_t axpy(void *x, const void *y, double b, const double *b_dev, int64_t n, ValueType vt, hipStream_t s,
+                const int *skip) {
     ELEMENTWISE(hipLaunchKernelGGL((k_axpy<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)x, (const V *)y,
-                                   (V)b, b_dev, n));
+                                   (V)b, b_dev, n, skip));
 }
 hipError_t scaled_sum3(void *r, const void *x, const void *y, double b, int64_t n, ValueType vt, hipStream_t s) {
     ELEMENTWISE(hipLaunchKernelGGL((k_scaled_sum3<V>), dim3(stream_grid(n)), dim3(kBlock), 0, s, (V *)r,

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -154,6 +155,13 @@ int honei_hip_shutdown(void) {
     release_workspaces();
     return HONEI_HIP_SUCCESS;
 }
+
+int honei_hip_host_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "ptr is null");
+    *ptr = nullptr;
+    return check(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+}
+int honei_hip_host_free(void *ptr) { return ptr ? check(hipHostFree(ptr), "hipHostFree") : HONEI_HIP_SUCCESS; }
 
 int honei_hip_malloc(void **ptr, size_t bytes) {
     if (!ptr) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "ptr is null");
@@ -325,6 +333,100 @@ PRODUCT_ENTRY64(honei_hip_product_smell_dv_double_idx64, VT_F64)
     }
 DEFECT_ENTRY64(honei_hip_defect_smell_dv_float_idx64, VT_F32)
 DEFECT_ENTRY64(honei_hip_defect_smell_dv_double_idx64, VT_F64)
+
+// ---- sliced ELL copy ----------------------------------------------------------
+struct SellHandle {
+    ValueType vt;
+    int64_t rows = 0, cols = 0, slices = 0, padded = 0;
+    int32_t *perm = nullptr, *width = nullptr, *Aj = nullptr;
+    uint32_t *len = nullptr;
+    int64_t *base = nullptr;
+    void *Ax = nullptr;
+    uint64_t bytes = 0;
+    ~SellHandle() {
+        for (void *p : {(void *)perm, (void *)width, (void *)Aj, (void *)len, (void *)base, Ax})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int honei_hip_sell_create(void **sell, int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                          size_t cols, size_t ncpr, size_t stride, size_t threads, void *stream) {
+    if (!sell) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sell is null");
+    *sell = nullptr;
+    if (vtype != HONEI_HIP_F32 && vtype != HONEI_HIP_F64) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "FP32 or FP64 values");
+    if (threads != 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "the sliced copy is made from a threads = 1 matrix");
+    int st = check_ell(Aj, Ax, Arl, 0, rows, stride, threads, 0);
+    if (st) return st;
+    if (cols > (size_t)INT32_MAX) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "columns exceed 32-bit index range");
+    hipStream_t s = S(stream);
+    auto *h = new (std::nothrow) SellHandle();
+    if (!h) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation");
+    h->vt = (ValueType)vtype;
+    h->rows = (int64_t)rows;
+    h->cols = (int64_t)cols;
+    const int64_t C = sell_slice_rows();
+    h->slices = (h->rows + C - 1) / C;
+    const size_t vs = vtype == HONEI_HIP_F64 ? 8 : 4;
+    auto dalloc = [&](void **p, size_t b) -> int {
+        h->bytes += b ? b : 1;
+        return check(hipMalloc(p, b ? b : 1), "sell allocation");
+    };
+    st = dalloc((void **)&h->perm, rows * 4);
+    if (!st) st = dalloc((void **)&h->len, rows * 4);
+    if (!st) st = dalloc((void **)&h->width, (size_t)h->slices * 4);
+    if (!st) st = dalloc((void **)&h->base, (size_t)h->slices * 8);
+    if (!st) st = check(sell_sort((const uint32_t *)Arl, h->rows, h->perm, h->len, s), "sell sort");
+    if (!st) st = check(sell_widths(h->len, h->rows, h->slices, h->width, s), "sell widths");
+    std::vector<int32_t> w((size_t)h->slices);
+    std::vector<int64_t> b((size_t)h->slices);
+    if (!st && h->slices)
+        st = check(hipMemcpyAsync(w.data(), h->width, w.size() * 4, hipMemcpyDeviceToHost, s), "sell widths readback");
+    if (!st) st = check(hipStreamSynchronize(s), "sell sync");
+    int64_t acc = 0;
+    for (int64_t k = 0; !st && k < h->slices; ++k) {
+        if (w[(size_t)k] < 0 || (size_t)w[(size_t)k] > ncpr) st = fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "row length exceeds num_cols_per_row");
+        b[(size_t)k] = acc;
+        acc += C * (int64_t)w[(size_t)k];
+    }
+    h->padded = acc;
+    if (!st && h->slices)
+        st = check(hipMemcpyAsync(h->base, b.data(), b.size() * 8, hipMemcpyHostToDevice, s), "sell base upload");
+    if (!st) st = dalloc((void **)&h->Aj, (size_t)acc * 4);
+    if (!st) st = dalloc(&h->Ax, (size_t)acc * vs);
+    if (!st)
+        st = check(sell_fill(h->vt, (const int32_t *)Aj, Ax, (int64_t)stride, h->rows, h->perm, h->len, h->width,
+                             h->base, h->Aj, h->Ax, s),
+                   "sell fill");
+    if (!st) st = check(hipStreamSynchronize(s), "sell sync");
+    if (st) {
+        delete h;
+        return st;
+    }
+    *sell = h;
+    return HONEI_HIP_SUCCESS;
+}
+
+int honei_hip_sell_info(void *sell, size_t *slices, size_t *padded_slots, size_t *device_bytes) {
+    if (!sell) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sell is null");
+    const SellHandle *h = (const SellHandle *)sell;
+    if (slices) *slices = (size_t)h->slices;
+    if (padded_slots) *padded_slots = (size_t)h->padded;
+    if (device_bytes) *device_bytes = h->bytes;
+    return HONEI_HIP_SUCCESS;
+}
+
+int honei_hip_product_sell(void *sell, void *x, void *y, void *stream) {
+    if (!sell) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sell is null");
+    const SellHandle *h = (const SellHandle *)sell;
+    if (h->rows > 0 && (!x || !y)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector");
+    return check(sell_spmv(h->vt, h->Aj, h->Ax, h->perm, h->len, h->width, h->base, h->rows, x, y, S(stream)),
+                 "product_sell");
+}
+
+int honei_hip_sell_destroy(void *sell) {
+    delete (SellHandle *)sell;
+    return HONEI_HIP_SUCCESS;
+}
 
 int honei_hip_stream_probe(int kind, void *dst, const void *src, size_t bytes, size_t grid, void *stream) {
     if (kind != 0 && kind != 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "kind must be 0 (read) or 1 (copy)");

@@ -40,6 +40,7 @@ struct KArgs {
     int64_t nlogical;
     ReduceSlots slots;
     uint32_t aj_bytes, ax_bytes;   // buffer extents of the fixed-width form
+    const int *gate;               // non-null: run only while *gate != 0
 };
 
 // One block serves `per_block` consecutive logical blocks of blockDim/T rows
@@ -57,6 +58,7 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     const V *__restrict__ x = (const V *)a.x;
     const V *__restrict__ rhs = (const V *)a.rhs;
     const V *__restrict__ dinv = (const V *)a.dinv;
+    if (a.gate && !block_uniform_flag(a.gate)) return;
 
     const int64_t p = xcd_block(blockIdx.x, gridDim.x);
     const int64_t per = a.nlogical / gridDim.x, rem = a.nlogical % gridDim.x;
@@ -114,7 +116,8 @@ __global__ __launch_bounds__(1024) void ell_row_kernel(KArgs a) {
     }
 }
 
-__global__ void k_finish_norm(const double *partials, int64_t nparts, double *out) {
+__global__ void k_finish_norm(const double *partials, int64_t nparts, double *out, const int *gate) {
+    if (gate && !block_uniform_flag(gate)) return;
     const double t = sum_partials(partials, nparts, 1, 0);
     if (threadIdx.x == 0) *out = t;
 }
@@ -374,9 +377,12 @@ hipError_t ell_apply(const EllView &A, const EllArgs &e, const ReduceSlots &slot
     k.nlogical = nlogical;
     k.slots = slots;
     k.aj_bytes = (uint32_t)(slots_total * 4 * iw);
+    k.gate = e.gate;
     k.ax_bytes = (uint32_t)(slots_total * vsz);
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, k);
-    if (norm) hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, (const double *)slots.partials, grid, e.norm2);
+    if (norm)
+        hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, (const double *)slots.partials, grid, e.norm2,
+                           e.gate);
     return hipGetLastError();
 }
 

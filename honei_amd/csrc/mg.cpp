@@ -403,12 +403,18 @@ hipError_t MGHierarchy::solve_fixed(const void *b, void *x, hipStream_t s, int64
     return hipStreamSynchronize(s);
 }
 
+// FP64 conjugate gradients preconditioned by one V-cycle of the inner
+// hierarchy per iteration (z = |r| V(r / |r|), the cycle lean: from zero, no
+// inner norms).  Every scalar stays on the device (MgPcgScalars): the host
+// enqueues iterations in chunks, sized from the convergence rate seen so far,
+// and reads the record once per chunk.  A residual that meets the tolerance
+// is confirmed by the true defect b - A x before the solve stops, restarting
+// CG from it when the two disagree.
 hipError_t MGHierarchy::solve_pcg(const double *b, double *x, hipStream_t s, int64_t *iters_out,
                                   double *rel_defect) {
     MGLevel &top = L[(size_t)max_level];
     if (!A_outer.Aj) return hipErrorInvalidValue;
     const int64_t n = top.n;
-    const ValueType vv = inner == VT_F64 ? VT_F64 : VT_F32;
     if (n64_ != n) {
         for (double **p : {&guess_, &d64_, &res64_, &p64_, &q64_}) {
             if (*p) (void)hipFree(*p);
@@ -418,64 +424,42 @@ hipError_t MGHierarchy::solve_pcg(const double *b, double *x, hipStream_t s, int
         n64_ = n;
     }
     double *r = d64_, *z = res64_, *p = p64_, *q = q64_;
-    double *sc = scal_ + 16;   // device scalars: |r|^2, r.z, p.q
+    static_assert(sizeof(MgPcgScalars) <= 32 * sizeof(double), "scalar record fits scal_[32..63]");
+    MgPcgScalars *sc = (MgPcgScalars *)(scal_ + 32);
+    const int cap = (int)(cfg.max_outer > 0 ? std::min<int64_t>(cfg.max_outer, 1 << 30) : 10000);
     // x0 with Dirichlet boundary values; r = b - A x0
     HX(boundary_guess(x, b, n, top.side, s));
-    auto defect_norm = [&](double *h) -> hipError_t {
+    auto defect_norm = [&](double *norm2, const int *gate) -> hipError_t {
         EllArgs a;
         a.mode = ELL_DEFECT;
         a.out = r;
         a.x = x;
         a.rhs = b;
-        a.norm2 = sc;
+        a.norm2 = norm2;
         a.row_start = 0;
         a.row_end = n;
-        HX(ell_apply(A_outer.v, a, slots_, s));
-        return read_scalar(h, sc, s);
+        a.gate = gate;
+        return ell_apply(A_outer.v, a, slots_, s);
     };
-    double h = 0.0;
-    HX(defect_norm(&h));
-    const double initial = std::sqrt(h);
-    double nrm = initial, rho_prev = 0.0;
-    int64_t it = 0;
-    const int64_t cap = cfg.max_outer > 0 ? cfg.max_outer : 10000;
+    HX(defect_norm(&sc->r2, nullptr));
+    HX(mgpcg_scalar(MGPCG_INIT, sc, cfg.tolerance, cap, s));
     // the inner cycle runs lean: one V-cycle from zero, no inner norms
     const MGConfig saved = cfg;
     cfg.n_max_iter = 1;
     cfg.initial_zero = 1;
     cfg.convergence_check = 0;
-    hipError_t err = hipSuccess;
-    while (initial > 0.0 && it < cap && std::isfinite(nrm)) {
-        // z = |r| V(r / |r|)
-        const double inv = 1.0 / nrm;
-        if (vv == VT_F64) {
-            err = vcopy(top.rhs, r, n, VT_F64, s);
-            if (err == hipSuccess) err = vscale(top.rhs, inv, n, VT_F64, s);
-        } else {
-            err = scale_to_f32((float *)top.rhs, r, inv, n, s);
-        }
-        if (err == hipSuccess) err = fill_bytes_zero(top.x, (size_t)n * vec_size(inner), s);
-        if (err != hipSuccess) break;
+    auto iteration = [&]() -> hipError_t {
+        HX(mgpcg_scale(inner, top.rhs, r, n, sc, s));
+        HX(fill_bytes_zero(top.x, (size_t)n * vec_size(inner), s));
         int64_t inner_it = 0;
         mixed_inner_ = true;
-        err = run_kernel(s, &inner_it, nullptr);
+        const hipError_t e = run_kernel(s, &inner_it, nullptr);
         mixed_inner_ = false;
-        if (err != hipSuccess) break;
-        if (vv == VT_F64) {
-            err = vcopy(z, top.x, n, VT_F64, s);
-            if (err == hipSuccess) err = vscale(z, nrm, n, VT_F64, s);
-        } else {
-            err = fill_bytes_zero(z, (size_t)n * 8, s);
-            if (err == hipSuccess) err = acc_scaled_f32(z, (const float *)top.x, nrm, n, s);
-        }
-        ++it;
-        double rho = 0.0, sigma = 0.0;
-        if (err == hipSuccess) err = dot(r, z, n, VT_F64, sc + 1, slots_, s);
-        if (err == hipSuccess) err = read_scalar(&rho, sc + 1, s);
-        if (err != hipSuccess) break;
-        const double beta = (rho_prev > 0.0 && rho > 0.0) ? rho / rho_prev : 0.0;
-        err = beta == 0.0 ? vcopy(p, z, n, VT_F64, s) : scaled_sum3(p, z, p, beta, n, VT_F64, s);
-        if (err != hipSuccess) break;
+        HX(e);
+        HX(mgpcg_z(inner, z, top.x, n, sc, s));
+        HX(dot(r, z, n, VT_F64, &sc->rz, slots_, s));
+        HX(mgpcg_scalar(MGPCG_BETA, sc, 0.0, 0, s));
+        HX(mgpcg_pupdate(p, z, n, sc, s));
         {
             EllArgs a;
             a.mode = ELL_PRODUCT;
@@ -483,34 +467,40 @@ hipError_t MGHierarchy::solve_pcg(const double *b, double *x, hipStream_t s, int
             a.x = p;
             a.row_start = 0;
             a.row_end = n;
-            err = ell_apply(A_outer.v, a, slots_, s);
+            HX(ell_apply(A_outer.v, a, slots_, s));
         }
-        if (err == hipSuccess) err = dot(p, q, n, VT_F64, sc + 2, slots_, s);
-        if (err == hipSuccess) err = read_scalar(&sigma, sc + 2, s);
-        if (err != hipSuccess) break;
-        if (!(sigma > 0.0) || !(rho > 0.0)) break;   // breakdown: not converged
-        const double alpha = rho / sigma;
-        err = axpy(x, p, alpha, nullptr, n, VT_F64, s);
-        if (err == hipSuccess) err = axpy_norm2(r, q, -alpha, nullptr, n, VT_F64, sc, slots_, s);
-        if (err == hipSuccess) err = read_scalar(&h, sc, s);
-        if (err != hipSuccess) break;
-        nrm = std::sqrt(h);
-        rho_prev = rho;
-        if (nrm < cfg.tolerance * initial) {   // confirm with the true residual
-            err = defect_norm(&h);
-            if (err != hipSuccess) break;
-            nrm = std::sqrt(h);
-            if (nrm < cfg.tolerance * initial) break;
-            rho_prev = 0.0;   // drifted: restart from the true residual
+        HX(dot(p, q, n, VT_F64, &sc->pq, slots_, s));
+        HX(mgpcg_scalar(MGPCG_ALPHA, sc, 0.0, 0, s));
+        HX(axpy(x, p, 0.0, &sc->alpha, n, VT_F64, s, &sc->done));
+        HX(axpy_norm2(r, q, 0.0, &sc->malpha, n, VT_F64, &sc->r2, slots_, s, &sc->done));
+        HX(mgpcg_scalar(MGPCG_CHECK, sc, 0.0, 0, s));
+        HX(defect_norm(&sc->r2_true, &sc->verify));
+        return mgpcg_scalar(MGPCG_CONFIRM, sc, 0.0, 0, s);
+    };
+    MgPcgScalars h{};
+    hipError_t err = hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    int64_t chunk = 4;
+    while (err == hipSuccess && !h.done) {
+        for (int64_t k = 0; k < chunk && err == hipSuccess; ++k) err = iteration();
+        if (err == hipSuccess) err = hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        if (err != hipSuccess || h.done) break;
+        // iterations still needed at the rate seen so far, 1..8 per chunk
+        const double rel = h.initial > 0.0 ? h.nrm / h.initial : 1.0;
+        int64_t need = 1;
+        if (h.iters > 0 && rel > 0.0 && rel < 1.0 && std::isfinite(rel)) {
+            const double per = std::log(rel) / (double)h.iters;
+            need = (int64_t)std::ceil(std::log(cfg.tolerance / rel) / per);
         }
+        chunk = std::max<int64_t>(1, std::min<int64_t>(8, need));
     }
     cfg = saved;
     HX(err);
-    HX(hipStreamSynchronize(s));
-    if (iters_out) *iters_out = it;
-    const double rd = initial > 0 ? nrm / initial : 0.0;
+    if (iters_out) *iters_out = h.iters;
+    const double rd = h.initial > 0 ? h.nrm / h.initial : 0.0;
     if (rel_defect) *rel_defect = rd;
-    return (initial > 0.0 && !(nrm < cfg.tolerance * initial)) ? hipErrorNotReady : hipSuccess;
+    return (h.initial > 0.0 && !(h.nrm < cfg.tolerance * h.initial)) ? hipErrorNotReady : hipSuccess;
 }
 
 hipError_t MGHierarchy::solve_mixed(const double *b, double *x, hipStream_t s, int64_t *outer_out,

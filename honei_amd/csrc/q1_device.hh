@@ -21,6 +21,57 @@ __device__ inline void q1_put<bf16, float>(bf16 *Ax, int64_t s, float v) {
     Ax[s].bits = (uint16_t)(u >> 16);
 }
 
+// The nine weights of interior node (i, j) of the variable-coefficient Q1
+// operator (columns (i-1, j-1) .. (i+1, j+1) in column order): the bilinear
+// element stiffness kappa/6 * [[4,-1,-2,-1],[-1,4,-1,-2],[-2,-1,4,-1],
+// [-1,-2,-1,4]] summed over the four cells around the node, cells row-major
+// in coeff[(side-1)^2], each weight formed in double and rounded once to V.
+// q1_row stores these values; the matrix-free batched kernels form them
+// again from the same coefficients, so both see the same bits.
+template <typename V>
+__device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, const double *coeff, V (&w)[9]) {
+    const int64_t nc = side - 1;
+    const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
+    const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+    w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
+    w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
+    w[2] = (V)(-se / 3.0);              // (i-1, j+1)
+    w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
+    w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
+    w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
+    w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
+    w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
+    w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+}
+
+// A float as a bf16 matrix stores it (round to nearest even) and widens it
+// again on load.
+__device__ inline float bf16_round(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float((u >> 16) << 16);
+}
+
+// The weights of node (i, j) as a kernel reading the stored matrix of value
+// type VT sees them (arithmetic type A: float for float / bf16 values,
+// double for double): interior nodes from the cell coefficients, boundary
+// nodes the identity row (w[0] = 1, the rest unused).
+template <typename VT, typename A>
+__device__ inline void q1_stored_weights(int64_t side, int64_t i, int64_t j, bool boundary, const double *coeff,
+                                         A (&w)[9]) {
+    if (boundary) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = A(0);
+        w[0] = A(1);
+        return;
+    }
+    q1_coeff_weights<A>(side, i, j, coeff, w);
+    if constexpr (sizeof(VT) == 2) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = bf16_round(w[k]);
+    }
+}
+
 // Row `row` of the Q1 operator on a side x side grid in the T-interleaved
 // ELL layout: entry t at slot (t % T) + row*T + (t / T)*stride, ncpr =
 // ceil(9 / T) chunks, unused slots (and padding rows n <= row < stride/T)
@@ -54,18 +105,7 @@ __device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT 
                 for (int k = 0; k < 9; ++k) w[k] = c;
                 w[4] = V(8.0 / 3.0);
             } else {
-                const int64_t nc = side - 1;
-                const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
-                const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
-                w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
-                w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
-                w[2] = (V)(-se / 3.0);              // (i-1, j+1)
-                w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
-                w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
-                w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
-                w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
-                w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
-                w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+                q1_coeff_weights<V>(side, i, j, coeff, w);
             }
             const int64_t c0[9] = {row - side - 1, row - side, row - side + 1, row - 1, row,
                                    row + 1,        row + side - 1, row + side, row + side + 1};
@@ -88,14 +128,29 @@ __device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT 
     if (Arl) Arl[row] = (uint32_t)((cnt + T - 1) / T);
     if (dinv) {
         V d = (cnt == 1) ? w[0] : w[4];
-        if constexpr (sizeof(VT) == 2) {   // diagonal as stored (bf16-rounded)
-            uint32_t u = __float_as_uint(d);
-            u += 0x7fffu + ((u >> 16) & 1u);
-            d = __uint_as_float((u >> 16) << 16);
-        }
+        if constexpr (sizeof(VT) == 2) d = bf16_round(d);   // diagonal as stored
         const V inv = V(1) / d;
         dinv[row] = inv * (V)omega;
     }
+}
+
+// dinv = omega / diag of row `row` exactly as q1_row stores it, for a level
+// whose values are not stored (interior rows: the coefficient operator).
+template <typename VT>
+__device__ inline void q1_dinv_row(int64_t side, int64_t row, const double *coeff, typename VecOf<VT>::type *dinv,
+                                   double omega) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t i = row / side, j = row % side;
+    const bool boundary = (i == 0 || i == side - 1 || j == 0 || j == side - 1);
+    V d = V(1.0);
+    if (!boundary) {
+        V w[9];
+        q1_coeff_weights<V>(side, i, j, coeff, w);
+        d = w[4];
+    }
+    if constexpr (sizeof(VT) == 2) d = bf16_round(d);   // diagonal as stored
+    const V inv = V(1) / d;
+    dinv[row] = inv * (V)omega;
 }
 
 // Coarse cell id = (I, J) of an nc x nc cell grid from fine cells (2I..2I+1,

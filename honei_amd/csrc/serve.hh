@@ -23,6 +23,13 @@ struct ServeRequest {
     int32_t level;         // side 2^level + 1
     uint64_t coeff_seed, rhs_seed;
     double *x_dev;         // optional: FP64 solution copied here on completion
+    // optional host payload (pinned memory for asynchronous copies): the
+    // coefficient ((side-1)^2) and right-hand side (side^2) are copied to the
+    // device when the request is admitted (instead of being drawn from the
+    // seeds), and the solution (side^2) is copied back before the request
+    // counts as done
+    const double *coeff_host = nullptr, *b_host = nullptr;
+    double *x_host = nullptr;
 };
 
 struct ServeRecord {
@@ -48,8 +55,9 @@ struct ServeConfig {
 };
 
 // Device bytes one slot of finest level `level` occupies (threads-per-row 1
-// layout; tuned layouts may need up to a third more matrix storage).
-uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method);
+// layout; tuned layouts may need up to a third more matrix storage), plus the
+// payload staging (coefficient and right-hand side) when `payload`.
+uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method, bool payload = false);
 
 hipError_t serve_run(const ServeConfig &cfg, const ServeRequest *reqs, int64_t n, ServeRecord *out,
                      ServeStats *stats, bool *over_budget);

@@ -396,6 +396,84 @@ __global__ void k_scale_to_f32(float *dst, const double *src, double inv, int64_
         dst[i] = (float)(src[i] * inv);
 }
 
+// ---- multigrid-preconditioned CG with device scalars (MGHierarchy::solve_pcg) --------
+// One thread updates the record; the vector kernels read it once per block.
+__global__ void k_mgpcg_scalar(int step, MgPcgScalars *sc, double tol, int cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MgPcgScalars &c = *sc;
+    if (step == MGPCG_INIT) {   // after r = b - A x0 with ||r||^2 in r2
+        c.initial = sqrt(c.r2);
+        c.nrm = c.initial;
+        c.rho_prev = 0.0;
+        c.tol = tol;
+        c.iters = 0;
+        c.cap = cap;
+        c.verify = 0;
+        c.alpha = c.malpha = c.beta = 0.0;
+        c.done = (c.initial > 0.0 && cap > 0 && isfinite(c.nrm)) ? 0 : 1;
+        return;
+    }
+    if (c.done) return;
+    if (step == MGPCG_BETA) {   // after r.z
+        c.iters += 1;
+        c.beta = (c.rho_prev > 0.0 && c.rz > 0.0) ? c.rz / c.rho_prev : 0.0;
+    } else if (step == MGPCG_ALPHA) {   // after p.q
+        if (!(c.pq > 0.0) || !(c.rz > 0.0)) {
+            c.done = 1;   // breakdown: not converged
+            return;
+        }
+        c.alpha = c.rz / c.pq;
+        c.malpha = -c.alpha;
+    } else if (step == MGPCG_CHECK) {   // after x += alpha p, r -= alpha q, ||r||^2
+        c.nrm = sqrt(c.r2);
+        c.rho_prev = c.rz;
+        if (c.nrm < c.tol * c.initial)
+            c.verify = 1;
+        else if (!(c.iters < c.cap) || !isfinite(c.nrm))
+            c.done = 1;
+    } else if (step == MGPCG_CONFIRM) {   // after the gated r = b - A x, ||r||^2 in r2_true
+        if (!c.verify) return;
+        c.verify = 0;
+        c.nrm = sqrt(c.r2_true);
+        if (c.nrm < c.tol * c.initial) {
+            c.done = 1;
+        } else {
+            c.rho_prev = 0.0;   // drifted: restart from the true residual
+            if (!(c.iters < c.cap) || !isfinite(c.nrm)) c.done = 1;
+        }
+    }
+}
+
+// rhs = r / |r| in the inner precision (the rounding of r * (1 / |r|))
+template <typename V>
+__global__ void k_mgpcg_scale(V *rhs, const double *r, int64_t n, const MgPcgScalars *sc) {
+    if (block_uniform_flag(&sc->done)) return;
+    const double inv = 1.0 / sc->nrm;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        rhs[i] = (V)(r[i] * inv);
+}
+
+// z = |r| x (FP32 inner solution: widened and scaled into a zero vector)
+template <typename V>
+__global__ void k_mgpcg_z(double *z, const V *x, int64_t n, const MgPcgScalars *sc) {
+    if (block_uniform_flag(&sc->done)) return;
+    const double nrm = sc->nrm;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if constexpr (sizeof(V) == 4)
+            z[i] = 0.0 + (double)x[i] * nrm;
+        else
+            z[i] = x[i] * nrm;
+    }
+}
+
+// p = z (beta 0) or z + p beta
+__global__ void k_mgpcg_pupdate(double *p, const double *z, int64_t n, const MgPcgScalars *sc) {
+    if (block_uniform_flag(&sc->done)) return;
+    const double beta = sc->beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = beta == 0.0 ? z[i] : z[i] + p[i] * beta;
+}
+
 inline int grid_for(int64_t n, int per = 1024) {
     int64_t g = (n + per - 1) / per;
     return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
@@ -557,6 +635,29 @@ hipError_t boundary_guess(double *guess, const double *b, int64_t n, int64_t sid
 }
 hipError_t acc_scaled_f32(double *result, const float *x, double scale, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_acc_scaled_f32, dim3(grid_for(n)), dim3(256), 0, s, result, x, scale, n);
+    return hipGetLastError();
+}
+hipError_t mgpcg_scalar(int step, MgPcgScalars *sc, double tol, int cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_mgpcg_scalar, dim3(1), dim3(64), 0, s, step, sc, tol, cap);
+    return hipGetLastError();
+}
+hipError_t mgpcg_scale(ValueType inner, void *rhs, const double *r, int64_t n, const MgPcgScalars *sc,
+                       hipStream_t s) {
+    if (inner == VT_F64)
+        hipLaunchKernelGGL(k_mgpcg_scale<double>, dim3(grid_for(n)), dim3(256), 0, s, (double *)rhs, r, n, sc);
+    else
+        hipLaunchKernelGGL(k_mgpcg_scale<float>, dim3(grid_for(n)), dim3(256), 0, s, (float *)rhs, r, n, sc);
+    return hipGetLastError();
+}
+hipError_t mgpcg_z(ValueType inner, double *z, const void *x, int64_t n, const MgPcgScalars *sc, hipStream_t s) {
+    if (inner == VT_F64)
+        hipLaunchKernelGGL(k_mgpcg_z<double>, dim3(grid_for(n)), dim3(256), 0, s, z, (const double *)x, n, sc);
+    else
+        hipLaunchKernelGGL(k_mgpcg_z<float>, dim3(grid_for(n)), dim3(256), 0, s, z, (const float *)x, n, sc);
+    return hipGetLastError();
+}
+hipError_t mgpcg_pupdate(double *p, const double *z, int64_t n, const MgPcgScalars *sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_mgpcg_pupdate, dim3(grid_for(n)), dim3(256), 0, s, p, z, n, sc);
     return hipGetLastError();
 }
 hipError_t scale_to_f32(float *dst, const double *src, double inv, int64_t n, hipStream_t s) {

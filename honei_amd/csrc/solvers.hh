@@ -31,6 +31,31 @@ struct CGScalars {
 
 hipError_t cg_init(ValueType vt, CGScalars *sc, double eps_rel, int max_iters, hipStream_t s);
 
+// Scalars of the multigrid-preconditioned CG of MGHierarchy::solve_pcg, kept
+// on the device: the host enqueues iterations in chunks and reads this
+// record once per chunk.  Once `done` is set no kernel of an iteration
+// changes x, r, p or a scalar any more.
+struct MgPcgScalars {
+    double r2;        // ||r||^2 of the recursive residual
+    double rz;        // r.z
+    double pq;        // p.q
+    double r2_true;   // ||b - A x||^2 of a verification
+    double initial, nrm, rho_prev, beta, alpha, malpha, tol;
+    int32_t done;     // converged, broken down, non-finite or out of iterations
+    int32_t verify;   // the recursive residual met the tolerance: the true defect decides
+    int32_t iters, cap;
+};
+// Steps of one iteration (see MGHierarchy::solve_pcg): init after the first
+// defect; scale (rhs = r / |r|, FP32 or FP64 inner right-hand side); z =
+// |r| x (FP32 or FP64 inner solution); beta (counts the iteration); p update;
+// alpha; check (after the update); confirm (after the gated true defect).
+enum MgPcgStep : int { MGPCG_INIT = 0, MGPCG_BETA = 1, MGPCG_ALPHA = 2, MGPCG_CHECK = 3, MGPCG_CONFIRM = 4 };
+hipError_t mgpcg_scalar(int step, MgPcgScalars *sc, double tol, int cap, hipStream_t s);
+hipError_t mgpcg_scale(ValueType inner, void *rhs, const double *r, int64_t n, const MgPcgScalars *sc,
+                       hipStream_t s);
+hipError_t mgpcg_z(ValueType inner, double *z, const void *x, int64_t n, const MgPcgScalars *sc, hipStream_t s);
+hipError_t mgpcg_pupdate(double *p, const double *z, int64_t n, const MgPcgScalars *sc, hipStream_t s);
+
 // Host side of CGSolver (solvers_capi.cpp), shared by honei_hip_cg_solve and
 // its allocation-free variant: argument checks (HONEI status code), the
 // bytes of one CG vector (256-byte multiple), the matrix view, and the solve

@@ -51,6 +51,10 @@ int honei_hip_device_count(int *count);
 int honei_hip_init(int device);
 int honei_hip_shutdown(void);
 int honei_hip_malloc(void **ptr, size_t bytes);
+/* Pinned host memory (asynchronous copies); replaces cuda_malloc_host /
+ * cuda_free_host (backends/cuda/transfer.cu:49-60).                     */
+int honei_hip_host_alloc(void **ptr, size_t bytes);
+int honei_hip_host_free(void *ptr);
 int honei_hip_free(void *ptr);
 int honei_hip_upload(void *dst_dev, const void *src_host, size_t bytes, void *stream);
 int honei_hip_download(void *dst_host, const void *src_dev, size_t bytes, void *stream);
@@ -186,6 +190,23 @@ int honei_hip_defect_smell_dv_double_idx64(void *rhs, void *result, const void *
  * (4 instead of 8 bytes per index slot).  Synchronous (reads back the
  * count of out-of-range values); no reference counterpart.               */
 int honei_hip_narrow_indices(void *dst_dev_u32, const void *src_dev_u64, size_t n, uint64_t bound, void *stream);
+
+/* ---- sliced copy of an ELL matrix (no reference counterpart) ---------------
+ * For matrices whose row lengths vary (ELL pads every row to the widest):
+ * honei_hip_sell_create builds, on the device, a copy of a threads = 1 ELL
+ * matrix (FP32 or FP64 values, int32 Aj, uint32 Arl) whose rows are sorted by
+ * length, longest first, within windows of 2048 rows and grouped into slices
+ * of 64 rows padded to their own longest row; each row keeps its entries in
+ * stored order.  honei_hip_product_sell computes y = A x over all rows with
+ * each row's terms added in that order: bit-identical to
+ * honei_hip_product_smell_dv_* and Product<tags::CPU> (la/product.hh:
+ * 148-206).  The handle owns its device memory (create allocates and
+ * synchronises; product neither allocates nor synchronises).            */
+int honei_hip_sell_create(void **sell, int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                          size_t cols, size_t num_cols_per_row, size_t stride, size_t threads, void *stream);
+int honei_hip_sell_info(void *sell, size_t *slices, size_t *padded_slots, size_t *device_bytes);
+int honei_hip_product_sell(void *sell, void *x, void *y, void *stream);
+int honei_hip_sell_destroy(void *sell);
 
 /* Bandwidth probe (measurement only, no reference counterpart): `bytes`
  * (multiple of 16, 16-byte aligned pointers) streamed with 16-byte
@@ -419,6 +440,10 @@ typedef struct {
     int32_t pair_sweeps;       /* 1 (default): on levels of 33^2 nodes and up, Jacobi0 + defect and
                                   the two post-smoothing sweeps each run as one launch that reads
                                   the matrix once (n_pre == 2, n_post == 2; same results) */
+    int32_t matrix_free;       /* 1 (default): those paired launches and the FP64 outer row sums
+                                  form each row's weights from the slot's cell coefficients
+                                  (8 bytes per node) with the assembly's expressions instead of
+                                  reading the nine stored values (same results); 0: read them */
 } honei_hip_batch_config;
 
 typedef struct {
@@ -499,6 +524,14 @@ typedef struct {
     int32_t pad;
     uint64_t coeff_seed, rhs_seed;
     double *x_dev;             /* optional: FP64 solution (side^2) copied here */
+    /* optional host payload (pinned memory, honei_hip_host_alloc, for
+       asynchronous copies): coefficient ((side-1)^2) and right-hand side
+       (side^2) copied to the device at admission instead of drawn from the
+       seeds (both or neither); the solution (side^2) copied to x_host, the
+       request counting as done when that copy has completed               */
+    const double *coeff_host;
+    const double *b_host;
+    double *x_host;
 } honei_hip_request;
 
 typedef struct {

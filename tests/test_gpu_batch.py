@@ -199,6 +199,26 @@ def test_paired_passes_equal_separate_launches_513_and_1025(inner):
         assert np.array_equal(sep[key][1], par[key][1]), key
 
 
+@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+@pytest.mark.parametrize("outer", [0, 1])
+def test_matrix_free_equals_stored_values(outer, inner):
+    """matrix_free = 1 (default): the paired launches and the FP64 outer row
+    sums form each row's weights from the slots' cell coefficients with the
+    assembly's expressions; every slot's iterations, defect and x equal those
+    of matrix_free = 0 (values read from the stored matrices), bit for bit,
+    at sides 9..513."""
+    levels = [9, 7, 5, 3]
+    systems = {(c, j): _system(lv, 800 + 10 * c + j) for c, lv in enumerate(levels) for j in range(2)}
+    kw = dict(outer_method=outer, inner_vtype=inner, iters_per_launch=3)
+    st, _, _ = _run_batch(levels, [2] * 4, systems, matrix_free=0, **kw)
+    mf, _, _ = _run_batch(levels, [2] * 4, systems, matrix_free=1, **kw)
+    for key in systems:
+        assert st[key][0] == mf[key][0], key
+        assert np.array_equal(st[key][1], mf[key][1]), key
+        if outer == 1:
+            assert mf[key][0][2] == 0, key
+
+
 def test_batch_slot_reuse_and_late_admission():
     """A slot reloaded after its first system retired, while another slot is
     still running, solves the new system exactly as a fresh batch does."""
@@ -241,7 +261,7 @@ def test_batch_not_converged_and_budget():
     assert e.value.code == 1
     for bad in (dict(fused_levels=7), dict(fused_levels=-1), dict(outer_method=2), dict(n_pre=1),
                 dict(iters_per_launch=0), dict(averaging=3), dict(use_graphs=2),
-                dict(pair_sweeps=2)):
+                dict(pair_sweeps=2), dict(matrix_free=2)):
         with pytest.raises(H.HipError) as e:
             H.Batch([5], [1], H.batch_config(**bad))
         assert e.value.code == 1, bad

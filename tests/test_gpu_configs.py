@@ -155,6 +155,37 @@ def test_serve_alone_equals_in_the_stream_257_and_1025():
         assert np.array_equal(xs[pos].get(), xa), pos
 
 
+def test_serve_host_payload_equals_seeded():
+    """Requests whose coefficient and right-hand side come from pinned host
+    memory and whose solution returns there, mixed with seeded requests in
+    one stream (sides 65..1025): every result has the bits, iterations and
+    residual of the same system served from its seeds; a payload request is
+    done only after its solution has reached host memory."""
+    stream = serving_requests(1.0)[:20]
+    res0, st0, xs0 = _serve_with_solutions(stream, [2] * 5)
+    assert st0.completed == len(stream)
+    mix, host = [], {}
+    for i, q in enumerate(stream):
+        if i % 2:
+            kappa, b = _device_fields(q[1], q[2], q[3])
+            pk, pb, px = H.PinnedArray(kappa.size), H.PinnedArray(b.size), H.PinnedArray(b.size)
+            pk.a[:] = kappa
+            pb.a[:] = b
+            px.a[:] = np.nan
+            host[i] = px
+            mix.append(q + (None, pk, pb, px))
+        else:
+            host[i] = H.DeviceArray(xs0[i].n, np.float64)
+            mix.append(q + (host[i],))
+    res, st = H.serve(list(SERVE_LEVELS), mix, counts=[2] * 5)
+    assert st.completed == len(mix) and st.not_converged == 0
+    for i, (r, r0) in enumerate(zip(res, res0)):
+        x = host[i].a.copy() if i % 2 else host[i].get()
+        assert (r.iters, r.rel, r.status) == (r0.iters, r0.rel, r0.status), i
+        assert np.array_equal(x, xs0[i].get()), i
+        assert r.admit_s <= r.done_s
+
+
 @pytest.mark.parametrize("levels", [(6, 7, 8, 9), (10,)])
 def test_serve_error_vs_fp64_direct_solve(levels):
     """One request per size of the stream: the served solution against the

@@ -144,6 +144,67 @@ def test_product_on_reference_index_arrays_q1_2049(dtype):
     assert np.array_equal(dy.get().view(np.uint8), oracle.product(A, x).view(np.uint8))
 
 
+def sell_padded_slots(lengths, window=2048, c=64, keys=64):
+    """Slots of the sliced copy: rows sorted by min(length, keys - 1)
+    descending, ties by row, within windows; each slice of c sorted rows
+    padded to its longest row."""
+    lengths = np.asarray(lengths, np.int64)
+    key = np.minimum(lengths, keys - 1)
+    n = lengths.size
+    perm = np.concatenate([np.arange(w0, min(n, w0 + window))[np.lexsort(
+        (np.arange(w0, min(n, w0 + window)), -key[w0:min(n, w0 + window)]))] for w0 in range(0, n, window)])
+    L = lengths[perm]
+    return int(sum(c * L[s:s + c].max() for s in range(0, n, c)))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n,max_len", [(1, 27), (63, 27), (2048, 27), (2049, 27), (20000, 27), (5000, 100),
+                                       (300001, 27)])
+def test_sliced_copy_product_bitexact(dtype, n, max_len):
+    """honei_hip_product_sell on the sliced copy of a ragged random ELL
+    matrix: y bit-identical to the ELL product and to Product<tags::CPU>
+    (every row's terms in stored order); the padded slot count is the one
+    the windowed sort implies (rows longer than 63 share the last key)."""
+    rp, col, val = random_band_csr(n, seed=40 + n % 7, max_len=max_len, band=min(4096, max(n - 1, 1)), dtype=dtype)
+    A = oracle.ell_from_sm(n, n, rp, col, val, 1)
+    x = uniform_pm1(6, n).astype(dtype)
+    D = H.DeviceEll.from_oracle(A)
+    S = H.Sell(D)
+    dx = H.DeviceArray.from_host(x)
+    ye, ys = H.DeviceArray(n, dtype), H.DeviceArray(n, dtype)
+    D.product(ye, dx)
+    S.product(ys, dx)
+    assert np.array_equal(ys.get().view(np.uint8), ye.get().view(np.uint8))
+    assert np.array_equal(ys.get().view(np.uint8), oracle.product(A, x).view(np.uint8))
+    assert S.padded_slots == sell_padded_slots(A.Arl)
+    assert S.padded_slots <= A.ncpr * A.stride
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sliced_copy_of_the_random_4m_matrix(dtype):
+    """The SpMV benchmark's random matrix (4,194,304 rows, lengths U{1..27},
+    columns within +-4096, generated in HBM): the sliced product equals the
+    ELL product bit for bit, and the sliced copy stores under 55 % of the
+    padded ELL slots."""
+    n, ncpr = 1 << 22, 27
+    vt = H.F64 if dtype == np.float64 else H.F32
+    Aj = H.DeviceArray(ncpr * n, np.int32)
+    Ax = H.DeviceArray(ncpr * n, dtype)
+    Arl = H.DeviceArray(n, np.uint32)
+    H.call("honei_hip_random_ell", vt, n, 27, 4096, 2, Aj.ptr, Ax.ptr, Arl.ptr, n, 1, None)
+    D = H.DeviceEll.__new__(H.DeviceEll)
+    D.rows, D.cols, D.stride, D.ncpr, D.threads, D.vtype = n, n, n, ncpr, 1, vt
+    D.Aj, D.Ax, D.Arl = Aj, Ax, Arl
+    S = H.Sell(D)
+    x = H.DeviceArray.from_host(uniform_pm1(2, n).astype(dtype))
+    ye, ys = H.DeviceArray(n, dtype), H.DeviceArray(n, dtype)
+    D.product(ye, x)
+    S.product(ys, x)
+    assert np.array_equal(ys.get().view(np.uint8), ye.get().view(np.uint8))
+    assert S.padded_slots == sell_padded_slots(Arl.get())
+    assert S.padded_slots < 0.55 * ncpr * n
+
+
 def test_product_row_range_offsets_output():
     """Sub-range convention of cuda_product_smell_dv_*: y[r - row_start]."""
     A = oracle.q1_matrix(65)
