@@ -7,6 +7,7 @@
 // Multiply and add stay separate roundings in everything that follows.
 #pragma clang fp contract(off)
 
+#include "exact_div.hh"
 #include "hip_common.hh"
 
 namespace honei {
@@ -33,15 +34,16 @@ __device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, cons
     const int64_t nc = side - 1;
     const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
     const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
-    w[0] = (V)(-sw / 3.0);              // (i-1, j-1)
-    w[1] = (V)(-(sw + se) / 6.0);       // (i-1, j)
-    w[2] = (V)(-se / 3.0);              // (i-1, j+1)
-    w[3] = (V)(-(sw + nw) / 6.0);       // (i, j-1)
-    w[4] = (V)(2.0 * (sw + se + nw + ne) / 3.0);
-    w[5] = (V)(-(se + ne) / 6.0);       // (i, j+1)
-    w[6] = (V)(-nw / 3.0);              // (i+1, j-1)
-    w[7] = (V)(-(nw + ne) / 6.0);       // (i+1, j)
-    w[8] = (V)(-ne / 3.0);              // (i+1, j+1)
+    // the values of -c / 3.0, -(a + b) / 6.0 and 2.0 * (sum) / 3.0
+    w[0] = (V)(-div3_rn(sw));                  // (i-1, j-1)
+    w[1] = (V)(-0.5 * div3_rn(sw + se));       // (i-1, j)
+    w[2] = (V)(-div3_rn(se));                  // (i-1, j+1)
+    w[3] = (V)(-0.5 * div3_rn(sw + nw));       // (i, j-1)
+    w[4] = (V)div3_rn(2.0 * (sw + se + nw + ne));
+    w[5] = (V)(-0.5 * div3_rn(se + ne));       // (i, j+1)
+    w[6] = (V)(-div3_rn(nw));                  // (i+1, j-1)
+    w[7] = (V)(-0.5 * div3_rn(nw + ne));       // (i+1, j)
+    w[8] = (V)(-div3_rn(ne));                  // (i+1, j+1)
 }
 
 // A float as a bf16 matrix stores it (round to nearest even) and widens it

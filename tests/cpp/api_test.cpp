@@ -248,7 +248,10 @@ void reference_glue() {
             DenseVector<DT> x = vec<DT>(rand_vec(cols)), rhs = vec<DT>(rand_vec(rows));
             DenseVector<DT> y32(rows, DT(0)), r32(rows, DT(0));
             Product<tags::GPU::HIP>::value(y32, a, x);
-            Defect<tags::GPU::HIP>::value(r32, rhs, a, x);
+            // Defect<> keeps the reference's size checks (rhs against the
+            // columns, math/defect-cuda.cc:530-537): square matrices only
+            const bool square = rows == cols;
+            if (square) Defect<tags::GPU::HIP>::value(r32, rhs, a, x);
             unsigned long *aj = a.Aj().elements(), *arl = a.Arl().elements();
             DT *ax = a.Ax().elements(), *xe = x.elements(), *re = rhs.elements();
             std::vector<DT> y(rows, DT(4711)), r(rows, DT(4711));
@@ -262,12 +265,13 @@ void reference_glue() {
             hip_check(product64(x_gpu, y_gpu, Aj_gpu, Ax_gpu, Arl_gpu, 0, rows, a.num_cols_per_row(), a.stride(), 0,
                                 a.threads(), nullptr),
                       "product idx64");
-            hip_check(defect64(rhs_gpu, r_gpu, Aj_gpu, Ax_gpu, Arl_gpu, x_gpu, 0, rows, a.num_cols_per_row(),
-                               a.stride(), 0, a.threads(), nullptr),
-                      "defect idx64");
+            if (square)
+                hip_check(defect64(rhs_gpu, r_gpu, Aj_gpu, Ax_gpu, Arl_gpu, x_gpu, 0, rows, a.num_cols_per_row(),
+                                   a.stride(), 0, a.threads(), nullptr),
+                          "defect idx64");
             mb->download(y.data(), y.data(), sizeof(DT) * rows);
             mb->download(r.data(), r.data(), sizeof(DT) * rows);
-            for (unsigned long i = 0; i < rows; ++i) TEST_CHECK(y[i] == y32[i] && r[i] == r32[i]);
+            for (unsigned long i = 0; i < rows; ++i) TEST_CHECK(y[i] == y32[i] && (!square || r[i] == r32[i]));
             // a sub-range writes y[i - row_start], as the int32 entry does
             const unsigned long r0 = rows / 3, r1 = rows - 5;
             hip_check(product64(x_gpu, y_gpu, Aj_gpu, Ax_gpu, Arl_gpu, r0, r1, a.num_cols_per_row(), a.stride(), 0,

@@ -12,8 +12,9 @@ each result with a reference solution):
     below the tolerance (x0: the boundary start vector of the driver);
   * the device's FP64 operator is the oracle's, bit for bit;
   * the error against the oracle's FP64 direct solve (sparse LU plus
-    refinement to a relative residual of 1e-14) is stated for a sample and
-    bounded;
+    refinement, aiming at a relative residual of 1e-14 and reaching at most
+    1e-13 -- the rounding of the FP64 defect itself -- five orders below the
+    solver tolerance) is stated for a sample and bounded;
   * a request's result is bit-identical served alone and served inside the
     stream, at 257^2 and 1025^2.
 """
@@ -73,7 +74,7 @@ def _fp64_check(level, kappa, b, x):
 
 def _err_vs_direct(A, b, x):
     x64, rel, _ = oracle.direct_solve(A, b)
-    assert rel <= 1e-14
+    assert rel <= 1e-13
     return float(np.abs(x - x64).max() / np.abs(x64).max())
 
 
