@@ -90,6 +90,8 @@ _SIGS = {
     "honei_hip_random_ell": [_i, _sz, _sz, _sz, _u64, _p, _p, _p, _sz, _sz, _p],
     "honei_hip_cg_solve": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
                            C.POINTER(_d), _p],
+    "honei_hip_cg_solve_noalloc": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _sz, _d, C.POINTER(_sz),
+                                   C.POINTER(_d), _p],
     "honei_hip_cg_konv": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _d, _sz, C.POINTER(_sz), _p],
     "honei_hip_pcg_jacobi": [_i, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _p, _sz, _d, C.POINTER(_sz),
                              C.POINTER(_d), _p],
@@ -664,6 +666,19 @@ def cg_solve(D: DeviceEll, b: DeviceArray, x: DeviceArray, max_iters=10000, eps=
                                   b.ptr, x.ptr, max_iters, eps, C.byref(used), C.byref(fd), stream)
     if rc != SUCCESS and (check or rc != 5):
         raise HipError(rc, "honei_hip_cg_solve", last_error())
+    return used.value, fd.value
+
+
+def cg_solve_noalloc(D: DeviceEll, b: DeviceArray, x: DeviceArray, max_iters=10000, eps=1e-8, stream=None,
+                     check=True):
+    """honei_hip_cg_solve_noalloc: cg_solve on the library's fixed device
+    scratch (no allocation at call time; rows bounded by that scratch)."""
+    used, fd = _sz(), _d()
+    rc = lib().honei_hip_cg_solve_noalloc(D.vtype, D.Aj.ptr, D.Ax.ptr, D.Arl.ptr, D.rows, D.ncpr, D.stride,
+                                          D.threads, b.ptr, x.ptr, max_iters, eps, C.byref(used), C.byref(fd),
+                                          stream)
+    if rc != SUCCESS and (check or rc != 5):
+        raise HipError(rc, "honei_hip_cg_solve_noalloc", last_error())
     return used.value, fd.value
 
 

@@ -147,7 +147,8 @@ int honei_hip_init(int device) {
     if (device < 0 || device >= n) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "device index out of range");
     st = check(hipSetDevice(device), "hipSetDevice");
     if (st) return st;
-    return workspace_for(nullptr) ? HONEI_HIP_SUCCESS : HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    if (!workspace_for(nullptr)) return HONEI_HIP_ERROR_OUT_OF_MEMORY;
+    return noalloc_setup(nullptr);
 }
 
 int honei_hip_shutdown(void) {

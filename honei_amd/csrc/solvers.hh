@@ -68,6 +68,9 @@ EllView cg_view(int vtype, const void *Aj, const void *Ax, const void *Arl, size
 int cg_solve_on(const EllView &A, const void *b, void *x, size_t max_iters, double eps_relative, size_t *used_iters,
                 double *final_defect, hipStream_t s, const ReduceSlots &slots, void *p, void *r, void *v,
                 CGScalars *sc);
+// Device address of the allocation-free variant's fixed scratch on the
+// current device (looked up once per device); HONEI status code.
+int noalloc_setup(void **out);
 // Iteration `it` (0, 1, ... in order) of the loop.
 hipError_t cg_iteration(const EllView &A, void *x, void *p, void *r, void *v, CGScalars *sc,
                         const ReduceSlots &sl, int it, hipStream_t s);

@@ -313,6 +313,23 @@ int honei_hip_cg_solve(int vtype, const void *Aj, const void *Ax, const void *Ar
                        size_t max_iters, double eps_relative, size_t *used_iters, double *final_defect,
                        void *stream);
 
+/* honei_hip_cg_solve without allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory at call time (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels, results and status codes as honei_hip_cg_solve;
+ * the solve's scratch (reduction partials, scalars, vectors p, r, v) is a
+ * fixed library-held device array of 4 MiB + 4 KiB + 32 MiB per device,
+ * reserved when the library is loaded and kept until the process ends.
+ * rows is limited to what three vectors fit in 32 MiB (1,398,080 rows for
+ * F64 and 2,796,160 for F32 / BF16, each vector rounded up to 256 bytes);
+ * larger systems return HONEI_HIP_ERROR_INVALID_ARGUMENT without running.
+ * Every call on one device uses the same scratch: calls that overlap on one
+ * device (other threads or streams) race and must be serialised.          */
+int honei_hip_cg_solve_noalloc(int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
+                               size_t num_cols_per_row, size_t stride, size_t threads, const void *b_dev,
+                               void *x_dev, size_t max_iters, double eps_relative, size_t *used_iters,
+                               double *final_defect, void *stream);
+
 /* ConjugateGradients<Tag, methods::JAC>::value(A, b, x, dd_inverted,
  * max_iters, used_iters, eps_relative) on ELL
  * (math/conjugate_gradients.hh:1110-1154, kernel :884-913): Jacobi-
