@@ -946,8 +946,10 @@ def main():
                          "(same kernels and results; for kernel-trace profiling)")
     ap.add_argument("--no-pair", action="store_true",
                     help="with --batch-only: one launch per smoothing pass (pair_sweeps = 0), for comparison")
-    ap.add_argument("--no-mf", action="store_true",
-                    help="with --batch-only: read the stored matrix values (matrix_free = 0), for comparison")
+    ap.add_argument("--weights", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="with --batch-only: weight source of the paired launches and FP64 outer row sums "
+                         "(matrix_free 0 stored values, 1 cell coefficients, 2 distinct-weight arrays; "
+                         "-1: the pool's default), for comparison")
     ap.add_argument("--iters-per-launch", type=int, default=0,
                     help="with --batch-only: outer iterations per graph launch (0: the pool's default)")
     ap.add_argument("--spmv-only", action="store_true", help="only the SpMV section")
@@ -994,8 +996,9 @@ def main():
 
     if args.batch_only:
         levels, kap, rhs = batched_workload()
-        kw = dict(use_graphs=0 if args.no_graphs else 1, pair_sweeps=0 if args.no_pair else 1,
-                  matrix_free=0 if args.no_mf else 1)
+        kw = dict(use_graphs=0 if args.no_graphs else 1, pair_sweeps=0 if args.no_pair else 1)
+        if args.weights >= 0:
+            kw["matrix_free"] = args.weights
         if args.iters_per_launch > 0:
             kw["iters_per_launch"] = args.iters_per_launch
         B, slots = batched_pool(levels, **kw)
@@ -1109,6 +1112,24 @@ def main():
             "all_converged": bool(all(info_b[g].status == 0 for g in slots_b))}
     log(f"[batch bf16] {bf16['solves_per_s']} solves/s, outer median {bf16['outer_iterations_median']}")
     del Bb
+    # the same batched step entirely in FP64 (FP64 V-cycle values and vectors):
+    # separates the precision gain from the batching gain
+    Bd, slots_d = batched_pool(levels, inner_vtype=H.F64)
+    batched_step(Bd, slots_d, kap, rhs)
+    td = []
+    for _ in range(max(1, args.steps)):
+        sec, _done, _nl, info_d = batched_step(Bd, slots_d, kap, rhs)
+        td.append(sec)
+    it_d = np.array([info_d[g].iters for g in slots_d])
+    err_samples["fp64_batched"] = sample_batch_results(Bd, levels, slots_d, info_d)
+    fp64b = {"solves_per_s": round(len(levels) / float(np.median(td)), 2),
+             "outer_iterations_median": int(np.median(it_d)), "outer_iterations_max": int(it_d.max()),
+             "max_rel_residual": float(max(info_d[g].rel for g in slots_d)),
+             "all_converged": bool(all(info_d[g].status == 0 for g in slots_d)),
+             "what": ("the same 256-system batched step (one pool, same launches, CG outer) with an FP64 "
+                      "V-cycle: FP64 values and vectors on every level")}
+    log(f"[batch fp64] {fp64b['solves_per_s']} solves/s, outer median {fp64b['outer_iterations_median']}")
+    del Bd
 
     # ---- serving stream ---------------------------------------------------------------
     serving = None
@@ -1263,6 +1284,8 @@ def main():
                   "max_rel_residual": float(rels.max()), "all_converged": bool((status == 0).all()),
                   "graph_launches_per_step": round(launches / args.steps, 2)},
         "bf16_inner": bf16,
+        "fp64_batched": fp64b,
+        "speedup_mixed_vs_fp64_batched": round(value / fp64b["solves_per_s"], 2),
         "fp64_single_gpu": fp64,
         "speedup_vs_fp64_single_gpu": round(value / fp64["solves_per_s"], 2) if fp64 else None,
         "serving": serving,
@@ -1295,7 +1318,8 @@ def main():
         out["cpu_baseline_one_core"] = one
         # final error of the GPU results against the oracle's FP64 direct solve
         errs = fp64_oracle_errors(err_samples, cpu_workers())
-        for key, obj in (("batch", out["solve"]), ("bf16_inner", bf16), ("serving", serving)):
+        for key, obj in (("batch", out["solve"]), ("bf16_inner", bf16), ("fp64_batched", fp64b),
+                         ("serving", serving)):
             if obj is not None and key in errs:
                 obj["max_err_vs_fp64_oracle"] = errs[key]["max_err_vs_fp64_oracle"]
                 obj["fp64_oracle_check"] = errs[key]

@@ -126,11 +126,11 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
     HX(hipEventCreate(&e0));
     HX(hipEventCreate(&e1));
     hipError_t err = hipSuccess;
-    const size_t vs = vt_size(vt);
+    const size_t vs = vt_size(vt), es = inner_vector_bytes();
     for (int T : {1, 2, 4}) {
         const Level0 g = level_geometry(lv.level, T, 256);
         void *Aj = nullptr, *Arl = nullptr, *Ax = nullptr, *vec = nullptr, *scd = nullptr, *lst = nullptr;
-        const size_t vbytes = (size_t)(ns * g.vld) * 4;
+        const size_t vbytes = (size_t)(ns * g.vld) * es;
         err = hipMalloc(&Aj, (size_t)g.ax_ld * 4);
         if (err == hipSuccess) err = hipMalloc(&Arl, (size_t)n * 4);
         if (err == hipSuccess) err = hipMalloc(&Ax, (size_t)(ns * g.ax_ld) * vs);
@@ -156,7 +156,8 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
                 if (block < T || group > ns) continue;
                 BMat m{(const int32_t *)Aj, (const uint32_t *)Arl, Ax, g.stride, g.ax_ld, n, g.vld,
                        (n * T + block - 1) / block};
-                float *dinv = (float *)vec, *x = dinv + ns * g.vld, *rhs = x + ns * g.vld, *out = rhs + ns * g.vld;
+                char *v0 = (char *)vec;
+                void *dinv = v0, *x = v0 + vbytes, *rhs = v0 + 2 * vbytes, *out = v0 + 3 * vbytes;
                 const int reps = n >= 65536 ? 10 : 30;
                 for (int r = 0; r < 3 && err == hipSuccess; ++r)
                     err = b_ell(vt, T, block, group, ELL_JACOBI, m, w, ns, out, x, rhs, dinv, s_main_);
@@ -191,19 +192,23 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
             tl.Aj = (int32_t *)Aj;
             tl.Arl = (uint32_t *)Arl;
             tl.Ax = Ax;
-            tl.dinv = (float *)vec;
-            tl.x = tl.dinv + ns * g.vld;
-            tl.rhs = tl.x + ns * g.vld;
-            float *out = tl.rhs + ns * g.vld;
+            char *v0 = (char *)vec;
+            tl.dinv = v0;
+            tl.x = v0 + vbytes;
+            tl.rhs = v0 + 2 * vbytes;
+            void *out = v0 + 3 * vbytes;
             tl.kld = round_up((g.side - 1) * (g.side - 1), 32);
             err = hipMalloc((void **)&tl.kappa, (size_t)(ns * tl.kld) * 8);
             if (err == hipSuccess) err = vfill(tl.kappa, 1.0, ns * tl.kld, VT_F64, s_main_);
-            // both forms: matrix values read (mf 0) and formed from the cell coefficients (mf 1)
-            for (int mf = 0; mf < 2; ++mf) {
+            if (err == hipSuccess) err = hipMalloc(&tl.st, (size_t)(ns * 4 * g.vld) * vs);
+            if (err == hipSuccess) err = b_stencil_fill(vt, g.side, tl.kappa, tl.kld, tl.st, g.vld, ns, s_main_);
+            // every weight source: the stored values (0), the cell
+            // coefficients (1), the distinct-weight arrays (2)
+            for (int mf = 0; mf < 3; ++mf) {
                 for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
                     auto launch = [&]() {
-                        return pass == 0 ? b_pair(vt, PAIR_PRE, tl, w, tl.x, out, nullptr, s_main_, mf == 1)
-                                         : b_pair(vt, PAIR_POST, tl, w, out, nullptr, tl.x, s_main_, mf == 1);
+                        return pass == 0 ? b_pair(vt, PAIR_PRE, tl, w, tl.x, out, nullptr, s_main_, mf)
+                                         : b_pair(vt, PAIR_POST, tl, w, out, nullptr, tl.x, s_main_, mf);
                     };
                     const int reps = n >= 65536 ? 10 : 30;
                     for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch();
@@ -218,6 +223,7 @@ hipError_t BatchMG::tune_level(BLevel &lv) {
             }
             (void)hipStreamSynchronize(s_main_);
             if (tl.kappa) (void)hipFree(tl.kappa);
+            if (tl.st) (void)hipFree(tl.st);
         }
         (void)hipStreamSynchronize(s_main_);
         for (void *p : {Aj, Arl, Ax, vec, scd, lst})
@@ -253,8 +259,9 @@ uint64_t BatchMG::plan_bytes() const {
             b += (uint64_t)lv.ax_ld * 4 + (uint64_t)lv.n * 4;                      // Aj, Arl
             b += (uint64_t)(lv.cap * lv.ax_ld) * vs;                                // Ax
         }
-        b += (uint64_t)(lv.cap * lv.vld) * 4 * (l == 1 ? 8 : 5);                    // vectors
+        b += (uint64_t)(lv.cap * lv.vld) * inner_vector_bytes() * (l == 1 ? 8 : 5);   // vectors
         b += (uint64_t)(lv.cap * lv.kld) * 8;                                       // cell coefficients
+        if (stores_stencil(lv)) b += (uint64_t)(lv.cap * 4 * lv.vld) * vs;          // distinct weights
     }
     for (const BClass &c : C_) {
         if (stores_outer_values()) b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4 + (uint64_t)(c.count * c.ax_ld) * 8;
@@ -271,14 +278,15 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     over_budget_ = false;
     cfg_ = cfg;
     if (nclass < 1 || nclass > kMaxClasses) return hipErrorInvalidValue;
-    if (cfg.inner_vtype != VT_F32 && cfg.inner_vtype != VT_BF16) return hipErrorInvalidValue;
+    if (cfg.inner_vtype != VT_F32 && cfg.inner_vtype != VT_BF16 && cfg.inner_vtype != VT_F64)
+        return hipErrorInvalidValue;
     if (cfg.n_pre < 2 || cfg.n_pre > 16 || cfg.n_post < 0 || cfg.n_post > 16) return hipErrorInvalidValue;
     if (cfg.iters_per_launch < 1 || cfg.iters_per_launch > 64 || cfg.max_outer < 1) return hipErrorInvalidValue;
     if (cfg.averaging < 0 || cfg.averaging > 2) return hipErrorInvalidValue;
     if (cfg.outer != 0 && cfg.outer != 1) return hipErrorInvalidValue;
     if (cfg.use_graphs != 0 && cfg.use_graphs != 1) return hipErrorInvalidValue;
     if (cfg.pair_sweeps != 0 && cfg.pair_sweeps != 1) return hipErrorInvalidValue;
-    if (cfg.matrix_free != 0 && cfg.matrix_free != 1) return hipErrorInvalidValue;
+    if (cfg.matrix_free < 0 || cfg.matrix_free > 2) return hipErrorInvalidValue;
     std::vector<int> order((size_t)nclass);
     for (int c = 0; c < nclass; ++c) {
         if (levels[c] < 2 || levels[c] > 13 || counts[c] < 1 || counts[c] > 65535) return hipErrorInvalidValue;
@@ -342,11 +350,12 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     const size_t vs = vt_size(vt);
     for (int64_t l = 1; l <= max_level_; ++l) {
         BLevel &lv = L_[(size_t)l];
-        const size_t vb = (size_t)(lv.cap * lv.vld) * 4;
-        for (float **p : {&lv.dinv, &lv.rhs, &lv.x, &lv.d, &lv.t}) HX(alloc((void **)p, vb));
+        const size_t vb = (size_t)(lv.cap * lv.vld) * inner_vector_bytes();
+        for (void **p : {&lv.dinv, &lv.rhs, &lv.x, &lv.d, &lv.t}) HX(alloc(p, vb));
         HX(alloc((void **)&lv.kappa, (size_t)(lv.cap * lv.kld) * 8));
+        if (stores_stencil(lv)) HX(alloc(&lv.st, (size_t)(lv.cap * 4 * lv.vld) * vs));
         if (l == 1)
-            for (float **p : {&lv.g, &lv.u, &lv.en}) HX(alloc((void **)p, vb));
+            for (void **p : {&lv.g, &lv.u, &lv.en}) HX(alloc(p, vb));
         if (!stores_values(lv)) continue;   // matrix-free paired level: no index arrays or values
         HX(alloc((void **)&lv.Aj, (size_t)lv.ax_ld * 4));
         HX(alloc((void **)&lv.Arl, (size_t)lv.n * 4));
@@ -405,21 +414,21 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         const BWork w = level_work(l);
         if (paired(lv) && cfg_.n_pre == 2) {
             // x = Jacobi0, d = rhs - A x: one launch
-            HX(b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s, cfg_.matrix_free != 0));
-            HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
+            HX(b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s, pair_weights(lv)));
+            HX(b_restrict(vt, lv, L_[(size_t)l - 1], w, s));
             continue;
         }
         // x = n_pre Jacobi sweeps from zero (first two fused), ending in x
         const int rest = cfg_.n_pre - 2;
-        float *src = (rest % 2 == 0) ? lv.x : lv.t;
+        void *src = (rest % 2 == 0) ? lv.x : lv.t;
         HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI0, m, w, lv.cap, src, nullptr, lv.rhs, lv.dinv, s));
         for (int k = 0; k < rest; ++k) {
-            float *dst = (src == lv.x) ? lv.t : lv.x;
+            void *dst = (src == lv.x) ? lv.t : lv.x;
             HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
             src = dst;
         }
         HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_DEFECT, m, w, lv.cap, lv.d, lv.x, lv.rhs, nullptr, s));
-        HX(b_restrict(lv, L_[(size_t)l - 1], w, s));
+        HX(b_restrict(vt, lv, L_[(size_t)l - 1], w, s));
     }
     if (lf >= 2) {
         FusedCycle fc{};
@@ -449,7 +458,7 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         fc.g = L_[1].g;
         fc.u = L_[1].u;
         fc.en = L_[1].en;
-        fc.konv = 1.1920928955078125e-07;   // FP32 epsilon, as the level-1 launch
+        fc.konv = coarse_konv();
         fc.max_iters = (int)(4 * L_[1].n + 64);
         HX(b_coarse_vcycle(vt, fc, level_work(1), L_[1].cap, s));
     } else {
@@ -461,17 +470,17 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         const BWork w = level_work(l);
         if (paired(lv) && cfg_.n_post == 2) {
             // t = x + P e, then x = J(J(t)): one launch for both sweeps
-            HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.t, s));
-            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s, cfg_.matrix_free != 0));
+            HX(b_prolong_axpy(vt, lv, L_[(size_t)l - 1], w, lv.t, s));
+            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s, pair_weights(lv)));
             continue;
         }
-        HX(b_prolong_axpy(lv, L_[(size_t)l - 1], w, lv.x, s));
-        float *src = lv.x, *dst = lv.t;
+        HX(b_prolong_axpy(vt, lv, L_[(size_t)l - 1], w, lv.x, s));
+        void *src = lv.x, *dst = lv.t;
         for (int k = 0; k < cfg_.n_post; ++k) {
             HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
             std::swap(src, dst);
         }
-        if (src != lv.x) HX(b_copy(lv, w, lv.x, src, s));
+        if (src != lv.x) HX(b_copy(vt, lv, w, lv.x, src, s));
     }
     return hipSuccess;
 }
@@ -490,6 +499,7 @@ BClassTab BatchMG::class_table() const {
         e.stride = c.stride;
         e.ax_ld = c.ax_ld;
         e.bps = c.bps;
+        e.side_magic = (uint32_t)((uint64_t(1) << 32) / (uint64_t)c.side + 1);
         e.pst = 4 * c.bps;
         e.nps = (c.n + 63) / 64;
         e.vld_top = top.vld;
@@ -503,6 +513,7 @@ BClassTab BatchMG::class_table() const {
         e.q = c.q;
         e.partials = c.partials;
         e.rhs_top = top.rhs;
+        e.top64 = cfg_.inner_vtype == VT_F64 ? 1 : 0;
         e.x_top = top.x;
         e.kappa = top.kappa;
         e.kld = top.kld;
@@ -628,6 +639,7 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
         P.side[l] = L_[(size_t)l].side;
         P.stride[l] = L_[(size_t)l].stride;
         P.T[l] = L_[(size_t)l].T;
+        P.vld[l] = L_[(size_t)l].vld;
     }
     for (const BClass &c : C_) P.stride64[c.level] = c.stride;
     for (int64_t e = 0; e < n; ++e) {
@@ -646,7 +658,8 @@ hipError_t BatchMG::load_many(int64_t n, const int64_t *slots, const double *con
             const BLevel &lv = L_[(size_t)l];
             b.kappa[l] = lv.kappa + g * lv.kld;
             b.Ax[l] = lv.Ax ? (char *)lv.Ax + (size_t)(g * lv.ax_ld) * vs : nullptr;
-            b.dinv[l] = lv.dinv + g * lv.vld;
+            b.dinv[l] = (char *)lv.dinv + (size_t)(g * lv.vld) * inner_vector_bytes();
+            b.st[l] = lv.st ? (char *)lv.st + (size_t)(g * 4 * lv.vld) * vs : nullptr;
         }
         b.Ax64 = c.Ax ? c.Ax + ls * c.ax_ld : nullptr;
         b.b = c.b + ls * c.vld;
@@ -767,10 +780,11 @@ hipError_t BatchMG::time_level(int64_t level, int reps, double us[5]) {
         if (err == hipSuccess) *out = 1000.0 * ms / reps;
     };
     const bool mf = cfg_.matrix_free != 0;
-    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_, mf); }, &us[0]);
-    timeit([&]() { return b_restrict(lv, lc, w, s_main_); }, &us[1]);
-    timeit([&]() { return b_prolong_axpy(lv, lc, w, lv.t, s_main_); }, &us[2]);
-    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, mf); }, &us[3]);
+    const int wm = pair_weights(lv);
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_, wm); }, &us[0]);
+    timeit([&]() { return b_restrict(vt, lv, lc, w, s_main_); }, &us[1]);
+    timeit([&]() { return b_prolong_axpy(vt, lv, lc, w, lv.t, s_main_); }, &us[2]);
+    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, wm); }, &us[3]);
     if (cfg_.outer == 1) {
         const BClassTab tab = class_table();
         int64_t blocks = 0;

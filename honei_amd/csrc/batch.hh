@@ -70,7 +70,9 @@ constexpr int kPairTileRows = 16, kPairRows = kPairTileRows - 2;
 constexpr int64_t kPairMinSide = 33;
 
 struct BatchConfig {
-    int32_t inner_vtype = VT_F32;    // VT_F32 or VT_BF16 matrix values (vectors FP32)
+    // VT_F32 or VT_BF16 matrix values with FP32 vectors (mixed precision),
+    // or VT_F64 values and vectors (the same solver entirely in FP64)
+    int32_t inner_vtype = VT_F32;
     int32_t n_pre = 2, n_post = 2;   // Jacobi sweeps (n_pre >= 2)
     double omega = 0.8;              // Jacobi damping
     double tolerance = 1e-8;         // relative FP64 defect
@@ -92,10 +94,14 @@ struct BatchConfig {
     // defect and the two post-smoothing sweeps each run as one launch that
     // reads the matrix once (n_pre == 2 / n_post == 2); 0: one launch per pass.
     int32_t pair_sweeps = 1;
-    // 1: the paired launches and the FP64 outer row sums form each row's
-    // nine weights from the slot's cell coefficients (8 bytes per node) with
-    // the assembly's own expressions instead of reading the stored values
-    // (9 per row): the same bits; 0: read the stored values.
+    // Where the paired launches and the FP64 outer row sums take each row's
+    // nine weights from (all forms give the stored matrix's bits):
+    //   0: the stored values (9 per row);
+    //   1: formed from the slot's cell coefficients (8 bytes per node) with
+    //      the assembly's own expressions, paired launches and FP64 outer;
+    //   2: paired launches gather them from four per-node arrays of the
+    //      distinct weights (q1_stencil_node; one per cell, edge and node),
+    //      built with the hierarchy; FP64 outer as 1.
     int32_t matrix_free = 1;
 };
 
@@ -107,10 +113,12 @@ struct BLevel {
     int32_t *Aj = nullptr;
     uint32_t *Arl = nullptr;
     void *Ax = nullptr;
-    float *dinv = nullptr, *rhs = nullptr, *x = nullptr, *d = nullptr, *t = nullptr;
-    float *g = nullptr, *u = nullptr, *en = nullptr;   // coarse CG scratch (level 1 only)
+    // vectors of the inner type (float, or double for an FP64 inner solve)
+    void *dinv = nullptr, *rhs = nullptr, *x = nullptr, *d = nullptr, *t = nullptr;
+    void *g = nullptr, *u = nullptr, *en = nullptr;   // coarse CG scratch (level 1 only)
     double *kappa = nullptr;   // cell coefficients [cap][kld] ((side-1)^2 per slot)
     int64_t kld = 0;
+    void *st = nullptr;        // distinct weights [cap][4][vld] of the value type (matrix_free 2)
 };
 
 // Class c: slots [first, first + count), finest level `level`, FP64 outer data.
@@ -129,9 +137,10 @@ struct LaunchShape {
     int T = 1, block = 256, group = 1;
     double us = 0.0;       // measured time of one batched Jacobi sweep
     int64_t slots = 0;     // slots of the tuning batch
-    // paired launches (b_pair) on the same batch, T = 1, with the matrix
-    // values read [0] and formed from cell coefficients [1]
-    double pre_us[2] = {0.0, 0.0}, post_us[2] = {0.0, 0.0};
+    // paired launches (b_pair) on the same batch, T = 1, with the weights
+    // from the stored values [0], the cell coefficients [1], the arrays of
+    // distinct weights [2]
+    double pre_us[3] = {0.0, 0.0, 0.0}, post_us[3] = {0.0, 0.0, 0.0};
 };
 
 // Batched kernels (batch_kernels.hip).
@@ -154,14 +163,16 @@ struct BWork {
 // every class in one launch (logical block = 256 rows of one slot).
 struct BClassEntry {
     int64_t first, n, side, vld, stride, ax_ld, bps, vld_top;
+    uint32_t side_magic;   // floor(2^32 / side) + 1: v / side = umulhi(v, side_magic) for v side < 2^32
     int64_t pst;   // partial stride per slot: 4 bps
     int64_t nps;   // partials per slot of the CG reductions: one per 64-row chunk
     const int32_t *Aj;
     const uint32_t *Arl;
     const double *Ax, *b;
     double *x, *r, *p, *q, *partials;
-    float *rhs_top;     // inner right-hand side of the class's finest level
-    const float *x_top; // inner solution of the class's finest level
+    void *rhs_top;      // inner right-hand side of the class's finest level
+    const void *x_top;  // inner solution of the class's finest level
+    int32_t top64;      // 1: the inner vectors are double, 0: float
     const double *kappa;   // cell coefficients of the finest level, [global slot][kld]
     int64_t kld;
 };
@@ -173,7 +184,7 @@ struct BClassTab {
 enum OuterOp : int32_t {
     OP_DEFECT_ALL = 0,     // r = b - A x (+ |r|^2) for every listed slot
     OP_DEFECT_SETTLE = 1,  // same op; CG lists only new / verifying slots for it
-    OP_SCALE = 2,          // rhs_top = (float)(r / |r|)
+    OP_SCALE = 2,          // rhs_top = r / |r| in the inner precision
     OP_ACC = 3,            // x = (first cycle ? 0 : x) + |r| * x_top   (defect correction)
     OP_DOT_RZ = 4,         // partial r.z, z = |r| * x_top
     OP_PUPDATE = 5,        // p = z + beta p
@@ -194,6 +205,7 @@ struct BMat {
     int64_t stride, ax_ld, n, vld, bps;
     const double *kappa;   // cell coefficients [slot][kld] (matrix-free launches)
     int64_t kld;
+    const void *st;        // distinct weights [slot][4][vld] (stencil-array launches)
 };
 
 // ---- bulk hierarchy build (many slots per launch) ----------------------------
@@ -209,14 +221,15 @@ struct BuildEntry {
     const double *coeff_src, *b_src;
     double *kappa[kMaxLevel + 1]; // coefficient of level l ((2^l)^2 cells) of this slot; kappa[level] finest
     void *Ax[kMaxLevel + 1];      // this slot's inner values per level
-    float *dinv[kMaxLevel + 1];   // this slot's omega / diag per level
+    void *dinv[kMaxLevel + 1];    // this slot's omega / diag per level (inner vector type)
+    void *st[kMaxLevel + 1];      // this slot's distinct-weight arrays per level, or null
     double *Ax64, *b, *x;         // FP64 outer values, right-hand side, start vector
 };
 struct BuildParams {
     int32_t entries, averaging, vt;
     int32_t outer_values;         // 0: the pool stores no FP64 outer values (matrix-free outer)
     double sigma, omega;
-    int64_t side[kMaxLevel + 1], stride[kMaxLevel + 1], stride64[kMaxLevel + 1];
+    int64_t side[kMaxLevel + 1], stride[kMaxLevel + 1], stride64[kMaxLevel + 1], vld[kMaxLevel + 1];
     int32_t T[kMaxLevel + 1];
 };
 
@@ -300,9 +313,21 @@ class BatchMG {
         const bool all_paired = paired(lv) && cfg_.n_pre == 2 && (cfg_.n_post == 2 || cfg_.n_post == 0);
         return !(cfg_.matrix_free && all_paired && lv.level > lf);
     }
+    // the level's paired launches gather the weights from the stencil arrays
+    bool stores_stencil(const BLevel &lv) const { return cfg_.matrix_free == 2 && !stores_values(lv); }
+    // weight source of the level's paired launches (b_pair's wmode)
+    int pair_weights(const BLevel &lv) const {
+        return cfg_.matrix_free == 0 ? 0 : (stores_stencil(lv) ? 2 : 1);
+    }
     // the FP64 outer values are read only when the outer row sums do not
     // form them from the cell coefficients
     bool stores_outer_values() const { return !cfg_.matrix_free; }
+    // bytes of one inner vector element (double for an FP64 inner solve)
+    size_t inner_vector_bytes() const { return cfg_.inner_vtype == VT_F64 ? 8 : 4; }
+    // konv_rad of the coarsest-level CG: the inner type's epsilon
+    double coarse_konv() const {
+        return cfg_.inner_vtype == VT_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+    }
   private:
     BatchConfig cfg_;
     std::vector<BLevel> L_;          // indexed by level (0 unused)
@@ -367,17 +392,23 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
 // One block per listed slot: reduce its partials and update its scalars.
 hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
                     double tol, int32_t max_outer, hipStream_t s);
-hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
+hipError_t b_restrict(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
 // out = fine.x + P coarse.x (out: fine.x or fine.t)
-hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, float *out, hipStream_t s);
+hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, void *out,
+                          hipStream_t s);
 // Two dependent row passes of a T = 1 Q1 level in one launch (k_b_pair):
 // PAIR_PRE writes x = Jacobi0 to out0 and d = rhs - A x to out1; PAIR_POST
 // writes two Jacobi sweeps of xin to out0 (xin != out0).  Bit-identical to
 // the separate b_ell launches.
-// mf: the matrix weights from lv.kappa (bit-identical to the stored ones).
-hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
-                  const float *xin, hipStream_t s, bool mf = false);
-hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s);
+// wmode: the weights from the stored values (0), from lv.kappa (1), from the
+// distinct-weight arrays lv.st (2): the same bits in every form.
+hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void *out0, void *out1,
+                  const void *xin, hipStream_t s, int wmode = 0);
+// The distinct-weight arrays of `count` slots of a side x side level from
+// their cell coefficients (kappa + g kld, st + g 4 vld).
+hipError_t b_stencil_fill(ValueType vt, int64_t side, const double *kappa, int64_t kld, void *st, int64_t vld,
+                          int64_t count, hipStream_t s);
+hipError_t b_copy(ValueType vt, const BLevel &lv, const BWork &w, void *dst, const void *src, hipStream_t s);
 hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream_t s);
 // Slot ids of one state-setting launch (kernel argument).
 constexpr int kMaxSlotIds = 256;
@@ -396,12 +427,12 @@ struct FusedLevel {
     const void *Ax;
     int64_t stride, ax_ld, n, vld, side, cap, ncpr;
     int T;
-    float *dinv, *rhs, *x, *d, *t;
+    void *dinv, *rhs, *x, *d, *t;   // inner vector type
 };
 struct FusedCycle {
     int top, n_pre, n_post;
     FusedLevel lv[kMaxFusedLevels + 1];   // index = level (1..top)
-    float *g, *u, *en;                    // level-1 CG scratch
+    void *g, *u, *en;                     // level-1 CG scratch
     double konv;
     int max_iters;
 };

@@ -94,8 +94,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
         total += counts[c];
     }
     if (total > 65535) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "more than 65535 slots");
-    if (cfg->inner_vtype != HONEI_HIP_F32 && cfg->inner_vtype != HONEI_HIP_BF16)
-        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "inner type must be F32 or BF16");
+    if (cfg->inner_vtype != HONEI_HIP_F32 && cfg->inner_vtype != HONEI_HIP_BF16 && cfg->inner_vtype != HONEI_HIP_F64)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "inner type must be F32, BF16 or F64");
     if (cfg->n_pre < 2 || cfg->n_pre > 16 || cfg->n_post < 0 || cfg->n_post > 16)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sweeps: n_pre 2..16, n_post 0..16");
     if (cfg->iters_per_launch < 1 || cfg->iters_per_launch > 64 || cfg->max_outer < 1)
@@ -109,8 +109,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "use_graphs 0 or 1");
     if (cfg->pair_sweeps != 0 && cfg->pair_sweeps != 1)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "pair_sweeps 0 or 1");
-    if (cfg->matrix_free != 0 && cfg->matrix_free != 1)
-        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "matrix_free 0 or 1");
+    if (cfg->matrix_free < 0 || cfg->matrix_free > 2)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "matrix_free 0, 1 or 2");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");
@@ -256,7 +256,7 @@ int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, doub
     const bool tuned = hipGetDevice(&dev) == hipSuccess &&
                        batch_tuned_shape(dev, level, b->config().inner_vtype, &sh) && b->config().autotune &&
                        b->paired(L[(size_t)level]);
-    const int mf = b->config().matrix_free != 0 ? 1 : 0;
+    const int mf = b->config().matrix_free;
     if (pre_us) *pre_us = tuned ? sh.pre_us[mf] : 0.0;
     if (post_us) *post_us = tuned ? sh.post_us[mf] : 0.0;
     return HONEI_HIP_SUCCESS;

@@ -175,20 +175,23 @@ struct PairGeom {
     int32_t side, width, strips, segs;
 };
 
-template <typename VT> __device__ inline float load_value_cached(const VT *p) { return *p; }
+template <typename VT> __device__ inline typename VecOf<VT>::type load_value_cached(const VT *p) { return *p; }
 template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { return bf16_to_float(p->bits); }
 
-template <typename VT, int PAIR, bool MF>
-__global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWork w, PairGeom pg, float *out0,
-                                                                  float *out1, const float *xin, const float *rhs,
-                                                                  const float *dinv) {
+// V: the vector and arithmetic type (float for float / bf16 values, double
+// for double values)
+// WM: weights from the stored values (0), formed from the cell coefficients
+// (1), gathered from the distinct-weight arrays (2).
+template <typename VT, int PAIR, int WM, typename V = typename VecOf<VT>::type>
+__global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWork w, PairGeom pg, V *out0, V *out1,
+                                                                  const V *xin, const V *rhs, const V *dinv) {
     // first-pass results of the tile and its halo
-    __shared__ float tile[kPairTileRows][kWave];
+    __shared__ V tile[kPairTileRows][kWave];
     // the first pass's gathered operand (PAIR_PRE: dinv .* rhs; PAIR_POST:
     // xin) on grid rows r0 - 2 .. r0 + kPairTileRows - 1 and columns
     // c0 - 2 .. c0 + kWave - 1 (entry [a][b] = grid (r0 - 2 + a, c0 - 2 + b),
     // positions outside the grid clamped to it: loaded, never used)
-    __shared__ float src[kPairTileRows + 2][kWave + 2];
+    __shared__ V src[kPairTileRows + 2][kWave + 2];
     constexpr int kExtra = 2 * kWave + 2 * (kPairTileRows + 2);   // src entries beyond one per thread
     const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;   // tile row, lane
     const int64_t a0 = active_first(w), na = active_count(w);
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         const int64_t ls = w.list[a0 + e] - w.base;
         const int64_t vo = ls * m.vld;
         const VT *ax = (const VT *)m.Ax + ls * m.ax_ld;
-        const float *xs = xin + vo, *rs = rhs + vo, *ds = dinv + vo;
+        const V *xs = xin + vo, *rs = rhs + vo, *ds = dinv + vo;
         const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
         const int c0 = (int)strip * pg.width, c1 = min(S, c0 + pg.width);
         const int col = c0 - 1 + lane;
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);       // clamped row: computed, unused
         const int64_t node = (int64_t)rc * S + colc;
         const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
-        auto operand = [&](int gr, int gc) -> float {
+        auto operand = [&](int gr, int gc) -> V {
             gr = gr < 0 ? 0 : (gr > S - 1 ? S - 1 : gr);
             gc = gc < 0 ? 0 : (gc > S - 1 ? S - 1 : gc);
             const int64_t j = (int64_t)gr * S + gc;
@@ -244,49 +247,69 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             src[a][b] = operand(r0 - 2 + a, c0 - 2 + b);
         }
         // ---- first pass, grid row r ----
-        float av[9];
-        if constexpr (MF) {
-            q1_stored_weights<VT, float>(S, rc, colc, bnd, m.kappa + ls * m.kld, av);
+        V av[9];
+        if constexpr (WM == 1) {
+            q1_stored_weights<VT, V>(S, rc, colc, bnd, m.kappa + ls * m.kld, av);
+        } else if constexpr (WM == 2) {
+            // cell weights at node - S - 1, node - S, node - 1, node; row-pair
+            // weights at node - S, node; column-pair weights at node - 1,
+            // node; the diagonal at node (boundary rows: addresses of the
+            // node itself, values replaced by the identity row)
+            const VT *wc = (const VT *)m.st + ls * 4 * m.vld, *wh = wc + m.vld, *wv = wh + m.vld, *wd = wv + m.vld;
+            const int64_t dS = bnd ? 0 : S, d1 = bnd ? 0 : 1;
+            av[0] = load_value_cached(wc + (node - dS - d1));
+            av[1] = load_value_cached(wh + (node - dS));
+            av[2] = load_value_cached(wc + (node - dS));
+            av[3] = load_value_cached(wv + (node - d1));
+            av[4] = load_value_cached(wd + node);
+            av[5] = load_value_cached(wv + node);
+            av[6] = load_value_cached(wc + (node - d1));
+            av[7] = load_value_cached(wh + node);
+            av[8] = load_value_cached(wc + node);
+            if (bnd) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) av[k] = V(k == 0 ? 1 : 0);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
         }
-        const float rh = rs[node], di = ds[node];
+        const V rh = rs[node], di = ds[node];
         __syncthreads();
         {
-            float xv[9];
+            V xv[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 const int a = bnd ? i + 1 : i + 1 + (k / 3 - 1);
                 const int b = bnd ? lane + 1 : lane + 1 + (k % 3 - 1);
                 xv[k] = src[a][b];
             }
-            float s = 0.f;
+            V s = V(0);
 #pragma unroll
             for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
-            const float x0 = (PAIR == PAIR_PRE) ? di * rh : xs[node];
-            const float t = rh - s;
-            const float v1 = x0 + t * di;
+            const V x0 = (PAIR == PAIR_PRE) ? di * rh : xs[node];
+            const V t = rh - s;
+            const V v1 = x0 + t * di;
             tile[i][lane] = v1;
             if (PAIR == PAIR_PRE && in2 && i >= 1 && r < r1) out0[vo + node] = v1;
         }
         __syncthreads();
         // ---- second pass, grid row r for tile rows 1 .. kPairRows ----
         if (i >= 1 && i <= kPairRows && r < r1 && in2) {
-            float xv[9];
+            V xv[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 const int ik = bnd ? i : i + (k / 3 - 1);
                 const int lk = bnd ? lane : lane + (k % 3 - 1);
                 xv[k] = tile[ik][lk];
             }
-            float s = 0.f;
+            V s = V(0);
 #pragma unroll
             for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
             if constexpr (PAIR == PAIR_PRE) {
                 out1[vo + node] = rh - s;
             } else {
-                const float t = rh - s;
+                const V t = rh - s;
                 out0[vo + node] = tile[i][lane] + t * di;
             }
         }
@@ -415,7 +438,12 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 // the layout of k_b_outer_rows): logical block = R consecutive 256-row blocks
 // of one listed slot, thread t takes row t of each with every load of the R
 // rows issued before any sum (rows past the end load the last row and store
-// nothing).  Same sums as k_b_outer_rows<OP_SPMV_DOT>.
+// nothing).  The block's class, slot and first row are uniform (scalar
+// registers); a row's grid position comes from the block's first row by one
+// multiply-high with the class's reciprocal of the side.  Every row forms the
+// interior stencil sum (a boundary row at clamped, in-range addresses) and a
+// boundary row then takes 0 + x[row], its identity row's sum: the terms and
+// order of the stored operator's row sums, hence the same bits.
 template <int R, bool MF>
 __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w) {
     __shared__ int64_t pre[kMaxClasses + 1];
@@ -432,39 +460,51 @@ __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w
     for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
         int c = 0;
         while (lb >= pre[c + 1]) ++c;
+        c = __builtin_amdgcn_readfirstlane(c);
         const BClassEntry &e = tab.c[c];
         int64_t gi, hb;   // listed slot of the class, group of R row blocks
         div32(lb - pre[c], (e.bps + R - 1) / R, gi, hb);
+        gi = __builtin_amdgcn_readfirstlane((int32_t)gi);
+        hb = __builtin_amdgcn_readfirstlane((int32_t)hb);
         const int64_t ls = w.list[w.bounds[c] + gi] - e.first;
         const double *ax = e.Ax + ls * e.ax_ld;
         const double *kap = MF ? e.kappa + (ls + e.first) * e.kld : nullptr;
         const double *pv = e.p + ls * e.vld;
-        const int64_t S = e.side;
+        const uint32_t S = (uint32_t)e.side, n = (uint32_t)e.n;
+        const uint32_t base = (uint32_t)(hb * R * 256);
+        const uint32_t gr0 = base / S, gc0 = base - gr0 * S;
         double av[R][9], xv[R][9], pr[R];
         bool bnd[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int64_t row0 = (hb * R + r) * 256 + threadIdx.x;
-            const int64_t row = row0 < e.n ? row0 : e.n - 1;
-            const uint32_t gr = (uint32_t)row / (uint32_t)S, gc = (uint32_t)row - gr * (uint32_t)S;
+            const uint32_t row0 = base + (uint32_t)(r * 256) + threadIdx.x;
+            const uint32_t row = row0 < n ? row0 : n - 1;
+            const uint32_t v = gc0 + (row - base);
+            const uint32_t q = __umulhi(v, e.side_magic);
+            const uint32_t gr = gr0 + q, gc = v - q * S;
             bnd[r] = gr == 0 || gr == S - 1 || gc == 0 || gc == S - 1;
-            if constexpr (MF) q1_stored_weights<double, double>(S, gr, gc, bnd[r], kap, av[r]);
+            // interior position used for the addresses (a boundary row's are unused)
+            const uint32_t ir = gr < 1 ? 1 : (gr > S - 2 ? S - 2 : gr);
+            const uint32_t ic = gc < 1 ? 1 : (gc > S - 2 ? S - 2 : gc);
+            const uint32_t irow = ir * S + ic;
+            if constexpr (MF) q1_coeff_weights<double>(S, ir, ic, kap, av[r]);
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                const int64_t j = bnd[r] ? row : row + (int64_t)(k / 3 - 1) * S + (k % 3 - 1);
-                if constexpr (!MF) av[r][k] = load_value_nt(ax + row + (int64_t)k * e.stride);
+                const uint32_t j = irow + (uint32_t)((k / 3) * S + (k % 3)) - (S + 1);
+                if constexpr (!MF) av[r][k] = load_value_nt(ax + irow + (int64_t)k * e.stride);
                 xv[r][k] = pv[j];
             }
             pr[r] = pv[row];
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int64_t row = (hb * R + r) * 256 + threadIdx.x;
+            const uint32_t row = base + (uint32_t)(r * 256) + threadIdx.x;
             double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) t = (k < (bnd[r] ? 1 : 9)) ? t + av[r][k] * xv[r][k] : t;
+            for (int k = 0; k < 9; ++k) t = t + av[r][k] * xv[r][k];
+            if (bnd[r]) t = 0.0 + pr[r];
             double part = 0.0;
-            if (row < e.n) {
+            if (row < n) {
                 e.q[ls * e.vld + row] = t;
                 part = pr[r] * t;
             }
@@ -479,7 +519,8 @@ __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w
 // listed slot; thread t takes row t of each, with every load of the R rows
 // issued before any use (rows past the end load the last row and are
 // dropped), so one slot lookup serves R rows and their loads overlap.
-template <int OP, int R>
+// Z: the inner vectors' type (float, or double for an FP64 inner cycle).
+template <int OP, int R, typename Z>
 __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, const SlotScalars *sc) {
     static_assert(OP == OP_SCALE || OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_PUPDATE || OP == OP_UPDATE,
                   "element-wise op");
@@ -506,7 +547,7 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
         const SlotScalars &st = sc[g];
         const double *rs = e.r + ls * e.vld, *ps = e.p + ls * e.vld, *qs = e.q + ls * e.vld;
         const double *xs = e.x + ls * e.vld;
-        const float *zs = e.x_top + g * e.vld_top;
+        const Z *zs = (const Z *)e.x_top + g * e.vld_top;
         int64_t row[R];
         bool in[R];
         double rv[R], zv[R], pv[R], qv[R], xo[R], part[R];
@@ -526,7 +567,7 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
             part[j] = 0.0;
             const int64_t vi = ls * e.vld + row[j];
             if constexpr (OP == OP_SCALE) {
-                if (in[j]) e.rhs_top[g * e.vld_top + row[j]] = (float)(rv[j] * st.inv);
+                if (in[j]) ((Z *)e.rhs_top)[g * e.vld_top + row[j]] = (Z)(rv[j] * st.inv);
             } else if constexpr (OP == OP_ACC) {
                 const double base = st.iters == 1 ? 0.0 : xo[j];
                 if (in[j]) e.x[vi] = base + zv[j] * st.scale;
@@ -656,30 +697,172 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
         }                                                                         \
     } while (0)
 
-// rhs_c[g] = R d_f[g] for active slots present on the fine level
-__global__ __launch_bounds__(256) void k_b_restrict(BWork w, float *rhs_c, int64_t vld_c, const float *d_f, int64_t vld_f, int64_t m,
-                             Mask8 mk) {
-    const int64_t nc = (m + 1) * (m + 1);
-    FOR_SLOT_ENTRIES(w, nc, restrict_node<float>(rhs_c + g * vld_c, d_f + g * vld_f, m, mk, i));
+// Row-wise transfers of the batch (every border Dirichlet, as the batched
+// V-cycle uses them).  Logical unit = 64 consecutive nodes of one grid row of
+// one listed slot, one wavefront per unit and four per work-group: the row,
+// and with it the coarse (or fine) rows it reads, is uniform across the
+// wavefront, so a node's position needs no division and its loads are
+// consecutive across lanes.
+struct RowUnit {
+    int64_t g;    // slot
+    int32_t row;  // grid row
+    int32_t col;  // first column of the unit
+};
+// Unit u of the launch over `rows` rows of `cols` columns per slot; the
+// arithmetic is wavefront-uniform (scalar registers).
+__device__ inline RowUnit row_unit(const BWork &w, int64_t u, int32_t rows, int32_t cols) {
+    const int32_t segs = (cols + kWave - 1) / kWave;
+    const uint32_t per = (uint32_t)rows * (uint32_t)segs;
+    const uint32_t uu = (uint32_t)u;
+    const uint32_t e = uu / per, rem = uu - e * per;
+    const uint32_t row = rem / (uint32_t)segs, seg = rem - row * (uint32_t)segs;
+    RowUnit ru;
+    ru.g = w.list[active_first(w) + (int64_t)e];
+    ru.row = (int32_t)row;
+    ru.col = (int32_t)seg * kWave;
+    return ru;
+}
+// units of a launch over up to max_slots slots
+inline int64_t row_units(int64_t max_slots, int64_t rows, int64_t cols) {
+    return max_slots * rows * ((cols + kWave - 1) / kWave);
 }
 
-// x_f[g] += P x_c[g]
-// x_f + P x_c -> out_f (out_f may be x_f)
-__global__ __launch_bounds__(256) void k_b_prolong_axpy(BWork w, float *x_f, int64_t vld_f, const float *x_c, int64_t vld_c, int64_t m,
-                                 Mask8 mk, float *out_f) {
-    const int64_t nf = (2 * m + 1) * (2 * m + 1);
-    FOR_SLOT_ENTRIES(w, nf, (prolong_node<float, true>(nullptr, x_c + g * vld_c, m, mk, x_f + g * vld_f, 1.0f, i,
-                                                       out_f + g * vld_f)));
+// Each wavefront takes kRowUnitsPerWave consecutive units per round and
+// issues every load of them before the first store, so a round costs one
+// memory round trip for several units (lanes past a row's end, and units past
+// the launch's last, load from a valid node of the first slot and store
+// nothing).
+constexpr int kRowUnitsPerWave = 4;
+
+// rhs_c[g] = R d_f[g] for active slots present on the fine level: coarse node
+// (r, c) of an interior row/column gathers its nine fine values and adds the
+// eight weighted terms to the centre value in the order restrict_node's
+// sorting network puts them (the CPU sweep's order): the row terms A, G, H,
+// B, E, F in that order, the column terms D and C placed by c - r (tags
+// A 8r, G 8r+6, H 8r+7, B 8r+9, E 8r+12, F 8r+13, D 8c+3, C 8c+10); border
+// nodes are Dirichlet (0).  The same bits as restrict_node with every border
+// Dirichlet.
+template <typename V>
+__global__ __launch_bounds__(256) void k_b_restrict_rows(BWork w, V *rhs_c, int64_t vld_c, const V *d_f,
+                                                         int64_t vld_f, int32_t m) {
+    constexpr int U = kRowUnitsPerWave;
+    const int32_t nc = m + 1, nf = 2 * m + 1;
+    const int64_t units = active_count(w) * (int64_t)nc * ((nc + kWave - 1) / kWave);
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int64_t lb = first_block(); lb * 4 * U < units; lb += gridDim.x) {
+        const int64_t u0 = __builtin_amdgcn_readfirstlane((int32_t)((lb * 4 + wv) * U));
+        V f[U][9];
+        int64_t dst[U];
+        int32_t dd[U];
+        bool store[U], border[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const bool uok = u0 + k < units;
+            const RowUnit ru = row_unit(w, uok ? u0 + k : 0, nc, nc);
+            const int32_t c = ru.col + lane;
+            store[k] = uok && c < nc;
+            const int32_t r = ru.row;
+            border[k] = r == 0 || r == m || c == 0 || c >= m;
+            // interior position for the loads (a border node's are unused)
+            const int32_t ir = r < 1 ? 1 : (r > m - 1 ? m - 1 : r);
+            const int32_t ic = c < 1 ? 1 : (c > m - 1 ? m - 1 : c);
+            dd[k] = c - r;
+            dst[k] = ru.g * vld_c + (int64_t)r * nc + c;
+            const V *fm = d_f + ru.g * vld_f + (int64_t)(2 * ir - 1) * nf;   // fine rows 2r - 1, 2r, 2r + 1
+            const V *f0r = fm + nf, *fp = f0r + nf;
+            const int32_t cc = 2 * ic;
+            f[k][0] = fm[cc];       // A
+            f[k][1] = fp[cc];       // B
+            f[k][2] = f0r[cc + 1];  // C
+            f[k][3] = f0r[cc - 1];  // D
+            f[k][4] = fp[cc + 1];   // E
+            f[k][5] = fp[cc - 1];   // F
+            f[k][6] = fm[cc + 1];   // G
+            f[k][7] = fm[cc - 1];   // H
+            f[k][8] = f0r[cc];      // centre
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const V h = V(0.5), q = V(0.25);
+            const V A = h * f[k][0], B = h * f[k][1], C = h * f[k][2], D = h * f[k][3];
+            const V E = q * f[k][4], F = q * f[k][5], G = q * f[k][6], Hh = q * f[k][7];
+            const int32_t d = dd[k];
+            V s = f[k][8];
+            s = d <= -1 ? s + D : s;
+            s = d <= -2 ? s + C : s;
+            s = s + A;
+            s = d == -1 ? s + C : s;
+            s = d == 0 ? s + D : s;
+            s = s + G;
+            s = s + Hh;
+            s = s + B;
+            s = d == 0 ? s + C : s;
+            s = d == 1 ? s + D : s;
+            s = s + E;
+            s = s + F;
+            s = d >= 2 ? s + D : s;
+            s = d >= 1 ? s + C : s;
+            if (store[k]) rhs_c[dst[k]] = border[k] ? V(0) : s;
+        }
+    }
 }
 
-__global__ __launch_bounds__(256) void k_b_copy(BWork w, float *dst, const float *src, int64_t vld, int64_t n) {
+// out_f[g] = x_f[g] + P x_c[g] (out_f may be x_f): fine node (R, C) from
+// its up to four coarse neighbours with prolong_node's terms and order
+// (border nodes Dirichlet: 0 correction).  The same bits as prolong_node
+// with every border Dirichlet.
+template <typename V>
+__global__ __launch_bounds__(256) void k_b_prolong_rows(BWork w, V *x_f, int64_t vld_f, const V *x_c, int64_t vld_c,
+                                                        int32_t m, V *out_f) {
+    constexpr int U = kRowUnitsPerWave;
+    const int32_t nc = m + 1, nf = 2 * m + 1;
+    const int64_t units = active_count(w) * (int64_t)nf * ((nf + kWave - 1) / kWave);
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int64_t lb = first_block(); lb * 4 * U < units; lb += gridDim.x) {
+        const int64_t u0 = __builtin_amdgcn_readfirstlane((int32_t)((lb * 4 + wv) * U));
+        V a[U], b[U], c[U], d[U], xv[U];
+        int64_t id[U];
+        bool store[U], oddR[U], oddC[U], border[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const bool uok = u0 + k < units;
+            const RowUnit ru = row_unit(w, uok ? u0 + k : 0, nf, nf);
+            const int32_t R = ru.row, C0 = ru.col + lane;
+            store[k] = uok && C0 < nf;
+            const int32_t C = C0 < nf ? C0 : nf - 1;
+            oddR[k] = (R & 1) != 0;
+            oddC[k] = (C & 1) != 0;
+            border[k] = R == 0 || R == nf - 1 || C == 0 || C == nf - 1;
+            const int32_t rlo = R >> 1, rhi = (R + 1) >> 1, clo = C >> 1, chi = (C + 1) >> 1;
+            const V *xc = x_c + ru.g * vld_c;
+            a[k] = xc[(int64_t)rlo * nc + clo];
+            b[k] = xc[(int64_t)rlo * nc + chi];
+            c[k] = xc[(int64_t)rhi * nc + clo];
+            d[k] = xc[(int64_t)rhi * nc + chi];
+            id[k] = ru.g * vld_f + (int64_t)R * nf + C;
+            xv[k] = x_f[id[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const V h = V(0.5), q = V(0.25);
+            const V f_rc = ((V(0) + a[k] * q + b[k] * q) + c[k] * q) + d[k] * q;
+            const V f_c = (V(0) + a[k] * h) + b[k] * h;
+            const V f_r = (V(0) + a[k] * h) + c[k] * h;
+            V f = oddR[k] ? (oddC[k] ? f_rc : f_r) : (oddC[k] ? f_c : a[k]);
+            if (border[k]) f = V(0);
+            if (store[k]) out_f[id[k]] = xv[k] + f * V(1);
+        }
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_b_copy(BWork w, V *dst, const V *src, int64_t vld, int64_t n) {
     FOR_SLOT_ENTRIES(w, n, dst[g * vld + i] = src[g * vld + i]);
 }
 
-template <typename VT>
-__global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t ax_ld, int64_t vld,
-                                                     const float *rhs, float *x, float *g, float *u, float *en,
-                                                     double konv, int max_iters) {
+template <typename VT, typename V = typename VecOf<VT>::type>
+__global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t ax_ld, int64_t vld, const V *rhs,
+                                                     V *x, V *g, V *u, V *en, double konv, int max_iters) {
     const int64_t a0 = active_first(w), na = active_count(w);
     const void *Ax0 = A.Ax;
     for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
@@ -695,19 +878,18 @@ __global__ __launch_bounds__(256) void k_b_coarse_cg(EllView A, BWork w, int64_t
 // One sweep of level L of slot `ls` by the whole work-group (T lanes per row;
 // the loop bound is a multiple of blockDim so every lane of a row joins its
 // lane reduction).  Same row sum and epilogue as k_b_ell.
-template <typename VT, int T, int MODE>
-__device__ inline void fused_sweep(const FusedLevel &L, int64_t ls, float *out, const float *x, const float *rhs,
-                                   const float *dinv) {
+template <typename VT, int T, int MODE, typename V = typename VecOf<VT>::type>
+__device__ inline void fused_sweep(const FusedLevel &L, int64_t ls, V *out, const V *x, const V *rhs, const V *dinv) {
     const VT *Ax = (const VT *)L.Ax + ls * L.ax_ld;
     const int64_t vo = ls * L.vld;
-    const float *xs = x + vo, *rs = rhs + vo, *ds = dinv + vo;
-    float *os = out + vo;
+    const V *xs = x + vo, *rs = rhs + vo, *ds = dinv + vo;
+    V *os = out + vo;
     const int64_t total = ((L.n * T + blockDim.x - 1) / blockDim.x) * blockDim.x;
     for (int64_t gid = threadIdx.x; gid < total; gid += blockDim.x) {
         const int lane = (int)(gid & (T - 1));
         const int64_t row = gid / T;
         const bool active = row < L.n;
-        float s = 0.f;
+        V s = V(0);
         if (active) {
             if constexpr (MODE == ELL_JACOBI0)
                 s = ell_lane_sum<VT, T, true, true>(L.Aj, Ax, row, lane, L.Arl[row], L.stride, rs, ds);
@@ -719,17 +901,17 @@ __device__ inline void fused_sweep(const FusedLevel &L, int64_t ls, float *out, 
             if constexpr (MODE == ELL_DEFECT) {
                 os[row] = rs[row] - s;
             } else {
-                const float xin = (MODE == ELL_JACOBI0) ? ds[row] * rs[row] : xs[row];
-                const float t = rs[row] - s;
+                const V xin = (MODE == ELL_JACOBI0) ? ds[row] * rs[row] : xs[row];
+                const V t = rs[row] - s;
                 os[row] = xin + t * ds[row];
             }
         }
     }
 }
 
-template <typename VT, int MODE>
-__device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, float *out, const float *x, const float *rhs,
-                                     const float *dinv) {
+template <typename VT, int MODE, typename V = typename VecOf<VT>::type>
+__device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, V *out, const V *x, const V *rhs,
+                                     const V *dinv) {
     switch (L.T) {
         case 1: fused_sweep<VT, 1, MODE>(L, ls, out, x, rhs, dinv); break;
         case 2: fused_sweep<VT, 2, MODE>(L, ls, out, x, rhs, dinv); break;
@@ -741,6 +923,7 @@ __device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, float *out
 
 template <typename VT>
 __global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w) {
+    typedef typename VecOf<VT>::type V;
     const int64_t a0 = active_first(w), na = active_count(w);
     const Mask8 mk{0xffu};   // all Dirichlet, as the level-by-level transfers
     for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
@@ -750,19 +933,21 @@ __global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w)
         // down: n_pre sweeps from zero (first two fused), defect, restriction
         for (int l = top; l >= 2; --l) {
             const FusedLevel &L = fc.lv[l];
+            V *Lx = (V *)L.x, *Lt = (V *)L.t, *Ld = (V *)L.d;
+            const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
             const int rest = fc.n_pre - 2;
-            float *src = (rest % 2 == 0) ? L.x : L.t;
-            fused_sweep_t<VT, ELL_JACOBI0>(L, g, src, nullptr, L.rhs, L.dinv);
+            V *src = (rest % 2 == 0) ? Lx : Lt;
+            fused_sweep_t<VT, ELL_JACOBI0>(L, g, src, (const V *)nullptr, Lr, Ldi);
             for (int k = 0; k < rest; ++k) {
-                float *dst = (src == L.x) ? L.t : L.x;
-                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, L.rhs, L.dinv);
+                V *dst = (src == Lx) ? Lt : Lx;
+                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
                 src = dst;
             }
-            fused_sweep_t<VT, ELL_DEFECT>(L, g, L.d, L.x, L.rhs, nullptr);
+            fused_sweep_t<VT, ELL_DEFECT>(L, g, Ld, Lx, Lr, (const V *)nullptr);
             const FusedLevel &C = fc.lv[l - 1];
             const int64_t m = C.side - 1;
             for (int64_t i = threadIdx.x; i < C.n; i += blockDim.x)
-                restrict_node<float>(C.rhs + g * C.vld, L.d + g * L.vld, m, mk, i);
+                restrict_node<V>((V *)C.rhs + g * C.vld, Ld + g * L.vld, m, mk, i);
             __syncthreads();
         }
         {   // coarse CG on level 1
@@ -777,26 +962,29 @@ __global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w)
             A.Ax = (const VT *)L.Ax + g * L.ax_ld;
             A.Arl = L.Arl;
             const int64_t o = g * L.vld;
-            (void)coarse_cg_block<VT>(A, L.rhs + o, L.x + o, fc.g + o, fc.u + o, fc.en + o, fc.konv, fc.max_iters);
+            (void)coarse_cg_block<VT>(A, (const V *)L.rhs + o, (V *)L.x + o, (V *)fc.g + o, (V *)fc.u + o,
+                                      (V *)fc.en + o, fc.konv, fc.max_iters);
             __syncthreads();
         }
         // up: x += P x_coarse, n_post sweeps (result in x)
         for (int l = 2; l <= top; ++l) {
             const FusedLevel &L = fc.lv[l];
             const FusedLevel &C = fc.lv[l - 1];
+            V *Lx = (V *)L.x, *Lt = (V *)L.t;
+            const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
             const int64_t m = C.side - 1;
             for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x)
-                prolong_node<float, true>(nullptr, C.x + g * C.vld, m, mk, L.x + g * L.vld, 1.0f, i);
+                prolong_node<V, true>(nullptr, (const V *)C.x + g * C.vld, m, mk, Lx + g * L.vld, V(1), i);
             __syncthreads();
-            float *src = L.x, *dst = L.t;
+            V *src = Lx, *dst = Lt;
             for (int k = 0; k < fc.n_post; ++k) {
-                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, L.rhs, L.dinv);
-                float *tmp = src;
+                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
+                V *tmp = src;
                 src = dst;
                 dst = tmp;
             }
-            if (src != L.x) {
-                for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x) L.x[g * L.vld + i] = src[g * L.vld + i];
+            if (src != Lx) {
+                for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x) Lx[g * L.vld + i] = src[g * L.vld + i];
                 __syncthreads();
             }
         }
@@ -860,11 +1048,16 @@ __global__ void k_bb_level(const BuildEntry *entries, BuildParams P, int l) {
     if (e.level < l) return;
     const int T = P.T[l];
     const int64_t rows = P.stride[l] / T, n = P.side[l] * P.side[l];
+    typedef typename VecOf<VT>::type V;
+    V *dinv = (V *)e.dinv[l];
+    VT *st = (VT *)e.st[l];
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-        if (e.Ax[l])
-            q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], e.dinv[l], P.omega);
-        else if (r < n)
-            q1_dinv_row<VT>(P.side[l], r, e.kappa[l], e.dinv[l], P.omega);
+        if (e.Ax[l]) {
+            q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], dinv, P.omega);
+        } else if (r < n) {
+            q1_dinv_row<VT>(P.side[l], r, e.kappa[l], dinv, P.omega);
+            if (st) q1_stencil_node<VT>(P.side[l], r, e.kappa[l], st, P.vld[l]);
+        }
     }
 }
 
@@ -877,6 +1070,15 @@ __global__ void k_bb_outer(const BuildEntry *entries, BuildParams P) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < P.stride64[L];
          r += (int64_t)gridDim.x * blockDim.x)
         q1_row<double>(P.side[L], 1, r, nullptr, e.Ax64, nullptr, P.stride64[L], e.kappa[L], nullptr, 0.0);
+}
+
+// distinct-weight arrays of slot blockIdx.y from its cell coefficients
+template <typename VT>
+__global__ void k_stencil_fill(int64_t side, const double *kappa, int64_t kld, void *st, int64_t vld) {
+    const int64_t g = blockIdx.y, n = side * side;
+    VT *sg = (VT *)st + g * 4 * vld;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        q1_stencil_node<VT>(side, r, kappa + g * kld, sg, vld);
 }
 
 typedef void (*BEllFn)(BMat, BWork, void *, const void *, const void *, const void *);
@@ -937,6 +1139,7 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
     BEllFn fn = nullptr;
     if (vt == VT_F32) fn = pick_threads<float>(T, group, mode);
     else if (vt == VT_BF16) fn = pick_threads<bf16>(T, group, mode);
+    else if (vt == VT_F64) fn = pick_threads<double>(T, group, mode);
     if (!fn || block % kWave || block > 1024 || block < T) return hipErrorInvalidValue;
     if (m.bps != (m.n * T + block - 1) / block) return hipErrorInvalidValue;
     if (max_slots <= 0) return hipSuccess;
@@ -979,11 +1182,19 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
             else
                 hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks, false>), gs, block, 0, s, tab, w);
             break;
-        case OP_SCALE: hipLaunchKernelGGL((k_b_outer_elem<OP_SCALE, R>), ge, block, 0, s, tab, w, sc); break;
-        case OP_ACC: hipLaunchKernelGGL((k_b_outer_elem<OP_ACC, R>), ge, block, 0, s, tab, w, sc); break;
-        case OP_DOT_RZ: hipLaunchKernelGGL((k_b_outer_elem<OP_DOT_RZ, R>), ge, block, 0, s, tab, w, sc); break;
-        case OP_PUPDATE: hipLaunchKernelGGL((k_b_outer_elem<OP_PUPDATE, R>), ge, block, 0, s, tab, w, sc); break;
-        case OP_UPDATE: hipLaunchKernelGGL((k_b_outer_elem<OP_UPDATE, R>), ge, block, 0, s, tab, w, sc); break;
+#define ELEM_LAUNCH(OPV)                                                                                 \
+    do {                                                                                                 \
+        if (tab.c[0].top64)                                                                              \
+            hipLaunchKernelGGL((k_b_outer_elem<OPV, R, double>), ge, block, 0, s, tab, w, sc);          \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_b_outer_elem<OPV, R, float>), ge, block, 0, s, tab, w, sc);           \
+    } while (0)
+        case OP_SCALE: ELEM_LAUNCH(OP_SCALE); break;
+        case OP_ACC: ELEM_LAUNCH(OP_ACC); break;
+        case OP_DOT_RZ: ELEM_LAUNCH(OP_DOT_RZ); break;
+        case OP_PUPDATE: ELEM_LAUNCH(OP_PUPDATE); break;
+        case OP_UPDATE: ELEM_LAUNCH(OP_UPDATE); break;
+#undef ELEM_LAUNCH
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1009,30 +1220,49 @@ hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t m
     return hipGetLastError();
 }
 
-hipError_t b_restrict(const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
+hipError_t b_restrict(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s) {
     const int64_t m = coarse.side - 1;
-    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || !entry_fits(fine.cap, coarse.n)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_b_restrict, dim3(entry_grid(fine.cap, coarse.n)), dim3(256), 0, s, w, coarse.rhs,
-                       coarse.vld, (const float *)fine.d, fine.vld, m, pack_mask(nullptr));
+    if (m < 2 || fine.side != 2 * m + 1 || fine.cap > coarse.cap || fine.side > kMaxTransferSide)
+        return hipErrorInvalidValue;
+    const int64_t units = row_units(fine.cap, coarse.side, coarse.side);
+    if (units > 0x7fffffff - 4 * kRowUnitsPerWave) return hipErrorInvalidValue;
+    const dim3 grid(batch_grid((units + 4 * kRowUnitsPerWave - 1) / (4 * kRowUnitsPerWave), 256)), block(256);
+    if (vt == VT_F64)
+        hipLaunchKernelGGL(k_b_restrict_rows<double>, grid, block, 0, s, w, (double *)coarse.rhs, coarse.vld,
+                           (const double *)fine.d, fine.vld, (int32_t)m);
+    else
+        hipLaunchKernelGGL(k_b_restrict_rows<float>, grid, block, 0, s, w, (float *)coarse.rhs, coarse.vld,
+                           (const float *)fine.d, fine.vld, (int32_t)m);
     return hipGetLastError();
 }
 
-hipError_t b_prolong_axpy(const BLevel &fine, const BLevel &coarse, const BWork &w, float *out, hipStream_t s) {
+hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, void *out,
+                          hipStream_t s) {
     const int64_t m = coarse.side - 1;
-    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || !entry_fits(fine.cap, fine.n)) return hipErrorInvalidValue;
+    if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || fine.side > kMaxTransferSide) return hipErrorInvalidValue;
     if (out != fine.x && out != fine.t) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_b_prolong_axpy, dim3(entry_grid(fine.cap, fine.n)), dim3(256), 0, s, w, fine.x, fine.vld,
-                       (const float *)coarse.x, coarse.vld, m, pack_mask(nullptr), out);
+    const int64_t units = row_units(fine.cap, fine.side, fine.side);
+    if (units > 0x7fffffff - 4 * kRowUnitsPerWave) return hipErrorInvalidValue;
+    const dim3 grid(batch_grid((units + 4 * kRowUnitsPerWave - 1) / (4 * kRowUnitsPerWave), 256)), block(256);
+    if (vt == VT_F64)
+        hipLaunchKernelGGL(k_b_prolong_rows<double>, grid, block, 0, s, w, (double *)fine.x, fine.vld,
+                           (const double *)coarse.x, coarse.vld, (int32_t)m, (double *)out);
+    else
+        hipLaunchKernelGGL(k_b_prolong_rows<float>, grid, block, 0, s, w, (float *)fine.x, fine.vld,
+                           (const float *)coarse.x, coarse.vld, (int32_t)m, (float *)out);
     return hipGetLastError();
 }
 
-hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, float *out0, float *out1,
-                  const float *xin, hipStream_t s, bool mf) {
+hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void *out0, void *out1,
+                  const void *xin, hipStream_t s, int wmode) {
     if (lv.T != 1 || lv.side < 3 || lv.ncpr != 9 || lv.side > 65535) return hipErrorInvalidValue;
     if (pair != PAIR_PRE && pair != PAIR_POST) return hipErrorInvalidValue;
     if (!out0 || (pair == PAIR_PRE && !out1) || (pair == PAIR_POST && (!xin || xin == out0)))
         return hipErrorInvalidValue;
-    if (mf && (!lv.kappa || lv.kld < (lv.side - 1) * (lv.side - 1))) return hipErrorInvalidValue;
+    if (wmode < 0 || wmode > 2) return hipErrorInvalidValue;
+    if (wmode == 1 && (!lv.kappa || lv.kld < (lv.side - 1) * (lv.side - 1))) return hipErrorInvalidValue;
+    if (wmode == 2 && !lv.st) return hipErrorInvalidValue;
+    if (wmode == 0 && (!lv.Ax || !lv.Aj)) return hipErrorInvalidValue;
     PairGeom pg;
     pg.side = (int32_t)lv.side;
     pg.strips = (int32_t)((lv.side + kPairWidth - 1) / kPairWidth);
@@ -1041,34 +1271,53 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, floa
     if (lv.cap <= 0) return hipSuccess;
     const int64_t units = lv.cap * (int64_t)pg.strips * pg.segs;
     if (units > kMaxLogicalBlocks) return hipErrorInvalidValue;
-    const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0, lv.kappa, lv.kld};
+    const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0, lv.kappa, lv.kld, lv.st};
     const int threads = kPairTileRows * kWave;
     const dim3 grid(batch_grid(units, threads)), block(threads);
-    const float *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
-#define PAIR_LAUNCH(VT, P, MFV) \
-    hipLaunchKernelGGL((k_b_pair<VT, P, MFV>), grid, block, 0, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv)
+    const void *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
+#define PAIR_LAUNCH(VT, P, MFV)                                                                              \
+    do {                                                                                                     \
+        typedef typename VecOf<VT>::type V;                                                                  \
+        hipLaunchKernelGGL((k_b_pair<VT, P, MFV>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,        \
+                           (const V *)x, (const V *)lv.rhs, (const V *)lv.dinv);                             \
+    } while (0)
+#define PAIR_WM(VT, P)                                                        \
+    do {                                                                      \
+        if (wmode == 2)                                                       \
+            PAIR_LAUNCH(VT, P, 2);                                            \
+        else if (wmode == 1)                                                  \
+            PAIR_LAUNCH(VT, P, 1);                                            \
+        else                                                                  \
+            PAIR_LAUNCH(VT, P, 0);                                            \
+    } while (0)
 #define PAIR_VT(VT)                                                           \
     do {                                                                      \
-        if (pair == PAIR_PRE) {                                               \
-            if (mf) PAIR_LAUNCH(VT, PAIR_PRE, true); else PAIR_LAUNCH(VT, PAIR_PRE, false);    \
-        } else {                                                              \
-            if (mf) PAIR_LAUNCH(VT, PAIR_POST, true); else PAIR_LAUNCH(VT, PAIR_POST, false);  \
-        }                                                                     \
+        if (pair == PAIR_PRE)                                                 \
+            PAIR_WM(VT, PAIR_PRE);                                            \
+        else                                                                  \
+            PAIR_WM(VT, PAIR_POST);                                           \
     } while (0)
     if (vt == VT_F32)
         PAIR_VT(float);
     else if (vt == VT_BF16)
         PAIR_VT(bf16);
+    else if (vt == VT_F64)
+        PAIR_VT(double);
     else
         return hipErrorInvalidValue;
 #undef PAIR_VT
+#undef PAIR_WM
 #undef PAIR_LAUNCH
     return hipGetLastError();
 }
 
-hipError_t b_copy(const BLevel &lv, const BWork &w, float *dst, const float *src, hipStream_t s) {
+hipError_t b_copy(ValueType vt, const BLevel &lv, const BWork &w, void *dst, const void *src, hipStream_t s) {
     if (!entry_fits(lv.cap, lv.n)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_b_copy, dim3(entry_grid(lv.cap, lv.n)), dim3(256), 0, s, w, dst, src, lv.vld, lv.n);
+    const dim3 grid(entry_grid(lv.cap, lv.n)), block(256);
+    if (vt == VT_F64)
+        hipLaunchKernelGGL(k_b_copy<double>, grid, block, 0, s, w, (double *)dst, (const double *)src, lv.vld, lv.n);
+    else
+        hipLaunchKernelGGL(k_b_copy<float>, grid, block, 0, s, w, (float *)dst, (const float *)src, lv.vld, lv.n);
     return hipGetLastError();
 }
 
@@ -1083,15 +1332,22 @@ hipError_t b_coarse_cg(ValueType vt, const BLevel &lv, const BWork &w, hipStream
     A.Ax = lv.Ax;
     A.Arl = lv.Arl;
     A.vtype = vt;
-    const double konv = 1.1920928955078125e-07;   // FP32 epsilon, as the single-system coarse solve
+    // the inner type's epsilon, as the single-system coarse solve
+    const double konv = vt == VT_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
     const int max_iters = (int)(4 * lv.n + 64);
     const int grid = (int)(lv.cap < 2048 ? lv.cap : 2048);
+    float *fr = (float *)lv.rhs, *fx = (float *)lv.x, *fg = (float *)lv.g, *fu = (float *)lv.u, *fe = (float *)lv.en;
+    double *dr = (double *)lv.rhs, *dx = (double *)lv.x, *dg = (double *)lv.g, *du = (double *)lv.u,
+           *de = (double *)lv.en;
     if (vt == VT_F32)
         hipLaunchKernelGGL(k_b_coarse_cg<float>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
-                           (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
+                           (const float *)fr, fx, fg, fu, fe, konv, max_iters);
     else if (vt == VT_BF16)
         hipLaunchKernelGGL(k_b_coarse_cg<bf16>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
-                           (const float *)lv.rhs, lv.x, lv.g, lv.u, lv.en, konv, max_iters);
+                           (const float *)fr, fx, fg, fu, fe, konv, max_iters);
+    else if (vt == VT_F64)
+        hipLaunchKernelGGL(k_b_coarse_cg<double>, dim3(grid), dim3(256), 0, s, A, w, lv.ax_ld, lv.vld,
+                           (const double *)dr, dx, dg, du, de, konv, max_iters);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1111,6 +1367,8 @@ hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, i
         hipLaunchKernelGGL(k_b_coarse_vcycle<float>, dim3(grid), dim3(256), 0, s, fc, w);
     else if (vt == VT_BF16)
         hipLaunchKernelGGL(k_b_coarse_vcycle<bf16>, dim3(grid), dim3(256), 0, s, fc, w);
+    else if (vt == VT_F64)
+        hipLaunchKernelGGL(k_b_coarse_vcycle<double>, dim3(grid), dim3(256), 0, s, fc, w);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1135,10 +1393,27 @@ hipError_t b_build(const BuildEntry *entries, const BuildParams &P, int max_leve
             hipLaunchKernelGGL(k_bb_level<float>, grid(rows), dim3(256), 0, s, entries, P, l);
         else if (P.vt == VT_BF16)
             hipLaunchKernelGGL(k_bb_level<bf16>, grid(rows), dim3(256), 0, s, entries, P, l);
+        else if (P.vt == VT_F64)
+            hipLaunchKernelGGL(k_bb_level<double>, grid(rows), dim3(256), 0, s, entries, P, l);
         else
             return hipErrorInvalidValue;
     }
     if (P.outer_values) hipLaunchKernelGGL(k_bb_outer, grid(P.stride64[max_level]), dim3(256), 0, s, entries, P);
+    return hipGetLastError();
+}
+
+hipError_t b_stencil_fill(ValueType vt, int64_t side, const double *kappa, int64_t kld, void *st, int64_t vld,
+                          int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (side < 3 || count > 65535 || vld < side * side) return hipErrorInvalidValue;
+    const int64_t n = side * side;
+    const dim3 grid((unsigned)std::min<int64_t>(2048, (n + 255) / 256), (unsigned)count), block(256);
+    switch (vt) {
+        case VT_F32: hipLaunchKernelGGL(k_stencil_fill<float>, grid, block, 0, s, side, kappa, kld, st, vld); break;
+        case VT_BF16: hipLaunchKernelGGL(k_stencil_fill<bf16>, grid, block, 0, s, side, kappa, kld, st, vld); break;
+        case VT_F64: hipLaunchKernelGGL(k_stencil_fill<double>, grid, block, 0, s, side, kappa, kld, st, vld); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -136,6 +136,37 @@ __device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT 
     }
 }
 
+// The distinct weights of the variable-coefficient operator, one per cell,
+// edge and node, in four arrays of side^2 entries (node indexing, unused
+// entries untouched) at st + k * vld:
+//   k = 0: -c / 3 of cell (i, j) at i * side + j (i, j < side - 1), the
+//          corner weights w0, w2, w6, w8 of the cell's four nodes;
+//   k = 1: -(c(i, j-1) + c(i, j)) / 6 of cell row i at i * side + j
+//          (1 <= j <= side - 2): w1 of node (i + 1, j), w7 of node (i, j);
+//   k = 2: -(c(i-1, j) + c(i, j)) / 6 of cell column j at i * side + j
+//          (1 <= i <= side - 2): w3 of node (i, j + 1), w5 of node (i, j);
+//   k = 3: the diagonal 2 (sw + se + nw + ne) / 3 of interior node (i, j).
+// Each is the value q1_coeff_weights forms for every node that uses it (same
+// operands, same order), rounded to the stored type as q1_row stores it, so
+// the nine weights gathered from these arrays are the stored row's bits.
+// Written for node index `node` < side^2.
+template <typename VT>
+__device__ inline void q1_stencil_node(int64_t side, int64_t node, const double *coeff, VT *st, int64_t vld) {
+    typedef typename VecOf<VT>::type V;
+    const int64_t nc = side - 1, i = node / side, j = node % side;
+    auto put = [&](int k, double v) {
+        q1_put<VT, V>(st + k * vld, node, (V)v);
+    };
+    if (i < nc && j < nc) put(0, -div3_rn(coeff[i * nc + j]));
+    if (i < nc && j >= 1 && j <= side - 2) put(1, -0.5 * div3_rn(coeff[i * nc + (j - 1)] + coeff[i * nc + j]));
+    if (j < nc && i >= 1 && i <= side - 2) put(2, -0.5 * div3_rn(coeff[(i - 1) * nc + j] + coeff[i * nc + j]));
+    if (i >= 1 && i <= side - 2 && j >= 1 && j <= side - 2) {
+        const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
+        const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+        put(3, div3_rn(2.0 * (sw + se + nw + ne)));
+    }
+}
+
 // dinv = omega / diag of row `row` exactly as q1_row stores it, for a level
 // whose values are not stored (interior rows: the coefficient operator).
 template <typename VT>

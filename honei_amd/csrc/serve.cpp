@@ -12,13 +12,13 @@ namespace honei {
 namespace hipx {
 
 uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method, bool payload) {
-    const uint64_t vs = vt_size(inner);
+    const uint64_t vs = vt_size(inner), es = inner == VT_F64 ? 8 : 4;   // value, inner vector element
     uint64_t b = 0;
     for (int64_t l = 1; l <= level; ++l) {
         const uint64_t side = (1ull << l) + 1, n = side * side;
         const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;
         const uint64_t kld = ((side - 1) * (side - 1) + 31) / 32 * 32;
-        b += 9 * stride * vs + vld * 4 * (l == 1 ? 8 : 5) + kld * 8;   // values, vectors, cell coefficients
+        b += 9 * stride * vs + vld * es * (l == 1 ? 8 : 5) + kld * 8;   // values, vectors, cell coefficients
     }
     const uint64_t side = (1ull << level) + 1, n = side * side;
     const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;

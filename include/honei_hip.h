@@ -438,7 +438,9 @@ int honei_hip_fill_lognormal(double *dst_dev, size_t n, double sigma, uint64_t s
  * sweeps.  The batch
  * owns its streams; calls on one batch must not overlap.                    */
 typedef struct {
-    int32_t inner_vtype;       /* HONEI_HIP_F32 or HONEI_HIP_BF16 (matrix values) */
+    int32_t inner_vtype;       /* HONEI_HIP_F32 or HONEI_HIP_BF16 matrix values with FP32
+                                  vectors (mixed precision), or HONEI_HIP_F64 (values and
+                                  vectors in FP64: the same solver without the FP32 inner) */
     int32_t n_pre, n_post;     /* Jacobi sweeps, n_pre >= 2 */
     double omega;              /* Jacobi damping */
     double tolerance;          /* relative FP64 defect */
@@ -457,10 +459,13 @@ typedef struct {
     int32_t pair_sweeps;       /* 1 (default): on levels of 33^2 nodes and up, Jacobi0 + defect and
                                   the two post-smoothing sweeps each run as one launch that reads
                                   the matrix once (n_pre == 2, n_post == 2; same results) */
-    int32_t matrix_free;       /* 1 (default): those paired launches and the FP64 outer row sums
-                                  form each row's weights from the slot's cell coefficients
-                                  (8 bytes per node) with the assembly's expressions instead of
-                                  reading the nine stored values (same results); 0: read them */
+    int32_t matrix_free;       /* where those paired launches and the FP64 outer row sums take
+                                  each row's weights (same results in every form): 0 the nine
+                                  stored values; 1 (default) formed from the slot's cell
+                                  coefficients (8 bytes per node) with the assembly's
+                                  expressions; 2 the paired launches gather them from four per-node arrays of
+                                  the distinct weights (one per cell, edge and node) built with
+                                  the hierarchy, the FP64 outer forms them as 1 */
 } honei_hip_batch_config;
 
 typedef struct {

@@ -165,7 +165,7 @@ def test_direct_enqueue_equals_graph_launch(outer):
         assert np.array_equal(g[key][1], d[key][1]), key
 
 
-@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+@pytest.mark.parametrize("inner", [H.F32, H.BF16, H.F64])
 @pytest.mark.parametrize("outer", [0, 1])
 def test_paired_passes_equal_separate_launches(outer, inner):
     """Levels of 33^2 nodes and up run Jacobi0 + defect and the two
@@ -183,7 +183,7 @@ def test_paired_passes_equal_separate_launches(outer, inner):
             assert sep[key][0][2] == 0, key   # converged
 
 
-@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+@pytest.mark.parametrize("inner", [H.F32, H.BF16, H.F64])
 def test_paired_passes_equal_separate_launches_513_and_1025(inner):
     """The tile shapes of the headline batch's finest level (513^2: 9 column
     strips of 57, 37 row bands) and of the serving stream's largest size
@@ -199,24 +199,26 @@ def test_paired_passes_equal_separate_launches_513_and_1025(inner):
         assert np.array_equal(sep[key][1], par[key][1]), key
 
 
-@pytest.mark.parametrize("inner", [H.F32, H.BF16])
+@pytest.mark.parametrize("inner", [H.F32, H.BF16, H.F64])
 @pytest.mark.parametrize("outer", [0, 1])
 def test_matrix_free_equals_stored_values(outer, inner):
     """matrix_free = 1 (default): the paired launches and the FP64 outer row
     sums form each row's weights from the slots' cell coefficients with the
-    assembly's expressions; every slot's iterations, defect and x equal those
-    of matrix_free = 0 (values read from the stored matrices), bit for bit,
-    at sides 9..513."""
+    assembly's expressions; matrix_free = 2: the paired launches gather them
+    from the per-node arrays of distinct weights.  Every slot's iterations,
+    defect and x equal those of matrix_free = 0 (values read from the stored
+    matrices), bit for bit, at sides 9..513."""
     levels = [9, 7, 5, 3]
     systems = {(c, j): _system(lv, 800 + 10 * c + j) for c, lv in enumerate(levels) for j in range(2)}
     kw = dict(outer_method=outer, inner_vtype=inner, iters_per_launch=3)
     st, _, _ = _run_batch(levels, [2] * 4, systems, matrix_free=0, **kw)
-    mf, _, _ = _run_batch(levels, [2] * 4, systems, matrix_free=1, **kw)
-    for key in systems:
-        assert st[key][0] == mf[key][0], key
-        assert np.array_equal(st[key][1], mf[key][1]), key
-        if outer == 1:
-            assert mf[key][0][2] == 0, key
+    for form in (1, 2):
+        mf, _, _ = _run_batch(levels, [2] * 4, systems, matrix_free=form, **kw)
+        for key in systems:
+            assert st[key][0] == mf[key][0], (form, key)
+            assert np.array_equal(st[key][1], mf[key][1]), (form, key)
+            if outer == 1:
+                assert mf[key][0][2] == 0, (form, key)
 
 
 def test_batch_slot_reuse_and_late_admission():
@@ -261,7 +263,7 @@ def test_batch_not_converged_and_budget():
     assert e.value.code == 1
     for bad in (dict(fused_levels=7), dict(fused_levels=-1), dict(outer_method=2), dict(n_pre=1),
                 dict(iters_per_launch=0), dict(averaging=3), dict(use_graphs=2),
-                dict(pair_sweeps=2), dict(matrix_free=2)):
+                dict(pair_sweeps=2), dict(matrix_free=3), dict(matrix_free=-1), dict(inner_vtype=9)):
         with pytest.raises(H.HipError) as e:
             H.Batch([5], [1], H.batch_config(**bad))
         assert e.value.code == 1, bad
@@ -318,19 +320,21 @@ def test_serve_budget_and_unknown_size():
     assert res[2].admit_s >= res[0].done_s - 1e-9   # one slot: the second level-4 request waits for it
 
 
-def test_batch_cg_matches_single_system_cg():
+@pytest.mark.parametrize("inner", [H.F32, H.F64])
+def test_batch_cg_matches_single_system_cg(inner):
     """Batched CG outer loop (default) against the single-system CG driver on
-    the same FP32 hierarchy: same iteration count within one, solutions
-    agree to the accuracy the tolerance implies."""
+    the same hierarchy (FP32 inner, or everything in FP64): same iteration
+    count within one, solutions agree to the accuracy the tolerance implies
+    (1e-5 relative with the FP32 inner cycle, 1e-7 in FP64)."""
     levels, counts = [6, 5], [1, 2]
     systems = {(c, j): _system(levels[c], 400 + 10 * c + j) for c in range(2) for j in range(counts[c])}
-    res, _, _ = _run_batch(levels, counts, systems, autotune=0)
+    res, _, _ = _run_batch(levels, counts, systems, autotune=0, inner_vtype=inner)
     for (c, j), ((state, iters, status, rel, level), x) in res.items():
         assert status == 0 and rel < TOL
         L = levels[c]
         n = (2 ** L + 1) ** 2
         kappa, b = systems[(c, j)]
-        mg = H.Multigrid(H.F32, 1, L)
+        mg = H.Multigrid(inner, 1, L)
         mg.configure(H.mg_config(tolerance=TOL, max_outer=200))
         mg.build_q1_coeff(H.DeviceArray.from_host(kappa, np.float64), OMEGA, AVG)
         db, dx = H.DeviceArray.from_host(b), H.DeviceArray(n, np.float64)
@@ -338,7 +342,8 @@ def test_batch_cg_matches_single_system_cg():
         it, rd = mg.solve_pcg(db, dx)
         assert abs(it - iters) <= 1, (c, j, it, iters)
         xs = dx.get()
-        assert np.abs(x - xs).max() <= 1e-5 * max(1.0, np.abs(xs).max())
+        tol = 1e-5 if inner == H.F32 else 1e-7
+        assert np.abs(x - xs).max() <= tol * max(1.0, np.abs(xs).max())
 
 
 def test_batch_cg_converges_where_defect_correction_diverges():
