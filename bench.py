@@ -132,12 +132,17 @@ def time_spmv(sets, reps, label, idx64=False):
             "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": len(sets)}
 
 
-def time_sell(sets, reps, label):
+def time_sell(sets, reps, label, stage_x=True):
     """time_spmv for the sliced copy of each operand matrix
     (honei_hip_product_sell; the copies are built outside the timed
     launches); the same algorithmic bytes as the ELL product, and the slots
-    actually stored beside them."""
+    actually stored beside them.  stage_x: each 2048-row window's column
+    range of x copied into LDS (the default) or x gathered from global
+    memory."""
     sells = [(H.Sell(D), x, y) for D, x, y in sets]
+    for S, _, _ in sells:
+        S.stage_x(stage_x)
+    label = label + ("" if stage_x else ", x from global memory")
     for k in range(2 * len(sells)):
         S, x, y = sells[k % len(sells)]
         S.product(y, x)
@@ -161,7 +166,9 @@ def time_sell(sets, reps, label):
            "pct_hbm_peak": round(100 * gbs / PEAK_GBS, 2), "operand_copies": len(sets),
            "stored_slots_per_nonzero": round(pad / nnz, 4),
            "what": ("honei_hip_product_sell: rows sorted by length within 2048-row windows, 64-row slices "
-                    "padded to their longest row, terms in stored order (bit-identical to the ELL product)")}
+                    "padded to their longest row, terms in stored order (bit-identical to the ELL product); "
+                    + ("x gathered from an LDS copy of each window's column range" if stage_x
+                       else "x gathered from global memory"))}
     del sells
     return out
 
@@ -415,7 +422,15 @@ def batched_workload(nsys=NSYS, seed=BATCH_SEED):
     return levels, kap, rhs
 
 
+# Outer iterations per graph launch of the measured step: enough for the whole
+# solve of the workload (median 12, max 17), so a step is one persistent graph
+# launch plus one poll (the library default, 4, suits serving, where new
+# requests join at launch boundaries).
+STEP_ITERS_PER_LAUNCH = 24
+
+
 def batched_pool(levels, **cfg):
+    cfg.setdefault("iters_per_launch", STEP_ITERS_PER_LAUNCH)
     lv = sorted(set(levels), reverse=True)
     counts = [levels.count(v) for v in lv]
     B = H.Batch(lv, counts, H.batch_config(**cfg))
@@ -956,6 +971,9 @@ def main():
     ap.add_argument("--pcg-only", action="store_true",
                     help="only the PCG-Jacobi iteration rate on Q1 2049^2 FP64 (for profiling)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream (for profiling)")
+    ap.add_argument("--pmc-spmv", action="store_true",
+                    help="5 launches each of the Q1 2049^2 FP64 ELL SpMV, the random 4M ELL SpMV (FP32, FP64) "
+                         "and its sliced copy, then 5 of k_scale<float> on 2^26 floats (for rocprofv3 --pmc)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 5 launches of the roofline SpMV and 5 of a 2^26-float scale kernel "
                          "(reads and writes 256 MiB each) that calibrates FETCH_SIZE/WRITE_SIZE")
@@ -967,7 +985,8 @@ def main():
 
     # the CPU legs fork host processes: start them before the GPU is touched
     cpu_all, ref_proc = None, None
-    full = not (args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe or args.pcg_only)
+    full = not (args.batch_only or args.serve_only or args.spmv_only or args.pmc_probe or args.pcg_only
+                or args.pmc_spmv)
     if full and not args.no_cpu:
         os.environ["OMP_NUM_THREADS"] = "1"
         cpu_all = cpu_baseline_all_cores([int(v) for v in batch_levels(NSYS, BATCH_SEED)], cpu_workers())
@@ -988,6 +1007,35 @@ def main():
             H.call("honei_hip_scale_one", H.F32, big.ptr, 1.0, 1 << 26, None)
         H.synchronize()
         print(json.dumps({"pmc_probe": "done"}))
+        return
+
+    if args.pmc_spmv:
+        D64 = device_q1(SIDE, H.F64)
+        x64 = H.DeviceArray.from_host(uniform_pm1(2, n))
+        y64 = H.DeviceArray(n, np.float64)
+        for _ in range(5):
+            H.call("honei_hip_product_smell_dv_double", x64.ptr, y64.ptr, D64.Aj.ptr, D64.Ax.ptr, D64.Arl.ptr, 0, n,
+                   9, D64.stride, 0, 1, None)
+        H.synchronize()
+        del D64, x64, y64
+        for vt, dt in ((H.F32, np.float32), (H.F64, np.float64)):
+            R = device_random_ell(1 << 22, vt, seed=2)
+            xr = H.DeviceArray.from_host(uniform_pm1(2, R.rows).astype(dt))
+            yr = H.DeviceArray(R.rows, dt)
+            for _ in range(5):
+                H.call("honei_hip_product_smell_dv_float" if vt == H.F32 else "honei_hip_product_smell_dv_double",
+                       xr.ptr, yr.ptr, R.Aj.ptr, R.Ax.ptr, R.Arl.ptr, 0, R.rows, R.ncpr, R.stride, 0, 1, None)
+            S = H.Sell(R)
+            for _ in range(5):
+                S.product(yr, xr)
+            H.synchronize()
+            del S, R, xr, yr
+        big = H.DeviceArray(1 << 26, np.float32)
+        big.zero()
+        for _ in range(5):
+            H.call("honei_hip_scale_one", H.F32, big.ptr, 1.0, 1 << 26, None)
+        H.synchronize()
+        print(json.dumps({"pmc_spmv": "done"}))
         return
 
     if args.pcg_only:
@@ -1045,6 +1093,7 @@ def main():
         spmv[key] = time_spmv(sets, 30, key)
         spmv[key]["nnz"] = sets[0][0].nnz
         spmv[key + "_sliced"] = time_sell(sets, 30, key)
+        spmv[key + "_sliced_global_x"] = time_sell(sets, 30, key, stage_x=False)
         R, xr, yr = sets[0]
         spmv[key + "_staged"] = time_spmv_staged(R, 30, xr, yr, key)
         del sets, R, xr, yr

@@ -66,6 +66,7 @@ _SIGS = {
     "honei_hip_sell_info": [_p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)],
     "honei_hip_product_sell": [_p, _p, _p, _p],
     "honei_hip_sell_destroy": [_p],
+    "honei_hip_sell_stage_x": [_p, _i],
     "honei_hip_defect_norm_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_jacobi_smell_dv": [_i, _p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p],
     "honei_hip_dot_product_two_float": [C.POINTER(C.c_float), _p, _p, _sz, _p],
@@ -349,6 +350,11 @@ class Sell:
 
     def product(self, y: "DeviceArray", x: "DeviceArray", stream=None):
         call("honei_hip_product_sell", self.h, x.ptr, y.ptr, stream)
+
+    def stage_x(self, on: bool):
+        """Gather x from an LDS copy of each window's column range (default)
+        or from global memory; the same bits either way."""
+        call("honei_hip_sell_stage_x", self.h, 1 if on else 0)
 
     def close(self):
         if getattr(self, "h", None):

@@ -118,6 +118,14 @@ hipError_t sell_fill(ValueType vt, const int32_t *Aj, const void *Ax, int64_t st
                      int32_t *sAj, void *sAx, hipStream_t s);
 hipError_t sell_spmv(ValueType vt, const int32_t *sAj, const void *sAx, const int32_t *perm, const uint32_t *len,
                      const int32_t *width, const int64_t *base, int64_t rows, const void *x, void *y, hipStream_t s);
+// Column range of each sort window's stored entries (ranges[2 w], [2 w + 1]).
+hipError_t sell_ranges(const int32_t *sAj, const uint32_t *len, const int32_t *width, const int64_t *base,
+                       int64_t rows, int32_t *ranges, hipStream_t s);
+// sell_spmv with each window's x range staged in LDS (`window` entries of
+// LDS per work-group, at most kMaxStageBytes); the same bits.
+hipError_t sell_spmv_staged(ValueType vt, const int32_t *sAj, const void *sAx, const int32_t *perm,
+                            const uint32_t *len, const int32_t *width, const int64_t *base, int64_t rows,
+                            const void *x, void *y, const int32_t *ranges, int64_t window, hipStream_t s);
 
 // Banded Q1 product y = A x (rhs null) or defect y = rhs - A x; bands[9] =
 // LL, LD, LU, DL, DD, DU, UL, UD, UU device arrays of n = m*m values.

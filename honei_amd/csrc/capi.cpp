@@ -344,8 +344,11 @@ struct SellHandle {
     int64_t *base = nullptr;
     void *Ax = nullptr;
     uint64_t bytes = 0;
+    int32_t *ranges = nullptr;   // column range of each sort window
+    int64_t window = 0;          // widest window (entries of x), 0 if none fits LDS
+    int stage = 1;               // gather x from LDS where the window fits
     ~SellHandle() {
-        for (void *p : {(void *)perm, (void *)width, (void *)Aj, (void *)len, (void *)base, Ax})
+        for (void *p : {(void *)perm, (void *)width, (void *)Aj, (void *)len, (void *)base, Ax, (void *)ranges})
             if (p) (void)hipFree(p);
     }
 };
@@ -398,7 +401,23 @@ int honei_hip_sell_create(void **sell, int vtype, const void *Aj, const void *Ax
         st = check(sell_fill(h->vt, (const int32_t *)Aj, Ax, (int64_t)stride, h->rows, h->perm, h->len, h->width,
                              h->base, h->Aj, h->Ax, s),
                    "sell fill");
+    // each sort window's column range, for staging x in LDS
+    const int64_t windows = (h->rows + sell_window() - 1) / sell_window();
+    std::vector<int32_t> rg((size_t)(2 * windows));
+    if (!st && windows) st = dalloc((void **)&h->ranges, (size_t)windows * 8);
+    if (!st && windows)
+        st = check(sell_ranges(h->Aj, h->len, h->width, h->base, h->rows, h->ranges, s), "sell ranges");
+    if (!st && windows)
+        st = check(hipMemcpyAsync(rg.data(), h->ranges, rg.size() * 4, hipMemcpyDeviceToHost, s), "sell ranges readback");
     if (!st) st = check(hipStreamSynchronize(s), "sell sync");
+    if (!st) {
+        const int64_t cap = (int64_t)(kMaxStageBytes / vs);
+        for (int64_t k = 0; k < windows; ++k) {
+            const int64_t wn = (int64_t)rg[(size_t)(2 * k + 1)] - rg[(size_t)(2 * k)] + 1;
+            if (wn > h->window) h->window = wn;
+        }
+        if (h->window > cap) h->window = cap;   // wider windows gather from global x
+    }
     if (st) {
         delete h;
         return st;
@@ -420,8 +439,19 @@ int honei_hip_product_sell(void *sell, void *x, void *y, void *stream) {
     if (!sell) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sell is null");
     const SellHandle *h = (const SellHandle *)sell;
     if (h->rows > 0 && (!x || !y)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null vector");
+    if (h->stage && h->window > 0)
+        return check(sell_spmv_staged(h->vt, h->Aj, h->Ax, h->perm, h->len, h->width, h->base, h->rows, x, y,
+                                      h->ranges, h->window, S(stream)),
+                     "product_sell (staged)");
     return check(sell_spmv(h->vt, h->Aj, h->Ax, h->perm, h->len, h->width, h->base, h->rows, x, y, S(stream)),
                  "product_sell");
+}
+
+int honei_hip_sell_stage_x(void *sell, int on) {
+    if (!sell) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "sell is null");
+    if (on != 0 && on != 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "on must be 0 or 1");
+    ((SellHandle *)sell)->stage = on;
+    return HONEI_HIP_SUCCESS;
 }
 
 int honei_hip_sell_destroy(void *sell) {

@@ -165,6 +165,117 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int32_t *__restrict__ s
     if (valid) y[perm[i]] = acc;
 }
 
+// Column range [lo, hi] of the stored entries (k < len) of sort window
+// blockIdx.x (its 32 slices); a window without entries gets lo > hi.
+__global__ __launch_bounds__(1024) void k_sell_ranges(const int32_t *__restrict__ sAj, const uint32_t *__restrict__ len,
+                                                      const int32_t *__restrict__ width,
+                                                      const int64_t *__restrict__ base, int64_t rows, int64_t slices,
+                                                      int32_t *ranges) {
+    __shared__ int32_t rl[1024 / kWave], rh[1024 / kWave];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    constexpr int kSlicesPerWave = kSellChunks / (1024 / kWave);
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int h = 0; h < kSlicesPerWave; ++h) {
+        const int64_t sl = (int64_t)blockIdx.x * kSellChunks + wave * kSlicesPerWave + h;
+        if (sl >= slices) break;
+        const int64_t i = sl * kSellC + lane;
+        const uint32_t l = i < rows ? len[i] : 0u;
+        const int64_t b = base[sl] + lane;
+        for (uint32_t k = 0; k < l; ++k) {
+            const int32_t j = sAj[b + (int64_t)k * kSellC];
+            lo = j < lo ? j : lo;
+            hi = j > hi ? j : hi;
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int32_t ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+    }
+    if (lane == 0) {
+        rl[wave] = lo;
+        rh[wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / kWave; ++w) {
+            lo = rl[w] < lo ? rl[w] : lo;
+            hi = rh[w] > hi ? rh[w] : hi;
+        }
+        ranges[2 * blockIdx.x] = lo;
+        ranges[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// y = A x with x staged in LDS: one 1024-thread work-group per sort window
+// (32 slices, two per wavefront).  The window's column range of x is copied
+// into LDS once (coalesced), then each slice streams its index and value
+// arrays and gathers x from LDS.  The terms and their order are k_sell_spmv's,
+// so y has the same bits; a window wider than `cap` entries, or a row with an
+// entry outside the recorded range, sums that row from global x instead.
+template <typename VT>
+__global__ __launch_bounds__(1024) void k_sell_spmv_staged(
+    const int32_t *__restrict__ sAj, const VT *__restrict__ sAx, const int32_t *__restrict__ perm,
+    const uint32_t *__restrict__ len, const int32_t *__restrict__ width, const int64_t *__restrict__ base,
+    int64_t rows, int64_t slices, const VT *__restrict__ x, VT *__restrict__ y, const int32_t *__restrict__ ranges,
+    int64_t cap) {
+    extern __shared__ unsigned char sell_stage_raw[];
+    VT *win = reinterpret_cast<VT *>(sell_stage_raw);
+    constexpr int U = sizeof(VT) == 8 ? 4 : 8;
+    constexpr int kSlicesPerWave = kSellChunks / (1024 / kWave);
+    const int64_t wdw = xcd_block(blockIdx.x, gridDim.x);   // neighbouring windows on one XCD
+    const int32_t lo = ranges[2 * wdw], hi = ranges[2 * wdw + 1];
+    const int64_t wn = (int64_t)hi - (int64_t)lo + 1;
+    const bool staged = hi >= lo && wn <= cap;
+    if (staged)
+        for (int64_t i = threadIdx.x; i < wn; i += blockDim.x) win[i] = x[lo + i];
+    __syncthreads();
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    for (int h = 0; h < kSlicesPerWave; ++h) {
+        const int64_t sl = wdw * kSellChunks + wave * kSlicesPerWave + h;
+        if (sl >= slices) break;
+        const int64_t i = sl * kSellC + lane;
+        const bool valid = i < rows;
+        const uint32_t l = valid ? len[i] : 0u;
+        const int w = width[sl];
+        const int64_t b = base[sl] + lane;
+        VT acc = VT(0);
+        bool outside = !staged;
+        if (staged) {
+            int k = 0;
+            for (; k + U <= w; k += U) {
+                int32_t j[U];
+                VT v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    j[u] = __builtin_nontemporal_load(sAj + b + (int64_t)(k + u) * kSellC);
+                    v[u] = __builtin_nontemporal_load(sAx + b + (int64_t)(k + u) * kSellC);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t o = (uint32_t)(j[u] - lo);
+                    const bool in = o < (uint32_t)wn, use = (uint32_t)(k + u) < l;
+                    outside |= use && !in;
+                    acc = use ? acc + v[u] * win[in ? o : 0u] : acc;
+                }
+            }
+            for (; k < w; ++k) {
+                const int32_t j = __builtin_nontemporal_load(sAj + b + (int64_t)k * kSellC);
+                const VT v = __builtin_nontemporal_load(sAx + b + (int64_t)k * kSellC);
+                const uint32_t o = (uint32_t)(j - lo);
+                const bool in = o < (uint32_t)wn, use = (uint32_t)k < l;
+                outside |= use && !in;
+                acc = use ? acc + v * win[in ? o : 0u] : acc;
+            }
+        }
+        if (outside) {   // the row from global x, the same terms in the same order
+            acc = VT(0);
+            for (uint32_t k = 0; k < l; ++k) acc = acc + sAx[b + (int64_t)k * kSellC] * x[sAj[b + (int64_t)k * kSellC]];
+        }
+        if (valid) y[perm[i]] = acc;
+    }
+}
+
 }  // namespace
 
 int sell_window() { return kSellWindow; }
@@ -196,6 +307,39 @@ hipError_t sell_fill(ValueType vt, const int32_t *Aj, const void *Ax, int64_t st
     else if (vt == VT_F64)
         hipLaunchKernelGGL(k_sell_fill<double>, g, b, 0, s, Aj, (const double *)Ax, stride, rows, perm, len, width,
                            base, sAj, (double *)sAx);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t sell_ranges(const int32_t *sAj, const uint32_t *len, const int32_t *width, const int64_t *base,
+                       int64_t rows, int32_t *ranges, hipStream_t s) {
+    const int64_t slices = (rows + kSellC - 1) / kSellC;
+    const int64_t windows = (rows + kSellWindow - 1) / kSellWindow;
+    if (windows <= 0) return hipSuccess;
+    if (windows > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sell_ranges, dim3((unsigned)windows), dim3(1024), 0, s, sAj, len, width, base, rows, slices,
+                       ranges);
+    return hipGetLastError();
+}
+
+hipError_t sell_spmv_staged(ValueType vt, const int32_t *sAj, const void *sAx, const int32_t *perm,
+                            const uint32_t *len, const int32_t *width, const int64_t *base, int64_t rows,
+                            const void *x, void *y, const int32_t *ranges, int64_t window, hipStream_t s) {
+    const int64_t slices = (rows + kSellC - 1) / kSellC;
+    const int64_t windows = (rows + kSellWindow - 1) / kSellWindow;
+    if (windows <= 0) return hipSuccess;
+    const size_t vs = vt == VT_F64 ? 8 : 4;
+    if (window < 1 || (size_t)window * vs > kMaxStageBytes || windows > 0x7fffffff) return hipErrorInvalidValue;
+    const size_t lds = (size_t)window * vs;
+    if (vt == VT_F32)
+        hipLaunchKernelGGL(k_sell_spmv_staged<float>, dim3((unsigned)windows), dim3(1024), lds, s, sAj,
+                           (const float *)sAx, perm, len, width, base, rows, slices, (const float *)x, (float *)y,
+                           ranges, window);
+    else if (vt == VT_F64)
+        hipLaunchKernelGGL(k_sell_spmv_staged<double>, dim3((unsigned)windows), dim3(1024), lds, s, sAj,
+                           (const double *)sAx, perm, len, width, base, rows, slices, (const double *)x, (double *)y,
+                           ranges, window);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -201,11 +201,16 @@ int honei_hip_narrow_indices(void *dst_dev_u32, const void *src_dev_u64, size_t 
  * each row's terms added in that order: bit-identical to
  * honei_hip_product_smell_dv_* and Product<tags::CPU> (la/product.hh:
  * 148-206).  The handle owns its device memory (create allocates and
- * synchronises; product neither allocates nor synchronises).            */
+ * synchronises; product neither allocates nor synchronises).  create also
+ * records each 2048-row window's column range; where it fits 96 KiB of LDS
+ * the product copies that range of x into LDS once per window and gathers
+ * from there (the same bits); honei_hip_sell_stage_x(sell, 0) turns this
+ * off (gathers from global x), 1 back on (default).                       */
 int honei_hip_sell_create(void **sell, int vtype, const void *Aj, const void *Ax, const void *Arl, size_t rows,
                           size_t cols, size_t num_cols_per_row, size_t stride, size_t threads, void *stream);
 int honei_hip_sell_info(void *sell, size_t *slices, size_t *padded_slots, size_t *device_bytes);
 int honei_hip_product_sell(void *sell, void *x, void *y, void *stream);
+int honei_hip_sell_stage_x(void *sell, int on);
 int honei_hip_sell_destroy(void *sell);
 
 /* Bandwidth probe (measurement only, no reference counterpart): `bytes`

@@ -163,9 +163,13 @@ def sell_padded_slots(lengths, window=2048, c=64, keys=64):
 def test_sliced_copy_product_bitexact(dtype, n, max_len):
     """honei_hip_product_sell on the sliced copy of a ragged random ELL
     matrix: y bit-identical to the ELL product and to Product<tags::CPU>
-    (every row's terms in stored order); the padded slot count is the one
-    the windowed sort implies (rows longer than 63 share the last key)."""
-    rp, col, val = random_band_csr(n, seed=40 + n % 7, max_len=max_len, band=min(4096, max(n - 1, 1)), dtype=dtype)
+    (every row's terms in stored order), with x staged in LDS per window and
+    gathered from global memory; the padded slot count is the one the
+    windowed sort implies (rows longer than 63 share the last key).  Bands
+    up to 300,000 columns: windows wider than the LDS stage fall back to
+    global gathers."""
+    band = max(n - 1, 1) if n == 300001 and dtype == np.float64 else min(4096, max(n - 1, 1))
+    rp, col, val = random_band_csr(n, seed=40 + n % 7, max_len=max_len, band=band, dtype=dtype)
     A = oracle.ell_from_sm(n, n, rp, col, val, 1)
     x = uniform_pm1(6, n).astype(dtype)
     D = H.DeviceEll.from_oracle(A)
@@ -176,6 +180,11 @@ def test_sliced_copy_product_bitexact(dtype, n, max_len):
     S.product(ys, dx)
     assert np.array_equal(ys.get().view(np.uint8), ye.get().view(np.uint8))
     assert np.array_equal(ys.get().view(np.uint8), oracle.product(A, x).view(np.uint8))
+    # x gathered from global memory instead of each window's LDS copy: the same bits
+    yg = H.DeviceArray(n, dtype)
+    S.stage_x(False)
+    S.product(yg, dx)
+    assert np.array_equal(yg.get().view(np.uint8), ye.get().view(np.uint8))
     assert S.padded_slots == sell_padded_slots(A.Arl)
     # no more than the padded ELL, rounded up to whole 64-row slices
     assert S.padded_slots <= A.ncpr * max(A.stride, 64 * ((n + 63) // 64))
@@ -200,6 +209,9 @@ def test_sliced_copy_of_the_random_4m_matrix(dtype):
     x = H.DeviceArray.from_host(uniform_pm1(2, n).astype(dtype))
     ye, ys = H.DeviceArray(n, dtype), H.DeviceArray(n, dtype)
     D.product(ye, x)
+    S.product(ys, x)
+    assert np.array_equal(ys.get().view(np.uint8), ye.get().view(np.uint8))
+    S.stage_x(False)
     S.product(ys, x)
     assert np.array_equal(ys.get().view(np.uint8), ye.get().view(np.uint8))
     assert S.padded_slots == sell_padded_slots(Arl.get())
