@@ -213,43 +213,49 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
         const int c0 = (int)strip * pg.width, c1 = min(S, c0 + pg.width);
         const int col = c0 - 1 + lane;
-        const bool in1 = col >= 0 && col < S && col <= c1;        // first-pass lanes
         const bool in2 = lane >= 1 && col < c1;                    // second-pass lanes
-        const int colc = in1 ? col : (col < 0 ? 0 : S - 1);       // clamped, for addresses
+        // this thread's node: grid (r, col) clamped into the grid (lanes and
+        // rows outside it are computed and never used)
         const int r = r0 - 1 + i;
-        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);       // clamped row: computed, unused
+        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);
+        const int colc = col < 0 ? 0 : (col > S - 1 ? S - 1 : col);
         const int64_t node = (int64_t)rc * S + colc;
         const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
-        auto operand = [&](int gr, int gc) -> V {
-            gr = gr < 0 ? 0 : (gr > S - 1 ? S - 1 : gr);
-            gc = gc < 0 ? 0 : (gc > S - 1 ? S - 1 : gc);
-            const int64_t j = (int64_t)gr * S + gc;
-            if constexpr (PAIR == PAIR_PRE)
-                return ds[j] * rs[j];
-            else
-                return xs[j];
-        };
-        // ---- stage the operand ----
-        src[i + 1][lane + 1] = operand(r, col);
+        // the staged entry beyond one per thread (threads < kExtra): grid
+        // rows r0 - 2 and r0 + kPairTileRows - 1, columns c0 - 2 and
+        // c0 + kWave - 1; other threads repeat their own node
+        int ea = i + 1, eb = lane + 1;
         if (threadIdx.x < kExtra) {
             const int t = threadIdx.x;
-            int a, b;
-            if (t < kWave) {                    // grid row r0 - 2
-                a = 0;
-                b = t + 1;
-            } else if (t < 2 * kWave) {         // grid row r0 + kPairTileRows - 1
-                a = kPairTileRows + 1;
-                b = t - kWave + 1;
-            } else {                            // grid columns c0 - 2 and c0 + kWave - 1
-                a = (t - 2 * kWave) / 2;
-                b = ((t - 2 * kWave) % 2) ? kWave + 1 : 0;
+            if (t < kWave) {
+                ea = 0;
+                eb = t + 1;
+            } else if (t < 2 * kWave) {
+                ea = kPairTileRows + 1;
+                eb = t - kWave + 1;
+            } else {
+                ea = (t - 2 * kWave) / 2;
+                eb = ((t - 2 * kWave) % 2) ? kWave + 1 : 0;
             }
-            src[a][b] = operand(r0 - 2 + a, c0 - 2 + b);
         }
-        // ---- first pass, grid row r ----
+        const int egr = min(max(r0 - 2 + ea, 0), S - 1), egc = min(max(c0 - 2 + eb, 0), S - 1);
+        const int64_t ej = (int64_t)egr * S + egc;
+        // ---- every global load of the tile issued before any use ----
+        const V rh = rs[node], di = ds[node];
+        const V xn = (PAIR == PAIR_PRE) ? V(0) : xs[node];
+        const V e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
+        const V e2 = (PAIR == PAIR_PRE) ? rs[ej] : V(0);
         V av[9];
+        double cw[4];
         if constexpr (WM == 1) {
-            q1_stored_weights<VT, V>(S, rc, colc, bnd, m.kappa + ls * m.kld, av);
+            // the node's four cells (a boundary node's: clamped, unused)
+            const uint32_t nc = (uint32_t)S - 1u;
+            const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
+            const double *cs = m.kappa + ls * m.kld + ((ir - 1u) * nc + (ic - 1u));
+            cw[0] = cs[0];
+            cw[1] = cs[1];
+            cw[2] = cs[nc];
+            cw[3] = cs[nc + 1];
         } else if constexpr (WM == 2) {
             // cell weights at node - S - 1, node - S, node - 1, node; row-pair
             // weights at node - S, node; column-pair weights at node - 1,
@@ -266,15 +272,22 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             av[6] = load_value_cached(wc + (node - d1));
             av[7] = load_value_cached(wh + node);
             av[8] = load_value_cached(wc + node);
-            if (bnd) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) av[k] = V(k == 0 ? 1 : 0);
-            }
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
         }
-        const V rh = rs[node], di = ds[node];
+        // ---- stage the first pass's operand ----
+        src[i + 1][lane + 1] = (PAIR == PAIR_PRE) ? di * rh : xn;
+        if (threadIdx.x < kExtra) src[ea][eb] = (PAIR == PAIR_PRE) ? e1 * e2 : e1;
+        // ---- the weights of the node's row ----
+        if constexpr (WM == 1) {
+            q1_stored_weights_cells<VT, V>(bnd, cw[0], cw[1], cw[2], cw[3], av);
+        } else if constexpr (WM == 2) {
+            if (bnd) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) av[k] = V(k == 0 ? 1 : 0);
+            }
+        }
         __syncthreads();
         {
             V xv[9];
@@ -287,7 +300,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             V s = V(0);
 #pragma unroll
             for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
-            const V x0 = (PAIR == PAIR_PRE) ? di * rh : xs[node];
+            const V x0 = (PAIR == PAIR_PRE) ? di * rh : xn;
             const V t = rh - s;
             const V v1 = x0 + t * di;
             tile[i][lane] = v1;

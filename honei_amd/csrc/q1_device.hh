@@ -28,12 +28,11 @@ __device__ inline void q1_put<bf16, float>(bf16 *Ax, int64_t s, float v) {
 // [-1,-2,-1,4]] summed over the four cells around the node, cells row-major
 // in coeff[(side-1)^2], each weight formed in double and rounded once to V.
 // q1_row stores these values; the matrix-free batched kernels form them
-// again from the same coefficients, so both see the same bits.
+// again from the same coefficients, so both see the same bits.  Here from
+// the four cell values around the node (sw = cell (i-1, j-1), se = (i-1, j),
+// nw = (i, j-1), ne = (i, j)).
 template <typename V>
-__device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, const double *coeff, V (&w)[9]) {
-    const int64_t nc = side - 1;
-    const double sw = coeff[(i - 1) * nc + (j - 1)], se = coeff[(i - 1) * nc + j];
-    const double nw = coeff[i * nc + (j - 1)], ne = coeff[i * nc + j];
+__device__ inline void q1_cell_weights(double sw, double se, double nw, double ne, V (&w)[9]) {
     // the values of -c / 3.0, -(a + b) / 6.0 and 2.0 * (sum) / 3.0
     w[0] = (V)(-div3_rn(sw));                  // (i-1, j-1)
     w[1] = (V)(-0.5 * div3_rn(sw + se));       // (i-1, j)
@@ -44,6 +43,15 @@ __device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, cons
     w[6] = (V)(-div3_rn(nw));                  // (i+1, j-1)
     w[7] = (V)(-0.5 * div3_rn(nw + ne));       // (i+1, j)
     w[8] = (V)(-div3_rn(ne));                  // (i+1, j+1)
+}
+
+// Node (i, j)'s cells read from coeff[(side-1)^2] (cell indices below 2^32:
+// 32-bit arithmetic) and its weights formed.
+template <typename V>
+__device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, const double *coeff, V (&w)[9]) {
+    const uint32_t nc = (uint32_t)side - 1u, b = ((uint32_t)i - 1u) * nc + ((uint32_t)j - 1u);
+    const double *cs = coeff + b;
+    q1_cell_weights<V>(cs[0], cs[1], cs[nc], cs[nc + 1], w);
 }
 
 // A float as a bf16 matrix stores it (round to nearest even) and widens it
@@ -68,6 +76,22 @@ __device__ inline void q1_stored_weights(int64_t side, int64_t i, int64_t j, boo
         return;
     }
     q1_coeff_weights<A>(side, i, j, coeff, w);
+    if constexpr (sizeof(VT) == 2) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = bf16_round(w[k]);
+    }
+}
+
+// q1_stored_weights from the node's four cell values, already loaded.
+template <typename VT, typename A>
+__device__ inline void q1_stored_weights_cells(bool boundary, double sw, double se, double nw, double ne, A (&w)[9]) {
+    if (boundary) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = A(0);
+        w[0] = A(1);
+        return;
+    }
+    q1_cell_weights<A>(sw, se, nw, ne, w);
     if constexpr (sizeof(VT) == 2) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) w[k] = bf16_round(w[k]);
