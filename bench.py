@@ -190,7 +190,7 @@ def hbm_ceiling(reps=20):
     nbytes = 1 << 30
     a = H.DeviceArray(nbytes // 4, np.float32)
     b = H.DeviceArray(nbytes // 4, np.float32)
-    part = H.DeviceArray(65536, np.float32)
+    part = H.DeviceArray(65536, np.float32)   # one sum per work-group (grids up to 65536)
     a.zero()
     b.zero()
     out = {}
@@ -200,6 +200,11 @@ def hbm_ceiling(reps=20):
                       lambda g=grid: H.call("honei_hip_stream_probe", 0, part.ptr, a.ptr, nbytes, g, None), nbytes))
         cases.append((f"probe_copy_grid{grid}",
                       lambda g=grid: H.call("honei_hip_stream_probe", 1, b.ptr, a.ptr, nbytes, g, None), 2 * nbytes))
+    for grid in (8192, 32768):   # one contiguous chunk per work-group, chunks contiguous per XCD
+        cases.append((f"probe_read_chunked{grid}",
+                      lambda g=grid: H.call("honei_hip_stream_probe", 2, part.ptr, a.ptr, nbytes, g, None), nbytes))
+        cases.append((f"probe_copy_chunked{grid}",
+                      lambda g=grid: H.call("honei_hip_stream_probe", 3, b.ptr, a.ptr, nbytes, g, None), 2 * nbytes))
     for label, fn, nb in cases:
         for _ in range(3):
             fn()
@@ -499,25 +504,32 @@ def q1_nnz(side):
     return 9 * (side - 2) ** 2 + 4 * (side - 1)
 
 
-def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5):
+def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5, mf=True):
     """Algorithmic bytes of every launch one slot of finest level L takes
     part in during a batched step with `iters` outer (CG) iterations, each
     launch counted by its own compulsory traffic (SURVEY-style: values and
-    vectors once per launch, index streams only where a launch reads one):
+    vectors once per launch, index streams only where a launch reads one).
+    mf (the pool's default, matrix_free 1): the paired passes and the FP64
+    row sums read the cell coefficients (8 bytes per cell) instead of
+    stored values, and those values are never built.
 
       build    : coefficient read from its source and written, every coarser
-                 coefficient from four finer cells, every level's values
-                 (vs bytes each) and dinv, the FP64 operator, b copied, x0;
+                 coefficient from four finer cells, every stored level's
+                 values (vs bytes each) and every level's dinv, the FP64
+                 operator (not with mf), b copied, x0;
       settle   : r = b - A x in FP64 at the start and at the verification
-                 (nnz*8 + 24n each; neighbours from the grid, no indices);
+                 (nnz*8 + 24n each, mf: 8 per cell + 24n; neighbours from
+                 the grid, no indices);
       per iteration, on the finest level n = n_L:
                  scale r/|r| -> FP32 (12n), r.z (12n), p = z + beta p (20n),
-                 q = A p with p.q (nnz*8 + 16n), x, r update with |r|^2 (48n);
+                 q = A p with p.q (nnz*8 + 16n, mf: 8 per cell + 16n), x, r
+                 update with |r|^2 (48n);
       per iteration, per level l (n_l, nnz_l):
                  l >= pair_from: paired Jacobi0 + defect and paired two
-                 sweeps (nnz_l*vs + 16 n_l each), restriction
-                 (4 n_l + 4 n_{l-1}), prolongation + correction
-                 (8 n_l + 4 n_{l-1});
+                 sweeps (nnz_l*vs + 16 n_l each, mf: 8 per cell + 16 n_l),
+                 restriction (4 n_l + 4 n_{l-1}), the prolongation fused
+                 into the paired sweeps (4 n_{l-1} more: the coarse result
+                 read, the fine x read in place of the corrected vector);
                  2 <= l <= fused: the fused coarse launch's passes, each
                  with its index stream: Jacobi0, one Jacobi sweep, the
                  defect, two post sweeps (nnz_l*(vs+4) + 16 n_l, the defect
@@ -533,18 +545,20 @@ def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5):
         return 4 ** l
 
     n, nnz = n_(L), nnz_(L)
-    build = 16 * cells(L) + 24 * n + nnz * 8
+    op64 = 8 * cells(L) if mf else nnz * 8     # bytes of the FP64 operator a row sum reads
+    build = 16 * cells(L) + 24 * n + (0 if mf else nnz * 8)
     for l in range(1, L + 1):
-        build += 8 * cells(l) + nnz_(l) * vs + 4 * n_(l)
+        stored = not (mf and l >= pair_from)
+        build += 8 * cells(l) + (nnz_(l) * vs if stored else 0) + 4 * n_(l)
         if l < L:
             build += 8 * cells(l + 1) + 8 * cells(l)
-    settle = 2 * (nnz * 8 + 24 * n)
-    it = 12 * n + 12 * n + 20 * n + (nnz * 8 + 16 * n) + 48 * n
+    settle = 2 * (op64 + 24 * n)
+    it = 12 * n + 12 * n + 20 * n + (op64 + 16 * n) + 48 * n
     for l in range(2, L + 1):
         nl, zl, nc = n_(l), nnz_(l), n_(l - 1)
         transfers = (4 * nl + 4 * nc) + (8 * nl + 4 * nc)
         if l >= pair_from:
-            it += 2 * (zl * vs + 16 * nl) + transfers
+            it += 2 * ((8 * cells(l) if mf else zl * vs) + 16 * nl) + (4 * nl + 4 * nc) + 4 * nc
         elif l <= fused:
             sweep = zl * (vs + 4) + 16 * nl
             it += 4 * sweep + (zl * (vs + 4) + 12 * nl) + transfers
@@ -601,12 +615,20 @@ def level_launch_rates(B, levels, level, reps=20):
     taken)."""
     pre, rst, pro, post, spmv = B.time_level(level, reps)
     side = 2 ** level + 1
-    n, nnz, nc = side * side, q1_nnz(side), (2 ** (level - 1) + 1) ** 2
+    n, nc = side * side, (2 ** (level - 1) + 1) ** 2
+    cells = (side - 1) ** 2
     cap = sum(1 for L in levels if L >= level)
-    rows = {"paired_pre": (pre, cap * (nnz * 4 + 16 * n), "k_b_pair<float,PRE>: Jacobi0 then the defect"),
-            "restriction": (rst, cap * (4 * n + 4 * nc), "k_b_restrict"),
-            "prolongation": (pro, cap * (8 * n + 4 * nc), "k_b_prolong_axpy: t = x + P e"),
-            "paired_post": (post, cap * (nnz * 4 + 16 * n), "k_b_pair<float,POST>: two Jacobi sweeps")}
+    # the pool's default weight source (matrix_free 1): 8 bytes per cell
+    # instead of the stored values
+    rows = {"paired_pre": (pre, cap * (8 * cells + 16 * n),
+                           "k_b_pair<float,PRE,weights from cells>: Jacobi0 then the defect"),
+            "restriction": (rst, cap * (4 * n + 4 * nc), "k_b_restrict_rows"),
+            "prolongation": (pro, cap * (8 * n + 4 * nc), "k_b_prolong_rows: t = x + P e"),
+            # levels that pair fuse the prolongation into the post-smoothing
+            # launch (pro = 0 there): the coarse result is read once more
+            "paired_post": (post, cap * (8 * cells + 16 * n + (4 * nc if pro == 0 else 0)),
+                            "k_b_pair<float,POST,weights from cells%s>: two Jacobi sweeps"
+                            % (", prolongation fused" if pro == 0 else ""))}
     out = {"level": level, "side": side, "slots": cap}
     for key, (us, nb, what) in rows.items():
         if us > 0:
@@ -614,9 +636,10 @@ def level_launch_rates(B, levels, level, reps=20):
             out[key] = {"kernel": what, "us": round(us, 2), "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
                         "frac": round(g / PEAK_GBS, 4)}
     if spmv > 0:
-        nb = sum(q1_nnz(2 ** L + 1) * 8 + 16 * (2 ** L + 1) ** 2 for L in levels)
+        nb = sum(8 * 4 ** L + 16 * (2 ** L + 1) ** 2 for L in levels)
         g = nb / (spmv * 1e-6) / 1e9
-        out["outer_spmv_dot"] = {"kernel": "k_b_outer_spmv_dot: FP64 q = A p, p.q, every slot", "us": round(spmv, 2),
+        out["outer_spmv_dot"] = {"kernel": "k_b_outer_spmv_dot<2, weights from cells>: FP64 q = A p, p.q, every slot",
+                                 "us": round(spmv, 2),
                                  "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
                                  "frac": round(g / PEAK_GBS, 4)}
     out["what"] = ("launches of the batched step's shape over every slot of the level (full activity), timed "
@@ -827,17 +850,17 @@ def batch_kernel_roofline(B, level):
                          "bytes_rule": "slots*(nnz*4 + n*16) + nnz*4 + n*4: the column indices are shared by all "
                                        "slots",
                          "note": "one pass per launch; levels from 33^2 up run the paired launches below"}}
-    pre, post = B.pair_timing(level)
-    pbytes = ns * (nnz * 4 + n * 16)
-    for name, t, what in (("paired_pre", pre, "k_b_pair<float,PRE>: Jacobi0 then the defect"),
-                          ("paired_post", post, "k_b_pair<float,POST>: two Jacobi sweeps")):
+    pre, post = B.pair_timing(level)   # the pool's weight source: the cell coefficients
+    pbytes = ns * (8 * (side - 1) ** 2 + n * 16)
+    for name, t, what in (("paired_pre", pre, "k_b_pair<float,PRE,weights from cells>: Jacobi0 then the defect"),
+                          ("paired_post", post, "k_b_pair<float,POST,weights from cells>: two Jacobi sweeps")):
         if t > 0:
             g = pbytes / (t * 1e-6) / 1e9
             out[name] = {"kernel": f"{what}, level {level} ({side}^2)", "slots": ns, "us": round(t, 2),
                          "algorithmic_bytes": int(pbytes), "achieved_gbs": round(g, 1),
                          "frac": round(g / PEAK_GBS, 4),
-                         "bytes_rule": "slots*(nnz*4 + n*16): values once for both passes, rhs, dinv, the "
-                                       "input or first output, the output; no index stream"}
+                         "bytes_rule": "slots*(8*cells + n*16): the cell coefficients once for both passes, rhs, "
+                                       "dinv, the input or first output, the output; no index or value stream"}
     return out
 
 
@@ -1325,10 +1348,18 @@ def main():
                    "outer": "FP64 conjugate gradients preconditioned by one V-cycle per iteration",
                    "rel_residual": 1e-8,
                    "step": "assemble all hierarchies from resident coefficients + solve all systems"},
-        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
-                       "p99": round(float(np.percentile(lat_ms, 99)), 3),
-                       "max": round(float(lat_ms.max()), 3),
-                       "what": "per-system completion time within a batched step"},
+        # the metric's latency is a request's: arrival to result in the
+        # serving stream (every system of a batched step completes with it)
+        "latency_ms": ({"p50": serving["latency_ms"]["p50"], "p99": serving["latency_ms"]["p99"],
+                        "max": serving["latency_ms"]["max"],
+                        "what": ("request latency, arrival to result, in the serving stream (Poisson arrivals "
+                                 "at %g/s, sides 65..1025, to 1e-8)" % serving["arrival_rate"])}
+                       if serving else None),
+        "batch_completion_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
+                                "p99": round(float(np.percentile(lat_ms, 99)), 3),
+                                "max": round(float(lat_ms.max()), 3),
+                                "what": ("per-system completion time within a batched step (one graph launch: "
+                                         "every system completes with the step)")},
         "solve": {"outer_iterations_median": int(np.median(iters)), "outer_iterations_max": int(iters.max()),
                   "max_rel_residual": float(rels.max()), "all_converged": bool((status == 0).all()),
                   "graph_launches_per_step": round(launches / args.steps, 2)},

@@ -468,13 +468,13 @@ hipError_t BatchMG::enqueue_vcycle(hipStream_t s) {
         BLevel &lv = L_[(size_t)l];
         const BMat m = mat(lv);
         const BWork w = level_work(l);
-        if (paired(lv) && cfg_.n_post == 2) {
-            // t = x + P e, then x = J(J(t)): one launch for both sweeps
-            HX(b_prolong_axpy(vt, lv, L_[(size_t)l - 1], w, lv.t, s));
-            HX(b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s, pair_weights(lv)));
+        const BLevel &lc = L_[(size_t)l - 1];
+        if (fused_post(lv)) {
+            // t = J(J(x + P e)): one launch for the correction and both sweeps
+            HX(b_pair(vt, PAIR_POST, lv, w, lv.t, nullptr, lv.x, s, pair_weights(lv), &lc, level_result(lc)));
             continue;
         }
-        HX(b_prolong_axpy(vt, lv, L_[(size_t)l - 1], w, lv.x, s));
+        HX(b_prolong_axpy(vt, lv, lc, level_result(lc), w, lv.x, s));
         void *src = lv.x, *dst = lv.t;
         for (int k = 0; k < cfg_.n_post; ++k) {
             HX(b_ell(vt, lv.T, lv.block, lv.group, ELL_JACOBI, m, w, lv.cap, dst, src, lv.rhs, lv.dinv, s));
@@ -514,7 +514,7 @@ BClassTab BatchMG::class_table() const {
         e.partials = c.partials;
         e.rhs_top = top.rhs;
         e.top64 = cfg_.inner_vtype == VT_F64 ? 1 : 0;
-        e.x_top = top.x;
+        e.x_top = level_result(top);
         e.kappa = top.kappa;
         e.kld = top.kld;
     }
@@ -783,8 +783,12 @@ hipError_t BatchMG::time_level(int64_t level, int reps, double us[5]) {
     const int wm = pair_weights(lv);
     if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_PRE, lv, w, lv.x, lv.d, nullptr, s_main_, wm); }, &us[0]);
     timeit([&]() { return b_restrict(vt, lv, lc, w, s_main_); }, &us[1]);
-    timeit([&]() { return b_prolong_axpy(vt, lv, lc, w, lv.t, s_main_); }, &us[2]);
-    if (paired(lv)) timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, wm); }, &us[3]);
+    if (fused_post(lv))
+        timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.t, nullptr, lv.x, s_main_, wm, &lc, lc.x); }, &us[3]);
+    else
+        timeit([&]() { return b_prolong_axpy(vt, lv, lc, lc.x, w, lv.t, s_main_); }, &us[2]);
+    if (paired(lv) && !fused_post(lv))
+        timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, wm); }, &us[3]);
     if (cfg_.outer == 1) {
         const BClassTab tab = class_table();
         int64_t blocks = 0;

@@ -296,7 +296,8 @@ class BatchMG {
     // stream, reps launches after one warm-up: us[0] paired pre-smoothing +
     // defect, us[1] restriction to level - 1, us[2] prolongation + correction
     // from level - 1, us[3] paired post-smoothing (0 where the level does not
-    // pair), us[4] the FP64 q = A p, p.q over every slot of every class.
+    // pair; with the prolongation fused in, us[2] is 0 and us[3] covers
+    // both), us[4] the FP64 q = A p, p.q over every slot of every class.
     // Overwrites the pool's vectors: call between steps only (a load
     // rebuilds them, and results must be copied out first).
     hipError_t time_level(int64_t level, int reps, double us[5]);
@@ -313,6 +314,14 @@ class BatchMG {
         const bool all_paired = paired(lv) && cfg_.n_pre == 2 && (cfg_.n_post == 2 || cfg_.n_post == 0);
         return !(cfg_.matrix_free && all_paired && lv.level > lf);
     }
+    // the up-leg of the level is one paired post-smoothing launch with the
+    // prolongation from the level below fused in (b_pair with xc); it leaves
+    // the level's result in t, every other level's in x
+    bool fused_post(const BLevel &lv) const {
+        const int64_t lf = cfg_.fused_levels >= 2 ? std::min<int64_t>(cfg_.fused_levels, max_level_) : 1;
+        return paired(lv) && cfg_.n_post == 2 && lv.level > lf;
+    }
+    void *level_result(const BLevel &lv) const { return fused_post(lv) ? lv.t : lv.x; }
     // the level's paired launches gather the weights from the stencil arrays
     bool stores_stencil(const BLevel &lv) const { return cfg_.matrix_free == 2 && !stores_values(lv); }
     // weight source of the level's paired launches (b_pair's wmode)
@@ -393,17 +402,22 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
 hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
                     double tol, int32_t max_outer, hipStream_t s);
 hipError_t b_restrict(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, hipStream_t s);
-// out = fine.x + P coarse.x (out: fine.x or fine.t)
-hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, void *out,
-                          hipStream_t s);
+// out = fine.x + P xc (out: fine.x or fine.t; xc: the coarse level's
+// result, coarse.x or coarse.t)
+hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const void *xc, const BWork &w,
+                          void *out, hipStream_t s);
 // Two dependent row passes of a T = 1 Q1 level in one launch (k_b_pair):
 // PAIR_PRE writes x = Jacobi0 to out0 and d = rhs - A x to out1; PAIR_POST
 // writes two Jacobi sweeps of xin to out0 (xin != out0).  Bit-identical to
 // the separate b_ell launches.
 // wmode: the weights from the stored values (0), from lv.kappa (1), from the
 // distinct-weight arrays lv.st (2): the same bits in every form.
+// xc (PAIR_POST, with `coarse`): the prolongation fused in, out0 =
+// J(J(xin + P xc)), the same bits as b_prolong_axpy into a third vector
+// followed by the unfused pair (xc: the coarse level's result).
 hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void *out0, void *out1,
-                  const void *xin, hipStream_t s, int wmode = 0);
+                  const void *xin, hipStream_t s, int wmode = 0, const BLevel *coarse = nullptr,
+                  const void *xc = nullptr);
 // The distinct-weight arrays of `count` slots of a side x side level from
 // their cell coefficients (kappa + g kld, st + g 4 vld).
 hipError_t b_stencil_fill(ValueType vt, int64_t side, const double *kappa, int64_t kld, void *st, int64_t vld,

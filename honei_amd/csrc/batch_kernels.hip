@@ -170,10 +170,39 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
 // epilogue), so the result is bit-identical to two separate launches.
 //   PAIR_PRE : x = Jacobi0 (two sweeps from zero) -> out0, then the defect
 //              d = rhs - A x -> out1;
-//   PAIR_POST: out0 = J(J(xin)) (xin != out0).
+//   PAIR_POST: out0 = J(J(xin)) (xin != out0); with FP (the prolongation
+//              fused in) out0 = J(J(xin + P xc)), the operand xin + P xc
+//              formed while staging with the terms and order of
+//              k_b_prolong_rows (so the same bits as that launch followed
+//              by the unfused pass pair).
 struct PairGeom {
     int32_t side, width, strips, segs;
 };
+
+// The four coarse values around fine node (R, C) of a (2 m + 1)^2 level
+// (coarse side nc = m + 1): (R/2, C/2), (R/2, (C+1)/2), ((R+1)/2, C/2),
+// ((R+1)/2, (C+1)/2), rounded down.
+template <typename V>
+__device__ inline void prolong_taps(const V *xc, int32_t nc, int32_t R, int32_t C, V (&t)[4]) {
+    const int32_t rlo = R >> 1, rhi = (R + 1) >> 1, clo = C >> 1, chi = (C + 1) >> 1;
+    t[0] = xc[rlo * nc + clo];
+    t[1] = xc[rlo * nc + chi];
+    t[2] = xc[rhi * nc + clo];
+    t[3] = xc[rhi * nc + chi];
+}
+// prolong_node's correction of fine node (R, C) of an nf x nf level from its
+// taps, every border Dirichlet (0): a coarse node's value, the mean of two
+// (odd row or odd column) or of four (both odd), summed in that order.
+template <typename V>
+__device__ inline V prolong_value(int32_t R, int32_t C, int32_t nf, const V (&t)[4]) {
+    const V h = V(0.5), q = V(0.25);
+    const V f_rc = ((V(0) + t[0] * q + t[1] * q) + t[2] * q) + t[3] * q;
+    const V f_c = (V(0) + t[0] * h) + t[1] * h;
+    const V f_r = (V(0) + t[0] * h) + t[2] * h;
+    const bool oddR = (R & 1) != 0, oddC = (C & 1) != 0;
+    const V f = oddR ? (oddC ? f_rc : f_r) : (oddC ? f_c : t[0]);
+    return (R == 0 || R == nf - 1 || C == 0 || C == nf - 1) ? V(0) : f;
+}
 
 template <typename VT> __device__ inline typename VecOf<VT>::type load_value_cached(const VT *p) { return *p; }
 template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { return bf16_to_float(p->bits); }
@@ -182,9 +211,12 @@ template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { ret
 // for double values)
 // WM: weights from the stored values (0), formed from the cell coefficients
 // (1), gathered from the distinct-weight arrays (2).
-template <typename VT, int PAIR, int WM, typename V = typename VecOf<VT>::type>
+// FP (PAIR_POST only): xc = the coarse level's result (vld_c per slot).
+template <typename VT, int PAIR, int WM, bool FP = false, typename V = typename VecOf<VT>::type>
 __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWork w, PairGeom pg, V *out0, V *out1,
-                                                                  const V *xin, const V *rhs, const V *dinv) {
+                                                                  const V *xin, const V *rhs, const V *dinv,
+                                                                  const V *xc, int64_t vld_c) {
+    static_assert(!FP || PAIR == PAIR_POST, "the prolongation fuses into the post-smoothing pair");
     // first-pass results of the tile and its halo
     __shared__ V tile[kPairTileRows][kWave];
     // the first pass's gathered operand (PAIR_PRE: dinv .* rhs; PAIR_POST:
@@ -242,9 +274,16 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         const int64_t ej = (int64_t)egr * S + egc;
         // ---- every global load of the tile issued before any use ----
         const V rh = rs[node], di = ds[node];
-        const V xn = (PAIR == PAIR_PRE) ? V(0) : xs[node];
-        const V e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
+        V xn = (PAIR == PAIR_PRE) ? V(0) : xs[node];
+        V e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
         const V e2 = (PAIR == PAIR_PRE) ? rs[ej] : V(0);
+        V tn[4], te[4];   // FP: coarse taps of the node and of the extra entry
+        if constexpr (FP) {
+            const int32_t nc = (S >> 1) + 1;
+            const V *xcs = xc + ls * vld_c;
+            prolong_taps(xcs, nc, rc, colc, tn);
+            prolong_taps(xcs, nc, egr, egc, te);
+        }
         V av[9];
         double cw[4];
         if constexpr (WM == 1) {
@@ -275,6 +314,10 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
+        }
+        if constexpr (FP) {   // t = x + P xc, as k_b_prolong_rows stores it
+            xn = xn + prolong_value(rc, colc, S, tn) * V(1);
+            e1 = e1 + prolong_value(egr, egc, S, te) * V(1);
         }
         // ---- stage the first pass's operand ----
         src[i + 1][lane + 1] = (PAIR == PAIR_PRE) ? di * rh : xn;
@@ -833,9 +876,10 @@ __global__ __launch_bounds__(256) void k_b_prolong_rows(BWork w, V *x_f, int64_t
     const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     for (int64_t lb = first_block(); lb * 4 * U < units; lb += gridDim.x) {
         const int64_t u0 = __builtin_amdgcn_readfirstlane((int32_t)((lb * 4 + wv) * U));
-        V a[U], b[U], c[U], d[U], xv[U];
+        V t[U][4], xv[U];
         int64_t id[U];
-        bool store[U], oddR[U], oddC[U], border[U];
+        int32_t Rk[U], Ck[U];
+        bool store[U];
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             const bool uok = u0 + k < units;
@@ -843,28 +887,15 @@ __global__ __launch_bounds__(256) void k_b_prolong_rows(BWork w, V *x_f, int64_t
             const int32_t R = ru.row, C0 = ru.col + lane;
             store[k] = uok && C0 < nf;
             const int32_t C = C0 < nf ? C0 : nf - 1;
-            oddR[k] = (R & 1) != 0;
-            oddC[k] = (C & 1) != 0;
-            border[k] = R == 0 || R == nf - 1 || C == 0 || C == nf - 1;
-            const int32_t rlo = R >> 1, rhi = (R + 1) >> 1, clo = C >> 1, chi = (C + 1) >> 1;
-            const V *xc = x_c + ru.g * vld_c;
-            a[k] = xc[(int64_t)rlo * nc + clo];
-            b[k] = xc[(int64_t)rlo * nc + chi];
-            c[k] = xc[(int64_t)rhi * nc + clo];
-            d[k] = xc[(int64_t)rhi * nc + chi];
+            Rk[k] = R;
+            Ck[k] = C;
+            prolong_taps(x_c + ru.g * vld_c, nc, R, C, t[k]);
             id[k] = ru.g * vld_f + (int64_t)R * nf + C;
             xv[k] = x_f[id[k]];
         }
 #pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const V h = V(0.5), q = V(0.25);
-            const V f_rc = ((V(0) + a[k] * q + b[k] * q) + c[k] * q) + d[k] * q;
-            const V f_c = (V(0) + a[k] * h) + b[k] * h;
-            const V f_r = (V(0) + a[k] * h) + c[k] * h;
-            V f = oddR[k] ? (oddC[k] ? f_rc : f_r) : (oddC[k] ? f_c : a[k]);
-            if (border[k]) f = V(0);
-            if (store[k]) out_f[id[k]] = xv[k] + f * V(1);
-        }
+        for (int k = 0; k < U; ++k)
+            if (store[k]) out_f[id[k]] = xv[k] + prolong_value(Rk[k], Ck[k], nf, t[k]) * V(1);
     }
 }
 
@@ -1249,29 +1280,49 @@ hipError_t b_restrict(ValueType vt, const BLevel &fine, const BLevel &coarse, co
     return hipGetLastError();
 }
 
-hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const BWork &w, void *out,
-                          hipStream_t s) {
+hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse, const void *xc, const BWork &w,
+                          void *out, hipStream_t s) {
     const int64_t m = coarse.side - 1;
     if (fine.side != 2 * m + 1 || fine.cap > coarse.cap || fine.side > kMaxTransferSide) return hipErrorInvalidValue;
-    if (out != fine.x && out != fine.t) return hipErrorInvalidValue;
+    if ((out != fine.x && out != fine.t) || (xc != coarse.x && xc != coarse.t)) return hipErrorInvalidValue;
     const int64_t units = row_units(fine.cap, fine.side, fine.side);
     if (units > 0x7fffffff - 4 * kRowUnitsPerWave) return hipErrorInvalidValue;
     const dim3 grid(batch_grid((units + 4 * kRowUnitsPerWave - 1) / (4 * kRowUnitsPerWave), 256)), block(256);
     if (vt == VT_F64)
         hipLaunchKernelGGL(k_b_prolong_rows<double>, grid, block, 0, s, w, (double *)fine.x, fine.vld,
-                           (const double *)coarse.x, coarse.vld, (int32_t)m, (double *)out);
+                           (const double *)xc, coarse.vld, (int32_t)m, (double *)out);
     else
         hipLaunchKernelGGL(k_b_prolong_rows<float>, grid, block, 0, s, w, (float *)fine.x, fine.vld,
-                           (const float *)coarse.x, coarse.vld, (int32_t)m, (float *)out);
+                           (const float *)xc, coarse.vld, (int32_t)m, (float *)out);
     return hipGetLastError();
 }
 
+template <typename VT, int P, int WM>
+void pair_launch(dim3 grid, dim3 block, hipStream_t s, const BMat &m, const BWork &w, const PairGeom &pg, void *out0,
+                 void *out1, const void *x, const void *rhs, const void *dinv, const void *xc, int64_t vld_c) {
+    typedef typename VecOf<VT>::type V;
+    if constexpr (P == PAIR_POST) {
+        if (xc) {
+            hipLaunchKernelGGL((k_b_pair<VT, P, WM, true>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,
+                               (const V *)x, (const V *)rhs, (const V *)dinv, (const V *)xc, vld_c);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_b_pair<VT, P, WM>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1, (const V *)x,
+                       (const V *)rhs, (const V *)dinv, (const V *)nullptr, (int64_t)0);
+}
+
 hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void *out0, void *out1,
-                  const void *xin, hipStream_t s, int wmode) {
+                  const void *xin, hipStream_t s, int wmode, const BLevel *coarse, const void *xc) {
     if (lv.T != 1 || lv.side < 3 || lv.ncpr != 9 || lv.side > 65535) return hipErrorInvalidValue;
     if (pair != PAIR_PRE && pair != PAIR_POST) return hipErrorInvalidValue;
     if (!out0 || (pair == PAIR_PRE && !out1) || (pair == PAIR_POST && (!xin || xin == out0)))
         return hipErrorInvalidValue;
+    const bool fp = xc != nullptr;   // the prolongation fused in
+    if (fp && (pair != PAIR_POST || !coarse || lv.side != 2 * (coarse->side - 1) + 1 || lv.cap > coarse->cap ||
+               lv.side > kMaxTransferSide || xc == out0))
+        return hipErrorInvalidValue;
+    const int64_t vld_c = fp ? coarse->vld : 0;
     if (wmode < 0 || wmode > 2) return hipErrorInvalidValue;
     if (wmode == 1 && (!lv.kappa || lv.kld < (lv.side - 1) * (lv.side - 1))) return hipErrorInvalidValue;
     if (wmode == 2 && !lv.st) return hipErrorInvalidValue;
@@ -1288,12 +1339,8 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void
     const int threads = kPairTileRows * kWave;
     const dim3 grid(batch_grid(units, threads)), block(threads);
     const void *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
-#define PAIR_LAUNCH(VT, P, MFV)                                                                              \
-    do {                                                                                                     \
-        typedef typename VecOf<VT>::type V;                                                                  \
-        hipLaunchKernelGGL((k_b_pair<VT, P, MFV>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,        \
-                           (const V *)x, (const V *)lv.rhs, (const V *)lv.dinv);                             \
-    } while (0)
+#define PAIR_LAUNCH(VT, P, MFV) \
+    pair_launch<VT, P, MFV>(grid, block, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv, xc, vld_c)
 #define PAIR_WM(VT, P)                                                        \
     do {                                                                      \
         if (wmode == 2)                                                       \

@@ -101,6 +101,41 @@ __global__ __launch_bounds__(256) void k_probe_copy(f4v *__restrict__ dst, const
     }
     for (; i < n4; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
+// Chunked forms: work-group b (remapped so each XCD takes one contiguous
+// range of chunks, as the SpMV's row blocks) reads (and copies) one
+// contiguous chunk of `per` 16-byte elements, eight loads in flight per
+// thread; the grid covers the array once.
+__global__ __launch_bounds__(256) void k_probe_read_chunked(const f4v *__restrict__ src, int64_t n4, int64_t per,
+                                                            float *partials) {
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i0 = b * per, i1 = min(n4, i0 + per);
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    int64_t i = i0 + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * 256 < i1; i += kProbeUnroll * 256) {
+        f4v v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(src + i + k * 256);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) acc += v[k];
+    }
+    for (; i < i1; i += 256) acc += __builtin_nontemporal_load(src + i);
+    const float t = block_sum(acc.x + acc.y + acc.z + acc.w);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void k_probe_copy_chunked(f4v *__restrict__ dst, const f4v *__restrict__ src,
+                                                            int64_t n4, int64_t per) {
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i0 = b * per, i1 = min(n4, i0 + per);
+    int64_t i = i0 + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * 256 < i1; i += kProbeUnroll * 256) {
+        f4v v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(src + i + k * 256);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) __builtin_nontemporal_store(v[k], dst + i + k * 256);
+    }
+    for (; i < i1; i += 256) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
 __global__ void k_zero_bytes(unsigned char *p, size_t n) {
     GRID_STRIDE(i, (int64_t)n) p[i] = 0;
 }
@@ -258,10 +293,24 @@ hipError_t narrow_indices(uint32_t *d, const uint64_t *src, int64_t n, uint64_t 
     return hipGetLastError();
 }
 hipError_t stream_probe(int kind, void *dst, const void *src, int64_t n4, int grid, hipStream_t s) {
-    if (kind == 0)
-        hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(kBlock), 0, s, (const f4v *)src, n4, (float *)dst);
-    else
-        hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(kBlock), 0, s, (f4v *)dst, (const f4v *)src, n4);
+    const int64_t per = (n4 + grid - 1) / grid;   // chunked forms: elements per work-group
+    switch (kind) {
+        case 0:
+            hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(kBlock), 0, s, (const f4v *)src, n4, (float *)dst);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(kBlock), 0, s, (f4v *)dst, (const f4v *)src, n4);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_probe_read_chunked, dim3(grid), dim3(256), 0, s, (const f4v *)src, n4, per,
+                               (float *)dst);
+            break;
+        case 3:
+            hipLaunchKernelGGL(k_probe_copy_chunked, dim3(grid), dim3(256), 0, s, (f4v *)dst, (const f4v *)src, n4,
+                               per);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 hipError_t fill_bytes_zero(void *p, size_t bytes, hipStream_t s) {

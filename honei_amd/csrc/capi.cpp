@@ -460,7 +460,8 @@ int honei_hip_sell_destroy(void *sell) {
 }
 
 int honei_hip_stream_probe(int kind, void *dst, const void *src, size_t bytes, size_t grid, void *stream) {
-    if (kind != 0 && kind != 1) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "kind must be 0 (read) or 1 (copy)");
+    if (kind < 0 || kind > 3)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "kind: 0 read, 1 copy (grid-stride), 2 read, 3 copy (chunked)");
     if (!dst || !src) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "null pointer");
     if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "bytes and pointers must be multiples of 16");

@@ -217,7 +217,9 @@ int honei_hip_sell_destroy(void *sell);
  * (multiple of 16, 16-byte aligned pointers) streamed with 16-byte
  * non-temporal loads, eight in flight per thread, on `grid` (1..65536)
  * 256-thread work-groups.  kind 0: read src, one float sum per work-group
- * into dst (dst holds `grid` floats); kind 1: copy src to dst.           */
+ * into dst (dst holds `grid` floats); kind 1: copy src to dst; both
+ * grid-stride.  kinds 2 / 3: the same, each work-group taking one
+ * contiguous chunk, chunks contiguous per XCD (as the SpMV's row blocks). */
 int honei_hip_stream_probe(int kind, void *dst, const void *src, size_t bytes, size_t grid, void *stream);
 
 /* Fused defect + norm (no reference counterpart; the reference runs
@@ -527,7 +529,8 @@ int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, doub
  * full activity -- with HIP events on the pool's stream, `reps` launches
  * after one warm-up: us[0] paired Jacobi0 + defect, us[1] restriction to
  * level - 1, us[2] prolongation + correction from level - 1, us[3] paired
- * two-sweep launch (0 where the level does not pair), us[4] the FP64
+ * two-sweep launch (0 where the level does not pair; where it pairs, the
+ * prolongation + correction runs inside it and us[2] is 0), us[4] the FP64
  * q = A p, p.q over every slot of every class (CG outer).  Overwrites the
  * pool's vectors and solutions: only between steps, with no slot loading or
  * running, after the results are copied out.  Synchronous.               */
