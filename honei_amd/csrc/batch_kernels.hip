@@ -167,7 +167,10 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
 // HBM once for both passes: halo rows and columns are read again by the
 // neighbouring tiles, through L2 (cached loads, not streaming ones).  Every
 // node's sum is the T = 1 sum of k_b_ell (same terms, same order, same
-// epilogue), so the result is bit-identical to two separate launches.
+// epilogue), so the result is bit-identical to two separate launches: an
+// interior node's nine terms in column order, a boundary node's 0 + 1 * x
+// (selected after every node has formed the interior sum over its staged
+// neighbours, so the LDS reads take fixed offsets).
 //   PAIR_PRE : x = Jacobi0 (two sweeps from zero) -> out0, then the defect
 //              d = rhs - A x -> out1;
 //   PAIR_POST: out0 = J(J(xin)) (xin != out0); with FP (the prolongation
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             // cell weights at node - S - 1, node - S, node - 1, node; row-pair
             // weights at node - S, node; column-pair weights at node - 1,
             // node; the diagonal at node (boundary rows: addresses of the
-            // node itself, values replaced by the identity row)
+            // node itself, values unused)
             const VT *wc = (const VT *)m.st + ls * 4 * m.vld, *wh = wc + m.vld, *wv = wh + m.vld, *wd = wv + m.vld;
             const int64_t dS = bnd ? 0 : S, d1 = bnd ? 0 : 1;
             av[0] = load_value_cached(wc + (node - dS - d1));
@@ -322,27 +325,21 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         // ---- stage the first pass's operand ----
         src[i + 1][lane + 1] = (PAIR == PAIR_PRE) ? di * rh : xn;
         if (threadIdx.x < kExtra) src[ea][eb] = (PAIR == PAIR_PRE) ? e1 * e2 : e1;
-        // ---- the weights of the node's row ----
-        if constexpr (WM == 1) {
-            q1_stored_weights_cells<VT, V>(bnd, cw[0], cw[1], cw[2], cw[3], av);
-        } else if constexpr (WM == 2) {
-            if (bnd) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) av[k] = V(k == 0 ? 1 : 0);
-            }
-        }
+        // ---- the weights of the node's row (a boundary node's: those of
+        // its clamped cells, unused) ----
+        if constexpr (WM == 1) q1_interior_weights_cells<VT, V>(cw[0], cw[1], cw[2], cw[3], av);
         __syncthreads();
         {
+            // every node forms the interior stencil sum over its staged
+            // neighbours; a boundary node then takes its identity row's sum
+            // 0 + 1 * x instead
             V xv[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int a = bnd ? i + 1 : i + 1 + (k / 3 - 1);
-                const int b = bnd ? lane + 1 : lane + 1 + (k % 3 - 1);
-                xv[k] = src[a][b];
-            }
+            for (int k = 0; k < 9; ++k) xv[k] = src[i + 1 + (k / 3 - 1)][lane + 1 + (k % 3 - 1)];
             V s = V(0);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
+            for (int k = 0; k < 9; ++k) s = s + av[k] * xv[k];
+            if (bnd) s = V(0) + V(1) * xv[4];
             const V x0 = (PAIR == PAIR_PRE) ? di * rh : xn;
             const V t = rh - s;
             const V v1 = x0 + t * di;
@@ -354,14 +351,11 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         if (i >= 1 && i <= kPairRows && r < r1 && in2) {
             V xv[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int ik = bnd ? i : i + (k / 3 - 1);
-                const int lk = bnd ? lane : lane + (k % 3 - 1);
-                xv[k] = tile[ik][lk];
-            }
+            for (int k = 0; k < 9; ++k) xv[k] = tile[i + (k / 3 - 1)][lane + (k % 3 - 1)];
             V s = V(0);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) s = (k < (bnd ? 1 : 9)) ? s + av[k] * xv[k] : s;
+            for (int k = 0; k < 9; ++k) s = s + av[k] * xv[k];
+            if (bnd) s = V(0) + V(1) * xv[4];
             if constexpr (PAIR == PAIR_PRE) {
                 out1[vo + node] = rh - s;
             } else {

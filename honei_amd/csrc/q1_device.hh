@@ -98,6 +98,19 @@ __device__ inline void q1_stored_weights_cells(bool boundary, double sw, double 
     }
 }
 
+// The interior weights from the node's four cell values, already loaded, as
+// a kernel reading the stored matrix of value type VT sees them (no boundary
+// case: a caller that forms every node's interior sum selects the identity
+// row's sum for boundary nodes itself).
+template <typename VT, typename A>
+__device__ inline void q1_interior_weights_cells(double sw, double se, double nw, double ne, A (&w)[9]) {
+    q1_cell_weights<A>(sw, se, nw, ne, w);
+    if constexpr (sizeof(VT) == 2) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = bf16_round(w[k]);
+    }
+}
+
 // Row `row` of the Q1 operator on a side x side grid in the T-interleaved
 // ELL layout: entry t at slot (t % T) + row*T + (t / T)*stride, ncpr =
 // ceil(9 / T) chunks, unused slots (and padding rows n <= row < stride/T)
