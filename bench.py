@@ -521,9 +521,10 @@ def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5, mf=True):
                  (nnz*8 + 24n each, mf: 8 per cell + 24n; neighbours from
                  the grid, no indices);
       per iteration, on the finest level n = n_L:
-                 scale r/|r| -> FP32 (12n), r.z (12n), p = z + beta p (20n),
-                 q = A p with p.q (nnz*8 + 16n, mf: 8 per cell + 16n), x, r
-                 update with |r|^2 (48n);
+                 scale r/|r| -> FP32 (12n), r.z (12n), p = z + beta p with
+                 q = A p and p.q in one launch (nnz*8 + 28n, mf: 8 per cell +
+                 28n: z and p read, the new p and q written), x, r update
+                 with |r|^2 (48n);
       per iteration, per level l (n_l, nnz_l):
                  l >= pair_from: paired Jacobi0 + defect and paired two
                  sweeps (nnz_l*vs + 16 n_l each, mf: 8 per cell + 16 n_l),
@@ -553,7 +554,7 @@ def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5, mf=True):
         if l < L:
             build += 8 * cells(l + 1) + 8 * cells(l)
     settle = 2 * (op64 + 24 * n)
-    it = 12 * n + 12 * n + 20 * n + (op64 + 16 * n) + 48 * n
+    it = 12 * n + 12 * n + (op64 + 28 * n) + 48 * n
     for l in range(2, L + 1):
         nl, zl, nc = n_(l), nnz_(l), n_(l - 1)
         transfers = (4 * nl + 4 * nc) + (8 * nl + 4 * nc)
@@ -636,9 +637,11 @@ def level_launch_rates(B, levels, level, reps=20):
             out[key] = {"kernel": what, "us": round(us, 2), "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
                         "frac": round(g / PEAK_GBS, 4)}
     if spmv > 0:
-        nb = sum(8 * 4 ** L + 16 * (2 ** L + 1) ** 2 for L in levels)
+        # z (4) and p (8) read, the new p and q (8 each) written, 8 per cell
+        nb = sum(8 * 4 ** L + 28 * (2 ** L + 1) ** 2 for L in levels)
         g = nb / (spmv * 1e-6) / 1e9
-        out["outer_spmv_dot"] = {"kernel": "k_b_outer_spmv_dot<2, weights from cells>: FP64 q = A p, p.q, every slot",
+        out["outer_spmv_dot"] = {"kernel": ("k_b_dir_spmv<weights from cells, float>: FP64 p = z + beta p, q = A p, "
+                                            "p.q, every slot"),
                                  "us": round(spmv, 2),
                                  "algorithmic_bytes": int(nb), "achieved_gbs": round(g, 1),
                                  "frac": round(g / PEAK_GBS, 4)}

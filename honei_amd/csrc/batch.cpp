@@ -16,6 +16,21 @@ namespace {
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// Tiles of k_b_dir_spmv over a side x side grid: the paired passes' tiling
+// (at most kPairWidth columns, kPairRows rows per tile).
+struct DirTiles {
+    int32_t tw, strips;
+    int64_t ntiles;
+};
+DirTiles dir_tiles(int64_t side) {
+    DirTiles t;
+    const int64_t strips = (side + kPairWidth - 1) / kPairWidth;
+    t.strips = (int32_t)strips;
+    t.tw = (int32_t)((side + strips - 1) / strips);
+    t.ntiles = strips * ((side + kPairRows - 1) / kPairRows);
+    return t;
+}
+
 // Launch shapes chosen by the autotuner, per (device, level, value type), for
 // the lifetime of the process: every batch of a process uses one shape per
 // level, so a system's result does not depend on the other systems in its
@@ -265,7 +280,7 @@ uint64_t BatchMG::plan_bytes() const {
     }
     for (const BClass &c : C_) {
         if (stores_outer_values()) b += (uint64_t)c.ax_ld * 4 + (uint64_t)c.n * 4 + (uint64_t)(c.count * c.ax_ld) * 8;
-        b += (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 5 : 3);
+        b += (uint64_t)(c.count * c.vld) * 8 * (cfg_.outer == 1 ? 6 : 3);
         b += (uint64_t)(c.count * 4 * c.bps) * 8;   // partials
     }
     b += (uint64_t)nslots_ * sizeof(SlotScalars);
@@ -365,8 +380,10 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     }
     for (BClass &c : C_) {
         for (double **p : {&c.b, &c.x, &c.d}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
-        if (cfg_.outer == 1)
-            for (double **p : {&c.p, &c.q}) HX(alloc((void **)p, (size_t)(c.count * c.vld) * 8));
+        if (cfg_.outer == 1) {
+            HX(alloc((void **)&c.p, (size_t)(2 * c.count * c.vld) * 8));   // two direction buffers
+            HX(alloc((void **)&c.q, (size_t)(c.count * c.vld) * 8));
+        }
         HX(alloc((void **)&c.partials, (size_t)(c.count * 4 * c.bps) * 8));   // 64-row chunks
         if (!stores_outer_values()) continue;
         HX(alloc((void **)&c.Aj, (size_t)c.ax_ld * 4));
@@ -379,6 +396,8 @@ hipError_t BatchMG::create(const int64_t *levels, const int64_t *counts, int ncl
     HX(alloc((void **)&bounds_, (size_t)(kMaxClasses + 1) * sizeof(int32_t)));
     boundaries_.clear();
     for (const BClass &c : C_) boundaries_.push_back(c.first);
+    dir_tiles_ = 0;
+    for (const BClass &c : C_) dir_tiles_ += c.count * dir_tiles(c.side).ntiles;
     boundaries_.push_back(nslots_);
     HX(b_fill_state(sc_, nslots_, SLOT_IDLE, s_main_));
     HX(hipHostMalloc((void **)&sc_host_, (size_t)nslots_ * sizeof(SlotScalars), hipHostMallocDefault));
@@ -517,6 +536,11 @@ BClassTab BatchMG::class_table() const {
         e.x_top = level_result(top);
         e.kappa = top.kappa;
         e.kld = top.kld;
+        e.palt = c.count * c.vld;
+        const DirTiles dt = dir_tiles(c.side);
+        e.tw = dt.tw;
+        e.strips = dt.strips;
+        e.ntiles = dt.ntiles;
     }
     return t;
 }
@@ -528,7 +552,8 @@ BClassTab BatchMG::class_table() const {
 //   list new / verifying -> r = b - A x for them -> list non-idle -> check
 //   (recursive |r| for running slots) -> list active -> rhs = r/|r| ->
 //   V-cycle -> rho = r.z, beta ->
-//   p = z + beta p -> q = A p, p.q -> alpha -> x += alpha p, r -= alpha q, |r|.
+//   p = z + beta p with q = A p, p.q (one tiled launch) -> alpha ->
+//   x += alpha p, r -= alpha q, |r|.
 hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     const int nb = (int)boundaries_.size();
     const BClassTab tab = class_table();
@@ -549,8 +574,7 @@ hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     if (!cg) return b_outer(OP_ACC, tab, w, blocks, sc_, s);
     HX(b_outer(OP_DOT_RZ, tab, w, blocks, sc_, s));
     HX(b_finish(FIN_BETA, tab, w, nslots_, sc_, 0.0, 0, s));
-    HX(b_outer(OP_PUPDATE, tab, w, blocks, sc_, s));
-    HX(b_outer(OP_SPMV_DOT, tab, w, blocks, sc_, s, mf));
+    HX(b_dir_spmv(tab, w, dir_tiles_, sc_, s, mf));
     HX(b_finish(FIN_ALPHA, tab, w, nslots_, sc_, 0.0, 0, s));
     return b_outer(OP_UPDATE, tab, w, blocks, sc_, s);
 }
@@ -791,9 +815,7 @@ hipError_t BatchMG::time_level(int64_t level, int reps, double us[5]) {
         timeit([&]() { return b_pair(vt, PAIR_POST, lv, w, lv.x, nullptr, lv.t, s_main_, wm); }, &us[3]);
     if (cfg_.outer == 1) {
         const BClassTab tab = class_table();
-        int64_t blocks = 0;
-        for (const BClass &c : C_) blocks += c.count * c.bps;
-        timeit([&]() { return b_outer(OP_SPMV_DOT, tab, wa, blocks, sc_, s_main_, mf); }, &us[4]);
+        timeit([&]() { return b_dir_spmv(tab, wa, dir_tiles_, sc_, s_main_, mf); }, &us[4]);
     }
     (void)hipStreamSynchronize(s_main_);
     if (e0) (void)hipEventDestroy(e0);

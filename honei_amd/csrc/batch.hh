@@ -55,7 +55,7 @@ struct SlotScalars {
     int32_t state;    // SlotState
     int32_t iters;    // outer iterations (V-cycles) applied
     int32_t status;   // SlotStatus
-    int32_t pad;
+    int32_t pcur;     // CG: which of the slot's two direction buffers holds p (low bit)
 };
 
 constexpr int kMaxClasses = 16;
@@ -128,7 +128,7 @@ struct BClass {
     int32_t *Aj = nullptr;
     uint32_t *Arl = nullptr;
     double *Ax = nullptr, *b = nullptr, *x = nullptr, *d = nullptr, *partials = nullptr;
-    double *p = nullptr, *q = nullptr;   // CG direction and A p
+    double *p = nullptr, *q = nullptr;   // CG direction (two buffers, [2][count][vld]) and A p
 };
 
 // Threads-per-row / block-size choice for one level (autotuned, cached per
@@ -175,6 +175,9 @@ struct BClassEntry {
     int32_t top64;      // 1: the inner vectors are double, 0: float
     const double *kappa;   // cell coefficients of the finest level, [global slot][kld]
     int64_t kld;
+    int64_t palt;          // CG: offset of the second direction buffer from p (count * vld)
+    int32_t tw, strips;    // tiles of k_b_dir_spmv: e.tw columns, `strips` tiles across
+    int64_t ntiles;        // tiles per slot (strips * row bands of kPairRows)
 };
 struct BClassTab {
     int32_t nclass;
@@ -187,8 +190,7 @@ enum OuterOp : int32_t {
     OP_SCALE = 2,          // rhs_top = r / |r| in the inner precision
     OP_ACC = 3,            // x = (first cycle ? 0 : x) + |r| * x_top   (defect correction)
     OP_DOT_RZ = 4,         // partial r.z, z = |r| * x_top
-    OP_PUPDATE = 5,        // p = z + beta p
-    OP_SPMV_DOT = 6,       // q = A p, partial p.q
+    // (p = z + beta p with q = A p, p.q: b_dir_spmv)
     OP_UPDATE = 7          // x += alpha p, r -= alpha q, partial |r|^2
 };
 enum FinishOp : int32_t { FIN_CHECK_DC = 0, FIN_CHECK_CG = 1, FIN_BETA = 2, FIN_ALPHA = 3 };
@@ -297,7 +299,8 @@ class BatchMG {
     // defect, us[1] restriction to level - 1, us[2] prolongation + correction
     // from level - 1, us[3] paired post-smoothing (0 where the level does not
     // pair; with the prolongation fused in, us[2] is 0 and us[3] covers
-    // both), us[4] the FP64 q = A p, p.q over every slot of every class.
+    // both), us[4] the FP64 p = z + beta p with q = A p, p.q (b_dir_spmv)
+    // over every slot of every class.
     // Overwrites the pool's vectors: call between steps only (a load
     // rebuilds them, and results must be copied out first).
     hipError_t time_level(int64_t level, int reps, double us[5]);
@@ -353,6 +356,7 @@ class BatchMG {
 
     hipStream_t s_main_ = nullptr, s_load_ = nullptr;
     hipGraph_t graph_ = nullptr, graph1_ = nullptr;      // iters_per_launch / one iteration
+    int64_t dir_tiles_ = 0;   // k_b_dir_spmv tiles of all slots of all classes
     hipGraphExec_t exec_ = nullptr, exec1_ = nullptr;
     std::vector<int> host_state_;      // 0 free, 1 loading, 2 running
     std::vector<int64_t> loading_;
@@ -398,6 +402,11 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
 // the stored FP64 values.
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s, bool mf = false);
+// CG: p = z + beta p into the slot's other direction buffer and q = A p with
+// one partial p.q per tile (k_b_dir_spmv), over the active slots of all
+// classes; max_tiles: the tiles of all slots of all classes.
+hipError_t b_dir_spmv(const BClassTab &tab, const BWork &w, int64_t max_tiles, const SlotScalars *sc, hipStream_t s,
+                      bool mf);
 // One block per listed slot: reduce its partials and update its scalars.
 hipError_t b_finish(FinishOp op, const BClassTab &tab, const BWork &w, int64_t max_slots, SlotScalars *sc,
                     double tol, int32_t max_outer, hipStream_t s);

@@ -419,7 +419,7 @@ __device__ inline void q1_stencil_sum_group(int64_t side, int64_t row, const dou
 // the launches use G = kOuterRowGroup.
 template <int OP, int G, bool DC, bool MF>
 __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, const SlotScalars *sc) {
-    static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE, "defect op (q = A p: k_b_outer_spmv_dot)");
+    static_assert(OP == OP_DEFECT_ALL || OP == OP_DEFECT_SETTLE, "defect op (q = A p: k_b_dir_spmv)");
     __shared__ int64_t pre[kMaxClasses + 1];
     if (threadIdx.x == 0) {
         int64_t acc = 0;
@@ -484,83 +484,101 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
     }
 }
 
-// q = A p with the CG partials of p.q (one wavefront sum per 64-row chunk,
-// the layout of k_b_outer_rows): logical block = R consecutive 256-row blocks
-// of one listed slot, thread t takes row t of each with every load of the R
-// rows issued before any sum (rows past the end load the last row and store
-// nothing).  The block's class, slot and first row are uniform (scalar
-// registers); a row's grid position comes from the block's first row by one
-// multiply-high with the class's reciprocal of the side.  Every row forms the
-// interior stencil sum (a boundary row at clamped, in-range addresses) and a
-// boundary row then takes 0 + x[row], its identity row's sum: the terms and
-// order of the stored operator's row sums, hence the same bits.
-template <int R, bool MF>
-__global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w) {
+// CG direction and q = A p in one tiled launch.  Logical block = one tile of
+// one listed slot's finest grid (kPairRows rows by e.tw columns, the tiling
+// of the paired passes), with a one-node halo: every thread forms
+// p = z + beta p_old at its node (z = |r| x_top; the node clamped into the
+// grid) from the same two loads and expression wherever that node is
+// formed, so the tiles agree on every p; the tile's own nodes store it into
+// the slot's other direction buffer (the current one stays intact for the
+// neighbouring tiles' halos).  After a barrier the own nodes form q = A p
+// from LDS with the row sums of k_b_outer_rows (interior: the nine terms in
+// column order; boundary: 0 + p), store q, and the tile stores one partial
+// p.q (wave sums, then its wavefronts in order) at its tile index.
+// FIN_ALPHA sums the slot's e.ntiles partials and makes the new buffer
+// current.  Z: the inner vectors' type.
+template <bool MF, typename Z>
+__global__ __launch_bounds__(kPairTileRows * kWave) void k_b_dir_spmv(BClassTab tab, BWork w,
+                                                                     const SlotScalars *sc) {
     __shared__ int64_t pre[kMaxClasses + 1];
+    __shared__ double sp[kPairTileRows][kWave];
+    __shared__ double red[kPairTileRows];
     if (threadIdx.x == 0) {
         int64_t acc = 0;
         pre[0] = 0;
         for (int c = 0; c < tab.nclass; ++c) {
-            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * ((tab.c[c].bps + R - 1) / R);
+            acc += (int64_t)(w.bounds[c + 1] - w.bounds[c]) * tab.c[c].ntiles;
             pre[c + 1] = acc;
         }
     }
     __syncthreads();
+    const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t total = pre[tab.nclass];
     for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
         int c = 0;
         while (lb >= pre[c + 1]) ++c;
-        c = __builtin_amdgcn_readfirstlane(c);
         const BClassEntry &e = tab.c[c];
-        int64_t gi, hb;   // listed slot of the class, group of R row blocks
-        div32(lb - pre[c], (e.bps + R - 1) / R, gi, hb);
-        gi = __builtin_amdgcn_readfirstlane((int32_t)gi);
-        hb = __builtin_amdgcn_readfirstlane((int32_t)hb);
-        const int64_t ls = w.list[w.bounds[c] + gi] - e.first;
-        const double *ax = e.Ax + ls * e.ax_ld;
-        const double *kap = MF ? e.kappa + (ls + e.first) * e.kld : nullptr;
-        const double *pv = e.p + ls * e.vld;
-        const uint32_t S = (uint32_t)e.side, n = (uint32_t)e.n;
-        const uint32_t base = (uint32_t)(hb * R * 256);
-        const uint32_t gr0 = base / S, gc0 = base - gr0 * S;
-        double av[R][9], xv[R][9], pr[R];
-        bool bnd[R];
+        int64_t si, tile, seg, strip;   // listed slot of the class, tile of the slot
+        div32(lb - pre[c], e.ntiles, si, tile);
+        div32(tile, e.strips, seg, strip);
+        const int64_t g = w.list[w.bounds[c] + si];
+        const int64_t ls = g - e.first;
+        const SlotScalars &st = sc[g];
+        const double beta = st.beta, scale = st.scale;
+        const int64_t cur = st.pcur & 1;
+        const int S = (int)e.side;
+        const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
+        const int c0 = (int)strip * e.tw, c1 = min(S, c0 + e.tw);
+        const int r = r0 - 1 + i, col = c0 - 1 + lane;
+        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);
+        const int colc = col < 0 ? 0 : (col > S - 1 ? S - 1 : col);
+        const int64_t node = (int64_t)rc * S + colc;
+        const bool own = i >= 1 && r < r1 && lane >= 1 && col < c1;
+        const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
+        // interior position of the row's weights (a boundary node's: unused)
+        const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
+        const double *pold = e.p + ls * e.vld + cur * e.palt;
+        double *pnew = e.p + ls * e.vld + (1 - cur) * e.palt;
+        // ---- every global load issued before any use ----
+        const double zv = (double)((const Z *)e.x_top)[g * e.vld_top + node];
+        const double po = beta == 0.0 ? 0.0 : pold[node];
+        double av[9];
+        double cw[4];
+        if constexpr (MF) {
+            const uint32_t nc = (uint32_t)S - 1u;
+            const double *cs = e.kappa + g * e.kld + ((ir - 1u) * nc + (ic - 1u));
+            cw[0] = cs[0];
+            cw[1] = cs[1];
+            cw[2] = cs[nc];
+            cw[3] = cs[nc + 1];
+        } else {
+            const double *ax = e.Ax + ls * e.ax_ld + ((int64_t)ir * S + ic);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t row0 = base + (uint32_t)(r * 256) + threadIdx.x;
-            const uint32_t row = row0 < n ? row0 : n - 1;
-            const uint32_t v = gc0 + (row - base);
-            const uint32_t q = __umulhi(v, e.side_magic);
-            const uint32_t gr = gr0 + q, gc = v - q * S;
-            bnd[r] = gr == 0 || gr == S - 1 || gc == 0 || gc == S - 1;
-            // interior position used for the addresses (a boundary row's are unused)
-            const uint32_t ir = gr < 1 ? 1 : (gr > S - 2 ? S - 2 : gr);
-            const uint32_t ic = gc < 1 ? 1 : (gc > S - 2 ? S - 2 : gc);
-            const uint32_t irow = ir * S + ic;
-            if constexpr (MF) q1_coeff_weights<double>(S, ir, ic, kap, av[r]);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const uint32_t j = irow + (uint32_t)((k / 3) * S + (k % 3)) - (S + 1);
-                if constexpr (!MF) av[r][k] = load_value_nt(ax + irow + (int64_t)k * e.stride);
-                xv[r][k] = pv[j];
-            }
-            pr[r] = pv[row];
+            for (int k = 0; k < 9; ++k) av[k] = load_value_nt(ax + (int64_t)k * e.stride);
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t row = base + (uint32_t)(r * 256) + threadIdx.x;
+        // ---- p at the tile and its halo ----
+        const double z = zv * scale;
+        const double pv = beta == 0.0 ? z : z + beta * po;
+        sp[i][lane] = pv;
+        if (own) pnew[node] = pv;
+        if constexpr (MF) q1_cell_weights<double>(cw[0], cw[1], cw[2], cw[3], av);
+        __syncthreads();
+        // ---- q = A p at the own nodes, the tile's p.q ----
+        double part = 0.0;
+        if (own) {
             double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) t = t + av[r][k] * xv[r][k];
-            if (bnd[r]) t = 0.0 + pr[r];
-            double part = 0.0;
-            if (row < n) {
-                e.q[ls * e.vld + row] = t;
-                part = pr[r] * t;
-            }
-            const int64_t chunk = (hb * R + r) * (256 / kWave) + threadIdx.x / kWave;
-            const double v = wave_sum(part);
-            if (threadIdx.x % kWave == 0 && chunk < e.nps) e.partials[ls * e.pst + chunk] = v;
+            for (int k = 0; k < 9; ++k) t = t + av[k] * sp[i + (k / 3 - 1)][lane + (k % 3 - 1)];
+            if (bnd) t = 0.0 + pv;
+            e.q[ls * e.vld + node] = t;
+            part = pv * t;
+        }
+        const double v = wave_sum(part);
+        if (lane == 0) red[i] = v;
+        __syncthreads();
+        if (i == 0) {
+            const double t = wave_sum(lane < kPairTileRows ? red[lane] : 0.0);
+            if (lane == 0) e.partials[ls * e.pst + tile] = t;
         }
     }
 }
@@ -572,7 +590,7 @@ __global__ __launch_bounds__(256) void k_b_outer_spmv_dot(BClassTab tab, BWork w
 // Z: the inner vectors' type (float, or double for an FP64 inner cycle).
 template <int OP, int R, typename Z>
 __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, const SlotScalars *sc) {
-    static_assert(OP == OP_SCALE || OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_PUPDATE || OP == OP_UPDATE,
+    static_assert(OP == OP_SCALE || OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_UPDATE,
                   "element-wise op");
     constexpr bool REDUCE = OP == OP_DOT_RZ || OP == OP_UPDATE;
     __shared__ int64_t pre[kMaxClasses + 1];
@@ -595,7 +613,8 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
         const int64_t g = w.list[w.bounds[c] + si];
         const int64_t ls = g - e.first;
         const SlotScalars &st = sc[g];
-        const double *rs = e.r + ls * e.vld, *ps = e.p + ls * e.vld, *qs = e.q + ls * e.vld;
+        const double *rs = e.r + ls * e.vld, *qs = e.q + ls * e.vld;
+        const double *ps = e.p + ls * e.vld + (OP == OP_UPDATE ? (int64_t)(st.pcur & 1) * e.palt : 0);
         const double *xs = e.x + ls * e.vld;
         const Z *zs = (const Z *)e.x_top + g * e.vld_top;
         int64_t row[R];
@@ -607,8 +626,8 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
             in[j] = row[j] < e.n;
             const int64_t rc = in[j] ? row[j] : e.n - 1;
             if constexpr (OP == OP_SCALE || OP == OP_DOT_RZ || OP == OP_UPDATE) rv[j] = rs[rc];
-            if constexpr (OP == OP_ACC || OP == OP_DOT_RZ || OP == OP_PUPDATE) zv[j] = (double)zs[rc];
-            if constexpr (OP == OP_PUPDATE || OP == OP_UPDATE) pv[j] = ps[rc];
+            if constexpr (OP == OP_ACC || OP == OP_DOT_RZ) zv[j] = (double)zs[rc];
+            if constexpr (OP == OP_UPDATE) pv[j] = ps[rc];
             if constexpr (OP == OP_UPDATE) qv[j] = qs[rc];
             if constexpr (OP == OP_ACC || OP == OP_UPDATE) xo[j] = xs[rc];
         }
@@ -623,9 +642,6 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
                 if (in[j]) e.x[vi] = base + zv[j] * st.scale;
             } else if constexpr (OP == OP_DOT_RZ) {
                 if (in[j]) part[j] = rv[j] * (zv[j] * st.scale);
-            } else if constexpr (OP == OP_PUPDATE) {
-                const double z = zv[j] * st.scale;
-                if (in[j]) e.p[vi] = st.beta == 0.0 ? z : z + st.beta * pv[j];
             } else if constexpr (OP == OP_UPDATE) {
                 const double r = rv[j] - st.alpha * qv[j];
                 if (in[j]) {
@@ -665,7 +681,8 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
         while (c + 1 < tab.nclass && g >= tab.c[c + 1].first) ++c;
         const BClassEntry &e = tab.c[c];
         const int64_t ls = g - e.first;
-        const double h = sum_partials(e.partials + ls * e.pst, OP == FIN_CHECK_DC ? e.bps : e.nps, 1, 0);
+        const int64_t np = OP == FIN_CHECK_DC ? e.bps : (OP == FIN_ALPHA ? e.ntiles : e.nps);
+        const double h = sum_partials(e.partials + ls * e.pst, np, 1, 0);
         if (threadIdx.x == 0) {
             SlotScalars s = sc[g];
             if constexpr (OP == FIN_CHECK_DC || OP == FIN_CHECK_CG) {
@@ -715,6 +732,7 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
                 s.rho = h;
             } else {
                 const double ap = h;
+                s.pcur ^= 1;   // the direction just formed (k_b_dir_spmv) is now current
                 if (ap > 0.0 && isfinite(ap) && s.rho > 0.0) {
                     s.alpha = s.rho / ap;
                 } else {                           // breakdown: give up on this slot
@@ -1189,7 +1207,7 @@ hipError_t b_ell(ValueType vt, int T, int block, int group, EllMode mode, const 
 
 // Slots per logical block of the row-sum launches, 256-row blocks per
 // logical block of the element-wise launches.
-constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4, kOuterSpmvBlocks = 2;
+constexpr int kOuterRowGroup = 1, kOuterElemBlocks = 4;
 
 hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max_blocks, const SlotScalars *sc,
                    hipStream_t s, bool mf) {
@@ -1200,7 +1218,6 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
             if (!tab.c[c].kappa || tab.c[c].kld < (tab.c[c].side - 1) * (tab.c[c].side - 1)) return hipErrorInvalidValue;
     constexpr int G = kOuterRowGroup, R = kOuterElemBlocks;
     const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)), ge(batch_grid((max_blocks + R - 1) / R, 256));
-    const dim3 gs(batch_grid((max_blocks + kOuterSpmvBlocks - 1) / kOuterSpmvBlocks, 256));
     switch (op) {
         case OP_DEFECT_ALL:
             if (mf)
@@ -1214,12 +1231,6 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
             else
                 hipLaunchKernelGGL((k_b_outer_rows<OP_DEFECT_SETTLE, G, false, false>), gr, block, 0, s, tab, w, sc);
             break;
-        case OP_SPMV_DOT:
-            if (mf)
-                hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks, true>), gs, block, 0, s, tab, w);
-            else
-                hipLaunchKernelGGL((k_b_outer_spmv_dot<kOuterSpmvBlocks, false>), gs, block, 0, s, tab, w);
-            break;
 #define ELEM_LAUNCH(OPV)                                                                                 \
     do {                                                                                                 \
         if (tab.c[0].top64)                                                                              \
@@ -1230,10 +1241,37 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
         case OP_SCALE: ELEM_LAUNCH(OP_SCALE); break;
         case OP_ACC: ELEM_LAUNCH(OP_ACC); break;
         case OP_DOT_RZ: ELEM_LAUNCH(OP_DOT_RZ); break;
-        case OP_PUPDATE: ELEM_LAUNCH(OP_PUPDATE); break;
         case OP_UPDATE: ELEM_LAUNCH(OP_UPDATE); break;
 #undef ELEM_LAUNCH
         default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t b_dir_spmv(const BClassTab &tab, const BWork &w, int64_t max_tiles, const SlotScalars *sc, hipStream_t s,
+                      bool mf) {
+    if (max_tiles <= 0) return hipSuccess;
+    if (max_tiles > kMaxLogicalBlocks) return hipErrorInvalidValue;
+    for (int c = 0; c < tab.nclass; ++c) {
+        const BClassEntry &e = tab.c[c];
+        const int64_t strips = (e.side + kPairWidth - 1) / kPairWidth;
+        if (e.side < 3 || e.side > 65535 || e.strips != strips || e.tw != (e.side + strips - 1) / strips ||
+            e.ntiles != strips * ((e.side + kPairRows - 1) / kPairRows) || e.ntiles > e.pst || !e.p || !e.q ||
+            (mf && (!e.kappa || e.kld < (e.side - 1) * (e.side - 1))) || (!mf && !e.Ax))
+            return hipErrorInvalidValue;
+    }
+    const int threads = kPairTileRows * kWave;
+    const dim3 grid(batch_grid(max_tiles, threads)), block(threads);
+    if (tab.c[0].top64) {
+        if (mf)
+            hipLaunchKernelGGL((k_b_dir_spmv<true, double>), grid, block, 0, s, tab, w, sc);
+        else
+            hipLaunchKernelGGL((k_b_dir_spmv<false, double>), grid, block, 0, s, tab, w, sc);
+    } else {
+        if (mf)
+            hipLaunchKernelGGL((k_b_dir_spmv<true, float>), grid, block, 0, s, tab, w, sc);
+        else
+            hipLaunchKernelGGL((k_b_dir_spmv<false, float>), grid, block, 0, s, tab, w, sc);
     }
     return hipGetLastError();
 }

@@ -22,7 +22,7 @@ uint64_t slot_bytes(int64_t level, ValueType inner, int outer_method, bool paylo
     }
     const uint64_t side = (1ull << level) + 1, n = side * side;
     const uint64_t stride = 32 * ((n + 31) / 32), vld = (n + 63) / 64 * 64;
-    b += 9 * stride * 8 + vld * 8 * (outer_method == 1 ? 5 : 3) + 4 * ((n + 255) / 256) * 8 + sizeof(SlotScalars) + 8;
+    b += 9 * stride * 8 + vld * 8 * (outer_method == 1 ? 6 : 3) + 4 * ((n + 255) / 256) * 8 + sizeof(SlotScalars) + 8;
     if (payload) b += ((side - 1) * (side - 1) + n) * 8;
     return b;
 }

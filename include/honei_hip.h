@@ -531,7 +531,8 @@ int honei_hip_batch_pair_timing(void *batch, int32_t level, double *pre_us, doub
  * level - 1, us[2] prolongation + correction from level - 1, us[3] paired
  * two-sweep launch (0 where the level does not pair; where it pairs, the
  * prolongation + correction runs inside it and us[2] is 0), us[4] the FP64
- * q = A p, p.q over every slot of every class (CG outer).  Overwrites the
+ * p = z + beta p with q = A p, p.q (one tiled launch) over every slot of
+ * every class (CG outer).  Overwrites the
  * pool's vectors and solutions: only between steps, with no slot loading or
  * running, after the results are copied out.  Synchronous.               */
 int honei_hip_batch_time_level(void *batch, int32_t level, int32_t reps, double *us);
