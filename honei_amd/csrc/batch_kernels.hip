@@ -182,6 +182,14 @@ struct PairGeom {
     int32_t side, width, strips, segs;
 };
 
+// Tiled launches stage the slots of a work-group's tiles in LDS at the
+// start; their grids give no work-group more than kTileSlots tiles.
+constexpr int kTileSlots = 1024;
+inline int tiled_grid(int64_t units, int threads) {
+    const int64_t g = batch_grid(units, threads), need = (units + kTileSlots - 1) / kTileSlots;
+    return (int)(g > need ? g : need);
+}
+
 // The four coarse values around fine node (R, C) of a (2 m + 1)^2 level
 // (coarse side nc = m + 1): (R/2, C/2), (R/2, (C+1)/2), ((R+1)/2, C/2),
 // ((R+1)/2, (C+1)/2), rounded down.
@@ -233,7 +241,16 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
     const int64_t ups = (int64_t)pg.strips * pg.segs;
     const int64_t total = na * ups;
     const int S = pg.side;
-    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+    // the slots of this work-group's tiles (at most kTileSlots: the launcher
+    // sizes the grid so), loaded at once: one list round trip per launch
+    // instead of one per tile
+    __shared__ int32_t tslot[kTileSlots];
+    const int64_t lb0 = first_block();
+    const int64_t mine = lb0 < total ? (total - 1 - lb0) / gridDim.x + 1 : 0;
+    for (int64_t k = threadIdx.x; k < mine; k += blockDim.x) tslot[k] = w.list[a0 + (lb0 + k * gridDim.x) / ups];
+    __syncthreads();
+    int64_t k = 0;
+    for (int64_t lb = lb0; lb < total; lb += gridDim.x, ++k) {
         int64_t e, rem, strip, seg;
         div32(lb, ups, e, rem);
         // strips minor: the tiles side by side in a band of rows are
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         // cache lines their rows share (a row of a tile starts at an arbitrary
         // column) are fetched once into that XCD's L2
         div32(rem, pg.strips, seg, strip);
-        const int64_t ls = w.list[a0 + e] - w.base;
+        const int64_t ls = tslot[k] - w.base;
         const int64_t vo = ls * m.vld;
         const VT *ax = (const VT *)m.Ax + ls * m.ax_ld;
         const V *xs = xin + vo, *rs = rhs + vo, *ds = dinv + vo;
@@ -514,18 +531,38 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_dir_spmv(BClassTab 
     __syncthreads();
     const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t total = pre[tab.nclass];
-    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+    // the slots and scalars of this work-group's tiles (at most kTileSlots:
+    // the launcher sizes the grid so), loaded at once: two round trips per
+    // launch instead of two per tile
+    __shared__ int32_t tslot[kTileSlots];
+    __shared__ double tbeta[kTileSlots], tscale[kTileSlots];
+    __shared__ uint8_t tcur[kTileSlots];
+    const int64_t lb0 = first_block();
+    const int64_t mine = lb0 < total ? (total - 1 - lb0) / gridDim.x + 1 : 0;
+    for (int64_t k = threadIdx.x; k < mine; k += blockDim.x) {
+        const int64_t lb = lb0 + k * gridDim.x;
+        int c = 0;
+        while (lb >= pre[c + 1]) ++c;
+        const int64_t g = w.list[w.bounds[c] + (lb - pre[c]) / tab.c[c].ntiles];
+        const SlotScalars &st = sc[g];
+        tslot[k] = (int32_t)g;
+        tbeta[k] = st.beta;
+        tscale[k] = st.scale;
+        tcur[k] = (uint8_t)(st.pcur & 1);
+    }
+    __syncthreads();
+    int64_t kt = 0;
+    for (int64_t lb = lb0; lb < total; lb += gridDim.x, ++kt) {
         int c = 0;
         while (lb >= pre[c + 1]) ++c;
         const BClassEntry &e = tab.c[c];
         int64_t si, tile, seg, strip;   // listed slot of the class, tile of the slot
         div32(lb - pre[c], e.ntiles, si, tile);
         div32(tile, e.strips, seg, strip);
-        const int64_t g = w.list[w.bounds[c] + si];
+        const int64_t g = tslot[kt];
         const int64_t ls = g - e.first;
-        const SlotScalars &st = sc[g];
-        const double beta = st.beta, scale = st.scale;
-        const int64_t cur = st.pcur & 1;
+        const double beta = tbeta[kt], scale = tscale[kt];
+        const int64_t cur = tcur[kt];
         const int S = (int)e.side;
         const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
         const int c0 = (int)strip * e.tw, c1 = min(S, c0 + e.tw);
@@ -1261,7 +1298,7 @@ hipError_t b_dir_spmv(const BClassTab &tab, const BWork &w, int64_t max_tiles, c
             return hipErrorInvalidValue;
     }
     const int threads = kPairTileRows * kWave;
-    const dim3 grid(batch_grid(max_tiles, threads)), block(threads);
+    const dim3 grid(tiled_grid(max_tiles, threads)), block(threads);
     if (tab.c[0].top64) {
         if (mf)
             hipLaunchKernelGGL((k_b_dir_spmv<true, double>), grid, block, 0, s, tab, w, sc);
@@ -1369,7 +1406,7 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void
     if (units > kMaxLogicalBlocks) return hipErrorInvalidValue;
     const BMat m{lv.Aj, lv.Arl, lv.Ax, lv.stride, lv.ax_ld, lv.n, lv.vld, 0, lv.kappa, lv.kld, lv.st};
     const int threads = kPairTileRows * kWave;
-    const dim3 grid(batch_grid(units, threads)), block(threads);
+    const dim3 grid(tiled_grid(units, threads)), block(threads);
     const void *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
 #define PAIR_LAUNCH(VT, P, MFV) \
     pair_launch<VT, P, MFV>(grid, block, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv, xc, vld_c)
