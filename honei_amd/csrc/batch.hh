@@ -452,12 +452,19 @@ struct FusedLevel {
     int T;
     void *dinv, *rhs, *x, *d, *t;   // inner vector type
 };
+// Byte offsets of a level's arrays in the LDS form of the fused cycle.
+struct FusedLds {
+    int32_t aj, arl, ax, dinv, rhs, x, d, t;
+};
 struct FusedCycle {
     int top, n_pre, n_post;
     FusedLevel lv[kMaxFusedLevels + 1];   // index = level (1..top)
     void *g, *u, *en;                     // level-1 CG scratch
     double konv;
     int max_iters;
+    // LDS form (set by b_coarse_vcycle): per level, and the CG scratch
+    FusedLds lo[kMaxFusedLevels + 1];
+    int32_t og, ou, oen;
 };
 hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, int64_t max_slots, hipStream_t s);
 

@@ -1014,73 +1014,140 @@ __device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, V *out, co
     __syncthreads();
 }
 
+// Level l of the cycle: fc's arrays, or (lds non-null) the LDS copies at
+// the offsets fc.lo[l] (a copy of one level, so no runtime-indexed local array).
+__device__ inline FusedLevel level_view(const FusedCycle &fc, int l, char *lds) {
+    FusedLevel L = fc.lv[l];
+    if (lds) {
+        const FusedLds &o = fc.lo[l];
+        L.Aj = (const int32_t *)(lds + o.aj);
+        L.Arl = (const uint32_t *)(lds + o.arl);
+        L.Ax = lds + o.ax;
+        L.dinv = lds + o.dinv;
+        L.rhs = lds + o.rhs;
+        L.x = lds + o.x;
+        L.d = lds + o.d;
+        L.t = lds + o.t;
+    }
+    return L;
+}
+
+// One V-cycle of slot g from level `top` down to the coarse CG and back
+// (fc's arrays indexed by slot g, or with lds the LDS copies, g = 0).
+template <typename VT>
+__device__ inline void fused_cycle_body(const FusedCycle &fc, int64_t g, int top, char *lds) {
+    typedef typename VecOf<VT>::type V;
+    const Mask8 mk{0xffu};   // all Dirichlet, as the level-by-level transfers
+    // down: n_pre sweeps from zero (first two fused), defect, restriction
+    for (int l = top; l >= 2; --l) {
+        const FusedLevel L = level_view(fc, l, lds);
+        V *Lx = (V *)L.x, *Lt = (V *)L.t, *Ld = (V *)L.d;
+        const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
+        const int rest = fc.n_pre - 2;
+        V *src = (rest % 2 == 0) ? Lx : Lt;
+        fused_sweep_t<VT, ELL_JACOBI0>(L, g, src, (const V *)nullptr, Lr, Ldi);
+        for (int k = 0; k < rest; ++k) {
+            V *dst = (src == Lx) ? Lt : Lx;
+            fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
+            src = dst;
+        }
+        fused_sweep_t<VT, ELL_DEFECT>(L, g, Ld, Lx, Lr, (const V *)nullptr);
+        const FusedLevel C = level_view(fc, l - 1, lds);
+        const int64_t m = C.side - 1;
+        for (int64_t i = threadIdx.x; i < C.n; i += blockDim.x)
+            restrict_node<V>((V *)C.rhs + g * C.vld, Ld + g * L.vld, m, mk, i);
+        __syncthreads();
+    }
+    {   // coarse CG on level 1
+        const FusedLevel L = level_view(fc, 1, lds);
+        EllView A{};
+        A.rows = L.n;
+        A.cols = L.n;
+        A.stride = L.stride;
+        A.ncpr = L.ncpr;
+        A.threads = L.T;
+        A.Aj = L.Aj;
+        A.Ax = (const VT *)L.Ax + g * L.ax_ld;
+        A.Arl = L.Arl;
+        const int64_t o = g * L.vld;
+        V *cg = (V *)(lds ? lds + fc.og : fc.g), *cu = (V *)(lds ? lds + fc.ou : fc.u);
+        V *cen = (V *)(lds ? lds + fc.oen : fc.en);
+        (void)coarse_cg_block<VT>(A, (const V *)L.rhs + o, (V *)L.x + o, cg + o, cu + o, cen + o, fc.konv,
+                                  fc.max_iters);
+        __syncthreads();
+    }
+    // up: x += P x_coarse, n_post sweeps (result in x)
+    for (int l = 2; l <= top; ++l) {
+        const FusedLevel L = level_view(fc, l, lds);
+        const FusedLevel C = level_view(fc, l - 1, lds);
+        V *Lx = (V *)L.x, *Lt = (V *)L.t;
+        const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
+        const int64_t m = C.side - 1;
+        for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x)
+            prolong_node<V, true>(nullptr, (const V *)C.x + g * C.vld, m, mk, Lx + g * L.vld, V(1), i);
+        __syncthreads();
+        V *src = Lx, *dst = Lt;
+        for (int k = 0; k < fc.n_post; ++k) {
+            fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
+            V *tmp = src;
+            src = dst;
+            dst = tmp;
+        }
+        if (src != Lx) {
+            for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x) Lx[g * L.vld + i] = src[g * L.vld + i];
+            __syncthreads();
+        }
+    }
+}
+
 template <typename VT>
 __global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w) {
-    typedef typename VecOf<VT>::type V;
     const int64_t a0 = active_first(w), na = active_count(w);
-    const Mask8 mk{0xffu};   // all Dirichlet, as the level-by-level transfers
     for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
         const int64_t g = w.list[a0 + a];
         int top = fc.top;
         while (top > 1 && g >= fc.lv[top].cap) --top;   // slots whose finest level is lower
-        // down: n_pre sweeps from zero (first two fused), defect, restriction
-        for (int l = top; l >= 2; --l) {
+        fused_cycle_body<VT>(fc, g, top, nullptr);
+    }
+}
+
+// The same cycle with the slot's levels in LDS (dynamic LDS, offsets
+// fc.lo, fc.og/ou/oen): the index arrays once per launch, then per slot its
+// values, dinv and finest right-hand side in, the cycle on the LDS copies
+// (slot index 0, the same code and arithmetic, so the same bits), and the
+// finest level's x back out.  Every phase then waits on LDS instead of an L2
+// round trip.
+__device__ inline void lds_copy_words(void *dst, const void *src, int64_t bytes) {
+    uint32_t *d = (uint32_t *)dst;
+    const uint32_t *s = (const uint32_t *)src;
+    for (int64_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+}
+template <typename VT>
+__global__ __launch_bounds__(256) void k_b_coarse_vcycle_lds(FusedCycle fc, BWork w) {
+    typedef typename VecOf<VT>::type V;
+    extern __shared__ __align__(16) char lds[];
+    const int64_t a0 = active_first(w), na = active_count(w);
+    if ((int64_t)blockIdx.x >= na) return;   // uniform per work-group
+    for (int l = 1; l <= fc.top; ++l) {
+        const FusedLevel &L = fc.lv[l];
+        lds_copy_words(lds + fc.lo[l].aj, L.Aj, 4 * L.ncpr * L.stride);
+        lds_copy_words(lds + fc.lo[l].arl, L.Arl, 4 * L.n);
+    }
+    for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
+        const int64_t g = w.list[a0 + a];
+        int top = fc.top;
+        while (top > 1 && g >= fc.lv[top].cap) --top;   // slots whose finest level is lower
+        for (int l = 1; l <= top; ++l) {
             const FusedLevel &L = fc.lv[l];
-            V *Lx = (V *)L.x, *Lt = (V *)L.t, *Ld = (V *)L.d;
-            const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
-            const int rest = fc.n_pre - 2;
-            V *src = (rest % 2 == 0) ? Lx : Lt;
-            fused_sweep_t<VT, ELL_JACOBI0>(L, g, src, (const V *)nullptr, Lr, Ldi);
-            for (int k = 0; k < rest; ++k) {
-                V *dst = (src == Lx) ? Lt : Lx;
-                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
-                src = dst;
-            }
-            fused_sweep_t<VT, ELL_DEFECT>(L, g, Ld, Lx, Lr, (const V *)nullptr);
-            const FusedLevel &C = fc.lv[l - 1];
-            const int64_t m = C.side - 1;
-            for (int64_t i = threadIdx.x; i < C.n; i += blockDim.x)
-                restrict_node<V>((V *)C.rhs + g * C.vld, Ld + g * L.vld, m, mk, i);
-            __syncthreads();
+            lds_copy_words(lds + fc.lo[l].ax, (const VT *)L.Ax + g * L.ax_ld, (int64_t)sizeof(VT) * L.ax_ld);
+            lds_copy_words(lds + fc.lo[l].dinv, (const V *)L.dinv + g * L.vld, (int64_t)sizeof(V) * L.n);
         }
-        {   // coarse CG on level 1
-            const FusedLevel &L = fc.lv[1];
-            EllView A{};
-            A.rows = L.n;
-            A.cols = L.n;
-            A.stride = L.stride;
-            A.ncpr = L.ncpr;
-            A.threads = L.T;
-            A.Aj = L.Aj;
-            A.Ax = (const VT *)L.Ax + g * L.ax_ld;
-            A.Arl = L.Arl;
-            const int64_t o = g * L.vld;
-            (void)coarse_cg_block<VT>(A, (const V *)L.rhs + o, (V *)L.x + o, (V *)fc.g + o, (V *)fc.u + o,
-                                      (V *)fc.en + o, fc.konv, fc.max_iters);
-            __syncthreads();
-        }
-        // up: x += P x_coarse, n_post sweeps (result in x)
-        for (int l = 2; l <= top; ++l) {
-            const FusedLevel &L = fc.lv[l];
-            const FusedLevel &C = fc.lv[l - 1];
-            V *Lx = (V *)L.x, *Lt = (V *)L.t;
-            const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
-            const int64_t m = C.side - 1;
-            for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x)
-                prolong_node<V, true>(nullptr, (const V *)C.x + g * C.vld, m, mk, Lx + g * L.vld, V(1), i);
-            __syncthreads();
-            V *src = Lx, *dst = Lt;
-            for (int k = 0; k < fc.n_post; ++k) {
-                fused_sweep_t<VT, ELL_JACOBI>(L, g, dst, src, Lr, Ldi);
-                V *tmp = src;
-                src = dst;
-                dst = tmp;
-            }
-            if (src != Lx) {
-                for (int64_t i = threadIdx.x; i < L.n; i += blockDim.x) Lx[g * L.vld + i] = src[g * L.vld + i];
-                __syncthreads();
-            }
-        }
+        lds_copy_words(lds + fc.lo[top].rhs, (const V *)fc.lv[top].rhs + g * fc.lv[top].vld,
+                       (int64_t)sizeof(V) * fc.lv[top].n);
+        __syncthreads();
+        fused_cycle_body<VT>(fc, 0, top, lds);
+        lds_copy_words((V *)fc.lv[top].x + g * fc.lv[top].vld, lds + fc.lo[top].x, (int64_t)sizeof(V) * fc.lv[top].n);
+        __syncthreads();
     }
 }
 
@@ -1492,14 +1559,48 @@ hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, i
             return hipErrorInvalidValue;
     }
     const int grid = (int)(max_slots < 2048 ? max_slots : 2048);
+    // the LDS form when every level of the cycle fits in 60 KiB (with the
+    // kernel's static LDS, within a work-group's 64 KiB)
+    FusedCycle f = fc;
+    const size_t vs = vt_size(vt), es = vt == VT_F64 ? 8 : 4;
+    int64_t at = 0;
+    auto take = [&](int64_t bytes) {
+        const int64_t o = at;
+        at += (bytes + 15) / 16 * 16;
+        return (int32_t)o;
+    };
+    for (int l = 1; l <= f.top; ++l) {
+        const FusedLevel &L = f.lv[l];
+        FusedLds &o = f.lo[l];
+        o.aj = take(4 * L.ncpr * L.stride);
+        o.arl = take(4 * L.n);
+        o.ax = take((int64_t)vs * L.ax_ld);
+        o.dinv = take((int64_t)es * L.n);
+        o.rhs = take((int64_t)es * L.n);
+        o.x = take((int64_t)es * L.n);
+        o.d = take((int64_t)es * L.n);
+        o.t = take((int64_t)es * L.n);
+    }
+    f.og = take((int64_t)es * f.lv[1].n);
+    f.ou = take((int64_t)es * f.lv[1].n);
+    f.oen = take((int64_t)es * f.lv[1].n);
+    const bool lds = at <= 60 * 1024;
+#define CYCLE_LAUNCH(VT)                                                                                  \
+    do {                                                                                                  \
+        if (lds)                                                                                          \
+            hipLaunchKernelGGL(k_b_coarse_vcycle_lds<VT>, dim3(grid), dim3(256), (size_t)at, s, f, w);   \
+        else                                                                                              \
+            hipLaunchKernelGGL(k_b_coarse_vcycle<VT>, dim3(grid), dim3(256), 0, s, f, w);                \
+    } while (0)
     if (vt == VT_F32)
-        hipLaunchKernelGGL(k_b_coarse_vcycle<float>, dim3(grid), dim3(256), 0, s, fc, w);
+        CYCLE_LAUNCH(float);
     else if (vt == VT_BF16)
-        hipLaunchKernelGGL(k_b_coarse_vcycle<bf16>, dim3(grid), dim3(256), 0, s, fc, w);
+        CYCLE_LAUNCH(bf16);
     else if (vt == VT_F64)
-        hipLaunchKernelGGL(k_b_coarse_vcycle<double>, dim3(grid), dim3(256), 0, s, fc, w);
+        CYCLE_LAUNCH(double);
     else
         return hipErrorInvalidValue;
+#undef CYCLE_LAUNCH
     return hipGetLastError();
 }
 
