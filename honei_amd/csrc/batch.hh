@@ -99,9 +99,9 @@ struct BatchConfig {
     //   0: the stored values (9 per row);
     //   1: formed from the slot's cell coefficients (8 bytes per node) with
     //      the assembly's own expressions, paired launches and FP64 outer;
-    //   2: paired launches gather them from four per-node arrays of the
-    //      distinct weights (q1_stencil_node; one per cell, edge and node),
-    //      built with the hierarchy; FP64 outer as 1.
+    //   2: paired launches gather them from the distinct weights stored
+    //      four per node, interleaved (q1_stencil_node; one per cell, edge
+    //      and node), built with the hierarchy; FP64 outer as 1.
     int32_t matrix_free = 1;
 };
 
@@ -118,7 +118,7 @@ struct BLevel {
     void *g = nullptr, *u = nullptr, *en = nullptr;   // coarse CG scratch (level 1 only)
     double *kappa = nullptr;   // cell coefficients [cap][kld] ((side-1)^2 per slot)
     int64_t kld = 0;
-    void *st = nullptr;        // distinct weights [cap][4][vld] of the value type (matrix_free 2)
+    void *st = nullptr;        // distinct weights [cap][vld][4] of the value type (matrix_free 2)
 };
 
 // Class c: slots [first, first + count), finest level `level`, FP64 outer data.
@@ -207,7 +207,7 @@ struct BMat {
     int64_t stride, ax_ld, n, vld, bps;
     const double *kappa;   // cell coefficients [slot][kld] (matrix-free launches)
     int64_t kld;
-    const void *st;        // distinct weights [slot][4][vld] (stencil-array launches)
+    const void *st;        // distinct weights [slot][vld][4] (stencil-array launches)
 };
 
 // ---- bulk hierarchy build (many slots per launch) ----------------------------

@@ -217,6 +217,29 @@ __device__ inline V prolong_value(int32_t R, int32_t C, int32_t nf, const V (&t)
 
 template <typename VT> __device__ inline typename VecOf<VT>::type load_value_cached(const VT *p) { return *p; }
 template <> __device__ inline float load_value_cached<bf16>(const bf16 *p) { return bf16_to_float(p->bits); }
+// A node's four interleaved distinct weights in one (float, bf16) or two
+// (double) vector loads; p is 4-entry aligned.
+__device__ inline void load_weights4(const float *p, float (&o)[4]) {
+    const float4 v = *(const float4 *)p;
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+__device__ inline void load_weights4(const bf16 *p, float (&o)[4]) {
+    const uint2 v = *(const uint2 *)p;
+    o[0] = bf16_to_float((uint16_t)(v.x & 0xffffu));
+    o[1] = bf16_to_float((uint16_t)(v.x >> 16));
+    o[2] = bf16_to_float((uint16_t)(v.y & 0xffffu));
+    o[3] = bf16_to_float((uint16_t)(v.y >> 16));
+}
+__device__ inline void load_weights4(const double *p, double (&o)[4]) {
+    const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+}
 
 // V: the vector and arithmetic type (float for float / bf16 values, double
 // for double values)
@@ -316,21 +339,27 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             cw[2] = cs[nc];
             cw[3] = cs[nc + 1];
         } else if constexpr (WM == 2) {
+            // the interleaved distinct weights (cell, row pair, column pair,
+            // diagonal) of the node and of its neighbours at -1, -S, -S-1:
             // cell weights at node - S - 1, node - S, node - 1, node; row-pair
             // weights at node - S, node; column-pair weights at node - 1,
             // node; the diagonal at node (boundary rows: addresses of the
             // node itself, values unused)
-            const VT *wc = (const VT *)m.st + ls * 4 * m.vld, *wh = wc + m.vld, *wv = wh + m.vld, *wd = wv + m.vld;
+            const VT *sw4 = (const VT *)m.st + ls * 4 * m.vld;
             const int64_t dS = bnd ? 0 : S, d1 = bnd ? 0 : 1;
-            av[0] = load_value_cached(wc + (node - dS - d1));
-            av[1] = load_value_cached(wh + (node - dS));
-            av[2] = load_value_cached(wc + (node - dS));
-            av[3] = load_value_cached(wv + (node - d1));
-            av[4] = load_value_cached(wd + node);
-            av[5] = load_value_cached(wv + node);
-            av[6] = load_value_cached(wc + (node - d1));
-            av[7] = load_value_cached(wh + node);
-            av[8] = load_value_cached(wc + node);
+            V q0[4], q1[4], q2[4];
+            load_weights4(sw4 + 4 * node, q0);
+            load_weights4(sw4 + 4 * (node - d1), q1);
+            load_weights4(sw4 + 4 * (node - dS), q2);
+            av[0] = load_value_cached(sw4 + 4 * (node - dS - d1));
+            av[1] = q2[1];
+            av[2] = q2[0];
+            av[3] = q1[2];
+            av[4] = q0[3];
+            av[5] = q0[2];
+            av[6] = q1[0];
+            av[7] = q0[1];
+            av[8] = q0[0];
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
@@ -1216,7 +1245,7 @@ __global__ void k_bb_level(const BuildEntry *entries, BuildParams P, int l) {
             q1_row<VT>(P.side[l], T, r, nullptr, (VT *)e.Ax[l], nullptr, P.stride[l], e.kappa[l], dinv, P.omega);
         } else if (r < n) {
             q1_dinv_row<VT>(P.side[l], r, e.kappa[l], dinv, P.omega);
-            if (st) q1_stencil_node<VT>(P.side[l], r, e.kappa[l], st, P.vld[l]);
+            if (st) q1_stencil_node<VT>(P.side[l], r, e.kappa[l], st);
         }
     }
 }
@@ -1238,7 +1267,7 @@ __global__ void k_stencil_fill(int64_t side, const double *kappa, int64_t kld, v
     const int64_t g = blockIdx.y, n = side * side;
     VT *sg = (VT *)st + g * 4 * vld;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-        q1_stencil_node<VT>(side, r, kappa + g * kld, sg, vld);
+        q1_stencil_node<VT>(side, r, kappa + g * kld, sg);
 }
 
 typedef void (*BEllFn)(BMat, BWork, void *, const void *, const void *, const void *);

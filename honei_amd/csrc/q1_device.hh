@@ -174,8 +174,8 @@ __device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT 
 }
 
 // The distinct weights of the variable-coefficient operator, one per cell,
-// edge and node, in four arrays of side^2 entries (node indexing, unused
-// entries untouched) at st + k * vld:
+// edge and node, interleaved per node (entry k of node r at st[4 r + k], so
+// one 4-wide load fetches a node's four; unused entries untouched):
 //   k = 0: -c / 3 of cell (i, j) at i * side + j (i, j < side - 1), the
 //          corner weights w0, w2, w6, w8 of the cell's four nodes;
 //   k = 1: -(c(i, j-1) + c(i, j)) / 6 of cell row i at i * side + j
@@ -188,11 +188,11 @@ __device__ inline void q1_row(int64_t side, int T, int64_t row, int32_t *Aj, VT 
 // the nine weights gathered from these arrays are the stored row's bits.
 // Written for node index `node` < side^2.
 template <typename VT>
-__device__ inline void q1_stencil_node(int64_t side, int64_t node, const double *coeff, VT *st, int64_t vld) {
+__device__ inline void q1_stencil_node(int64_t side, int64_t node, const double *coeff, VT *st) {
     typedef typename VecOf<VT>::type V;
     const int64_t nc = side - 1, i = node / side, j = node % side;
     auto put = [&](int k, double v) {
-        q1_put<VT, V>(st + k * vld, node, (V)v);
+        q1_put<VT, V>(st, 4 * node + k, (V)v);
     };
     if (i < nc && j < nc) put(0, -div3_rn(coeff[i * nc + j]));
     if (i < nc && j >= 1 && j <= side - 2) put(1, -0.5 * div3_rn(coeff[i * nc + (j - 1)] + coeff[i * nc + j]));
