@@ -470,9 +470,10 @@ typedef struct {
                                   each row's weights (same results in every form): 0 the nine
                                   stored values; 1 (default) formed from the slot's cell
                                   coefficients (8 bytes per node) with the assembly's
-                                  expressions; 2 the paired launches gather them from four per-node arrays of
-                                  the distinct weights (one per cell, edge and node) built with
-                                  the hierarchy, the FP64 outer forms them as 1 */
+                                  expressions; 2 the paired launches gather them from the
+                                  distinct weights (one per cell, edge and node) stored four per
+                                  node, interleaved, built with the hierarchy; the FP64 outer
+                                  forms them as 1 */
 } honei_hip_batch_config;
 
 typedef struct {
