@@ -1636,9 +1636,15 @@ hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, i
 hipError_t b_build(const BuildEntry *entries, const BuildParams &P, int max_level, hipStream_t s) {
     if (P.entries <= 0) return hipSuccess;
     if (P.entries > 65535 || max_level < 1 || max_level > kMaxLevel) return hipErrorInvalidValue;
+    // blocks per entry: enough for the largest entry's work, but about 8192
+    // blocks in all (each thread strides over more of an entry when many are
+    // built at once), so the entries smaller than the largest do not launch
+    // thousands of empty blocks each
+    const int64_t per_cap = std::max<int64_t>(16, 8192 / P.entries);
     auto grid = [&](int64_t work) {
         int64_t g = (work + 255) / 256;
-        return dim3((unsigned)(g < 1 ? 1 : (g > 2048 ? 2048 : g)), (unsigned)P.entries);
+        g = std::min<int64_t>(g, std::min<int64_t>(2048, per_cap));
+        return dim3((unsigned)(g < 1 ? 1 : g), (unsigned)P.entries);
     };
     const int64_t nmax = P.side[max_level] * P.side[max_level];
     hipLaunchKernelGGL(k_bb_fields, grid(nmax), dim3(256), 0, s, entries, P);
