@@ -548,12 +548,17 @@ BClassTab BatchMG::class_table() const {
 // One outer iteration of every slot.  Defect correction (outer 0):
 //   list non-idle -> r = b - A x, |r| -> check -> list active -> rhs = r/|r|
 //   -> V-cycle -> x += |r| c.
-// Preconditioned CG (outer 1), z = |r| V(r/|r|):
+// Preconditioned CG (outer 1), z = s V(r/s):
 //   list new / verifying -> r = b - A x for them -> list non-idle -> check
-//   (recursive |r| for running slots) -> list active -> rhs = r/|r| ->
-//   V-cycle -> rho = r.z, beta ->
-//   p = z + beta p with q = A p, p.q (one tiled launch) -> alpha ->
-//   x += alpha p, r -= alpha q, |r|.
+//   (recursive |r| for running slots) -> list active -> rhs = r/|r| for the
+//   slots whose r was just recomputed (s = |r|) -> V-cycle -> rho = r.z,
+//   beta -> p = z + beta p with q = A p, p.q (one tiled launch) -> alpha ->
+//   x += alpha p, r -= alpha q, |r|, and the next cycle's rhs = r / s' with
+//   s' = |r| of this iteration's check (the new norm is not known yet). The
+//   cycle is linear in its right-hand side apart from the coarse CG's
+//   stopping rule, which compares absolute changes with the type's epsilon;
+//   s' keeps that input within one iteration's reduction factor of unit
+//   size, as r/|r| does.
 hipError_t BatchMG::enqueue_iteration(hipStream_t s) {
     const int nb = (int)boundaries_.size();
     const BClassTab tab = class_table();

@@ -52,10 +52,15 @@ struct SlotScalars {
     double rho;       // CG: r.z of the previous iteration (0: restart)
     double beta;      // CG: direction update factor of this iteration
     double alpha;     // CG: step length of this iteration
+    // CG: the scaling OP_UPDATE gives the next cycle's right-hand side
+    // (r_new * inv_w, z then taken as scale_w * V): 1/|r| and |r| of the latest check
+    double scale_w, inv_w;
     int32_t state;    // SlotState
     int32_t iters;    // outer iterations (V-cycles) applied
     int32_t status;   // SlotStatus
     int32_t pcur;     // CG: which of the slot's two direction buffers holds p (low bit)
+    int32_t fresh;    // 1: r was (re)computed from b - A x at this check, OP_SCALE writes the rhs
+    int32_t pad;
 };
 
 constexpr int kMaxClasses = 16;
@@ -187,11 +192,12 @@ struct BClassTab {
 enum OuterOp : int32_t {
     OP_DEFECT_ALL = 0,     // r = b - A x (+ |r|^2) for every listed slot
     OP_DEFECT_SETTLE = 1,  // same op; CG lists only new / verifying slots for it
-    OP_SCALE = 2,          // rhs_top = r / |r| in the inner precision
+    OP_SCALE = 2,          // rhs_top = r / |r| in the inner precision (slots with `fresh`; CG's
+                           // running slots get theirs from OP_UPDATE)
     OP_ACC = 3,            // x = (first cycle ? 0 : x) + |r| * x_top   (defect correction)
     OP_DOT_RZ = 4,         // partial r.z, z = |r| * x_top
     // (p = z + beta p with q = A p, p.q: b_dir_spmv)
-    OP_UPDATE = 7          // x += alpha p, r -= alpha q, partial |r|^2
+    OP_UPDATE = 7          // x += alpha p, r -= alpha q, partial |r|^2, rhs_top = r * inv_w
 };
 enum FinishOp : int32_t { FIN_CHECK_DC = 0, FIN_CHECK_CG = 1, FIN_BETA = 2, FIN_ALPHA = 3 };
 

@@ -679,6 +679,8 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
         const int64_t g = w.list[w.bounds[c] + si];
         const int64_t ls = g - e.first;
         const SlotScalars &st = sc[g];
+        // a running CG slot's rhs was written by OP_UPDATE (uniform per block)
+        if (OP == OP_SCALE && !st.fresh) continue;
         const double *rs = e.r + ls * e.vld, *qs = e.q + ls * e.vld;
         const double *ps = e.p + ls * e.vld + (OP == OP_UPDATE ? (int64_t)(st.pcur & 1) * e.palt : 0);
         const double *xs = e.x + ls * e.vld;
@@ -709,10 +711,13 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
             } else if constexpr (OP == OP_DOT_RZ) {
                 if (in[j]) part[j] = rv[j] * (zv[j] * st.scale);
             } else if constexpr (OP == OP_UPDATE) {
+                // and the next cycle's right-hand side, scaled by 1/|r| of
+                // this iteration's check (the norm of r_new is not known yet)
                 const double r = rv[j] - st.alpha * qv[j];
                 if (in[j]) {
                     e.x[vi] = xo[j] + st.alpha * pv[j];
                     e.r[vi] = r;
+                    ((Z *)e.rhs_top)[g * e.vld_top + row[j]] = (Z)(r * st.inv_w);
                     part[j] = r * r;
                 }
             }
@@ -769,6 +774,9 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
                     } else {
                         s.scale = OP == FIN_CHECK_DC ? 1.0 : nrm;
                         s.inv = OP == FIN_CHECK_DC ? 1.0 : 1.0 / nrm;
+                        s.scale_w = nrm;
+                        s.inv_w = 1.0 / nrm;
+                        s.fresh = 1;
                         s.iters = 1;
                         s.state = SLOT_ACTIVE;
                     }
@@ -784,12 +792,25 @@ __global__ __launch_bounds__(1024) void k_b_finish(BClassTab tab, BWork w, SlotS
                         s.state = SLOT_IDLE;
                         s.status = SLOT_NOT_CONVERGED;
                     } else {
+                        // DC, and CG after a recomputed residual: OP_SCALE
+                        // writes rhs = r / |r|; a running CG slot's rhs was
+                        // written by OP_UPDATE as r * inv_w, so its z takes
+                        // that scale (|r| of the previous check)
+                        const bool recomputed = OP == FIN_CHECK_DC || s.state == SLOT_VERIFY;
                         if (s.state == SLOT_VERIFY) {   // recursive residual drifted: restart
                             s.state = SLOT_ACTIVE;
                             s.rho = 0.0;
                         }
-                        s.scale = nrm;
-                        s.inv = 1.0 / nrm;
+                        if (recomputed) {
+                            s.scale = nrm;
+                            s.inv = 1.0 / nrm;
+                        } else {
+                            s.scale = s.scale_w;
+                            s.inv = s.inv_w;
+                        }
+                        s.scale_w = nrm;
+                        s.inv_w = 1.0 / nrm;
+                        s.fresh = recomputed ? 1 : 0;
                         s.iters += 1;
                     }
                 }
