@@ -83,7 +83,7 @@ struct BatchConfig {
     double tolerance = 1e-8;         // relative FP64 defect
     int32_t max_outer = 200;
     int32_t averaging = 1;           // coefficient coarsening: 0 arith, 1 geometric, 2 harmonic
-    int32_t iters_per_launch = 4;    // outer iterations per graph launch
+    int32_t iters_per_launch = 2;    // outer iterations per graph launch
     int32_t autotune = 1;            // 0: threads-per-row 1, block 256 everywhere
     // Outer iteration: 0 defect correction x += V(b - A x) (the reference's
     // MIXED driver), 1 conjugate gradients in FP64 preconditioned by the

@@ -453,7 +453,7 @@ typedef struct {
     double tolerance;          /* relative FP64 defect */
     int32_t max_outer;         /* outer iterations before a slot gives up */
     int32_t averaging;         /* coefficient coarsening: 0 arithmetic, 1 geometric, 2 harmonic */
-    int32_t iters_per_launch;  /* outer iterations per graph launch (1..64) */
+    int32_t iters_per_launch;  /* outer iterations per graph launch (1..64, default 2) */
     int32_t autotune;
     int32_t outer_method;      /* 0 defect correction x += V(b - A x) (the reference's MIXED
                                   driver); 1 FP64 conjugate gradients preconditioned by the V-cycle,
