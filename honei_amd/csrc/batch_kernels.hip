@@ -318,14 +318,14 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         // ---- every global load of the tile issued before any use ----
         const V rh = rs[node], di = ds[node];
         V xn = (PAIR == PAIR_PRE) ? V(0) : xs[node];
-        V e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
-        const V e2 = (PAIR == PAIR_PRE) ? rs[ej] : V(0);
         V tn[4], te[4];   // FP: coarse taps of the node and of the extra entry
-        if constexpr (FP) {
-            const int32_t nc = (S >> 1) + 1;
-            const V *xcs = xc + ls * vld_c;
-            prolong_taps(xcs, nc, rc, colc, tn);
-            prolong_taps(xcs, nc, egr, egc, te);
+        if constexpr (FP) prolong_taps(xc + ls * vld_c, (S >> 1) + 1, rc, colc, tn);
+        // the extra entry's loads only where there is one (waves 0..2)
+        V e1 = V(0), e2 = V(0);
+        if (threadIdx.x < kExtra) {
+            e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
+            if (PAIR == PAIR_PRE) e2 = rs[ej];
+            if constexpr (FP) prolong_taps(xc + ls * vld_c, (S >> 1) + 1, egr, egc, te);
         }
         V av[9];
         double cw[4];
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         }
         if constexpr (FP) {   // t = x + P xc, as k_b_prolong_rows stores it
             xn = xn + prolong_value(rc, colc, S, tn) * V(1);
-            e1 = e1 + prolong_value(egr, egc, S, te) * V(1);
+            if (threadIdx.x < kExtra) e1 = e1 + prolong_value(egr, egc, S, te) * V(1);
         }
         // ---- stage the first pass's operand ----
         src[i + 1][lane + 1] = (PAIR == PAIR_PRE) ? di * rh : xn;
