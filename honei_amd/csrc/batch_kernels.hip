@@ -196,10 +196,15 @@ inline int tiled_grid(int64_t units, int threads) {
 template <typename V>
 __device__ inline void prolong_taps(const V *xc, int32_t nc, int32_t R, int32_t C, V (&t)[4]) {
     const int32_t rlo = R >> 1, rhi = (R + 1) >> 1, clo = C >> 1, chi = (C + 1) >> 1;
-    t[0] = xc[rlo * nc + clo];
-    t[1] = xc[rlo * nc + chi];
-    t[2] = xc[rhi * nc + clo];
-    t[3] = xc[rhi * nc + chi];
+    // the two rows' values at columns c0, c0 + 1 (two adjacent loads each,
+    // merged into one), c0 = clo except on the last column (nc >= 2)
+    const int32_t c0 = clo < nc - 1 ? clo : nc - 2;
+    const V *p0 = xc + rlo * nc + c0, *p1 = xc + rhi * nc + c0;
+    const V a0 = p0[0], a1 = p0[1], b0 = p1[0], b1 = p1[1];
+    t[0] = clo == c0 ? a0 : a1;
+    t[1] = chi == c0 ? a0 : a1;
+    t[2] = clo == c0 ? b0 : b1;
+    t[3] = chi == c0 ? b0 : b1;
 }
 // prolong_node's correction of fine node (R, C) of an nf x nf level from its
 // taps, every border Dirichlet (0): a coarse node's value, the mean of two
@@ -333,11 +338,11 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
             // the node's four cells (a boundary node's: clamped, unused)
             const uint32_t nc = (uint32_t)S - 1u;
             const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
-            const double *cs = m.kappa + ls * m.kld + ((ir - 1u) * nc + (ic - 1u));
+            const double *cs = m.kappa + ls * m.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
             cw[0] = cs[0];
             cw[1] = cs[1];
-            cw[2] = cs[nc];
-            cw[3] = cs[nc + 1];
+            cw[2] = cs2[0];
+            cw[3] = cs2[1];
         } else if constexpr (WM == 2) {
             // the interleaved distinct weights (cell, row pair, column pair,
             // diagonal) of the node and of its neighbours at -1, -S, -S-1:
