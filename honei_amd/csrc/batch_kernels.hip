@@ -617,11 +617,11 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_dir_spmv(BClassTab 
         double cw[4];
         if constexpr (MF) {
             const uint32_t nc = (uint32_t)S - 1u;
-            const double *cs = e.kappa + g * e.kld + ((ir - 1u) * nc + (ic - 1u));
+            const double *cs = e.kappa + g * e.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
             cw[0] = cs[0];
             cw[1] = cs[1];
-            cw[2] = cs[nc];
-            cw[3] = cs[nc + 1];
+            cw[2] = cs2[0];
+            cw[3] = cs2[1];
         } else {
             const double *ax = e.Ax + ls * e.ax_ld + ((int64_t)ir * S + ic);
 #pragma unroll

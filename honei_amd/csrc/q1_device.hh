@@ -50,8 +50,8 @@ __device__ inline void q1_cell_weights(double sw, double se, double nw, double n
 template <typename V>
 __device__ inline void q1_coeff_weights(int64_t side, int64_t i, int64_t j, const double *coeff, V (&w)[9]) {
     const uint32_t nc = (uint32_t)side - 1u, b = ((uint32_t)i - 1u) * nc + ((uint32_t)j - 1u);
-    const double *cs = coeff + b;
-    q1_cell_weights<V>(cs[0], cs[1], cs[nc], cs[nc + 1], w);
+    const double *cs = coeff + b, *cs2 = cs + nc;   // two adjacent pairs: two vector loads
+    q1_cell_weights<V>(cs[0], cs[1], cs2[0], cs2[1], w);
 }
 
 // A float as a bf16 matrix stores it (round to nearest even) and widens it
