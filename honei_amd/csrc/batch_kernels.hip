@@ -1071,9 +1071,10 @@ __device__ inline void fused_sweep_t(const FusedLevel &L, int64_t ls, V *out, co
 
 // Level l of the cycle: fc's arrays, or (lds non-null) the LDS copies at
 // the offsets fc.lo[l] (a copy of one level, so no runtime-indexed local array).
+template <bool LDSM>
 __device__ inline FusedLevel level_view(const FusedCycle &fc, int l, char *lds) {
     FusedLevel L = fc.lv[l];
-    if (lds) {
+    if constexpr (LDSM) {
         const FusedLds &o = fc.lo[l];
         L.Aj = (const int32_t *)(lds + o.aj);
         L.Arl = (const uint32_t *)(lds + o.arl);
@@ -1088,14 +1089,15 @@ __device__ inline FusedLevel level_view(const FusedCycle &fc, int l, char *lds) 
 }
 
 // One V-cycle of slot g from level `top` down to the coarse CG and back
-// (fc's arrays indexed by slot g, or with lds the LDS copies, g = 0).
-template <typename VT>
+// (fc's arrays indexed by slot g, or with LDSM the LDS copies at lds, g = 0;
+// a compile-time choice, so the LDS form's accesses are LDS instructions).
+template <typename VT, bool LDSM>
 __device__ inline void fused_cycle_body(const FusedCycle &fc, int64_t g, int top, char *lds) {
     typedef typename VecOf<VT>::type V;
     const Mask8 mk{0xffu};   // all Dirichlet, as the level-by-level transfers
     // down: n_pre sweeps from zero (first two fused), defect, restriction
     for (int l = top; l >= 2; --l) {
-        const FusedLevel L = level_view(fc, l, lds);
+        const FusedLevel L = level_view<LDSM>(fc, l, lds);
         V *Lx = (V *)L.x, *Lt = (V *)L.t, *Ld = (V *)L.d;
         const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
         const int rest = fc.n_pre - 2;
@@ -1107,14 +1109,14 @@ __device__ inline void fused_cycle_body(const FusedCycle &fc, int64_t g, int top
             src = dst;
         }
         fused_sweep_t<VT, ELL_DEFECT>(L, g, Ld, Lx, Lr, (const V *)nullptr);
-        const FusedLevel C = level_view(fc, l - 1, lds);
+        const FusedLevel C = level_view<LDSM>(fc, l - 1, lds);
         const int64_t m = C.side - 1;
         for (int64_t i = threadIdx.x; i < C.n; i += blockDim.x)
             restrict_node<V>((V *)C.rhs + g * C.vld, Ld + g * L.vld, m, mk, i);
         __syncthreads();
     }
     {   // coarse CG on level 1
-        const FusedLevel L = level_view(fc, 1, lds);
+        const FusedLevel L = level_view<LDSM>(fc, 1, lds);
         EllView A{};
         A.rows = L.n;
         A.cols = L.n;
@@ -1125,16 +1127,16 @@ __device__ inline void fused_cycle_body(const FusedCycle &fc, int64_t g, int top
         A.Ax = (const VT *)L.Ax + g * L.ax_ld;
         A.Arl = L.Arl;
         const int64_t o = g * L.vld;
-        V *cg = (V *)(lds ? lds + fc.og : fc.g), *cu = (V *)(lds ? lds + fc.ou : fc.u);
-        V *cen = (V *)(lds ? lds + fc.oen : fc.en);
+        V *cg = (V *)(LDSM ? lds + fc.og : fc.g), *cu = (V *)(LDSM ? lds + fc.ou : fc.u);
+        V *cen = (V *)(LDSM ? lds + fc.oen : fc.en);
         (void)coarse_cg_block<VT>(A, (const V *)L.rhs + o, (V *)L.x + o, cg + o, cu + o, cen + o, fc.konv,
                                   fc.max_iters);
         __syncthreads();
     }
     // up: x += P x_coarse, n_post sweeps (result in x)
     for (int l = 2; l <= top; ++l) {
-        const FusedLevel L = level_view(fc, l, lds);
-        const FusedLevel C = level_view(fc, l - 1, lds);
+        const FusedLevel L = level_view<LDSM>(fc, l, lds);
+        const FusedLevel C = level_view<LDSM>(fc, l - 1, lds);
         V *Lx = (V *)L.x, *Lt = (V *)L.t;
         const V *Lr = (const V *)L.rhs, *Ldi = (const V *)L.dinv;
         const int64_t m = C.side - 1;
@@ -1162,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w)
         const int64_t g = w.list[a0 + a];
         int top = fc.top;
         while (top > 1 && g >= fc.lv[top].cap) --top;   // slots whose finest level is lower
-        fused_cycle_body<VT>(fc, g, top, nullptr);
+        fused_cycle_body<VT, false>(fc, g, top, nullptr);
     }
 }
 
@@ -1200,7 +1202,7 @@ __global__ __launch_bounds__(256) void k_b_coarse_vcycle_lds(FusedCycle fc, BWor
         lds_copy_words(lds + fc.lo[top].rhs, (const V *)fc.lv[top].rhs + g * fc.lv[top].vld,
                        (int64_t)sizeof(V) * fc.lv[top].n);
         __syncthreads();
-        fused_cycle_body<VT>(fc, 0, top, lds);
+        fused_cycle_body<VT, true>(fc, 0, top, lds);
         lds_copy_words((V *)fc.lv[top].x + g * fc.lv[top].vld, lds + fc.lo[top].x, (int64_t)sizeof(V) * fc.lv[top].n);
         __syncthreads();
     }
