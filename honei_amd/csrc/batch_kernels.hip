@@ -690,14 +690,21 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
         const double *ps = e.p + ls * e.vld + (OP == OP_UPDATE ? (int64_t)(st.pcur & 1) * e.palt : 0);
         const double *xs = e.x + ls * e.vld;
         const Z *zs = (const Z *)e.x_top + g * e.vld_top;
+        // slot scalars and output bases in registers: the stores below could
+        // alias the slot record, which would make every row reload them
+        const double alpha = st.alpha, inv = st.inv, inv_w = st.inv_w, scale = st.scale;
+        const bool first_acc = st.iters == 1;
+        double *const xw = e.x + ls * e.vld, *const rw = e.r + ls * e.vld;
+        Z *const tw = (Z *)e.rhs_top + g * e.vld_top;
+        const int64_t n = e.n;
         int64_t row[R];
         bool in[R];
         double rv[R], zv[R], pv[R], qv[R], xo[R], part[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {   // every load first
             row[j] = (sb * R + j) * 256 + threadIdx.x;
-            in[j] = row[j] < e.n;
-            const int64_t rc = in[j] ? row[j] : e.n - 1;
+            in[j] = row[j] < n;
+            const int64_t rc = in[j] ? row[j] : n - 1;
             if constexpr (OP == OP_SCALE || OP == OP_DOT_RZ || OP == OP_UPDATE) rv[j] = rs[rc];
             if constexpr (OP == OP_ACC || OP == OP_DOT_RZ) zv[j] = (double)zs[rc];
             if constexpr (OP == OP_UPDATE) pv[j] = ps[rc];
@@ -707,22 +714,22 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             part[j] = 0.0;
-            const int64_t vi = ls * e.vld + row[j];
+            const int64_t vi = row[j];
             if constexpr (OP == OP_SCALE) {
-                if (in[j]) ((Z *)e.rhs_top)[g * e.vld_top + row[j]] = (Z)(rv[j] * st.inv);
+                if (in[j]) tw[vi] = (Z)(rv[j] * inv);
             } else if constexpr (OP == OP_ACC) {
-                const double base = st.iters == 1 ? 0.0 : xo[j];
-                if (in[j]) e.x[vi] = base + zv[j] * st.scale;
+                const double base = first_acc ? 0.0 : xo[j];
+                if (in[j]) xw[vi] = base + zv[j] * scale;
             } else if constexpr (OP == OP_DOT_RZ) {
-                if (in[j]) part[j] = rv[j] * (zv[j] * st.scale);
+                if (in[j]) part[j] = rv[j] * (zv[j] * scale);
             } else if constexpr (OP == OP_UPDATE) {
                 // and the next cycle's right-hand side, scaled by 1/|r| of
                 // this iteration's check (the norm of r_new is not known yet)
-                const double r = rv[j] - st.alpha * qv[j];
+                const double r = rv[j] - alpha * qv[j];
                 if (in[j]) {
-                    e.x[vi] = xo[j] + st.alpha * pv[j];
-                    e.r[vi] = r;
-                    ((Z *)e.rhs_top)[g * e.vld_top + row[j]] = (Z)(r * st.inv_w);
+                    xw[vi] = xo[j] + alpha * pv[j];
+                    rw[vi] = r;
+                    tw[vi] = (Z)(r * inv_w);
                     part[j] = r * r;
                 }
             }
