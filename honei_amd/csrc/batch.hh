@@ -68,11 +68,13 @@ constexpr int kMaxClasses = 16;
 // Paired row passes (b_pair): tiles of at most kPairWidth columns (62 + a
 // halo column each side fill a wavefront) and kPairRows rows (kPairTileRows
 // with the halo, one wavefront per row: 1024-thread work-groups), on levels
-// from kPairMinSide^2 nodes up.
+// from kPairMinSide^2 nodes up; from kPairTallMinSide^2 nodes up every thread
+// takes two grid rows (tiles of 2 kPairTileRows - 2 rows).  The CG direction
+// launch tiles the finest grids in kPairRows-row bands.
 enum PairKind { PAIR_PRE = 0, PAIR_POST = 1 };
 constexpr int64_t kPairWidth = 62;
 constexpr int kPairTileRows = 16, kPairRows = kPairTileRows - 2;
-constexpr int64_t kPairMinSide = 33;
+constexpr int64_t kPairMinSide = 33, kPairTallMinSide = 129;
 
 struct BatchConfig {
     // VT_F32 or VT_BF16 matrix values with FP32 vectors (mixed precision),

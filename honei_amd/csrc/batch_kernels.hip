@@ -157,12 +157,13 @@ __global__ __launch_bounds__(1024) void k_b_ell(BMat m, BWork w, void *out, cons
 // grid offset (k/3 - 1, k%3 - 1)), a boundary row its diagonal alone
 // (q1_row), T = 1 layout.  k_b_pair runs two passes that depend on each
 // other in one launch.  A work-group of kPairTileRows wavefronts owns a tile
-// of one slot's grid: kPairRows rows by `width` (<= 62) columns.  The
-// work-group first stages the first pass's gathered operand for the tile and
-// a two-node halo in LDS; after a barrier, wavefront i forms the first pass
-// for grid row r0 - 1 + i on the tile's columns plus one halo column each
-// side (a lane per column) from it into LDS; after a second barrier,
-// wavefronts 1 .. kPairRows form the second pass for their row from LDS,
+// of one slot's grid: kPairTileRows * NR - 2 rows (NR = 1 or 2 grid rows per
+// thread) by `width` (<= 62) columns.  The work-group first stages the first
+// pass's gathered operand for the tile and a two-node halo in LDS; after a
+// barrier, tile row I (thread row I mod kPairTileRows) forms the first pass
+// for grid row r0 - 1 + I on the tile's columns plus one halo column each
+// side (a lane per column) from it into LDS; after a second barrier, tile
+// rows 1 .. kPairTileRows * NR - 2 form the second pass for their row from LDS,
 // with the matrix row still in registers.  So the matrix is read from
 // HBM once for both passes: halo rows and columns are read again by the
 // neighbouring tiles, through L2 (cached loads, not streaming ones).  Every
@@ -251,20 +252,25 @@ __device__ inline void load_weights4(const double *p, double (&o)[4]) {
 // WM: weights from the stored values (0), formed from the cell coefficients
 // (1), gathered from the distinct-weight arrays (2).
 // FP (PAIR_POST only): xc = the coarse level's result (vld_c per slot).
-template <typename VT, int PAIR, int WM, bool FP = false, typename V = typename VecOf<VT>::type>
+// NR: grid rows per thread; thread (i, lane) takes tile rows i + u *
+// kPairTileRows (u < NR), so a tile is kPairTileRows * NR - 2 rows and every
+// thread keeps NR nodes' loads in flight.
+template <typename VT, int PAIR, int WM, bool FP = false, int NR = 1, typename V = typename VecOf<VT>::type>
 __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWork w, PairGeom pg, V *out0, V *out1,
                                                                   const V *xin, const V *rhs, const V *dinv,
                                                                   const V *xc, int64_t vld_c) {
     static_assert(!FP || PAIR == PAIR_POST, "the prolongation fuses into the post-smoothing pair");
+    static_assert(NR == 1 || NR == 2, "one or two rows per thread");
+    constexpr int TR = kPairTileRows * NR, OR = TR - 2;   // staged rows, tile rows
     // first-pass results of the tile and its halo
-    __shared__ V tile[kPairTileRows][kWave];
+    __shared__ V tile[TR][kWave];
     // the first pass's gathered operand (PAIR_PRE: dinv .* rhs; PAIR_POST:
-    // xin) on grid rows r0 - 2 .. r0 + kPairTileRows - 1 and columns
-    // c0 - 2 .. c0 + kWave - 1 (entry [a][b] = grid (r0 - 2 + a, c0 - 2 + b),
-    // positions outside the grid clamped to it: loaded, never used)
-    __shared__ V src[kPairTileRows + 2][kWave + 2];
-    constexpr int kExtra = 2 * kWave + 2 * (kPairTileRows + 2);   // src entries beyond one per thread
-    const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;   // tile row, lane
+    // xin) on grid rows r0 - 2 .. r0 + TR - 1 and columns c0 - 2 ..
+    // c0 + kWave - 1 (entry [a][b] = grid (r0 - 2 + a, c0 - 2 + b), positions
+    // outside the grid clamped to it: loaded, never used)
+    __shared__ V src[TR + 2][kWave + 2];
+    constexpr int kExtra = 2 * kWave + 2 * (TR + 2);   // src entries beyond one per node
+    const int i = threadIdx.x / kWave, lane = threadIdx.x % kWave;   // tile row (u = 0), lane
     const int64_t a0 = active_first(w), na = active_count(w);
     const int64_t ups = (int64_t)pg.strips * pg.segs;
     const int64_t total = na * ups;
@@ -277,6 +283,22 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
     const int64_t mine = lb0 < total ? (total - 1 - lb0) / gridDim.x + 1 : 0;
     for (int64_t k = threadIdx.x; k < mine; k += blockDim.x) tslot[k] = w.list[a0 + (lb0 + k * gridDim.x) / ups];
     __syncthreads();
+    // the staged entry beyond one per node (threads < kExtra): grid rows
+    // r0 - 2 and r0 + TR - 1, columns c0 - 2 and c0 + kWave - 1
+    int ea = 0, eb = 0;
+    if (threadIdx.x < kExtra) {
+        const int t = threadIdx.x;
+        if (t < kWave) {
+            ea = 0;
+            eb = t + 1;
+        } else if (t < 2 * kWave) {
+            ea = TR + 1;
+            eb = t - kWave + 1;
+        } else {
+            ea = (t - 2 * kWave) / 2;
+            eb = ((t - 2 * kWave) % 2) ? kWave + 1 : 0;
+        }
+    }
     int64_t k = 0;
     for (int64_t lb = lb0; lb < total; lb += gridDim.x, ++k) {
         int64_t e, rem, strip, seg;
@@ -290,128 +312,138 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_pair(BMat m, BWo
         const int64_t vo = ls * m.vld;
         const VT *ax = (const VT *)m.Ax + ls * m.ax_ld;
         const V *xs = xin + vo, *rs = rhs + vo, *ds = dinv + vo;
-        const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
+        const int r0 = (int)seg * OR, r1 = min(S, r0 + OR);
         const int c0 = (int)strip * pg.width, c1 = min(S, c0 + pg.width);
         const int col = c0 - 1 + lane;
         const bool in2 = lane >= 1 && col < c1;                    // second-pass lanes
-        // this thread's node: grid (r, col) clamped into the grid (lanes and
-        // rows outside it are computed and never used)
-        const int r = r0 - 1 + i;
-        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);
         const int colc = col < 0 ? 0 : (col > S - 1 ? S - 1 : col);
-        const int64_t node = (int64_t)rc * S + colc;
-        const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
-        // the staged entry beyond one per thread (threads < kExtra): grid
-        // rows r0 - 2 and r0 + kPairTileRows - 1, columns c0 - 2 and
-        // c0 + kWave - 1; other threads repeat their own node
-        int ea = i + 1, eb = lane + 1;
-        if (threadIdx.x < kExtra) {
-            const int t = threadIdx.x;
-            if (t < kWave) {
-                ea = 0;
-                eb = t + 1;
-            } else if (t < 2 * kWave) {
-                ea = kPairTileRows + 1;
-                eb = t - kWave + 1;
-            } else {
-                ea = (t - 2 * kWave) / 2;
-                eb = ((t - 2 * kWave) % 2) ? kWave + 1 : 0;
-            }
+        // this thread's nodes: grid (r[u], col) clamped into the grid (lanes
+        // and rows outside it are computed and never used)
+        int r[NR], rc[NR];
+        int64_t node[NR];
+        bool bnd[NR];
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            r[u] = r0 - 1 + i + u * kPairTileRows;
+            rc[u] = r[u] < 0 ? 0 : (r[u] > S - 1 ? S - 1 : r[u]);
+            node[u] = (int64_t)rc[u] * S + colc;
+            bnd[u] = rc[u] == 0 || rc[u] == S - 1 || colc == 0 || colc == S - 1;
         }
         const int egr = min(max(r0 - 2 + ea, 0), S - 1), egc = min(max(c0 - 2 + eb, 0), S - 1);
         const int64_t ej = (int64_t)egr * S + egc;
         // ---- every global load of the tile issued before any use ----
-        const V rh = rs[node], di = ds[node];
-        V xn = (PAIR == PAIR_PRE) ? V(0) : xs[node];
-        V tn[4], te[4];   // FP: coarse taps of the node and of the extra entry
-        if constexpr (FP) prolong_taps(xc + ls * vld_c, (S >> 1) + 1, rc, colc, tn);
-        // the extra entry's loads only where there is one (waves 0..2)
+        V rh[NR], di[NR], xn[NR];
+        V tn[NR][4], te[4];   // FP: coarse taps of the nodes and of the extra entry
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            rh[u] = rs[node[u]];
+            di[u] = ds[node[u]];
+            xn[u] = (PAIR == PAIR_PRE) ? V(0) : xs[node[u]];
+            if constexpr (FP) prolong_taps(xc + ls * vld_c, (S >> 1) + 1, rc[u], colc, tn[u]);
+        }
+        // the extra entry's loads only where there is one (the first waves)
         V e1 = V(0), e2 = V(0);
         if (threadIdx.x < kExtra) {
             e1 = (PAIR == PAIR_PRE) ? ds[ej] : xs[ej];
             if (PAIR == PAIR_PRE) e2 = rs[ej];
             if constexpr (FP) prolong_taps(xc + ls * vld_c, (S >> 1) + 1, egr, egc, te);
         }
-        V av[9];
-        double cw[4];
-        if constexpr (WM == 1) {
-            // the node's four cells (a boundary node's: clamped, unused)
-            const uint32_t nc = (uint32_t)S - 1u;
-            const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
-            const double *cs = m.kappa + ls * m.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
-            cw[0] = cs[0];
-            cw[1] = cs[1];
-            cw[2] = cs2[0];
-            cw[3] = cs2[1];
-        } else if constexpr (WM == 2) {
-            // the interleaved distinct weights (cell, row pair, column pair,
-            // diagonal) of the node and of its neighbours at -1, -S, -S-1:
-            // cell weights at node - S - 1, node - S, node - 1, node; row-pair
-            // weights at node - S, node; column-pair weights at node - 1,
-            // node; the diagonal at node (boundary rows: addresses of the
-            // node itself, values unused)
-            const VT *sw4 = (const VT *)m.st + ls * 4 * m.vld;
-            const int64_t dS = bnd ? 0 : S, d1 = bnd ? 0 : 1;
-            V q0[4], q1[4], q2[4];
-            load_weights4(sw4 + 4 * node, q0);
-            load_weights4(sw4 + 4 * (node - d1), q1);
-            load_weights4(sw4 + 4 * (node - dS), q2);
-            av[0] = load_value_cached(sw4 + 4 * (node - dS - d1));
-            av[1] = q2[1];
-            av[2] = q2[0];
-            av[3] = q1[2];
-            av[4] = q0[3];
-            av[5] = q0[2];
-            av[6] = q1[0];
-            av[7] = q0[1];
-            av[8] = q0[0];
-        } else {
+        V av[NR][9];
+        double cw[NR][4];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) av[k] = load_value_cached(ax + node + (int64_t)k * m.stride);
+        for (int u = 0; u < NR; ++u) {
+            if constexpr (WM == 1) {
+                // the node's four cells (a boundary node's: clamped, unused)
+                const uint32_t nc = (uint32_t)S - 1u;
+                const uint32_t ir = (uint32_t)min(max(rc[u], 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
+                const double *cs = m.kappa + ls * m.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
+                cw[u][0] = cs[0];
+                cw[u][1] = cs[1];
+                cw[u][2] = cs2[0];
+                cw[u][3] = cs2[1];
+            } else if constexpr (WM == 2) {
+                // the interleaved distinct weights (cell, row pair, column
+                // pair, diagonal) of the node and of its neighbours at -1, -S,
+                // -S-1: cell weights at node - S - 1, node - S, node - 1,
+                // node; row-pair weights at node - S, node; column-pair
+                // weights at node - 1, node; the diagonal at node (boundary
+                // rows: addresses of the node itself, values unused)
+                const VT *sw4 = (const VT *)m.st + ls * 4 * m.vld;
+                const int64_t nd = node[u], dS = bnd[u] ? 0 : S, d1 = bnd[u] ? 0 : 1;
+                V q0[4], q1[4], q2[4];
+                load_weights4(sw4 + 4 * nd, q0);
+                load_weights4(sw4 + 4 * (nd - d1), q1);
+                load_weights4(sw4 + 4 * (nd - dS), q2);
+                av[u][0] = load_value_cached(sw4 + 4 * (nd - dS - d1));
+                av[u][1] = q2[1];
+                av[u][2] = q2[0];
+                av[u][3] = q1[2];
+                av[u][4] = q0[3];
+                av[u][5] = q0[2];
+                av[u][6] = q1[0];
+                av[u][7] = q0[1];
+                av[u][8] = q0[0];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) av[u][q] = load_value_cached(ax + node[u] + (int64_t)q * m.stride);
+            }
         }
         if constexpr (FP) {   // t = x + P xc, as k_b_prolong_rows stores it
-            xn = xn + prolong_value(rc, colc, S, tn) * V(1);
+#pragma unroll
+            for (int u = 0; u < NR; ++u) xn[u] = xn[u] + prolong_value(rc[u], colc, S, tn[u]) * V(1);
             if (threadIdx.x < kExtra) e1 = e1 + prolong_value(egr, egc, S, te) * V(1);
         }
         // ---- stage the first pass's operand ----
-        src[i + 1][lane + 1] = (PAIR == PAIR_PRE) ? di * rh : xn;
+#pragma unroll
+        for (int u = 0; u < NR; ++u)
+            src[i + u * kPairTileRows + 1][lane + 1] = (PAIR == PAIR_PRE) ? di[u] * rh[u] : xn[u];
         if (threadIdx.x < kExtra) src[ea][eb] = (PAIR == PAIR_PRE) ? e1 * e2 : e1;
-        // ---- the weights of the node's row (a boundary node's: those of
+        // ---- the weights of the nodes' rows (a boundary node's: those of
         // its clamped cells, unused) ----
-        if constexpr (WM == 1) q1_interior_weights_cells<VT, V>(cw[0], cw[1], cw[2], cw[3], av);
+        if constexpr (WM == 1) {
+#pragma unroll
+            for (int u = 0; u < NR; ++u)
+                q1_interior_weights_cells<VT, V>(cw[u][0], cw[u][1], cw[u][2], cw[u][3], av[u]);
+        }
         __syncthreads();
-        {
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
             // every node forms the interior stencil sum over its staged
             // neighbours; a boundary node then takes its identity row's sum
             // 0 + 1 * x instead
+            const int I = i + u * kPairTileRows;
             V xv[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) xv[k] = src[i + 1 + (k / 3 - 1)][lane + 1 + (k % 3 - 1)];
-            V s = V(0);
+            for (int q = 0; q < 9; ++q) xv[q] = src[I + 1 + (q / 3 - 1)][lane + 1 + (q % 3 - 1)];
+            V sm = V(0);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) s = s + av[k] * xv[k];
-            if (bnd) s = V(0) + V(1) * xv[4];
-            const V x0 = (PAIR == PAIR_PRE) ? di * rh : xn;
-            const V t = rh - s;
-            const V v1 = x0 + t * di;
-            tile[i][lane] = v1;
-            if (PAIR == PAIR_PRE && in2 && i >= 1 && r < r1) out0[vo + node] = v1;
+            for (int q = 0; q < 9; ++q) sm = sm + av[u][q] * xv[q];
+            if (bnd[u]) sm = V(0) + V(1) * xv[4];
+            const V x0 = (PAIR == PAIR_PRE) ? di[u] * rh[u] : xn[u];
+            const V t = rh[u] - sm;
+            const V v1 = x0 + t * di[u];
+            tile[I][lane] = v1;
+            if (PAIR == PAIR_PRE && in2 && I >= 1 && r[u] < r1) out0[vo + node[u]] = v1;
         }
         __syncthreads();
-        // ---- second pass, grid row r for tile rows 1 .. kPairRows ----
-        if (i >= 1 && i <= kPairRows && r < r1 && in2) {
-            V xv[9];
+        // ---- second pass, grid row r[u] for tile rows 1 .. OR ----
 #pragma unroll
-            for (int k = 0; k < 9; ++k) xv[k] = tile[i + (k / 3 - 1)][lane + (k % 3 - 1)];
-            V s = V(0);
+        for (int u = 0; u < NR; ++u) {
+            const int I = i + u * kPairTileRows;
+            if (I >= 1 && I <= OR && r[u] < r1 && in2) {
+                V xv[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) s = s + av[k] * xv[k];
-            if (bnd) s = V(0) + V(1) * xv[4];
-            if constexpr (PAIR == PAIR_PRE) {
-                out1[vo + node] = rh - s;
-            } else {
-                const V t = rh - s;
-                out0[vo + node] = tile[i][lane] + t * di;
+                for (int q = 0; q < 9; ++q) xv[q] = tile[I + (q / 3 - 1)][lane + (q % 3 - 1)];
+                V sm = V(0);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) sm = sm + av[u][q] * xv[q];
+                if (bnd[u]) sm = V(0) + V(1) * xv[4];
+                if constexpr (PAIR == PAIR_PRE) {
+                    out1[vo + node[u]] = rh[u] - sm;
+                } else {
+                    const V t = rh[u] - sm;
+                    out0[vo + node[u]] = tile[I][lane] + t * di[u];
+                }
             }
         }
         __syncthreads();
@@ -721,16 +753,19 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
                 const double base = first_acc ? 0.0 : xo[j];
                 if (in[j]) xw[vi] = base + zv[j] * scale;
             } else if constexpr (OP == OP_DOT_RZ) {
-                if (in[j]) part[j] = rv[j] * (zv[j] * scale);
+                // selects, not branches: a load used only inside a branch is
+                // moved into it, which serialises the row groups
+                const double t = rv[j] * (zv[j] * scale);
+                part[j] = in[j] ? t : 0.0;
             } else if constexpr (OP == OP_UPDATE) {
                 // and the next cycle's right-hand side, scaled by 1/|r| of
                 // this iteration's check (the norm of r_new is not known yet)
-                const double r = rv[j] - alpha * qv[j];
+                const double r = rv[j] - alpha * qv[j], xn = xo[j] + alpha * pv[j];
+                part[j] = in[j] ? r * r : 0.0;
                 if (in[j]) {
-                    xw[vi] = xo[j] + alpha * pv[j];
+                    xw[vi] = xn;
                     rw[vi] = r;
                     tw[vi] = (Z)(r * inv_w);
-                    part[j] = r * r;
                 }
             }
         }
@@ -1497,19 +1532,32 @@ hipError_t b_prolong_axpy(ValueType vt, const BLevel &fine, const BLevel &coarse
     return hipGetLastError();
 }
 
-template <typename VT, int P, int WM>
-void pair_launch(dim3 grid, dim3 block, hipStream_t s, const BMat &m, const BWork &w, const PairGeom &pg, void *out0,
-                 void *out1, const void *x, const void *rhs, const void *dinv, const void *xc, int64_t vld_c) {
+template <typename VT, int P, int WM, bool FP>
+void pair_launch_rows(dim3 grid, dim3 block, hipStream_t s, const BMat &m, const BWork &w, const PairGeom &pg,
+                      void *out0, void *out1, const void *x, const void *rhs, const void *dinv, const void *xc,
+                      int64_t vld_c, int nr) {
     typedef typename VecOf<VT>::type V;
-    if constexpr (P == PAIR_POST) {
-        if (xc) {
-            hipLaunchKernelGGL((k_b_pair<VT, P, WM, true>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,
+    if constexpr (sizeof(V) == 4) {   // FP64 vectors: two rows per thread would spill
+        if (nr == 2) {
+            hipLaunchKernelGGL((k_b_pair<VT, P, WM, FP, 2>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,
                                (const V *)x, (const V *)rhs, (const V *)dinv, (const V *)xc, vld_c);
             return;
         }
     }
-    hipLaunchKernelGGL((k_b_pair<VT, P, WM>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1, (const V *)x,
-                       (const V *)rhs, (const V *)dinv, (const V *)nullptr, (int64_t)0);
+    hipLaunchKernelGGL((k_b_pair<VT, P, WM, FP, 1>), grid, block, 0, s, m, w, pg, (V *)out0, (V *)out1,
+                       (const V *)x, (const V *)rhs, (const V *)dinv, (const V *)xc, vld_c);
+}
+
+template <typename VT, int P, int WM>
+void pair_launch(dim3 grid, dim3 block, hipStream_t s, const BMat &m, const BWork &w, const PairGeom &pg, void *out0,
+                 void *out1, const void *x, const void *rhs, const void *dinv, const void *xc, int64_t vld_c, int nr) {
+    if constexpr (P == PAIR_POST) {
+        if (xc) {
+            pair_launch_rows<VT, P, WM, true>(grid, block, s, m, w, pg, out0, out1, x, rhs, dinv, xc, vld_c, nr);
+            return;
+        }
+    }
+    pair_launch_rows<VT, P, WM, false>(grid, block, s, m, w, pg, out0, out1, x, rhs, dinv, nullptr, 0, nr);
 }
 
 hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void *out0, void *out1,
@@ -1531,7 +1579,11 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void
     pg.side = (int32_t)lv.side;
     pg.strips = (int32_t)((lv.side + kPairWidth - 1) / kPairWidth);
     pg.width = (int32_t)((lv.side + pg.strips - 1) / pg.strips);
-    pg.segs = (int32_t)((lv.side + kPairRows - 1) / kPairRows);
+    // two grid rows per thread on the larger levels (taller tiles: half the
+    // halo rows per node, twice the loads in flight per thread)
+    const int nr = lv.side >= kPairTallMinSide && vt != VT_F64 ? 2 : 1;
+    const int64_t rows = (int64_t)kPairTileRows * nr - 2;
+    pg.segs = (int32_t)((lv.side + rows - 1) / rows);
     if (lv.cap <= 0) return hipSuccess;
     const int64_t units = lv.cap * (int64_t)pg.strips * pg.segs;
     if (units > kMaxLogicalBlocks) return hipErrorInvalidValue;
@@ -1540,7 +1592,7 @@ hipError_t b_pair(ValueType vt, int pair, const BLevel &lv, const BWork &w, void
     const dim3 grid(tiled_grid(units, threads)), block(threads);
     const void *x = pair == PAIR_POST ? xin : lv.rhs;   // unused by PAIR_PRE
 #define PAIR_LAUNCH(VT, P, MFV) \
-    pair_launch<VT, P, MFV>(grid, block, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv, xc, vld_c)
+    pair_launch<VT, P, MFV>(grid, block, s, m, w, pg, out0, out1, x, lv.rhs, lv.dinv, xc, vld_c, nr)
 #define PAIR_WM(VT, P)                                                        \
     do {                                                                      \
         if (wmode == 2)                                                       \

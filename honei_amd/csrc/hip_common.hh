@@ -62,14 +62,46 @@ __device__ inline int64_t xcd_unblock(int64_t rb, int64_t nblocks) {
     return i * kXcds + x;
 }
 
+// Lane l < off of `v` receives lane l + off's value (other lanes: unspecified):
+// off 32 / 16 through the gfx950 lane-swap permutes, off 8 .. 1 through DPP
+// row shifts, every 32-bit half on its own -- register moves, no LDS trip.
+template <int OFF> __device__ inline uint32_t lane_down32(uint32_t v) {
+    if constexpr (OFF == 32) {
+        return __builtin_amdgcn_permlane32_swap(v, v, false, false)[1];   // src lanes 0..31 <- lanes 32..63
+    } else if constexpr (OFF == 16) {
+        return __builtin_amdgcn_permlane16_swap(v, v, false, false)[1];   // src row 0 <- row 1, row 2 <- row 3
+    } else {
+        static_assert(OFF >= 1 && OFF <= 8, "row shift");
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + OFF, 0xF, 0xF, true);   // row_shl:OFF
+    }
+}
+template <int OFF, typename T> __device__ inline T lane_down(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, lane_down32<OFF>(__builtin_bit_cast(uint32_t, v)));
+    } else {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = lane_down32<OFF>((uint32_t)b), hi = lane_down32<OFF>((uint32_t)(b >> 32));
+        return __builtin_bit_cast(T, lo | (hi << 32));
+    }
+}
+
+// Sum of a wavefront's values, in lane 0 (other lanes: partial sums).  Lane
+// 0 adds the pairs of the xor butterfly (lanes l and l + 32, then + 16, ...
+// + 1), so its bits are those of that tree.  Every lane of the wavefront
+// must be active.
 template <typename T> __device__ inline T wave_sum(T v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    v += lane_down<32>(v);
+    v += lane_down<16>(v);
+    v += lane_down<8>(v);
+    v += lane_down<4>(v);
+    v += lane_down<2>(v);
+    v += lane_down<1>(v);
     return v;
 }
 
 // Block-wide sum (any blockDim multiple of 64, at most 1024); result valid in
-// every thread.  Fixed order, hence deterministic.
+// every thread (wave_sum's lane 0 values, broadcast through LDS).  Fixed order, hence deterministic.
 template <typename T> __device__ inline T block_sum(T v) {
     __shared__ T red[1024 / kWave];
     v = wave_sum(v);

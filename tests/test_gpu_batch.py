@@ -186,9 +186,10 @@ def test_paired_passes_equal_separate_launches(outer, inner):
 @pytest.mark.parametrize("inner", [H.F32, H.BF16, H.F64])
 def test_paired_passes_equal_separate_launches_513_and_1025(inner):
     """The tile shapes of the headline batch's finest level (513^2: 9 column
-    strips of 57, 37 row bands) and of the serving stream's largest size
-    (1025^2: 17 strips of 61, 74 bands), one slot each: paired launches and
-    one launch per pass give the same iterations, defect and x."""
+    strips of 57, 18 row bands of 30 with FP32 / BF16 vectors, 37 of 14 with
+    FP64 ones) and of the serving stream's largest size (1025^2: 17 strips of
+    61, 35 or 74 bands), one slot each: paired launches and one launch per
+    pass give the same iterations, defect and x."""
     systems = {(0, 0): _system(10, 900), (1, 0): _system(9, 901)}
     kw = dict(outer_method=1, inner_vtype=inner, autotune=0, iters_per_launch=4)
     sep, _, _ = _run_batch([10, 9], [1, 1], systems, pair_sweeps=0, **kw)
