@@ -16,8 +16,8 @@ namespace {
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
-// Tiles of k_b_dir_spmv over a side x side grid: the paired passes' tiling
-// (at most kPairWidth columns, kPairRows rows per tile).
+// Tiles of k_b_dir_spmv over a side x side grid: the paired passes' column
+// strips (at most kPairWidth columns) in bands of kDirRows rows.
 struct DirTiles {
     int32_t tw, strips;
     int64_t ntiles;
@@ -27,7 +27,7 @@ DirTiles dir_tiles(int64_t side) {
     const int64_t strips = (side + kPairWidth - 1) / kPairWidth;
     t.strips = (int32_t)strips;
     t.tw = (int32_t)((side + strips - 1) / strips);
-    t.ntiles = strips * ((side + kPairRows - 1) / kPairRows);
+    t.ntiles = strips * ((side + kDirRows - 1) / kDirRows);
     return t;
 }
 

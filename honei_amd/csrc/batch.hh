@@ -70,11 +70,13 @@ constexpr int kMaxClasses = 16;
 // with the halo, one wavefront per row: 1024-thread work-groups), on levels
 // from kPairMinSide^2 nodes up; from kPairTallMinSide^2 nodes up every thread
 // takes two grid rows (tiles of 2 kPairTileRows - 2 rows).  The CG direction
-// launch tiles the finest grids in kPairRows-row bands.
+// launch tiles the finest grids in the same column strips and bands of
+// kDirRows rows (kDirNodes grid rows per thread).
 enum PairKind { PAIR_PRE = 0, PAIR_POST = 1 };
 constexpr int64_t kPairWidth = 62;
 constexpr int kPairTileRows = 16, kPairRows = kPairTileRows - 2;
 constexpr int64_t kPairMinSide = 33, kPairTallMinSide = 129;
+constexpr int kDirNodes = 2, kDirRows = kPairTileRows * kDirNodes - 2;
 
 struct BatchConfig {
     // VT_F32 or VT_BF16 matrix values with FP32 vectors (mixed precision),
@@ -184,7 +186,7 @@ struct BClassEntry {
     int64_t kld;
     int64_t palt;          // CG: offset of the second direction buffer from p (count * vld)
     int32_t tw, strips;    // tiles of k_b_dir_spmv: e.tw columns, `strips` tiles across
-    int64_t ntiles;        // tiles per slot (strips * row bands of kPairRows)
+    int64_t ntiles;        // tiles per slot (strips * row bands of kDirRows)
 };
 struct BClassTab {
     int32_t nclass;

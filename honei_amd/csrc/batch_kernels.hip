@@ -568,23 +568,25 @@ __global__ __launch_bounds__(256) void k_b_outer_rows(BClassTab tab, BWork w, co
 }
 
 // CG direction and q = A p in one tiled launch.  Logical block = one tile of
-// one listed slot's finest grid (kPairRows rows by e.tw columns, the tiling
-// of the paired passes), with a one-node halo: every thread forms
-// p = z + beta p_old at its node (z = |r| x_top; the node clamped into the
-// grid) from the same two loads and expression wherever that node is
-// formed, so the tiles agree on every p; the tile's own nodes store it into
-// the slot's other direction buffer (the current one stays intact for the
-// neighbouring tiles' halos).  After a barrier the own nodes form q = A p
-// from LDS with the row sums of k_b_outer_rows (interior: the nine terms in
-// column order; boundary: 0 + p), store q, and the tile stores one partial
-// p.q (wave sums, then its wavefronts in order) at its tile index.
-// FIN_ALPHA sums the slot's e.ntiles partials and makes the new buffer
-// current.  Z: the inner vectors' type.
+// one listed slot's finest grid (kDirRows rows by e.tw columns: the paired
+// passes' column strips, kDirNodes grid rows per thread), with a one-node
+// halo: every thread forms p = z + beta p_old at its nodes (z = |r| x_top;
+// the node clamped into the grid) from the same two loads and expression
+// wherever that node is formed, so the tiles agree on every p; the tile's own
+// nodes store it into the slot's other direction buffer (the current one
+// stays intact for the neighbouring tiles' halos).  After a barrier the own
+// nodes form q = A p from LDS with the row sums of k_b_outer_rows (interior:
+// the nine terms in column order; boundary: 0 + p), store q, and the tile
+// stores one partial p.q (each thread's nodes in order, wave sums, then its
+// wavefronts in order) at its tile index.  FIN_ALPHA sums the slot's
+// e.ntiles partials and makes the new buffer current.  Z: the inner vectors'
+// type.
 template <bool MF, typename Z>
-__global__ __launch_bounds__(kPairTileRows * kWave) void k_b_dir_spmv(BClassTab tab, BWork w,
-                                                                     const SlotScalars *sc) {
+__global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_dir_spmv(BClassTab tab, BWork w,
+                                                                        const SlotScalars *sc) {
+    constexpr int NR = kDirNodes, TR = kPairTileRows * NR;   // grid rows per thread, staged rows
     __shared__ int64_t pre[kMaxClasses + 1];
-    __shared__ double sp[kPairTileRows][kWave];
+    __shared__ double sp[TR][kWave];
     __shared__ double red[kPairTileRows];
     if (threadIdx.x == 0) {
         int64_t acc = 0;
@@ -630,51 +632,67 @@ __global__ __launch_bounds__(kPairTileRows * kWave) void k_b_dir_spmv(BClassTab 
         const double beta = tbeta[kt], scale = tscale[kt];
         const int64_t cur = tcur[kt];
         const int S = (int)e.side;
-        const int r0 = (int)seg * kPairRows, r1 = min(S, r0 + kPairRows);
+        const int r0 = (int)seg * kDirRows, r1 = min(S, r0 + kDirRows);
         const int c0 = (int)strip * e.tw, c1 = min(S, c0 + e.tw);
-        const int r = r0 - 1 + i, col = c0 - 1 + lane;
-        const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);
+        const int col = c0 - 1 + lane;
         const int colc = col < 0 ? 0 : (col > S - 1 ? S - 1 : col);
-        const int64_t node = (int64_t)rc * S + colc;
-        const bool own = i >= 1 && r < r1 && lane >= 1 && col < c1;
-        const bool bnd = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
-        // interior position of the row's weights (a boundary node's: unused)
-        const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2), ic = (uint32_t)min(max(colc, 1), S - 2);
+        const uint32_t ic = (uint32_t)min(max(colc, 1), S - 2);
         const double *pold = e.p + ls * e.vld + cur * e.palt;
         double *pnew = e.p + ls * e.vld + (1 - cur) * e.palt;
+        const Z *zs = (const Z *)e.x_top + g * e.vld_top;
+        int64_t node[NR];
+        bool own[NR], bnd[NR];
+        double zv[NR], po[NR], pv[NR];
+        double av[NR][9];
+        double cw[NR][4];
         // ---- every global load issued before any use ----
-        const double zv = (double)((const Z *)e.x_top)[g * e.vld_top + node];
-        const double po = beta == 0.0 ? 0.0 : pold[node];
-        double av[9];
-        double cw[4];
-        if constexpr (MF) {
-            const uint32_t nc = (uint32_t)S - 1u;
-            const double *cs = e.kappa + g * e.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
-            cw[0] = cs[0];
-            cw[1] = cs[1];
-            cw[2] = cs2[0];
-            cw[3] = cs2[1];
-        } else {
-            const double *ax = e.Ax + ls * e.ax_ld + ((int64_t)ir * S + ic);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) av[k] = load_value_nt(ax + (int64_t)k * e.stride);
+        for (int u = 0; u < NR; ++u) {
+            const int r = r0 - 1 + i + u * kPairTileRows;
+            const int rc = r < 0 ? 0 : (r > S - 1 ? S - 1 : r);
+            node[u] = (int64_t)rc * S + colc;
+            own[u] = r >= r0 && r < r1 && lane >= 1 && col < c1;
+            bnd[u] = rc == 0 || rc == S - 1 || colc == 0 || colc == S - 1;
+            // interior position of the row's weights (a boundary node's: unused)
+            const uint32_t ir = (uint32_t)min(max(rc, 1), S - 2);
+            zv[u] = (double)zs[node[u]];
+            po[u] = beta == 0.0 ? 0.0 : pold[node[u]];
+            if constexpr (MF) {
+                const uint32_t nc = (uint32_t)S - 1u;
+                const double *cs = e.kappa + g * e.kld + ((ir - 1u) * nc + (ic - 1u)), *cs2 = cs + nc;
+                cw[u][0] = cs[0];
+                cw[u][1] = cs[1];
+                cw[u][2] = cs2[0];
+                cw[u][3] = cs2[1];
+            } else {
+                const double *ax = e.Ax + ls * e.ax_ld + ((int64_t)ir * S + ic);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) av[u][k] = load_value_nt(ax + (int64_t)k * e.stride);
+            }
         }
         // ---- p at the tile and its halo ----
-        const double z = zv * scale;
-        const double pv = beta == 0.0 ? z : z + beta * po;
-        sp[i][lane] = pv;
-        if (own) pnew[node] = pv;
-        if constexpr (MF) q1_cell_weights<double>(cw[0], cw[1], cw[2], cw[3], av);
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            const double z = zv[u] * scale;
+            pv[u] = beta == 0.0 ? z : z + beta * po[u];
+            sp[i + u * kPairTileRows][lane] = pv[u];
+            if (own[u]) pnew[node[u]] = pv[u];
+        }
         __syncthreads();
         // ---- q = A p at the own nodes, the tile's p.q ----
         double part = 0.0;
-        if (own) {
-            double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) t = t + av[k] * sp[i + (k / 3 - 1)][lane + (k % 3 - 1)];
-            if (bnd) t = 0.0 + pv;
-            e.q[ls * e.vld + node] = t;
-            part = pv * t;
+        for (int u = 0; u < NR; ++u) {
+            if constexpr (MF) q1_cell_weights<double>(cw[u][0], cw[u][1], cw[u][2], cw[u][3], av[u]);
+            const int I = i + u * kPairTileRows;
+            if (own[u]) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) t = t + av[u][k] * sp[I + (k / 3 - 1)][lane + (k % 3 - 1)];
+                if (bnd[u]) t = 0.0 + pv[u];
+                e.q[ls * e.vld + node[u]] = t;
+                part = part + pv[u] * t;
+            }
         }
         const double v = wave_sum(part);
         if (lane == 0) red[i] = v;
@@ -1459,7 +1477,7 @@ hipError_t b_dir_spmv(const BClassTab &tab, const BWork &w, int64_t max_tiles, c
         const BClassEntry &e = tab.c[c];
         const int64_t strips = (e.side + kPairWidth - 1) / kPairWidth;
         if (e.side < 3 || e.side > 65535 || e.strips != strips || e.tw != (e.side + strips - 1) / strips ||
-            e.ntiles != strips * ((e.side + kPairRows - 1) / kPairRows) || e.ntiles > e.pst || !e.p || !e.q ||
+            e.ntiles != strips * ((e.side + kDirRows - 1) / kDirRows) || e.ntiles > e.pst || !e.p || !e.q ||
             (mf && (!e.kappa || e.kld < (e.side - 1) * (e.side - 1))) || (!mf && !e.Ax))
             return hipErrorInvalidValue;
     }
