@@ -186,6 +186,9 @@ struct PairGeom {
 // Tiled launches stage the slots of a work-group's tiles in LDS at the
 // start; their grids give no work-group more than kTileSlots tiles.
 constexpr int kTileSlots = 1024;
+// The element-wise outer launches stage at most kElemSlots logical blocks'
+// slots and scalars per work-group.
+constexpr int kElemSlots = 64;
 inline int tiled_grid(int64_t units, int threads) {
     const int64_t g = batch_grid(units, threads), need = (units + kTileSlots - 1) / kTileSlots;
     return (int)(g > need ? g : need);
@@ -707,7 +710,10 @@ __global__ __launch_bounds__(kPairTileRows * kWave, 8) void k_b_dir_spmv(BClassT
 // Element-wise ops: logical block = R consecutive 256-row blocks of one
 // listed slot; thread t takes row t of each, with every load of the R rows
 // issued before any use (rows past the end load the last row and are
-// dropped), so one slot lookup serves R rows and their loads overlap.
+// dropped), so one slot lookup serves R rows and their loads overlap.  The
+// slots of a work-group's logical blocks (at most kElemSlots: the launcher
+// sizes the grid so) and the slot scalars the op uses are staged in LDS at
+// the start, so a logical block waits on its row loads alone.
 // Z: the inner vectors' type (float, or double for an FP64 inner cycle).
 template <int OP, int R, typename Z>
 __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, const SlotScalars *sc) {
@@ -725,25 +731,44 @@ __global__ __launch_bounds__(256) void k_b_outer_elem(BClassTab tab, BWork w, co
     }
     __syncthreads();
     const int64_t total = pre[tab.nclass];
-    for (int64_t lb = first_block(); lb < total; lb += gridDim.x) {
+    // per logical block: the slot, two scalars (UPDATE: alpha, 1/|r| of the
+    // check; SCALE: 1/|r|; ACC, DOT_RZ: the cycle's scale) and a flag
+    // (UPDATE: the current direction buffer; SCALE: r recomputed; ACC: the
+    // first iteration)
+    __shared__ int32_t tslot[kElemSlots], tflag[kElemSlots];
+    __shared__ double ta[kElemSlots], tb[kElemSlots];
+    const int64_t lb0 = first_block();
+    const int64_t mine = lb0 < total ? (total - 1 - lb0) / gridDim.x + 1 : 0;
+    for (int64_t k = threadIdx.x; k < mine; k += blockDim.x) {
+        const int64_t lb = lb0 + k * gridDim.x;
+        int c = 0;
+        while (lb >= pre[c + 1]) ++c;
+        const int64_t g = w.list[w.bounds[c] + (lb - pre[c]) / ((tab.c[c].bps + R - 1) / R)];
+        const SlotScalars &st = sc[g];
+        tslot[k] = (int32_t)g;
+        ta[k] = OP == OP_UPDATE ? st.alpha : (OP == OP_SCALE ? st.inv : st.scale);
+        tb[k] = st.inv_w;
+        tflag[k] = OP == OP_UPDATE ? (st.pcur & 1) : (OP == OP_SCALE ? st.fresh : (st.iters == 1));
+    }
+    __syncthreads();
+    int64_t kt = 0;
+    for (int64_t lb = lb0; lb < total; lb += gridDim.x, ++kt) {
         int c = 0;
         while (lb >= pre[c + 1]) ++c;
         const BClassEntry &e = tab.c[c];
         int64_t si, sb;   // slot of the class list, group of R row blocks
         div32(lb - pre[c], (e.bps + R - 1) / R, si, sb);
-        const int64_t g = w.list[w.bounds[c] + si];
+        const int64_t g = tslot[kt];
         const int64_t ls = g - e.first;
-        const SlotScalars &st = sc[g];
+        const int32_t flag = tflag[kt];
         // a running CG slot's rhs was written by OP_UPDATE (uniform per block)
-        if (OP == OP_SCALE && !st.fresh) continue;
+        if (OP == OP_SCALE && !flag) continue;
         const double *rs = e.r + ls * e.vld, *qs = e.q + ls * e.vld;
-        const double *ps = e.p + ls * e.vld + (OP == OP_UPDATE ? (int64_t)(st.pcur & 1) * e.palt : 0);
+        const double *ps = e.p + ls * e.vld + (OP == OP_UPDATE ? (int64_t)flag * e.palt : 0);
         const double *xs = e.x + ls * e.vld;
         const Z *zs = (const Z *)e.x_top + g * e.vld_top;
-        // slot scalars and output bases in registers: the stores below could
-        // alias the slot record, which would make every row reload them
-        const double alpha = st.alpha, inv = st.inv, inv_w = st.inv_w, scale = st.scale;
-        const bool first_acc = st.iters == 1;
+        const double alpha = ta[kt], inv = ta[kt], scale = ta[kt], inv_w = tb[kt];
+        const bool first_acc = flag != 0;
         double *const xw = e.x + ls * e.vld, *const rw = e.r + ls * e.vld;
         Z *const tw = (Z *)e.rhs_top + g * e.vld_top;
         const int64_t n = e.n;
@@ -1438,7 +1463,10 @@ hipError_t b_outer(OuterOp op, const BClassTab &tab, const BWork &w, int64_t max
         for (int c = 0; c < tab.nclass; ++c)
             if (!tab.c[c].kappa || tab.c[c].kld < (tab.c[c].side - 1) * (tab.c[c].side - 1)) return hipErrorInvalidValue;
     constexpr int G = kOuterRowGroup, R = kOuterElemBlocks;
-    const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)), ge(batch_grid((max_blocks + R - 1) / R, 256));
+    const int64_t ue = (max_blocks + R - 1) / R;
+    const int ge_min = (int)((ue + kElemSlots - 1) / kElemSlots);   // at most kElemSlots logical blocks each
+    const dim3 block(256), gr(batch_grid((max_blocks + G - 1) / G, 256)),
+        ge(std::max(batch_grid(ue, 256), ge_min));
     switch (op) {
         case OP_DEFECT_ALL:
             if (mf)
