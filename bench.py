@@ -519,12 +519,13 @@ def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5, mf=True):
                  operator (not with mf), b copied, x0;
       settle   : r = b - A x in FP64 at the start and at the verification
                  (nnz*8 + 24n each, mf: 8 per cell + 24n; neighbours from
-                 the grid, no indices);
+                 the grid, no indices), each followed by r/|r| -> FP32
+                 (12n);
       per iteration, on the finest level n = n_L:
-                 scale r/|r| -> FP32 (12n), r.z (12n), p = z + beta p with
-                 q = A p and p.q in one launch (nnz*8 + 28n, mf: 8 per cell +
-                 28n: z and p read, the new p and q written), x, r update
-                 with |r|^2 (48n);
+                 r.z (12n), p = z + beta p with q = A p and p.q in one
+                 launch (nnz*8 + 28n, mf: 8 per cell + 28n: z and p read,
+                 the new p and q written), x, r update with |r|^2 and the
+                 next cycle's FP32 right-hand side (52n);
       per iteration, per level l (n_l, nnz_l):
                  l >= pair_from: paired Jacobi0 + defect and paired two
                  sweeps (nnz_l*vs + 16 n_l each, mf: 8 per cell + 16 n_l),
@@ -553,8 +554,8 @@ def slot_solve_bytes(L, iters, vs=4, fused=4, pair_from=5, mf=True):
         build += 8 * cells(l) + (nnz_(l) * vs if stored else 0) + 4 * n_(l)
         if l < L:
             build += 8 * cells(l + 1) + 8 * cells(l)
-    settle = 2 * (op64 + 24 * n)
-    it = 12 * n + 12 * n + (op64 + 28 * n) + 48 * n
+    settle = 2 * (op64 + 24 * n + 12 * n)
+    it = 12 * n + (op64 + 28 * n) + 52 * n
     for l in range(2, L + 1):
         nl, zl, nc = n_(l), nnz_(l), n_(l - 1)
         transfers = (4 * nl + 4 * nc) + (8 * nl + 4 * nc)

@@ -8,7 +8,11 @@ global index in bounds, every output node written exactly once, and every
 staged entry a pass actually uses belonging to the neighbour it stands for.
 The GPU test test_paired_passes_equal_separate_launches checks the kernel
 itself; this one pins the indexing on the CPU for sides the GPU tests do not
-cover (including sides that are not 2^l + 1).
+cover (including sides that are not 2^l + 1), for both tile heights: 16
+staged rows (one grid row per thread, 14-row tiles) and 32 (two grid rows per
+thread, tile row I on thread row I mod 16: 30-row tiles).  The thread that
+computes a tile row does not enter the arithmetic, so the model indexes by
+tile row.
 """
 import numpy as np
 import pytest
@@ -165,9 +169,10 @@ def pair_kernel(Ax, stride, S, rhs, dinv, xin, pair, W=KPAIR_WIDTH, TR=KPAIR_TIL
     return out0, out1
 
 
-@pytest.mark.parametrize("side", [3, 5, 17, 33, 40, 65, 97])
+@pytest.mark.parametrize("side,tr", [(3, 16), (5, 16), (17, 16), (33, 16), (40, 16), (65, 16), (97, 16),
+                                     (5, 32), (33, 32), (40, 32), (65, 32), (97, 32)])
 @pytest.mark.parametrize("pair", ["pre", "post"])
-def test_pair_tiles_equal_two_passes(side, pair):
+def test_pair_tiles_equal_two_passes(side, tr, pair):
     rng = np.random.default_rng(side)
     Ax, stride = q1_level(side, rng)
     n = side * side
@@ -175,7 +180,7 @@ def test_pair_tiles_equal_two_passes(side, pair):
     dinv = rng.uniform(0.2, 0.4, n).astype(f32)
     xin = rng.uniform(-1, 1, n).astype(f32)
     a0, a1 = two_pass(Ax, stride, side, rhs, dinv, xin, pair)
-    b0, b1 = pair_kernel(Ax, stride, side, rhs, dinv, xin, pair)
+    b0, b1 = pair_kernel(Ax, stride, side, rhs, dinv, xin, pair, TR=tr)
     assert np.array_equal(a0.view(np.uint32), b0.view(np.uint32))
     if a1 is not None:
         assert np.array_equal(a1.view(np.uint32), b1.view(np.uint32))
@@ -193,9 +198,18 @@ def xcd_block(b, nblocks):
     return x * q + min(x, r) + i
 
 
+KTILE_SLOTS = 1024            # kTileSlots (batch_kernels.hip)
+
+
 def batch_grid(items, block):
     cap = KRESIDENT_THREADS // block
     return 1 if items < 1 else min(items, cap)
+
+
+def tiled_grid(units, block):
+    """batch_kernels.hip tiled_grid: enough work-groups that none walks more
+    than kTileSlots logical blocks."""
+    return max(batch_grid(units, block), (units + KTILE_SLOTS - 1) // KTILE_SLOTS)
 
 
 def pair_geometry(S, W=KPAIR_WIDTH, R=KPAIR_TILE_ROWS - 2):
@@ -205,24 +219,30 @@ def pair_geometry(S, W=KPAIR_WIDTH, R=KPAIR_TILE_ROWS - 2):
     return strips, width, segs
 
 
-@pytest.mark.parametrize("side,slots", [(33, 61), (65, 3), (257, 53), (513, 1), (513, 51), (1025, 1), (1025, 7)])
-def test_pair_launch_visits_every_tile_once(side, slots):
+@pytest.mark.parametrize("nr", [1, 2])
+@pytest.mark.parametrize("side,slots", [(33, 61), (65, 3), (257, 53), (513, 1), (513, 51), (1025, 1), (1025, 7),
+                                        (1025, 300)])
+def test_pair_launch_visits_every_tile_once(side, slots, nr):
     """The launch's grid of 1024-thread work-groups, each walking logical
     blocks lb = xcd_block(b) + k * grid, with lb -> (listed slot e, seg,
     strip) by the kernel's two divisions (strips minor): every tile of every
-    listed slot is visited exactly once, consecutive logical blocks are the
-    side-by-side tiles of one band of rows, and the tiles of a slot cover its
-    grid exactly once with widths <= 62."""
-    strips, width, segs = pair_geometry(side)
+    listed slot is visited exactly once, no work-group walks more than the
+    kTileSlots logical blocks whose slots it stages in LDS, consecutive
+    logical blocks are the side-by-side tiles of one band of rows, and the
+    tiles of a slot cover its grid exactly once with widths <= 62.  nr: grid
+    rows per thread (tiles of 16 nr - 2 rows)."""
+    R = KPAIR_TILE_ROWS * nr - 2
+    strips, width, segs = pair_geometry(side, R=R)
     assert width <= KPAIR_WIDTH and (strips - 1) * width < side
-    R = KPAIR_TILE_ROWS - 2
     ups = strips * segs
     total = slots * ups
-    grid = batch_grid(total, KPAIR_TILE_ROWS * 64)
+    grid = tiled_grid(total, KPAIR_TILE_ROWS * 64)
     seen = np.zeros((slots, segs, strips), np.int32)
     first_lb = {}
     for b in range(grid):
         lb = xcd_block(b, grid)
+        mine = (total - 1 - lb) // grid + 1 if lb < total else 0
+        assert mine <= KTILE_SLOTS
         while lb < total:
             e, rem = divmod(lb, ups)
             seg, strip = divmod(rem, strips)
