@@ -436,6 +436,9 @@ STEP_ITERS_PER_LAUNCH = 24
 
 def batched_pool(levels, **cfg):
     cfg.setdefault("iters_per_launch", STEP_ITERS_PER_LAUNCH)
+    # repeated batches: each step's launch sized from the previous step's
+    # largest iteration count (the first step runs STEP_ITERS_PER_LAUNCH)
+    cfg.setdefault("first_launch_from_history", 1)
     lv = sorted(set(levels), reverse=True)
     counts = [levels.count(v) for v in lv]
     B = H.Batch(lv, counts, H.batch_config(**cfg))

@@ -463,7 +463,7 @@ class BatchConfig(C.Structure):
                 ("tolerance", _d), ("max_outer", C.c_int32), ("averaging", C.c_int32),
                 ("iters_per_launch", C.c_int32), ("autotune", C.c_int32), ("outer_method", C.c_int32),
                 ("fused_levels", C.c_int32), ("use_graphs", C.c_int32), ("pair_sweeps", C.c_int32),
-                ("matrix_free", C.c_int32)]
+                ("matrix_free", C.c_int32), ("first_launch_from_history", C.c_int32)]
 
 
 class SlotInfo(C.Structure):

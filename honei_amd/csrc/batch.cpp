@@ -69,8 +69,10 @@ void BatchMG::release() {
     if (graph_) (void)hipGraphDestroy(graph_);
     if (exec1_) (void)hipGraphExecDestroy(exec1_);
     if (graph1_) (void)hipGraphDestroy(graph1_);
-    exec_ = exec1_ = nullptr;
-    graph_ = graph1_ = nullptr;
+    if (execf_) (void)hipGraphExecDestroy(execf_);
+    if (graphf_) (void)hipGraphDestroy(graphf_);
+    exec_ = exec1_ = execf_ = nullptr;
+    graph_ = graph1_ = graphf_ = nullptr;
     for (hipEvent_t e : load_ev_)
         if (e) (void)hipEventDestroy(e);
     load_ev_.clear();
@@ -730,14 +732,36 @@ hipError_t BatchMG::activate_ready(bool wait) {
 
 hipError_t BatchMG::step() {
     HX(activate_ready(false));
+    // the first launch of a batch (none since the pool last ran empty),
+    // sized from the last batch
+    const bool first = cfg_.first_launch_from_history && !batch_launched_ && running_ > 0 && first_iters_ > 0;
+    if (running_ > 0) batch_launched_ = true;
     // a tail: few slots still running and none waiting to join
     const bool tail = cfg_.iters_per_launch > 1 && loading_.empty() && running_ <= std::max<int64_t>(1, nslots_ / 32);
     if (cfg_.use_graphs) {
-        HX(hipGraphLaunch(tail ? exec1_ : exec_, s_main_));
+        HX(hipGraphLaunch(first ? execf_ : (tail ? exec1_ : exec_), s_main_));
     } else {
-        for (int i = 0; i < (tail ? 1 : cfg_.iters_per_launch); ++i) HX(enqueue_iteration(s_main_));
+        const int n = first ? first_iters_ : (tail ? 1 : cfg_.iters_per_launch);
+        for (int i = 0; i < n; ++i) HX(enqueue_iteration(s_main_));
     }
     ++launches_;
+    return hipSuccess;
+}
+
+// Once a batch has retired entirely: the iteration count its first launch
+// would have needed, and (with graphs) that launch captured now, outside the
+// next batch's time.
+hipError_t BatchMG::note_batch_end() {
+    const int32_t k = std::min<int32_t>(batch_max_iters_ + 2, cfg_.iters_per_launch);
+    batch_max_iters_ = 0;
+    if (k == first_iters_) return hipSuccess;
+    if (execf_) (void)hipGraphExecDestroy(execf_);
+    if (graphf_) (void)hipGraphDestroy(graphf_);
+    execf_ = nullptr;
+    graphf_ = nullptr;
+    first_iters_ = 0;
+    if (cfg_.use_graphs) HX(capture_iterations(k, &graphf_, &execf_));
+    first_iters_ = k;
     return hipSuccess;
 }
 
@@ -746,12 +770,19 @@ hipError_t BatchMG::poll(std::vector<SlotScalars> &out, std::vector<int64_t> *re
     HX(hipStreamSynchronize(s_main_));
     out.assign(sc_host_, sc_host_ + nslots_);
     if (retired) retired->clear();
+    bool any = false;
     for (int64_t g = 0; g < nslots_; ++g) {
         if (host_state_[(size_t)g] == 2 && sc_host_[g].state == SLOT_IDLE) {
             host_state_[(size_t)g] = 0;
             --running_;
+            batch_max_iters_ = std::max(batch_max_iters_, sc_host_[g].iters);
+            any = true;
             if (retired) retired->push_back(g);
         }
+    }
+    if (running_ == 0 && loading_.empty()) {
+        batch_launched_ = false;
+        if (cfg_.first_launch_from_history && any) HX(note_batch_end());
     }
     return hipSuccess;
 }

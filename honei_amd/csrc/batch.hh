@@ -112,6 +112,12 @@ struct BatchConfig {
     //      four per node, interleaved (q1_stencil_node; one per cell, edge
     //      and node), built with the hierarchy; FP64 outer as 1.
     int32_t matrix_free = 1;
+    // 1: a launch that finds no slot running (the first of a batch) runs as
+    // many outer iterations as the last such batch needed until its last
+    // slot retired (its largest iteration count + 2: the check that meets
+    // the tolerance and the verification), at most iters_per_launch; later
+    // launches unchanged.  0: every launch runs iters_per_launch.
+    int32_t first_launch_from_history = 0;
 };
 
 // Inner level l of the pool: slots [0, cap) present.
@@ -368,6 +374,13 @@ class BatchMG {
     hipGraph_t graph_ = nullptr, graph1_ = nullptr;      // iters_per_launch / one iteration
     int64_t dir_tiles_ = 0;   // k_b_dir_spmv tiles of all slots of all classes
     hipGraphExec_t exec_ = nullptr, exec1_ = nullptr;
+    // first_launch_from_history: the first launch's iteration count (0: none
+    // known yet), its graph, the largest iteration count of the slots retired
+    // since the pool last ran empty, and whether it launched since then
+    int32_t first_iters_ = 0, batch_max_iters_ = 0;
+    bool batch_launched_ = false;
+    hipGraph_t graphf_ = nullptr;
+    hipGraphExec_t execf_ = nullptr;
     std::vector<int> host_state_;      // 0 free, 1 loading, 2 running
     std::vector<int64_t> loading_;
     int64_t running_ = 0, launches_ = 0;
@@ -383,6 +396,7 @@ class BatchMG {
     hipError_t tune_level(BLevel &lv);
     hipError_t capture();
     hipError_t capture_iterations(int iters, hipGraph_t *graph, hipGraphExec_t *exec);
+    hipError_t note_batch_end();
     hipError_t enqueue_iteration(hipStream_t s);
     hipError_t enqueue_vcycle(hipStream_t s);
     BWork level_work(int64_t level) const;

@@ -45,6 +45,7 @@ BatchConfig to_config(const honei_hip_batch_config &c) {
     b.use_graphs = c.use_graphs;
     b.pair_sweeps = c.pair_sweeps;
     b.matrix_free = c.matrix_free;
+    b.first_launch_from_history = c.first_launch_from_history;
     return b;
 }
 
@@ -79,6 +80,7 @@ int honei_hip_batch_config_default(honei_hip_batch_config *cfg) {
     cfg->use_graphs = b.use_graphs;
     cfg->pair_sweeps = b.pair_sweeps;
     cfg->matrix_free = b.matrix_free;
+    cfg->first_launch_from_history = b.first_launch_from_history;
     return HONEI_HIP_SUCCESS;
 }
 
@@ -111,6 +113,8 @@ int honei_hip_batch_create(void **batch, const int64_t *levels, const int64_t *c
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "pair_sweeps 0 or 1");
     if (cfg->matrix_free < 0 || cfg->matrix_free > 2)
         return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "matrix_free 0, 1 or 2");
+    if (cfg->first_launch_from_history != 0 && cfg->first_launch_from_history != 1)
+        return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "first_launch_from_history 0 or 1");
     if (!(cfg->tolerance > 0.0) || !(cfg->omega > 0.0)) return fail(HONEI_HIP_ERROR_INVALID_ARGUMENT, "tolerance, omega > 0");
     BatchMG *b = new (std::nothrow) BatchMG();
     if (!b) return fail(HONEI_HIP_ERROR_OUT_OF_MEMORY, "host allocation failed");

@@ -474,6 +474,15 @@ typedef struct {
                                   distinct weights (one per cell, edge and node) stored four per
                                   node, interleaved, built with the hierarchy; the FP64 outer
                                   forms them as 1 */
+    int32_t first_launch_from_history; /* 1: a honei_hip_batch_step that finds no slot running
+                                  (the first launch of a batch) runs as many outer iterations
+                                  as the last such batch needed until its last slot retired
+                                  (its largest outer iteration count + 2, at most
+                                  iters_per_launch; with graphs captured when that batch's
+                                  retirement is polled), so a repeated workload finishes in one
+                                  launch without trailing iterations over empty lists; later
+                                  launches unchanged, results identical.  0 (default): every
+                                  launch runs iters_per_launch */
 } honei_hip_batch_config;
 
 typedef struct {

@@ -78,6 +78,44 @@ def _run_batch(levels, counts, systems, **cfg):
     return out, launches, B
 
 
+def test_first_launch_from_history():
+    """first_launch_from_history = 1: a batch's first launch runs the last
+    batch's largest outer iteration count + 2 iterations.  After a batch whose
+    one system has b = 0 (retired at its first check: 2 iterations), the next
+    batch needs more than its first launch; the batch after that, the same
+    systems again, finishes in its first launch.  Every slot's iterations,
+    defect and x equal those of launches of iters_per_launch."""
+    levels, counts = [7, 5], [2, 3]
+    systems = {(c, j): _system(levels[c], 520 + 10 * c + j) for c in range(2) for j in range(counts[c])}
+    ref, nref, _ = _run_batch(levels, counts, systems, iters_per_launch=24)
+    assert nref == 1
+    B = H.Batch(levels, counts, H.batch_config(iters_per_launch=24, first_launch_from_history=1))
+    keep = []
+
+    def load(items):
+        for (c, j), (kappa, b) in items.items():
+            dk, db = H.DeviceArray.from_host(kappa, np.float64), H.DeviceArray.from_host(b)
+            keep.extend([dk, db])
+            B.load(B.slot(c, j), dk, db)
+
+    kappa0, b0 = systems[(1, 0)]
+    load({(1, 0): (kappa0, np.zeros_like(b0))})
+    assert B.run() == 1
+    assert B.poll()[B.slot(1, 0)].iters == 0
+    for expect_one in (False, True):
+        load(systems)
+        n = B.run()
+        assert (n == 1) if expect_one else (n >= 2), n
+        info = B.poll()
+        for (c, j) in systems:
+            g = B.slot(c, j)
+            x = H.DeviceArray(len(systems[(c, j)][1]), np.float64)
+            B.result(g, x)
+            i = info[g]
+            assert (i.state, i.iters, i.status, i.rel, i.level) == ref[(c, j)][0], (expect_one, c, j)
+            assert np.array_equal(x.get(), ref[(c, j)][1]), (expect_one, c, j)
+
+
 def test_fill_fields_match_host_generators():
     n = 10007
     u = H.DeviceArray(n, np.float64)
