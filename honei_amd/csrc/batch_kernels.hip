@@ -65,15 +65,30 @@ __global__ __launch_bounds__(1024) void k_b_compact(const SlotScalars *sc, int64
     const int64_t s0 = t * chunk, s1 = (s0 + chunk < nslots) ? s0 + chunk : nslots;
     int32_t cnt = 0;
     for (int64_t g = s0; g < s1; ++g) cnt += listed(sc[g], mode);
-    pre[t] = cnt;
-    __syncthreads();
-    // inclusive scan (Hillis-Steele) over 1024 counts
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t v = t >= off ? pre[t - off] : 0;
-        __syncthreads();
-        pre[t] += v;
-        __syncthreads();
+    // inclusive scan over the 1024 counts: within each wavefront, then over
+    // the wavefront totals (integer sums: any order gives the same counts)
+    __shared__ int32_t wtot[1024 / kWave];
+    const int lane = t % kWave, wv = t / kWave;
+    int32_t v = cnt;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
     }
+    if (lane == kWave - 1) wtot[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        int32_t x = lane < 1024 / kWave ? wtot[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < 1024 / kWave; off <<= 1) {
+            const int32_t u = __shfl_up(x, off);
+            if (lane >= off) x += u;
+        }
+        if (lane < 1024 / kWave) wtot[lane] = x;
+    }
+    __syncthreads();
+    pre[t] = v + (wv > 0 ? wtot[wv - 1] : 0);
+    __syncthreads();
     int32_t pos = pre[t] - cnt;
     for (int64_t g = s0; g < s1; ++g)
         if (listed(sc[g], mode)) list[pos++] = (int32_t)g;
