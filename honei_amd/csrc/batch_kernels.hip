@@ -204,6 +204,9 @@ constexpr int kTileSlots = 1024;
 // The element-wise outer launches stage at most kElemSlots logical blocks'
 // slots and scalars per work-group.
 constexpr int kElemSlots = 64;
+// The fused coarse cycle's work-group: one per slot, five wavefronts, so a
+// sweep of the largest fused level (17^2 = 289 nodes) is one round.
+constexpr int kFusedThreads = 320;
 inline int tiled_grid(int64_t units, int threads) {
     const int64_t g = batch_grid(units, threads), need = (units + kTileSlots - 1) / kTileSlots;
     return (int)(g > need ? g : need);
@@ -1258,7 +1261,7 @@ __device__ inline void fused_cycle_body(const FusedCycle &fc, int64_t g, int top
 }
 
 template <typename VT>
-__global__ __launch_bounds__(256) void k_b_coarse_vcycle(FusedCycle fc, BWork w) {
+__global__ __launch_bounds__(kFusedThreads) void k_b_coarse_vcycle(FusedCycle fc, BWork w) {
     const int64_t a0 = active_first(w), na = active_count(w);
     for (int64_t a = blockIdx.x; a < na; a += gridDim.x) {
         const int64_t g = w.list[a0 + a];
@@ -1280,7 +1283,7 @@ __device__ inline void lds_copy_words(void *dst, const void *src, int64_t bytes)
     for (int64_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
 }
 template <typename VT>
-__global__ __launch_bounds__(256) void k_b_coarse_vcycle_lds(FusedCycle fc, BWork w) {
+__global__ __launch_bounds__(kFusedThreads) void k_b_coarse_vcycle_lds(FusedCycle fc, BWork w) {
     typedef typename VecOf<VT>::type V;
     extern __shared__ __align__(16) char lds[];
     const int64_t a0 = active_first(w), na = active_count(w);
@@ -1765,9 +1768,9 @@ hipError_t b_coarse_vcycle(ValueType vt, const FusedCycle &fc, const BWork &w, i
 #define CYCLE_LAUNCH(VT)                                                                                  \
     do {                                                                                                  \
         if (lds)                                                                                          \
-            hipLaunchKernelGGL(k_b_coarse_vcycle_lds<VT>, dim3(grid), dim3(256), (size_t)at, s, f, w);   \
+            hipLaunchKernelGGL(k_b_coarse_vcycle_lds<VT>, dim3(grid), dim3(kFusedThreads), (size_t)at, s, f, w); \
         else                                                                                              \
-            hipLaunchKernelGGL(k_b_coarse_vcycle<VT>, dim3(grid), dim3(256), 0, s, f, w);                \
+            hipLaunchKernelGGL(k_b_coarse_vcycle<VT>, dim3(grid), dim3(kFusedThreads), 0, s, f, w);      \
     } while (0)
     if (vt == VT_F32)
         CYCLE_LAUNCH(float);
